@@ -1,0 +1,239 @@
+/* kpt_abi.h — C-ABI of libkpt_hip.so, the MI355X (gfx950) kernel library
+ * behind the KittlesPT renderer API.
+ *
+ * The reference has no C-ABI: its boundary is the C++ class
+ * KittlesPT::Renderer (KittlesPT/src/renderer.hpp:13-58) over POD scene types
+ * (KittlesPT/src/pod_types.hpp:9-219), and below it the host->kernel seam
+ * launch*Kernel(const ShaderData&) (KittlesPT/src/shaders/kernels.cuh:10-19).
+ * Every entry point below names the reference call it replaces.  Plain
+ * pointers and sizes only; every call returns an int status (KPT_OK == 0) and
+ * never exits the process (the reference calls exit(99) on any CUDA error,
+ * KittlesPT/src/error_check.cu:4-13).  A context owns one HIP stream and all of
+ * its device memory; it is not thread-safe, separate contexts may be used from
+ * separate threads.
+ */
+#ifndef KPT_ABI_H
+#define KPT_ABI_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KPT_ABI_VERSION 1
+
+enum kpt_status {
+    KPT_OK = 0,
+    KPT_ERR_INVALID_ARG = 1,
+    KPT_ERR_HIP = 2,
+    KPT_ERR_OUT_OF_MEMORY = 3,
+    KPT_ERR_STATE = 4,        /* call order violated (e.g. render before upload) */
+    KPT_ERR_OVERFLOW = 5,     /* traversal stack or queue capacity exceeded */
+    KPT_ERR_NO_DEVICE = 6,
+    KPT_ERR_UNSUPPORTED = 7
+};
+
+typedef struct kpt_ctx kpt_ctx;
+
+/* MaterialSceneEntity (pod_types.hpp:60-123) without the name string. */
+typedef struct kpt_material {
+    int32_t albedo_texture_id;
+    float albedo[3];
+    int32_t orm_texture_id;
+    float metallic;
+    float roughness;
+    int32_t transmission_texture_id;
+    float transmission;
+    float ior;
+    int32_t emission_texture_id;
+    float emission_factor[3];
+    float emission_scale_nits;
+    int32_t normal_texture_id;
+    float normal_scale;
+} kpt_material;
+
+/* TextureSceneEntity (pod_types.hpp:163-175): 8-bit texels at `offset` bytes
+ * into kpt_scene_desc.pixels. */
+typedef struct kpt_texture_desc {
+    uint32_t width;
+    uint32_t height;
+    uint32_t channels;
+    uint32_t reserved;
+    uint64_t offset;
+} kpt_texture_desc;
+
+/* Flattened BasicScene (pod_types.hpp:192-219).  Triangles of all meshes are
+ * stored back to back; mesh m owns [mesh_first[m], mesh_first[m]+mesh_count[m]).
+ * Instances place a mesh with a model matrix (column-major, glm layout).  The
+ * reference has one instance per mesh and requires setMeshTransform(i, M_i)
+ * after loadScene (renderer.cu:250-269, blas.cuh:190-200); the upload performs
+ * both steps.  num_instances == 0 means "one instance per mesh" using
+ * mesh_model. */
+typedef struct kpt_scene_desc {
+    int32_t num_triangles;
+    const float* positions;     /* [num_triangles][3][3] */
+    const float* normals;       /* [num_triangles][3][3] */
+    const float* uvs;           /* [num_triangles][3][2] */
+    const int32_t* material_ids;/* [num_triangles] */
+    int32_t num_meshes;
+    const int32_t* mesh_first;  /* [num_meshes] */
+    const int32_t* mesh_count;  /* [num_meshes] */
+    const float* mesh_model;    /* [num_meshes][16], used when num_instances == 0 */
+    int32_t num_instances;
+    const int32_t* instance_mesh;   /* [num_instances] */
+    const float* instance_model;    /* [num_instances][16] */
+    int32_t num_materials;
+    const kpt_material* materials;
+    int32_t num_textures;
+    const kpt_texture_desc* textures;
+    const uint8_t* pixels;
+    uint64_t num_pixel_bytes;
+} kpt_scene_desc;
+
+/* A ray for the trace entry points: origin, tmin, direction, tmax. */
+typedef struct kpt_ray {
+    float org[3];
+    float tmin;
+    float dir[3];
+    float tmax;
+} kpt_ray;
+
+/* Closest-hit result (Intersection, interaction.cuh:69-88). */
+typedef struct kpt_hit {
+    float t;            /* +inf on a miss */
+    int32_t prim_id;    /* global triangle index, -1 on a miss */
+    int32_t inst_id;    /* instance index, -1 on a miss */
+    float bary[3];
+} kpt_hit;
+
+/* Traversal event counters; bytes follow the reference node/triangle layout. */
+typedef struct kpt_trace_counters {
+    uint64_t blas_interior_pops;
+    uint64_t blas_leaf_pops;
+    uint64_t tri_tests;
+    uint64_t blas_entries;
+    uint64_t tlas_pops;
+    uint64_t rays;
+} kpt_trace_counters;
+
+/* Camera (camera.cuh:9-55): inverse matrices, column-major. */
+typedef struct kpt_camera {
+    float inv_view[16];
+    float inv_proj[16];
+    float prev_inv_view[16];
+    float prev_inv_proj[16];
+} kpt_camera;
+
+enum kpt_seed_mode {
+    KPT_SEED_REFERENCE = 0,   /* (x + y*W) * float(max(frame,1)), samplers.cuh:24-29 */
+    KPT_SEED_XOR = 1          /* pixel_index ^ frame */
+};
+
+enum kpt_integrator {
+    KPT_INTEGRATOR_MEGAKERNEL = 0,
+    KPT_INTEGRATOR_WAVEFRONT = 1
+};
+
+/* RendererSettings (pod_types.hpp:46-57) + ProceduralEnvironmentSettings
+ * (pod_types.hpp:37-44) + the extensions of this library. */
+typedef struct kpt_settings {
+    int32_t max_depth;
+    int32_t use_temporal_accumulation;
+    int32_t bloom_generate_bloom;
+    int32_t bloom_use_karis_average;
+    float bloom_final_blend;
+    float bloom_internal_blend;
+    int32_t enable_auto_exposure;
+    float sun_angular_diameter_rad;
+    float sun_phi_rad;
+    float sun_theta_rad;
+    float sun_emission_factor;
+    int32_t seed_mode;        /* kpt_seed_mode */
+    int32_t integrator;       /* kpt_integrator */
+    int32_t reserved[3];
+} kpt_settings;
+
+/* ---- context ---- */
+int kpt_get_abi_version(void);
+/* Renderer::initialize (renderer.cu:22-46). device < 0 picks the current one. */
+int kpt_ctx_create(int device, kpt_ctx** out);
+/* Renderer::shutdown (renderer.cu:48-53). */
+int kpt_ctx_destroy(kpt_ctx* ctx);
+/* Last error text of this context (never NULL). */
+const char* kpt_last_error(const kpt_ctx* ctx);
+
+/* ---- scene ---- */
+/* Renderer::loadScene + setMeshTransform per instance (renderer.cu:356-436,
+ * 250-269): BLAS binned-SAH and TLAS agglomerative builds (as_builder.cuh),
+ * upload of nodes, triangles, instances, materials and textures to HBM. */
+int kpt_upload_scene(kpt_ctx* ctx, const kpt_scene_desc* scene);
+/* Renderer::setMeshTransform (renderer.cu:250-269) + TLAS rebuild
+ * (renderer.cu:115): updates one instance and rebuilds the TLAS on the host. */
+int kpt_set_instance_transform(kpt_ctx* ctx, int32_t instance, const float model[16]);
+/* Renderer::setMaterial (renderer.cu:207-226). */
+int kpt_set_material(kpt_ctx* ctx, int32_t index, const kpt_material* m);
+/* Scene statistics: [0]=triangles [1]=blas nodes [2]=tlas nodes [3]=instances
+ * [4]=max BLAS depth [5]=max TLAS depth [6]=lights [7]=materials */
+int kpt_scene_info(const kpt_ctx* ctx, int64_t out[8]);
+/* Copies the built BVH in the reference's layout (BVH2Node, 32 B:
+ * i32 child_or_first, u32 count, f32 pmin[3], f32 pmax[3]) and the
+ * triangle index permutation; for parity tests of the host builders. */
+int kpt_get_blas(const kpt_ctx* ctx, int32_t mesh, void* nodes32, int64_t cap_nodes,
+                 int32_t* tri_index, int64_t cap_tris, int64_t out_counts[2]);
+/* TLAS nodes in the reference layout (TLASNode, 32 B: f32 pmin[3], f32 pmax[3],
+ * u32 left_right_index, i32 blas_index); node 0 is the root copy. */
+int kpt_get_tlas(const kpt_ctx* ctx, void* nodes32, int64_t cap_nodes, int64_t* out_count);
+
+/* ---- traversal (TLAS::intersect / intersectP, tlas.cu:8-125) ---- */
+/* rays/hits are DEVICE pointers; runs on the context stream. */
+int kpt_trace_closest(kpt_ctx* ctx, const kpt_ray* rays, kpt_hit* hits, int64_t n);
+/* occluded[i] = 1 when any triangle is hit in (tmin, tmax). */
+int kpt_trace_any(kpt_ctx* ctx, const kpt_ray* rays, int32_t* occluded, int64_t n);
+/* Same traversal with event counting (slower; measurement only). */
+int kpt_trace_closest_counted(kpt_ctx* ctx, const kpt_ray* rays, kpt_hit* hits, int64_t n,
+                              kpt_trace_counters* host_counters);
+
+/* ---- rendering (Renderer::executeRendering, renderer.cu:103-195) ---- */
+/* Renderer::setRendererSettings + setProceduralEnvironmentData
+ * (renderer.cu:290-305); resets accumulation. */
+int kpt_set_settings(kpt_ctx* ctx, const kpt_settings* s);
+int kpt_get_settings(const kpt_ctx* ctx, kpt_settings* out);
+/* Renderer::setView (renderer.cu:342-354): stores glm::inverse of both
+ * matrices, the previous camera becomes prev (Camera::setView,
+ * camera.cuh:28-36), resets accumulation unless temporal. */
+int kpt_set_view(kpt_ctx* ctx, const float projection[16], const float view[16]);
+/* Direct access to the camera inverses (current and previous). */
+int kpt_set_camera(kpt_ctx* ctx, const kpt_camera* cam);
+int kpt_get_camera(const kpt_ctx* ctx, kpt_camera* out);
+/* Renderer::resizeResolution (renderer.cu:55-101): (re)allocates frame buffers. */
+int kpt_resize(kpt_ctx* ctx, int32_t width, int32_t height);
+/* Renderer::resetAccumulation (renderer.cu:336-340). */
+int kpt_reset_accumulation(kpt_ctx* ctx);
+/* Renders `spp` reference frames of one sample each (static accumulation
+ * semantics of computePathTraceSamplesMegaKernel, kernels.cu:385-451) starting
+ * at the current frame index, then advances it by spp. */
+int kpt_render(kpt_ctx* ctx, int32_t spp);
+/* Frame index (ShaderData::frame_index, containers.cuh:40). */
+int kpt_get_frame_index(const kpt_ctx* ctx, int32_t* out);
+/* Copies a frame buffer to host memory. which: 0 = render (estimate, RGBA32F),
+ * 1 = accumulation (running sum, RGBA32F), 2 = gbuffer (packed, 4xu32),
+ * 3 = vbuffer (RGBA32F), 4 = output (post-processed sRGB, RGBA32F). */
+int kpt_readback(kpt_ctx* ctx, int32_t which, void* host_dst, int64_t bytes);
+/* Synchronises the context stream. */
+int kpt_sync(kpt_ctx* ctx);
+/* Milliseconds of the last kpt_render (HIP events on the context stream) and
+ * of its dominant closest-hit trace kernel launches. out[0]=render ms,
+ * out[1]=trace ms summed, out[2]=trace launches. */
+int kpt_last_timing(const kpt_ctx* ctx, double out[4]);
+/* Measurement mode: when enabled, the wavefront's closest-hit launches count
+ * traversal events; kpt_get_render_counters returns the totals of the last
+ * kpt_render.  No reference counterpart (the reference keeps per-ray
+ * DebugData counters, blas.cuh:39-42). */
+int kpt_set_trace_counting(kpt_ctx* ctx, int32_t enable);
+int kpt_get_render_counters(const kpt_ctx* ctx, kpt_trace_counters* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KPT_ABI_H */

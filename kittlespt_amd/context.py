@@ -1,0 +1,214 @@
+"""Python handle over one kpt_ctx (include/kpt_abi.h).
+
+Device memory for the trace entry points comes straight from the HIP runtime
+(hipMalloc / hipMemcpy through ctypes); no framework tensors are involved.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import ctypes.util
+import os
+
+import numpy as np
+
+from . import abi
+
+_hip = None
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        cands = [os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"),
+                 ctypes.util.find_library("amdhip64") or "libamdhip64.so"]
+        err = None
+        for c in cands:
+            try:
+                _hip = C.CDLL(c)
+                break
+            except OSError as e:  # pragma: no cover
+                err = e
+        if _hip is None:
+            raise RuntimeError(f"HIP runtime not found: {err}")
+        _hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        _hip.hipFree.argtypes = [C.c_void_p]
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipDeviceSynchronize.argtypes = []
+    return _hip
+
+
+H2D, D2H = 1, 2
+
+
+class DeviceBuffer:
+    def __init__(self, nbytes):
+        self.ptr = C.c_void_p()
+        self.nbytes = max(16, int(nbytes))
+        rc = hip().hipMalloc(C.byref(self.ptr), self.nbytes)
+        if rc != 0:
+            raise RuntimeError(f"hipMalloc({self.nbytes}) failed: {rc}")
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        rc = hip().hipMemcpy(self.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes, H2D)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy H2D failed: {rc}")
+
+    def download(self, out):
+        rc = hip().hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes, D2H)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy D2H failed: {rc}")
+        return out
+
+    def free(self):
+        if self.ptr:
+            hip().hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One renderer context.  device=-2 makes a host-only context (builders
+    and queries work, GPU entry points return KPT_ERR_NO_DEVICE)."""
+
+    def __init__(self, device=-1):
+        self.L = abi.lib()
+        h = C.c_void_p()
+        rc = self.L.kpt_ctx_create(device, C.byref(h))
+        if rc != abi.KPT_OK:
+            raise abi.KptError(rc, "kpt_ctx_create failed")
+        self.h = h
+        self.scene = None
+
+    def close(self):
+        if self.h:
+            self.L.kpt_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc):
+        return abi.check(self.h, rc)
+
+    # ---- scene
+    def upload(self, scene):
+        desc, keep = scene.desc()
+        self._ck(self.L.kpt_upload_scene(self.h, C.byref(desc)))
+        self.scene = scene
+
+    def info(self):
+        out = (C.c_int64 * 8)()
+        self._ck(self.L.kpt_scene_info(self.h, out))
+        return list(out)
+
+    def blas(self, mesh):
+        n = int(self.scene.mesh_count[mesh])
+        nodes = np.zeros(max(1, 2 * n - 1), abi.BVH2NODE_DTYPE)
+        tri = np.zeros(max(1, n), np.int32)
+        cnt = (C.c_int64 * 2)()
+        self._ck(self.L.kpt_get_blas(self.h, mesh, nodes.ctypes.data_as(C.c_void_p), len(nodes),
+                                     tri.ctypes.data_as(C.POINTER(C.c_int32)), len(tri), cnt))
+        return nodes[:cnt[0]], tri[:cnt[1]]
+
+    def tlas(self):
+        nodes = np.zeros(2 * self.scene.num_instances + 2, abi.TLASNODE_DTYPE)
+        cnt = C.c_int64()
+        self._ck(self.L.kpt_get_tlas(self.h, nodes.ctypes.data_as(C.c_void_p), len(nodes), C.byref(cnt)))
+        return nodes[:cnt.value]
+
+    def set_instance_transform(self, i, model16):
+        m = np.ascontiguousarray(model16, np.float32)
+        self._ck(self.L.kpt_set_instance_transform(self.h, i, abi.fptr(m)))
+
+    # ---- traversal (host arrays in, host arrays out)
+    def trace_closest(self, rays, counted=False):
+        rays = np.ascontiguousarray(rays, abi.RAY_DTYPE)
+        n = len(rays)
+        dr = DeviceBuffer.from_array(rays)
+        dh = DeviceBuffer(n * abi.HIT_DTYPE.itemsize)
+        cnt = abi.KptTraceCounters()
+        if counted:
+            self._ck(self.L.kpt_trace_closest_counted(self.h, dr.ptr, dh.ptr, n, C.byref(cnt)))
+        else:
+            self._ck(self.L.kpt_trace_closest(self.h, dr.ptr, dh.ptr, n))
+        self._ck(self.L.kpt_sync(self.h))
+        hits = dh.download(np.zeros(n, abi.HIT_DTYPE))
+        return (hits, cnt.as_dict()) if counted else hits
+
+    def trace_any(self, rays):
+        rays = np.ascontiguousarray(rays, abi.RAY_DTYPE)
+        n = len(rays)
+        dr = DeviceBuffer.from_array(rays)
+        do = DeviceBuffer(n * 4)
+        self._ck(self.L.kpt_trace_any(self.h, dr.ptr, do.ptr, n))
+        self._ck(self.L.kpt_sync(self.h))
+        return do.download(np.zeros(n, np.int32))
+
+    # ---- rendering
+    def settings(self):
+        s = abi.KptSettings()
+        self._ck(self.L.kpt_get_settings(self.h, C.byref(s)))
+        return s
+
+    def set_settings(self, s):
+        self._ck(self.L.kpt_set_settings(self.h, C.byref(s)))
+
+    def set_view(self, proj16, view16):
+        p = np.ascontiguousarray(proj16, np.float32)
+        v = np.ascontiguousarray(view16, np.float32)
+        self._ck(self.L.kpt_set_view(self.h, abi.fptr(p), abi.fptr(v)))
+
+    def camera(self):
+        c = abi.KptCamera()
+        self._ck(self.L.kpt_get_camera(self.h, C.byref(c)))
+        return c
+
+    def resize(self, w, h):
+        self._ck(self.L.kpt_resize(self.h, w, h))
+        self.width, self.height = w, h
+
+    def reset(self):
+        self._ck(self.L.kpt_reset_accumulation(self.h))
+
+    def render(self, spp=1):
+        self._ck(self.L.kpt_render(self.h, spp))
+
+    def frame_index(self):
+        v = C.c_int32()
+        self._ck(self.L.kpt_get_frame_index(self.h, C.byref(v)))
+        return v.value
+
+    def readback(self, which=0):
+        dt = np.uint32 if which == 2 else np.float32
+        out = np.zeros((self.height, self.width, 4), dt)
+        self._ck(self.L.kpt_readback(self.h, which, out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
+
+    def timing(self):
+        out = (C.c_double * 4)()
+        self._ck(self.L.kpt_last_timing(self.h, out))
+        return list(out)
+
+    def set_trace_counting(self, enable):
+        self._ck(self.L.kpt_set_trace_counting(self.h, 1 if enable else 0))
+
+    def render_counters(self):
+        c = abi.KptTraceCounters()
+        self._ck(self.L.kpt_get_render_counters(self.h, C.byref(c)))
+        return c.as_dict()

@@ -1,0 +1,799 @@
+// kpt_abi.hip — implementation of the C-ABI in include/kpt_abi.h: context,
+// scene upload (host builders + HBM layout), render orchestration.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../../include/kpt_abi.h"
+#include "../host/kpt_build.h"
+#include "../host/kpt_hostmath.h"
+#include "kpt_integrator.h"
+#include "kpt_launch.h"
+
+using namespace kpt;
+
+namespace {
+
+template <class T> struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace
+
+struct kpt_ctx {
+    int device = -1;
+    bool host_only = false;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // host copy of the scene
+    std::vector<float> pos, uv;
+    std::vector<int32_t> mat_ids, mesh_first, mesh_count, inst_mesh;
+    std::vector<float> inst_model;
+    std::vector<kpt::Material> materials;
+    std::vector<kpt::Texture> textures;
+    std::vector<BlasTree> blas;
+    std::vector<int32_t> geom_base;           // global node index of each BLAS root
+    std::vector<Aabb> inst_bounds;
+    TlasTree tlas;
+    std::vector<Node> nodes;                  // global BLAS node array
+    std::vector<InstTrav> itrav;
+    std::vector<InstShade> ishade;
+    std::vector<kpt::Light> lights;
+    std::vector<float> light_verts;
+    int64_t num_pixel_bytes = 0;
+    bool scene_ready = false;
+    // device scene
+    Node* d_blas = nullptr;
+    Node* d_tlas = nullptr;
+    TriPos* d_tris = nullptr;
+    TriAttr* d_attrs = nullptr;
+    InstTrav* d_itrav = nullptr;
+    InstShade* d_ishade = nullptr;
+    kpt::Material* d_mat = nullptr;
+    kpt::Texture* d_tex = nullptr;
+    uint8_t* d_pix = nullptr;
+    kpt::Light* d_lights = nullptr;
+    float* d_lverts = nullptr;
+    int32_t* d_overflow = nullptr;
+    // rendering
+    kpt_settings settings{};
+    kpt_camera cam{};
+    int32_t frame_index = 0;
+    int32_t width = 0, height = 0;
+    FrameBuffers fb{};
+    WavefrontState ws{};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, tev0 = nullptr, tev1 = nullptr;
+    WfInstrument ins{};
+    bool count_trace = false;
+    unsigned long long* d_counts = nullptr;
+    kpt_trace_counters render_counts{};
+    double timing[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+int fail(kpt_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+#define HIPCHK(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(ctx, KPT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T> hipError_t dev_upload(T** dst, const std::vector<T>& src) {
+    if (*dst) { (void)hipFree(*dst); *dst = nullptr; }
+    size_t bytes = std::max<size_t>(sizeof(T), src.size() * sizeof(T));
+    hipError_t e = hipMalloc((void**)dst, bytes);
+    if (e != hipSuccess) return e;
+    if (!src.empty()) e = hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+    return e;
+}
+template <class T> void dev_free(T** p) { if (*p) { (void)hipFree(*p); *p = nullptr; } }
+
+void default_settings(kpt_settings* s) {
+    std::memset(s, 0, sizeof(*s));
+    // RendererSettings / ProceduralEnvironmentSettings defaults (pod_types.hpp:37-57)
+    s->max_depth = 3;
+    s->bloom_final_blend = 0.175f;
+    s->bloom_internal_blend = 0.65f;
+    s->sun_angular_diameter_rad = 0.0087f;
+    s->sun_phi_rad = 4.18879f;
+    s->sun_theta_rad = 17.0f * (3.14159265358979323846f / 180.0f);
+    s->sun_emission_factor = 1.0f;
+    s->seed_mode = KPT_SEED_REFERENCE;
+    s->integrator = KPT_INTEGRATOR_MEGAKERNEL;
+}
+
+SceneView scene_view(const kpt_ctx* c) {
+    SceneView S;
+    S.blas_nodes = c->d_blas;
+    S.tlas_nodes = c->d_tlas;
+    S.tris = c->d_tris;
+    S.attrs = c->d_attrs;
+    S.inst_trav = c->d_itrav;
+    S.inst_shade = c->d_ishade;
+    S.materials = c->d_mat;
+    S.textures = c->d_tex;
+    S.pixels = c->d_pix;
+    S.lights = c->d_lights;
+    S.light_verts = c->d_lverts;
+    S.num_pixel_bytes = c->num_pixel_bytes;
+    S.tlas_root = c->tlas.root;
+    S.num_lights = (int32_t)c->lights.size();
+    return S;
+}
+
+void store_mat(const hm::Mat4& m, float* dst) { std::memcpy(dst, m.m, 64); }
+
+// Per-instance records from a model matrix (Renderer::setMeshTransform,
+// renderer.cu:250-269; BLAS2::setTransform, blas.cuh:190-200).
+void make_instance(kpt_ctx* c, int32_t i, const float* model, bool keep_prev) {
+    hm::Mat4 M = hm::load(model);
+    hm::Mat4 trav = hm::ref_inverse(M);
+    InstTrav& it = c->itrav[i];
+    for (int k = 0; k < 4; ++k) { it.r0[k] = trav.m[4 * k + 0]; it.r1[k] = trav.m[4 * k + 1]; it.r2[k] = trav.m[4 * k + 2]; }
+    int32_t g = c->inst_mesh[i];
+    it.root = c->blas[g].used > 0 ? c->geom_base[g] : -1;
+    it.geom = g;
+    InstShade& sh = c->ishade[i];
+    hm::Mat4 curr_inv = hm::glm_inverse(M);
+    hm::Mat4 prev_inv = keep_prev ? hm::load(sh.curr_inv) : curr_inv;
+    store_mat(hm::ref_inverse(curr_inv), sh.model);
+    store_mat(curr_inv, sh.curr_inv);
+    store_mat(hm::ref_inverse(prev_inv), sh.prev_model);
+    c->inst_bounds[i] = transform_bounds(c->blas[g].bounds, model);
+}
+
+int upload_tlas(kpt_ctx* ctx) {
+    ctx->tlas = build_tlas(ctx->inst_bounds);
+    if (ctx->tlas.nodes.size() > 65535) return fail(ctx, KPT_ERR_UNSUPPORTED, "TLAS exceeds 65535 nodes (16-bit child fields)");
+    if (ctx->host_only) return KPT_OK;
+    HIPCHK(dev_upload(&ctx->d_tlas, ctx->tlas.nodes));
+    HIPCHK(dev_upload(&ctx->d_itrav, ctx->itrav));
+    HIPCHK(dev_upload(&ctx->d_ishade, ctx->ishade));
+    return KPT_OK;
+}
+
+FrameConst make_frame(const kpt_ctx* c, int32_t /*frame*/) {
+    FrameConst fc;
+    std::memset(&fc, 0, sizeof(fc));
+    std::memcpy(fc.inv_view, c->cam.inv_view, 64);
+    std::memcpy(fc.inv_proj, c->cam.inv_proj, 64);
+    hm::Mat4 cvp = hm::mul(hm::ref_inverse(hm::load(c->cam.inv_proj)), hm::ref_inverse(hm::load(c->cam.inv_view)));
+    hm::Mat4 pvp = hm::mul(hm::ref_inverse(hm::load(c->cam.prev_inv_proj)), hm::ref_inverse(hm::load(c->cam.prev_inv_view)));
+    store_mat(cvp, fc.curr_vp);
+    store_mat(pvp, fc.prev_vp);
+    fc.cam_pos[0] = c->cam.inv_view[12];
+    fc.cam_pos[1] = c->cam.inv_view[13];
+    fc.cam_pos[2] = c->cam.inv_view[14];
+    const kpt_settings& s = c->settings;
+    // getSunDirection (integrator.cuh:222-230)
+    float phi = s.sun_phi_rad, th = s.sun_theta_rad;
+    v3 sun = normalize(mk3(dm::cosf(phi) * dm::cosf(th), dm::sinf(th), dm::sinf(phi) * dm::cosf(th)));
+    fc.sun_dir[0] = sun.x; fc.sun_dir[1] = sun.y; fc.sun_dir[2] = sun.z;
+    fc.sun_emission = s.sun_emission_factor;
+    fc.sun_diam = s.sun_angular_diameter_rad;
+    // The sky radiance toward the sun that sampleLdSun evaluates per call
+    // (integrator.cuh:169) does not depend on the path: computed once per frame
+    // by the same routine.
+    v3 srad = sky_li(observer_pos(), sun, sun, s.sun_emission_factor);
+    fc.sun_radiance[0] = srad.x; fc.sun_radiance[1] = srad.y; fc.sun_radiance[2] = srad.z;
+    // angularDiameterToPhysicalDiameter(...) / 2 (atmosphere.cuh:79-82, integrator.cuh:153)
+    fc.sun_radius = (2.0f * KPT_SOLAR_DIST * dm::tanf(s.sun_angular_diameter_rad / 2.0f)) / 2.0f;
+    fc.min_sun_cos = dm::cosf(s.sun_angular_diameter_rad / 2.0f);
+    fc.width = c->width;
+    fc.height = c->height;
+    fc.max_depth = s.max_depth;
+    fc.seed_mode = s.seed_mode;
+    fc.num_lights = (int32_t)c->lights.size();
+    fc.num_instances = (int32_t)c->itrav.size();
+    fc.tlas_root = c->tlas.root;
+    return fc;
+}
+
+void free_frame(kpt_ctx* c) {
+    dev_free(&c->fb.accum); dev_free(&c->fb.render); dev_free(&c->fb.gbuffer);
+    dev_free(&c->fb.prev_gbuffer); dev_free(&c->fb.vbuffer); dev_free(&c->fb.output);
+}
+
+void free_wavefront(kpt_ctx* c) {
+    WavefrontState& w = c->ws;
+    dev_free(&w.ray_o); dev_free(&w.ray_d); dev_free(&w.beta); dev_free(&w.L); dev_free(&w.prev_pos);
+    dev_free(&w.vis_a); dev_free(&w.vis_n); dev_free(&w.vis_p); dev_free(&w.hit); dev_free(&w.hit_v);
+    dev_free(&w.queue[0]); dev_free(&w.queue[1]); dev_free(&w.shadow_queue); dev_free(&w.shadow_o);
+    dev_free(&w.shadow_d); dev_free(&w.shadow_c); dev_free(&w.shadow_z); dev_free(&w.sort_keys);
+    dev_free(&w.sort_tmp); dev_free(&w.counters); dev_free(&w.hist);
+    w.capacity = 0;
+}
+
+int alloc_wavefront(kpt_ctx* ctx, int64_t n) {
+    WavefrontState& w = ctx->ws;
+    if (w.capacity >= n) return KPT_OK;
+    free_wavefront(ctx);
+    size_t f4 = (size_t)n * sizeof(float4);
+    size_t i4 = (size_t)n * sizeof(int32_t);
+    HIPCHK(hipMalloc((void**)&w.ray_o, f4)); HIPCHK(hipMalloc((void**)&w.ray_d, f4));
+    HIPCHK(hipMalloc((void**)&w.beta, f4)); HIPCHK(hipMalloc((void**)&w.L, f4));
+    HIPCHK(hipMalloc((void**)&w.prev_pos, f4)); HIPCHK(hipMalloc((void**)&w.vis_a, f4));
+    HIPCHK(hipMalloc((void**)&w.vis_n, f4)); HIPCHK(hipMalloc((void**)&w.vis_p, f4));
+    HIPCHK(hipMalloc((void**)&w.hit, f4)); HIPCHK(hipMalloc((void**)&w.hit_v, (size_t)n * sizeof(float)));
+    HIPCHK(hipMalloc((void**)&w.queue[0], i4)); HIPCHK(hipMalloc((void**)&w.queue[1], i4));
+    HIPCHK(hipMalloc((void**)&w.shadow_queue, i4));
+    HIPCHK(hipMalloc((void**)&w.shadow_o, f4)); HIPCHK(hipMalloc((void**)&w.shadow_d, f4));
+    HIPCHK(hipMalloc((void**)&w.shadow_c, f4)); HIPCHK(hipMalloc((void**)&w.shadow_z, f4));
+    HIPCHK(hipMalloc((void**)&w.sort_keys, i4)); HIPCHK(hipMalloc((void**)&w.sort_tmp, i4));
+    HIPCHK(hipMalloc((void**)&w.counters, 256 * sizeof(int32_t)));
+    HIPCHK(hipMalloc((void**)&w.hist, (size_t)64 * ((n + 255) / 256) * sizeof(int32_t)));
+    w.capacity = n;
+    return KPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kpt_get_abi_version(void) { return KPT_ABI_VERSION; }
+
+int kpt_ctx_create(int device, kpt_ctx** out) {
+    if (!out) return KPT_ERR_INVALID_ARG;
+    *out = nullptr;
+    kpt_ctx* ctx = new kpt_ctx();
+    default_settings(&ctx->settings);
+    hm::Mat4 I = hm::identity();
+    std::memcpy(ctx->cam.inv_view, I.m, 64); std::memcpy(ctx->cam.inv_proj, I.m, 64);
+    std::memcpy(ctx->cam.prev_inv_view, I.m, 64); std::memcpy(ctx->cam.prev_inv_proj, I.m, 64);
+    if (device == -2) {  // host-only context: builders and queries, no GPU work
+        ctx->host_only = true;
+        *out = ctx;
+        return KPT_OK;
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        delete ctx;
+        return KPT_ERR_NO_DEVICE;
+    }
+    if (device < 0) { if (hipGetDevice(&device) != hipSuccess) device = 0; }
+    if (device >= count) { delete ctx; return KPT_ERR_INVALID_ARG; }
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_overflow, sizeof(int32_t)) != hipSuccess ||
+        hipMemset(ctx->d_overflow, 0, sizeof(int32_t)) != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||
+        hipEventCreate(&ctx->ev1) != hipSuccess || hipEventCreate(&ctx->tev0) != hipSuccess ||
+        hipEventCreate(&ctx->tev1) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_counts, 8 * sizeof(unsigned long long)) != hipSuccess) {
+        delete ctx;
+        return KPT_ERR_HIP;
+    }
+    for (int k = 0; k < WF_MAX_TRACE_LAUNCHES; ++k) {
+        if (hipEventCreate(&ctx->ins.begin[k]) != hipSuccess || hipEventCreate(&ctx->ins.end[k]) != hipSuccess) {
+            delete ctx;
+            return KPT_ERR_HIP;
+        }
+    }
+    *out = ctx;
+    return KPT_OK;
+}
+
+int kpt_ctx_destroy(kpt_ctx* ctx) {
+    if (!ctx) return KPT_ERR_INVALID_ARG;
+    if (!ctx->host_only) {
+        (void)hipSetDevice(ctx->device);
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        dev_free(&ctx->d_blas); dev_free(&ctx->d_tlas); dev_free(&ctx->d_tris); dev_free(&ctx->d_attrs);
+        dev_free(&ctx->d_itrav); dev_free(&ctx->d_ishade); dev_free(&ctx->d_mat); dev_free(&ctx->d_tex);
+        dev_free(&ctx->d_pix); dev_free(&ctx->d_lights); dev_free(&ctx->d_lverts); dev_free(&ctx->d_overflow);
+        free_frame(ctx);
+        free_wavefront(ctx);
+        if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+        if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+        if (ctx->tev0) (void)hipEventDestroy(ctx->tev0);
+        if (ctx->tev1) (void)hipEventDestroy(ctx->tev1);
+        for (int k = 0; k < WF_MAX_TRACE_LAUNCHES; ++k) {
+            if (ctx->ins.begin[k]) (void)hipEventDestroy(ctx->ins.begin[k]);
+            if (ctx->ins.end[k]) (void)hipEventDestroy(ctx->ins.end[k]);
+        }
+        dev_free(&ctx->d_counts);
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+    return KPT_OK;
+}
+
+const char* kpt_last_error(const kpt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int kpt_upload_scene(kpt_ctx* ctx, const kpt_scene_desc* d) {
+    if (!ctx || !d) return KPT_ERR_INVALID_ARG;
+    const int32_t T = d->num_triangles;
+    if (T < 0 || d->num_meshes < 0 || d->num_materials < 0 || d->num_textures < 0)
+        return fail(ctx, KPT_ERR_INVALID_ARG, "negative count");
+    if (T > 0 && (!d->positions || !d->normals || !d->uvs || !d->material_ids))
+        return fail(ctx, KPT_ERR_INVALID_ARG, "missing triangle arrays");
+    // meshes must tile the triangle array in order (the reference's builder
+    // indexes its centroid cache by global triangle id, as_builder.cuh:238-248)
+    int32_t expect = 0;
+    for (int32_t m = 0; m < d->num_meshes; ++m) {
+        if (d->mesh_first[m] != expect || d->mesh_count[m] < 0)
+            return fail(ctx, KPT_ERR_INVALID_ARG, "meshes must be contiguous and in triangle order");
+        expect += d->mesh_count[m];
+    }
+    if (expect != T) return fail(ctx, KPT_ERR_INVALID_ARG, "mesh triangle counts do not sum to num_triangles");
+    for (int32_t t = 0; t < T; ++t)
+        if (d->material_ids[t] < 0 || d->material_ids[t] >= d->num_materials)
+            return fail(ctx, KPT_ERR_INVALID_ARG, "triangle material id out of range");
+    const int32_t NI = d->num_instances > 0 ? d->num_instances : d->num_meshes;
+    if (d->num_instances == 0 && d->num_meshes > 0 && !d->mesh_model)
+        return fail(ctx, KPT_ERR_INVALID_ARG, "mesh_model required when num_instances == 0");
+    for (int32_t i = 0; i < d->num_instances; ++i)
+        if (d->instance_mesh[i] < 0 || d->instance_mesh[i] >= d->num_meshes)
+            return fail(ctx, KPT_ERR_INVALID_ARG, "instance mesh index out of range");
+    for (int32_t i = 0; i < d->num_textures; ++i) {
+        const kpt_texture_desc& t = d->textures[i];
+        uint64_t need = (uint64_t)t.width * t.height * t.channels;
+        if (t.offset + need > d->num_pixel_bytes || t.width == 0 || t.height == 0 || t.channels == 0)
+            return fail(ctx, KPT_ERR_INVALID_ARG, "texture outside the pixel buffer");
+    }
+    // texture ids of materials must name a texture or be -1 (the reference
+    // indexes its texture buffer unchecked, material.cu:23-48)
+    for (int32_t i = 0; i < d->num_materials; ++i) {
+        const kpt_material& m = d->materials[i];
+        const int32_t ids[5] = {m.albedo_texture_id, m.orm_texture_id, m.transmission_texture_id,
+                                m.emission_texture_id, m.normal_texture_id};
+        for (int k = 0; k < 5; ++k)
+            if (ids[k] < -1 || ids[k] >= d->num_textures)
+                return fail(ctx, KPT_ERR_INVALID_ARG, "material texture id out of range");
+    }
+    ctx->scene_ready = false;
+    ctx->pos.assign(d->positions, d->positions + 9 * (size_t)T);
+    ctx->uv.assign(d->uvs, d->uvs + 6 * (size_t)T);
+    ctx->mat_ids.assign(d->material_ids, d->material_ids + T);
+    ctx->mesh_first.assign(d->mesh_first, d->mesh_first + d->num_meshes);
+    ctx->mesh_count.assign(d->mesh_count, d->mesh_count + d->num_meshes);
+    ctx->inst_mesh.resize(NI);
+    ctx->inst_model.resize(16 * (size_t)NI);
+    for (int32_t i = 0; i < NI; ++i) {
+        ctx->inst_mesh[i] = d->num_instances > 0 ? d->instance_mesh[i] : i;
+        const float* src = d->num_instances > 0 ? d->instance_model + 16 * (size_t)i : d->mesh_model + 16 * (size_t)i;
+        std::memcpy(&ctx->inst_model[16 * (size_t)i], src, 64);
+    }
+    // materials (Material ctor from MaterialSceneEntity, renderer.cu:372-382)
+    ctx->materials.resize(d->num_materials);
+    for (int32_t i = 0; i < d->num_materials; ++i) {
+        const kpt_material& s = d->materials[i];
+        kpt::Material& m = ctx->materials[i];
+        std::memset(&m, 0, sizeof(m));
+        m.albedo_tex = s.albedo_texture_id;
+        for (int k = 0; k < 3; ++k) { m.albedo[k] = s.albedo[k]; m.emission[k] = s.emission_factor[k]; }
+        m.orm_tex = s.orm_texture_id; m.metallic = s.metallic; m.roughness = s.roughness;
+        m.transmission_tex = s.transmission_texture_id; m.transmission = s.transmission;
+        m.ior = s.ior; m.emission_tex = s.emission_texture_id; m.emission_scale = s.emission_scale_nits;
+        m.normal_tex = s.normal_texture_id; m.normal_scale = s.normal_scale;
+    }
+    ctx->textures.resize(d->num_textures);
+    for (int32_t i = 0; i < d->num_textures; ++i) {
+        kpt::Texture& t = ctx->textures[i];
+        std::memset(&t, 0, sizeof(t));
+        t.width = (int32_t)d->textures[i].width; t.height = (int32_t)d->textures[i].height;
+        t.channels = (int32_t)d->textures[i].channels; t.bit_depth = 8; t.offset = (int64_t)d->textures[i].offset;
+    }
+    ctx->num_pixel_bytes = (int64_t)d->num_pixel_bytes;
+    // BLAS per mesh, built in parallel
+    ctx->blas.assign(d->num_meshes, BlasTree());
+    {
+        std::vector<std::thread> pool;
+        const float* P = ctx->pos.data();
+        for (int32_t m = 0; m < d->num_meshes; ++m)
+            pool.emplace_back([ctx, P, m]() { ctx->blas[m] = build_blas(P, ctx->mesh_first[m], ctx->mesh_count[m]); });
+        for (auto& th : pool) th.join();
+    }
+    // global node array: each root at an odd index so every sibling pair
+    // starts at an even index (one 64-byte line)
+    ctx->nodes.clear();
+    ctx->geom_base.assign(d->num_meshes, -1);
+    for (int32_t m = 0; m < d->num_meshes; ++m) {
+        const BlasTree& b = ctx->blas[m];
+        if (b.used == 0) continue;
+        if ((ctx->nodes.size() & 1) == 0) { Node pad; std::memset(&pad, 0, sizeof(pad)); ctx->nodes.push_back(pad); }
+        int32_t base = (int32_t)ctx->nodes.size();
+        ctx->geom_base[m] = base;
+        for (int32_t k = 0; k < b.used; ++k) {
+            Node n = b.nodes[k];
+            if (n.b == 0u) n.a = (uint32_t)(base + (int32_t)n.a);
+            else n.a = (uint32_t)(ctx->mesh_first[m] + (int32_t)n.a);
+            ctx->nodes.push_back(n);
+        }
+        if (ctx->nodes.size() & 1) { Node pad; std::memset(&pad, 0, sizeof(pad)); ctx->nodes.push_back(pad); }
+    }
+    // triangles in slot order, attributes in primitive order
+    std::vector<TriPos> tris(T);
+    std::vector<TriAttr> attrs(T);
+    for (int32_t m = 0; m < d->num_meshes; ++m) {
+        const BlasTree& b = ctx->blas[m];
+        for (int32_t s = 0; s < ctx->mesh_count[m]; ++s) {
+            int32_t prim = b.order[s];
+            const float* p = &ctx->pos[9 * (size_t)prim];
+            TriPos& tp = tris[ctx->mesh_first[m] + s];
+            for (int k = 0; k < 3; ++k) { tp.v0[k] = p[k]; tp.e1[k] = p[3 + k] - p[k]; tp.e2[k] = p[6 + k] - p[k]; }
+            tp.prim = prim; tp.pad0 = 0.0f; tp.pad1 = 0.0f;
+        }
+    }
+    // lights: emissive materials (renderer.cu:391-401), placed by the first
+    // instance of their mesh
+    std::vector<int32_t> first_inst(d->num_meshes, -1);
+    for (int32_t i = 0; i < NI; ++i) if (first_inst[ctx->inst_mesh[i]] < 0) first_inst[ctx->inst_mesh[i]] = i;
+    ctx->lights.clear();
+    ctx->light_verts.clear();
+    for (int32_t t = 0; t < T; ++t) {
+        const float* p = &ctx->pos[9 * (size_t)t];
+        const float* n = d->normals + 9 * (size_t)t;
+        TriAttr& a = attrs[t];
+        // Triangle constructor (triangle.cuh:54-64): geometric normal flipped
+        // toward the mean vertex normal, then normalised
+        v3 p0 = mk3(p[0], p[1], p[2]), p1 = mk3(p[3], p[4], p[5]), p2 = mk3(p[6], p[7], p[8]);
+        v3 gn = cross(p1 - p0, p2 - p0);
+        v3 avg = (mk3(n[0], n[1], n[2]) + mk3(n[3], n[4], n[5]) + mk3(n[6], n[7], n[8])) / 3.0f;
+        float dd = dot(gn, avg);
+        v3 lg = normalize((dd < 0.0f) ? -gn : gn);
+        a.lgn[0] = lg.x; a.lgn[1] = lg.y; a.lgn[2] = lg.z;
+        a.material = ctx->mat_ids[t];
+        const float* u = &ctx->uv[6 * (size_t)t];
+        a.uv0[0] = u[0]; a.uv0[1] = u[1]; a.uv1[0] = u[2]; a.uv1[1] = u[3]; a.uv2[0] = u[4]; a.uv2[1] = u[5];
+        a.light = -1;
+        a.inst_of_light = -1;
+        const kpt::Material& mat = ctx->materials[a.material];
+        v3 e = mk3(mat.emission[0], mat.emission[1], mat.emission[2]) * mat.emission_scale;
+        if (e.x > 0.0f || e.y > 0.0f || e.z > 0.0f) {
+            int32_t mesh = (int32_t)(std::upper_bound(ctx->mesh_first.begin(), ctx->mesh_first.end(), t) - ctx->mesh_first.begin()) - 1;
+            kpt::Light l;
+            l.area = 0.5f * length(cross(p1 - p0, p2 - p0));
+            l.prim = t;
+            l.inst = first_inst[mesh];
+            l.pad = 0;
+            if (l.inst < 0) continue;   // mesh never instanced: not a light
+            a.light = (int32_t)ctx->lights.size();
+            a.inst_of_light = l.inst;
+            ctx->lights.push_back(l);
+            ctx->light_verts.insert(ctx->light_verts.end(), p, p + 9);
+        }
+    }
+    // instances + TLAS
+    ctx->itrav.assign(NI, InstTrav());
+    ctx->ishade.assign(NI, InstShade());
+    ctx->inst_bounds.assign(NI, Aabb());
+    for (int32_t i = 0; i < NI; ++i) make_instance(ctx, i, &ctx->inst_model[16 * (size_t)i], false);
+    if (!ctx->host_only) {
+        HIPCHK(hipSetDevice(ctx->device));
+        HIPCHK(dev_upload(&ctx->d_blas, ctx->nodes));
+        HIPCHK(dev_upload(&ctx->d_tris, tris));
+        HIPCHK(dev_upload(&ctx->d_attrs, attrs));
+        HIPCHK(dev_upload(&ctx->d_mat, ctx->materials));
+        HIPCHK(dev_upload(&ctx->d_tex, ctx->textures));
+        // Texture::evaluate reads three bytes per texel whatever the channel
+        // count (texture.cuh:54-58): pad so a 1- or 2-channel last texel stays
+        // inside the allocation; the bounds checks still use num_pixel_bytes.
+        std::vector<uint8_t> pix(d->num_pixel_bytes + 16, 0);
+        if (d->num_pixel_bytes) std::memcpy(pix.data(), d->pixels, d->num_pixel_bytes);
+        HIPCHK(dev_upload(&ctx->d_pix, pix));
+        HIPCHK(dev_upload(&ctx->d_lights, ctx->lights));
+        HIPCHK(dev_upload(&ctx->d_lverts, ctx->light_verts));
+    }
+    int rc = upload_tlas(ctx);
+    if (rc != KPT_OK) return rc;
+    ctx->scene_ready = true;
+    ctx->frame_index = 0;
+    return KPT_OK;
+}
+
+int kpt_set_instance_transform(kpt_ctx* ctx, int32_t instance, const float model[16]) {
+    if (!ctx || !model) return KPT_ERR_INVALID_ARG;
+    if (!ctx->scene_ready) return fail(ctx, KPT_ERR_STATE, "no scene");
+    if (instance < 0 || instance >= (int32_t)ctx->itrav.size()) return fail(ctx, KPT_ERR_INVALID_ARG, "instance out of range");
+    std::memcpy(&ctx->inst_model[16 * (size_t)instance], model, 64);
+    make_instance(ctx, instance, model, true);
+    // setMeshTransform resets accumulation unless temporal (renderer.cu:264-266)
+    if (!ctx->settings.use_temporal_accumulation) ctx->frame_index = 0;
+    if (!ctx->host_only && ctx->fb.accum && !ctx->settings.use_temporal_accumulation) {
+        HIPCHK(launch_clear(ctx->stream, ctx->fb.accum, (int64_t)ctx->width * ctx->height));
+    }
+    if (!ctx->host_only) HIPCHK(hipStreamSynchronize(ctx->stream));
+    return upload_tlas(ctx);
+}
+
+int kpt_set_material(kpt_ctx* ctx, int32_t index, const kpt_material* s) {
+    if (!ctx || !s) return KPT_ERR_INVALID_ARG;
+    if (index < 0 || index >= (int32_t)ctx->materials.size()) return fail(ctx, KPT_ERR_INVALID_ARG, "material out of range");
+    kpt::Material& m = ctx->materials[index];
+    m.albedo_tex = s->albedo_texture_id;
+    for (int k = 0; k < 3; ++k) { m.albedo[k] = s->albedo[k]; m.emission[k] = s->emission_factor[k]; }
+    m.orm_tex = s->orm_texture_id; m.metallic = s->metallic; m.roughness = s->roughness;
+    m.transmission_tex = s->transmission_texture_id; m.transmission = s->transmission;
+    m.ior = s->ior; m.emission_tex = s->emission_texture_id; m.emission_scale = s->emission_scale_nits;
+    m.normal_tex = s->normal_texture_id; m.normal_scale = s->normal_scale;
+    ctx->frame_index = 0;   // renderer.cu:223
+    if (ctx->host_only) return KPT_OK;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(dev_upload(&ctx->d_mat, ctx->materials));
+    if (ctx->fb.accum) HIPCHK(launch_clear(ctx->stream, ctx->fb.accum, (int64_t)ctx->width * ctx->height));
+    return KPT_OK;
+}
+
+int kpt_scene_info(const kpt_ctx* ctx, int64_t out[8]) {
+    if (!ctx || !out) return KPT_ERR_INVALID_ARG;
+    int64_t md = 0;
+    for (const auto& b : ctx->blas) md = std::max<int64_t>(md, b.max_depth);
+    out[0] = (int64_t)ctx->pos.size() / 9;
+    out[1] = (int64_t)ctx->nodes.size();
+    out[2] = (int64_t)ctx->tlas.nodes.size();
+    out[3] = (int64_t)ctx->itrav.size();
+    out[4] = md;
+    out[5] = ctx->tlas.max_depth;
+    out[6] = (int64_t)ctx->lights.size();
+    out[7] = (int64_t)ctx->materials.size();
+    return KPT_OK;
+}
+
+int kpt_get_blas(const kpt_ctx* ctx, int32_t mesh, void* nodes32, int64_t cap_nodes, int32_t* tri_index,
+                 int64_t cap_tris, int64_t out_counts[2]) {
+    if (!ctx || !nodes32 || !tri_index || !out_counts) return KPT_ERR_INVALID_ARG;
+    if (mesh < 0 || mesh >= (int32_t)ctx->blas.size()) return KPT_ERR_INVALID_ARG;
+    const BlasTree& b = ctx->blas[mesh];
+    if (b.used > cap_nodes || (int64_t)b.order.size() > cap_tris) return KPT_ERR_INVALID_ARG;
+    struct RefNode { int32_t first; uint32_t count; float pmin[3]; float pmax[3]; };
+    RefNode* out = (RefNode*)nodes32;
+    for (int32_t k = 0; k < b.used; ++k) {
+        const Node& n = b.nodes[k];
+        out[k].first = (int32_t)n.a;
+        out[k].count = n.b;
+        for (int j = 0; j < 3; ++j) { out[k].pmin[j] = n.lo[j]; out[k].pmax[j] = n.hi[j]; }
+    }
+    for (size_t s = 0; s < b.order.size(); ++s) tri_index[s] = b.order[s];
+    out_counts[0] = b.used;
+    out_counts[1] = (int64_t)b.order.size();
+    return KPT_OK;
+}
+
+int kpt_get_tlas(const kpt_ctx* ctx, void* nodes32, int64_t cap_nodes, int64_t* out_count) {
+    if (!ctx || !nodes32 || !out_count) return KPT_ERR_INVALID_ARG;
+    if ((int64_t)ctx->tlas.nodes.size() > cap_nodes) return KPT_ERR_INVALID_ARG;
+    struct RefT { float pmin[3]; float pmax[3]; uint32_t lr; int32_t blas; };
+    RefT* out = (RefT*)nodes32;
+    for (size_t k = 0; k < ctx->tlas.nodes.size(); ++k) {
+        const Node& n = ctx->tlas.nodes[k];
+        for (int j = 0; j < 3; ++j) { out[k].pmin[j] = n.lo[j]; out[k].pmax[j] = n.hi[j]; }
+        out[k].lr = n.a;
+        out[k].blas = (int32_t)n.b;
+    }
+    *out_count = (int64_t)ctx->tlas.nodes.size();
+    return KPT_OK;
+}
+
+int kpt_trace_closest(kpt_ctx* ctx, const kpt_ray* rays, kpt_hit* hits, int64_t n) {
+    if (!ctx || (n > 0 && (!rays || !hits)) || n < 0) return KPT_ERR_INVALID_ARG;
+    if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
+    if (!ctx->scene_ready) return fail(ctx, KPT_ERR_STATE, "no scene");
+    HIPCHK(launch_trace_closest(ctx->stream, scene_view(ctx), rays, hits, n, ctx->d_overflow, nullptr));
+    return KPT_OK;
+}
+
+int kpt_trace_any(kpt_ctx* ctx, const kpt_ray* rays, int32_t* occluded, int64_t n) {
+    if (!ctx || (n > 0 && (!rays || !occluded)) || n < 0) return KPT_ERR_INVALID_ARG;
+    if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
+    if (!ctx->scene_ready) return fail(ctx, KPT_ERR_STATE, "no scene");
+    HIPCHK(launch_trace_any(ctx->stream, scene_view(ctx), rays, occluded, n, ctx->d_overflow));
+    return KPT_OK;
+}
+
+int kpt_trace_closest_counted(kpt_ctx* ctx, const kpt_ray* rays, kpt_hit* hits, int64_t n,
+                              kpt_trace_counters* host_counters) {
+    if (!ctx || !host_counters || n < 0 || (n > 0 && (!rays || !hits))) return KPT_ERR_INVALID_ARG;
+    if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
+    if (!ctx->scene_ready) return fail(ctx, KPT_ERR_STATE, "no scene");
+    kpt_trace_counters* d = nullptr;
+    HIPCHK(hipMalloc((void**)&d, sizeof(kpt_trace_counters)));
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(kpt_trace_counters), ctx->stream);
+    if (e == hipSuccess) e = launch_trace_closest(ctx->stream, scene_view(ctx), rays, hits, n, ctx->d_overflow, d);
+    if (e == hipSuccess) e = hipMemcpyAsync(host_counters, d, sizeof(kpt_trace_counters), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(ctx, KPT_ERR_HIP, std::string("counted trace: ") + hipGetErrorString(e));
+    host_counters->rays = (uint64_t)n;
+    return KPT_OK;
+}
+
+int kpt_set_settings(kpt_ctx* ctx, const kpt_settings* s) {
+    if (!ctx || !s) return KPT_ERR_INVALID_ARG;
+    if (s->max_depth < 0 || s->seed_mode < 0 || s->seed_mode > 1 || s->integrator < 0 || s->integrator > 1)
+        return fail(ctx, KPT_ERR_INVALID_ARG, "invalid settings");
+    ctx->settings = *s;
+    return kpt_reset_accumulation(ctx);   // renderer.cu:301-305
+}
+
+int kpt_get_settings(const kpt_ctx* ctx, kpt_settings* out) {
+    if (!ctx || !out) return KPT_ERR_INVALID_ARG;
+    *out = ctx->settings;
+    return KPT_OK;
+}
+
+int kpt_set_view(kpt_ctx* ctx, const float projection[16], const float view[16]) {
+    if (!ctx || !projection || !view) return KPT_ERR_INVALID_ARG;
+    hm::Mat4 ip = hm::glm_inverse(hm::load(projection));
+    hm::Mat4 iv = hm::glm_inverse(hm::load(view));
+    std::memcpy(ctx->cam.prev_inv_view, ctx->cam.inv_view, 64);
+    std::memcpy(ctx->cam.prev_inv_proj, ctx->cam.inv_proj, 64);
+    std::memcpy(ctx->cam.inv_proj, ip.m, 64);
+    std::memcpy(ctx->cam.inv_view, iv.m, 64);
+    if (!ctx->settings.use_temporal_accumulation) return kpt_reset_accumulation(ctx);
+    return KPT_OK;
+}
+
+int kpt_set_camera(kpt_ctx* ctx, const kpt_camera* cam) {
+    if (!ctx || !cam) return KPT_ERR_INVALID_ARG;
+    ctx->cam = *cam;
+    return KPT_OK;
+}
+
+int kpt_get_camera(const kpt_ctx* ctx, kpt_camera* out) {
+    if (!ctx || !out) return KPT_ERR_INVALID_ARG;
+    *out = ctx->cam;
+    return KPT_OK;
+}
+
+int kpt_resize(kpt_ctx* ctx, int32_t width, int32_t height) {
+    if (!ctx || width <= 0 || height <= 0 || (int64_t)width * height > (int64_t)1 << 28)
+        return KPT_ERR_INVALID_ARG;
+    if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    free_frame(ctx);
+    size_t n = (size_t)width * height;
+    HIPCHK(hipMalloc((void**)&ctx->fb.accum, n * sizeof(float4)));
+    HIPCHK(hipMalloc((void**)&ctx->fb.render, n * sizeof(float4)));
+    HIPCHK(hipMalloc((void**)&ctx->fb.gbuffer, n * sizeof(uint4)));
+    HIPCHK(hipMalloc((void**)&ctx->fb.prev_gbuffer, n * sizeof(uint4)));
+    HIPCHK(hipMalloc((void**)&ctx->fb.vbuffer, n * sizeof(float4)));
+    HIPCHK(hipMalloc((void**)&ctx->fb.output, n * sizeof(float4)));
+    ctx->width = width;
+    ctx->height = height;
+    HIPCHK(launch_clear(ctx->stream, (float4*)ctx->fb.prev_gbuffer, (int64_t)n));
+    HIPCHK(launch_clear(ctx->stream, ctx->fb.vbuffer, (int64_t)n));
+    HIPCHK(launch_clear(ctx->stream, (float4*)ctx->fb.gbuffer, (int64_t)n));
+    HIPCHK(launch_clear(ctx->stream, ctx->fb.render, (int64_t)n));
+    HIPCHK(launch_clear(ctx->stream, ctx->fb.output, (int64_t)n));
+    return kpt_reset_accumulation(ctx);
+}
+
+int kpt_reset_accumulation(kpt_ctx* ctx) {
+    if (!ctx) return KPT_ERR_INVALID_ARG;
+    ctx->frame_index = 0;
+    if (ctx->host_only || !ctx->fb.accum) return KPT_OK;
+    HIPCHK(launch_clear(ctx->stream, ctx->fb.accum, (int64_t)ctx->width * ctx->height));
+    return KPT_OK;
+}
+
+int kpt_render(kpt_ctx* ctx, int32_t spp) {
+    if (!ctx || spp < 0) return KPT_ERR_INVALID_ARG;
+    if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
+    if (!ctx->scene_ready) return fail(ctx, KPT_ERR_STATE, "no scene");
+    if (!ctx->fb.accum) return fail(ctx, KPT_ERR_STATE, "no frame buffers (kpt_resize first)");
+    if (ctx->settings.use_temporal_accumulation) return fail(ctx, KPT_ERR_UNSUPPORTED, "temporal accumulation not built");
+    SceneView S = scene_view(ctx);
+    const int64_t npx = (int64_t)ctx->width * ctx->height;
+    if (ctx->settings.integrator == KPT_INTEGRATOR_WAVEFRONT) {
+        int rc = alloc_wavefront(ctx, npx);
+        if (rc != KPT_OK) return rc;
+    }
+    if (ctx->count_trace) {
+        HIPCHK(hipMemsetAsync(ctx->d_counts, 0, 8 * sizeof(unsigned long long), ctx->stream));
+    }
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    double trace_ms = 0.0;
+    int trace_launches = 0;
+    for (int32_t k = 0; k < spp; ++k) {
+        FrameConst fc = make_frame(ctx, ctx->frame_index);
+        if (ctx->settings.integrator == KPT_INTEGRATOR_WAVEFRONT) {
+            ctx->ins.counters = ctx->count_trace ? ctx->d_counts : nullptr;
+            HIPCHK(wavefront_frame(ctx->stream, S, fc, ctx->frame_index, ctx->fb, ctx->ws, ctx->d_overflow,
+                                   (int32_t)ctx->materials.size(), 1, &ctx->ins));
+            // per-launch trace times of this frame (events already recorded)
+            HIPCHK(hipEventSynchronize(ctx->ins.end[ctx->ins.launches - 1]));
+            for (int j = 0; j < ctx->ins.launches; ++j) {
+                float t = 0.0f;
+                HIPCHK(hipEventElapsedTime(&t, ctx->ins.begin[j], ctx->ins.end[j]));
+                trace_ms += t;
+            }
+            trace_launches += ctx->ins.launches;
+        } else {
+            HIPCHK(launch_megakernel(ctx->stream, S, fc, ctx->frame_index, ctx->fb, ctx->d_overflow));
+            trace_launches += 1;
+        }
+        // end of executeRendering: gbuffer -> prev_gbuffer, camera prev <- curr,
+        // frame_index++ (renderer.cu:186-194)
+        HIPCHK(hipMemcpyAsync(ctx->fb.prev_gbuffer, ctx->fb.gbuffer, (size_t)npx * sizeof(uint4),
+                              hipMemcpyDeviceToDevice, ctx->stream));
+        std::memcpy(ctx->cam.prev_inv_view, ctx->cam.inv_view, 64);
+        std::memcpy(ctx->cam.prev_inv_proj, ctx->cam.inv_proj, 64);
+        ctx->frame_index++;
+    }
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->timing[0] = ms;
+    ctx->timing[1] = (ctx->settings.integrator == KPT_INTEGRATOR_WAVEFRONT) ? trace_ms : ms;
+    ctx->timing[2] = trace_launches;
+    if (ctx->count_trace) {
+        unsigned long long h[8];
+        HIPCHK(hipMemcpy(h, ctx->d_counts, sizeof(h), hipMemcpyDeviceToHost));
+        ctx->render_counts.blas_interior_pops = h[0];
+        ctx->render_counts.blas_leaf_pops = h[1];
+        ctx->render_counts.tri_tests = h[2];
+        ctx->render_counts.blas_entries = h[3];
+        ctx->render_counts.tlas_pops = h[4];
+        ctx->render_counts.rays = 0;
+    }
+    int32_t ovf = 0;
+    HIPCHK(hipMemcpy(&ovf, ctx->d_overflow, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (ovf) return fail(ctx, KPT_ERR_OVERFLOW, "traversal stack overflow (depth > 128)");
+    return KPT_OK;
+}
+
+int kpt_get_frame_index(const kpt_ctx* ctx, int32_t* out) {
+    if (!ctx || !out) return KPT_ERR_INVALID_ARG;
+    *out = ctx->frame_index;
+    return KPT_OK;
+}
+
+int kpt_readback(kpt_ctx* ctx, int32_t which, void* host_dst, int64_t bytes) {
+    if (!ctx || !host_dst) return KPT_ERR_INVALID_ARG;
+    if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
+    const void* src = nullptr;
+    switch (which) {
+        case 0: src = ctx->fb.render; break;
+        case 1: src = ctx->fb.accum; break;
+        case 2: src = ctx->fb.gbuffer; break;
+        case 3: src = ctx->fb.vbuffer; break;
+        case 4: src = ctx->fb.output; break;
+        default: return fail(ctx, KPT_ERR_INVALID_ARG, "unknown buffer");
+    }
+    if (!src) return fail(ctx, KPT_ERR_STATE, "no frame buffers");
+    int64_t need = (int64_t)ctx->width * ctx->height * 16;
+    if (bytes < need) return fail(ctx, KPT_ERR_INVALID_ARG, "destination too small");
+    HIPCHK(hipMemcpyAsync(host_dst, src, (size_t)need, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return KPT_OK;
+}
+
+int kpt_sync(kpt_ctx* ctx) {
+    if (!ctx) return KPT_ERR_INVALID_ARG;
+    if (ctx->host_only) return KPT_OK;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    int32_t ovf = 0;
+    HIPCHK(hipMemcpy(&ovf, ctx->d_overflow, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (ovf) return fail(ctx, KPT_ERR_OVERFLOW, "traversal stack overflow (depth > 128)");
+    return KPT_OK;
+}
+
+int kpt_set_trace_counting(kpt_ctx* ctx, int32_t enable) {
+    if (!ctx) return KPT_ERR_INVALID_ARG;
+    ctx->count_trace = enable != 0;
+    return KPT_OK;
+}
+
+int kpt_get_render_counters(const kpt_ctx* ctx, kpt_trace_counters* out) {
+    if (!ctx || !out) return KPT_ERR_INVALID_ARG;
+    *out = ctx->render_counts;
+    return KPT_OK;
+}
+
+int kpt_last_timing(const kpt_ctx* ctx, double out[4]) {
+    if (!ctx || !out) return KPT_ERR_INVALID_ARG;
+    for (int i = 0; i < 4; ++i) out[i] = ctx->timing[i];
+    return KPT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,224 @@
+// kpt_integrator.h — the per-path integrator (Integrator::Li,
+// integrator.cuh:241-402) as device code, split into the pieces the
+// megakernel and the wavefront stages share.
+#pragma once
+#include "kpt_shade.h"
+
+namespace kpt {
+
+struct VisOut {
+    v3 albedo;
+    v3 normal;
+    v3 pos;
+    float depth;
+    int32_t inst;
+    int32_t prim;
+};
+
+KD v3 observer_pos() { return mk3(0.0f, KPT_EARTH_R + 1.0f, 0.0f); }
+
+// Sky radiance for a miss, including LeSun on the primary ray
+// (integrator.cuh:315-326, 135-143).
+KD v3 miss_radiance(const FrameConst& fc, v3 dir, bool first) {
+    const v3 sun = ld3(fc.sun_dir);
+    v3 sky = sky_li(observer_pos(), dir, sun, fc.sun_emission);
+    if (first) {
+        float sim = dot(dir, sun);
+        bool on_sun = sim > fc.min_sun_cos;
+        float sr = fc.sun_emission * KPT_SUN_NITS;
+        float le = sr * dm::powf(1.0f / sr, on_sun ? 0.0f : 1.0f);
+        sky = sky * mk3(le, le, le);
+    }
+    return sky;
+}
+
+// A deferred next-event estimate: the shadow ray and the two possible
+// additions to L (unoccluded / occluded).  When no shadow ray is needed,
+// `trace` is false and `occl` is the addition.
+struct NeeResult {
+    bool trace;
+    v3 org;
+    v3 dir;
+    float tmax;
+    v3 unoccl;
+    v3 occl;
+};
+
+// sampleLdSun / sampleLd and the L += Ld*beta/p of integrator.cuh:365-375,
+// with the visibility test left to the caller.  Consumes the same random
+// numbers, in the same order, as the reference.
+KD NeeResult next_event(const SceneView& S, const FrameConst& fc, const Surf& s, const BSDF& b, v3 ray_d, v3 beta,
+                        uint32_t& seed) {
+    NeeResult r;
+    r.trace = false;
+    const float sun_p = 0.5f;
+    const v3 zero = mk3(0.0f, 0.0f, 0.0f);
+    if (rng1(seed) < sun_p) {
+        // sampleLdSun, integrator.cuh:147-186
+        const v3 sun = ld3(fc.sun_dir);
+        v3 sun_pos = sun * KPT_SOLAR_DIST;
+        float sun_radius = fc.sun_radius;
+        float ux = rng1(seed);
+        float uy = rng1(seed);
+        float uz = rng1(seed);
+        v3 off = mk3(ux * 2.0f - 1.0f, uy * 2.0f - 1.0f, uz * 2.0f - 1.0f);
+        v3 target = sun_pos + (off * sun_radius);
+        v3 wo = -ray_d;
+        v3 fcos = bsdf_f(b, wo, sun) * fmaxf_(0.0f, dot(sun, (s.backface ? -1.0f : 1.0f) * s.n));
+        r.occl = zero * beta / sun_p;
+        if (is_zero(fcos)) return r;
+        v3 srad = ld3(fc.sun_radiance);
+        if (is_zero(srad)) return r;
+        // spawnRayTo + Unoccluded (interaction.cuh:43-53, integrator.cuh:124-131)
+        v3 org = s.backface ? s.pos - (s.n * KPT_HIT_EPS) : s.pos + (s.n * KPT_HIT_EPS);
+        r.org = org;
+        r.dir = normalize(target - org);
+        r.tmax = distance(target, org) - 0.11f;
+        float sun_area = KPT_PI * sqr(sun_radius);
+        v3 snrm = normalize(target - sun_pos), wi = normalize(target - s.pos);
+        float cth = absdot(snrm, -wi);
+        float pdf = (1.0f / sun_area) / (cth / sqr(KPT_SOLAR_DIST));
+        v3 Ld = (fcos * srad * KPT_SUN_NITS) / pdf;
+        r.unoccl = Ld * beta / sun_p;
+        r.trace = true;
+        return r;
+    }
+    // sampleLd, integrator.cuh:188-220
+    const float p_other = 1.0f - sun_p;
+    r.occl = zero * beta / p_other;
+    float ul = rng1(seed);
+    const int nl = fc.num_lights;
+    if (nl < 1) return r;
+    int li = (int)(ul * (float)nl);
+    if (li > nl - 1) li = nl - 1;
+    const Light l = S.lights[li];
+    float pmf = 1.0f / (float)nl;
+    float u2x = rng1(seed);
+    float u2y = rng1(seed);
+    // Triangle::sample (triangle.cuh:104-119) through Light::sampleLi (light.cu:33-45)
+    const TriAttr ta = S.attrs[l.prim];
+    const float* model = S.inst_shade[l.inst].model;
+    float bx, by;
+    if (u2x < u2y) { bx = u2x / 2.0f; by = u2y - bx; } else { by = u2y / 2.0f; bx = u2x - by; }
+    float bz = 1.0f - bx - by;
+    v2 luv;
+    luv.x = (bx * ta.uv0[0]) + (by * ta.uv1[0]) + (bz * ta.uv2[0]);
+    luv.y = (bx * ta.uv0[1]) + (by * ta.uv1[1]) + (bz * ta.uv2[1]);
+    // object-space vertices of the light triangle (v0 + e1 need not equal v1
+    // in float, so lights keep their own vertex copy)
+    v3 p0 = ld3(S.light_verts + 9 * (size_t)li), p1 = ld3(S.light_verts + 9 * (size_t)li + 3),
+       p2 = ld3(S.light_verts + 9 * (size_t)li + 6);
+    v3 p = (p0 * bx) + (p1 * by) + (p2 * bz);
+    p = mul4_xyz(model, mk4(p, 1.0f));
+    v3 gn = mul4_xyz(model, mk4(ta.lgn[0], ta.lgn[1], ta.lgn[2], 0.0f));
+    float area = 0.5f * length(cross(p1 - p0, p2 - p0));
+    float spdf = 1.0f / area;
+    spdf *= sqr(distance(p, s.pos));
+    spdf /= absdot(normalize(p - s.pos), gn);
+    v3 wi = normalize(p - s.pos);
+    v3 lL = light_L(S, l, luv);
+    if (spdf <= 0.0f || is_zero(lL)) return r;
+    v3 wo = -ray_d;
+    v3 fcos = bsdf_f(b, wo, wi) * fmaxf_(0.0f, dot(wi, (s.backface ? -1.0f : 1.0f) * s.n));
+    if (is_zero(fcos)) return r;
+    v3 org = s.backface ? s.pos - (s.n * KPT_HIT_EPS) : s.pos + (s.n * KPT_HIT_EPS);
+    r.org = org;
+    r.dir = normalize(p - org);
+    r.tmax = distance(p, org) - 0.11f;
+    float pl = pmf * spdf;
+    float pb = bsdf_pdf(b, wo, wi);
+    float f = 1.0f * pl, g = 1.0f * pb;
+    float wl = sqr(f) / (sqr(f) + sqr(g));
+    v3 Ld = (lL * fcos * wl) / pl;
+    r.unoccl = Ld * beta / p_other;
+    r.trace = true;
+    return r;
+}
+
+// Emission of a hit with MIS against the previous BSDF sample
+// (integrator.cuh:333-343).  Returns the addition to L (zero when none).
+KD bool emitted(const SceneView& S, const FrameConst& fc, const Surf& s, bool first, float p_b, v3 prev_pos,
+                v3 beta, v3* Le_out, v3* add) {
+    v3 Le = mk3(0.0f, 0.0f, 0.0f);
+    if (s.light >= 0) Le = light_L(S, S.lights[s.light], s.uv);
+    *Le_out = Le;
+    if (is_zero(Le)) return false;
+    float wl = 1.0f;
+    if (s.light >= 0 && !first) {
+        float lpdf = (1.0f / (float)fc.num_lights) * light_pdf_li(S.lights[s.light], prev_pos, s.pos, s.n);
+        float f = 1.0f * p_b, g = 1.0f * lpdf;
+        wl = sqr(f) / (sqr(f) + sqr(g));
+    }
+    *add = Le * wl * beta;
+    return true;
+}
+
+// Megakernel form of Li: the whole path in one thread.
+KD v3 li_path(const SceneView& S, const FrameConst& fc, Ray ray, uint32_t& seed, uint32_t* lds, bool* ovf, VisOut* vs) {
+    v3 L = mk3(0.0f, 0.0f, 0.0f), beta = mk3(1.0f, 1.0f, 1.0f);
+    bool any_nonspec = false;
+    const float eta_scale = 1.0f;
+    int32_t depth = 0;
+    float p_b = 1.0f;
+    v3 prev_pos = mk3(0.0f, 0.0f, 0.0f);
+    while (!is_zero(beta)) {
+        seed = seed + (uint32_t)depth;
+        const bool first = (depth == 0);
+        bool o = false;
+        Hit h = trace_closest<false>(S, ray, 0.0f, __builtin_inff(), lds, &o, nullptr);
+        *ovf = *ovf || o;
+        if (h.slot < 0) {
+            v3 sky = miss_radiance(fc, ray.d, first);
+            if (first) {
+                vs->albedo = sky; vs->depth = __builtin_inff(); vs->normal = mk3(0.0f, 0.0f, 0.0f);
+                vs->pos = mk3(0.0f, 0.0f, 0.0f); vs->inst = -1; vs->prim = -1;
+            }
+            L = L + sky * beta;
+            break;
+        }
+        Surf s = make_surf(S, h, ray);
+        v3 Le, add;
+        if (emitted(S, fc, s, first, p_b, prev_pos, beta, &Le, &add)) L = L + add;
+        BSDF b = get_bsdf(S, s);
+        if (first) {
+            vs->albedo = !is_zero(Le) ? Le : b.albedo;
+            vs->depth = s.distance; vs->normal = s.n; vs->pos = s.pos; vs->inst = s.inst; vs->prim = s.prim;
+            b.albedo = mk3(1.0f, 1.0f, 1.0f);
+        }
+        if (any_nonspec) bsdf_regularize(b);
+        if (!(b.alpha < 0.001f)) {
+            NeeResult ne = next_event(S, fc, s, b, ray.d, beta, seed);
+            if (ne.trace) {
+                bool oo = false;
+                Ray sr = make_ray(ne.org, ne.dir);
+                bool occluded = trace_any<false>(S, sr, 0.0f, ne.tmax, lds, &oo, nullptr);
+                *ovf = *ovf || oo;
+                L = L + (occluded ? ne.occl : ne.unoccl);
+            } else {
+                L = L + ne.occl;
+            }
+        }
+        if (depth++ == fc.max_depth) break;
+        v3 wo = -ray.d;
+        v2 u2, X2;
+        u2.x = rng1(seed); u2.y = rng1(seed);
+        X2.x = rng1(seed); X2.y = rng1(seed);
+        BSample bs = bsdf_sample(b, wo, u2, X2);
+        if (is_zero(bs.f)) break;
+        beta = beta * (bs.f * absdot(s.n, bs.wi) / bs.pdf);
+        p_b = bsdf_pdf(b, wo, bs.wi);
+        any_nonspec = any_nonspec || !((bs.scatter & SC_SPECULAR) != 0);
+        prev_pos = s.pos;
+        v3 org = (bs.scatter & SC_TRANSMITTED) ? s.pos - (s.n * KPT_HIT_EPS) : s.pos + (s.n * KPT_HIT_EPS);
+        ray = make_ray(org, bs.wi);
+        v3 rr = beta * eta_scale;
+        if (max_comp(rr) < 1.0f && depth > 1) {
+            float q = fmaxf_(0.0f, 1.0f - max_comp(rr));
+            if (rng1(seed) < q) break;
+            beta = beta / (1.0f - q);
+        }
+    }
+    return L;
+}
+
+}  // namespace kpt

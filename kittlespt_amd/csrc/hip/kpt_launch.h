@@ -1,0 +1,73 @@
+// kpt_launch.h — host-side launch wrappers of the gfx950 kernels (defined in
+// kpt_kernels.hip, called by the C-ABI implementation).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../../include/kpt_abi.h"
+#include "../common/kpt_device_types.h"
+
+namespace kpt {
+
+// Frame buffers of one context, device pointers.
+struct FrameBuffers {
+    float4* accum;     // running sum (static) or history (temporal)
+    float4* render;    // current estimate
+    uint4* gbuffer;
+    uint4* prev_gbuffer;
+    float4* vbuffer;
+    float4* output;
+};
+
+// Per-path state of the wavefront tracer (struct of arrays in HBM).
+struct WavefrontState {
+    int64_t capacity;
+    // path records
+    float4* ray_o;        // xyz origin, w = bit-cast pixel index
+    float4* ray_d;        // xyz direction, w = bit-cast seed
+    float4* beta;         // xyz throughput, w = p_b
+    float4* L;            // xyz radiance, w = bit-cast (depth | flags<<16)
+    float4* prev_pos;     // xyz previous hit position
+    float4* vis_a;        // visible surface: albedo xyz, depth
+    float4* vis_n;        // normal xyz, bit-cast instance
+    float4* vis_p;        // position xyz, bit-cast primitive
+    // hits of the current bounce
+    float4* hit;          // t, bit-cast slot, bit-cast instance, u ; v kept in hit_v
+    float* hit_v;
+    // queues (path indices)
+    int32_t* queue[2];
+    int32_t* shadow_queue;
+    float4* shadow_o;     // xyz origin, w = tmax
+    float4* shadow_d;     // xyz direction
+    float4* shadow_c;     // contribution when unoccluded
+    float4* shadow_z;     // contribution when occluded ((0*beta)/p)
+    int32_t* sort_keys;
+    int32_t* sort_tmp;
+    int32_t* counters;    // [0] = next queue size, [1] = shadow count, [2] = overflow
+    int32_t* hist;        // material histogram for the shading sort
+};
+
+hipError_t launch_trace_closest(hipStream_t s, const SceneView& S, const kpt_ray* rays, kpt_hit* hits, int64_t n,
+                                int32_t* overflow, kpt_trace_counters* counters);
+hipError_t launch_trace_any(hipStream_t s, const SceneView& S, const kpt_ray* rays, int32_t* occluded, int64_t n,
+                            int32_t* overflow);
+hipError_t launch_megakernel(hipStream_t s, const SceneView& S, const FrameConst& fc, int32_t frame,
+                             const FrameBuffers& fb, int32_t* overflow);
+hipError_t launch_clear(hipStream_t s, float4* p, int64_t n);
+
+// Events bracketing every closest-hit trace launch of a frame (recorded on the
+// context stream), and optional traversal event counters.
+constexpr int WF_MAX_TRACE_LAUNCHES = 40;
+struct WfInstrument {
+    hipEvent_t begin[WF_MAX_TRACE_LAUNCHES];
+    hipEvent_t end[WF_MAX_TRACE_LAUNCHES];
+    int launches;                          // filled by wavefront_frame
+    unsigned long long* counters;          // device [5] or nullptr (counted traversal)
+};
+
+// Wavefront path tracer: one reference frame (one sample per pixel).
+hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& fc, int32_t frame,
+                           const FrameBuffers& fb, WavefrontState& W, int32_t* overflow, int32_t num_materials,
+                           int32_t sort_by_material, WfInstrument* ins);
+
+}  // namespace kpt

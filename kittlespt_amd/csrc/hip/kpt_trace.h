@@ -1,0 +1,278 @@
+// kpt_trace.h — two-level BVH2 traversal for gfx950 (device code).
+//
+// Semantics are those of TLAS::intersect / TLAS::intersectP (tlas.cu:8-125)
+// with BLAS2::intersect / intersectP (blas.cuh:50-188), restructured for the
+// GPU:
+//   * one loop and one stack for both levels; a stack entry carries a level
+//     bit, so the BLAS of a TLAS leaf is traversed to completion before the
+//     next TLAS entry is popped (LIFO), exactly as the nested reference calls;
+//   * the per-lane stack lives in LDS ([entry][lane] layout: lanes of a wave
+//     hit 64 distinct banks) with a private-memory (scratch) spill region for
+//     entries beyond the LDS depth, instead of the reference's fixed 16-entry
+//     register array that overflows on deep trees;
+//   * the BLAS is entered with tmax = min(tmax, closest-so-far).  The reference
+//     starts every BLAS with the original tmax and a local closest of +inf;
+//     nodes and triangles this extra bound skips can only produce hits that
+//     TLAS::intersect would reject (it keeps a hit only if strictly closer),
+//     and the traversal order of the nodes that remain is unchanged, so the
+//     closest hit (including which of two equal-distance hits wins) is the
+//     same;
+//   * any-hit needs no ordering: its result is "some reachable triangle is
+//     hit", which does not depend on the visiting order.
+#pragma once
+#include "kpt_vec.h"
+#include "../common/kpt_device_types.h"
+
+namespace kpt {
+
+constexpr int TRACE_BLOCK = 256;      // threads per workgroup of trace-bearing kernels
+constexpr int LDS_STACK = 16;         // stack entries per lane held in LDS
+constexpr int SPILL_STACK = 112;      // further entries in private memory
+constexpr uint32_t TLAS_BIT = 0x80000000u;
+
+struct Ray {
+    v3 o, d, inv;
+};
+KD Ray make_ray(v3 o, v3 d) { Ray r; r.o = o; r.d = d; r.inv = 1.0f / d; return r; }
+
+struct Hit {
+    float t;
+    int32_t slot;   // triangle slot (-1 = miss)
+    int32_t inst;
+    float u, v;
+};
+
+struct TraceCounts {
+    uint32_t interior, leaf, tri, entries, tlas;
+};
+
+// Bounds3f::intersectP, bounds.cuh:38-57
+KD bool slab(const float4& lo, const float4& hi, const Ray& r, float tmin, float tmax, float* tenter) {
+    float t0x = (lo.x - r.o.x) * r.inv.x, t0y = (lo.y - r.o.y) * r.inv.y, t0z = (lo.z - r.o.z) * r.inv.z;
+    float t1x = (hi.x - r.o.x) * r.inv.x, t1y = (hi.y - r.o.y) * r.inv.y, t1z = (hi.z - r.o.z) * r.inv.z;
+    float ex = fminf_(t0x, t1x), ey = fminf_(t0y, t1y), ez = fminf_(t0z, t1z);
+    float xx = fmaxf_(t0x, t1x), xy = fmaxf_(t0y, t1y), xz = fmaxf_(t0z, t1z);
+    float te = fmaxf_(fmaxf_(ex, ey), ez);
+    float tx = fminf_(fminf_(xx, xy), xz);
+    *tenter = te;
+    return (te <= tx) && (tx >= tmin) && (te <= tmax);
+}
+
+KD void load_node(const Node* nodes, uint32_t i, float4* lo, float4* hi) {
+    const float4* p = reinterpret_cast<const float4*>(nodes + i);
+    *lo = p[0];
+    *hi = p[1];
+}
+
+// Object-space ray: Ray::transform (ray.cuh:25-29) with the full 4-term rows.
+KD Ray object_ray(const InstTrav& it, const Ray& w) {
+    v3 o = mk3(it.r0[0] * w.o.x + it.r0[1] * w.o.y + it.r0[2] * w.o.z + it.r0[3] * 1.0f,
+               it.r1[0] * w.o.x + it.r1[1] * w.o.y + it.r1[2] * w.o.z + it.r1[3] * 1.0f,
+               it.r2[0] * w.o.x + it.r2[1] * w.o.y + it.r2[2] * w.o.z + it.r2[3] * 1.0f);
+    v3 d = mk3(it.r0[0] * w.d.x + it.r0[1] * w.d.y + it.r0[2] * w.d.z + it.r0[3] * 0.0f,
+               it.r1[0] * w.d.x + it.r1[1] * w.d.y + it.r1[2] * w.d.z + it.r1[3] * 0.0f,
+               it.r2[0] * w.d.x + it.r2[1] * w.d.y + it.r2[2] * w.d.z + it.r2[3] * 0.0f);
+    return make_ray(o, d);
+}
+
+// Triangle::intersect (triangle.cuh:66-102).  Returns t (> tmin, < tmax) or -1.
+KD bool tri_test(const TriPos& tp, const Ray& r, float tmin, float tmax, float* t_out, float* u_out, float* v_out) {
+    const float EPS = 1.0e-5f;
+    v3 v0 = mk3(tp.v0[0], tp.v0[1], tp.v0[2]);
+    v3 e1 = mk3(tp.e1[0], tp.e1[1], tp.e1[2]);
+    v3 e2 = mk3(tp.e2[0], tp.e2[1], tp.e2[2]);
+    v3 pvec = cross(r.d, e2);
+    float det = dot(e1, pvec);
+    if (det > -EPS && det < EPS) return false;
+    float inv = 1.0f / det;
+    v3 tvec = r.o - v0;
+    float u = inv * dot(tvec, pvec);
+    if (u < 0.0f || u > 1.0f) return false;
+    v3 q = cross(tvec, e1);
+    float v = inv * dot(r.d, q);
+    if (v < 0.0f || u + v > 1.0f) return false;
+    float t = inv * dot(e2, q);
+    if (t > EPS && t < tmax && t > tmin) {
+        *t_out = t; *u_out = u; *v_out = v;
+        return true;
+    }
+    return false;
+}
+
+// Per-lane stack: LDS part [LDS_STACK][TRACE_BLOCK] + private spill.
+struct LaneStack {
+    uint32_t* lds;      // &s_stack[0][lane]
+    uint32_t spill[SPILL_STACK];
+    int sp;
+    bool overflow;
+    KD void push(uint32_t e) {
+        if (sp < LDS_STACK) lds[sp * TRACE_BLOCK] = e;
+        else if (sp < LDS_STACK + SPILL_STACK) spill[sp - LDS_STACK] = e;
+        else { overflow = true; return; }
+        ++sp;
+    }
+    KD uint32_t pop() {
+        --sp;
+        return sp < LDS_STACK ? lds[sp * TRACE_BLOCK] : spill[sp - LDS_STACK];
+    }
+};
+
+// Closest hit.  COUNT adds traversal event counts.
+template <bool COUNT>
+KD Hit trace_closest(const SceneView& S, const Ray& w, float tmin, float tmax, uint32_t* lds_lane,
+                     bool* overflow, TraceCounts* cnt) {
+    Hit h;
+    h.t = __builtin_inff(); h.slot = -1; h.inst = -1; h.u = 0.0f; h.v = 0.0f;
+    if (S.tlas_root < 0) return h;
+    float4 rlo, rhi;
+    load_node(S.tlas_nodes, 0, &rlo, &rhi);
+    float te;
+    if (!slab(rlo, rhi, w, tmin, tmax, &te)) return h;
+    LaneStack st;
+    st.lds = lds_lane; st.sp = 0; st.overflow = false;
+    st.push(TLAS_BIT | 0u);
+    Ray o = w;
+    int32_t cur_inst = -1;
+    while (st.sp > 0) {
+        uint32_t e = st.pop();
+        if (e & TLAS_BIT) {
+            uint32_t ni = e & ~TLAS_BIT;
+            const Node nd = S.tlas_nodes[ni];
+            if (COUNT) cnt->tlas++;
+            if (nd.a != 0u) {
+                uint32_t i1 = nd.a & 0xFFFFu, i2 = (nd.a >> 16) & 0xFFFFu;
+                float4 l1, h1, l2, h2;
+                load_node(S.tlas_nodes, i1, &l1, &h1);
+                load_node(S.tlas_nodes, i2, &l2, &h2);
+                float c1, c2;
+                bool k1 = slab(l1, h1, w, tmin, tmax, &c1);
+                bool k2 = slab(l2, h2, w, tmin, tmax, &c2);
+                k1 = k1 && c1 < h.t;
+                k2 = k2 && c2 < h.t;
+                if (k1 && k2) {
+                    if (c1 > c2) { st.push(TLAS_BIT | i1); st.push(TLAS_BIT | i2); }
+                    else { st.push(TLAS_BIT | i2); st.push(TLAS_BIT | i1); }
+                } else if (k1) {
+                    st.push(TLAS_BIT | i1);
+                } else if (k2) {
+                    st.push(TLAS_BIT | i2);
+                }
+            } else {
+                if (COUNT) cnt->entries++;
+                cur_inst = (int32_t)nd.b;
+                const InstTrav it = S.inst_trav[cur_inst];
+                o = object_ray(it, w);
+                if (it.root >= 0) st.push((uint32_t)it.root);
+            }
+        } else {
+            const float bt = fminf_(tmax, h.t);
+            float4 lo, hi;
+            load_node(S.blas_nodes, e, &lo, &hi);
+            uint32_t a = __builtin_bit_cast(uint32_t, lo.w), b = __builtin_bit_cast(uint32_t, hi.w);
+            if (b == 0u) {
+                if (COUNT) cnt->interior++;
+                float4 l1, h1, l2, h2;
+                load_node(S.blas_nodes, a, &l1, &h1);
+                load_node(S.blas_nodes, a + 1, &l2, &h2);
+                float c1, c2;
+                bool k1 = slab(l1, h1, o, tmin, bt, &c1) && c1 < h.t;
+                bool k2 = slab(l2, h2, o, tmin, bt, &c2) && c2 < h.t;
+                if (k1 && k2) {
+                    if (c1 > c2) { st.push(a); st.push(a + 1); }
+                    else { st.push(a + 1); st.push(a); }
+                } else if (k1) {
+                    st.push(a);
+                } else if (k2) {
+                    st.push(a + 1);
+                }
+            } else {
+                if (COUNT) cnt->leaf++;
+                for (uint32_t k = 0; k < b; ++k) {
+                    if (COUNT) cnt->tri++;
+                    const TriPos tp = S.tris[a + k];
+                    float t, u, v;
+                    if (tri_test(tp, o, tmin, fminf_(tmax, h.t), &t, &u, &v) && t < h.t) {
+                        h.t = t; h.slot = (int32_t)(a + k); h.inst = cur_inst; h.u = u; h.v = v;
+                    }
+                }
+            }
+        }
+        if (st.overflow) break;
+    }
+    *overflow = st.overflow;
+    return h;
+}
+
+// Any hit in (tmin, tmax).
+template <bool COUNT>
+KD bool trace_any(const SceneView& S, const Ray& w, float tmin, float tmax, uint32_t* lds_lane, bool* overflow,
+                  TraceCounts* cnt) {
+    if (S.tlas_root < 0) return false;
+    float4 rlo, rhi;
+    load_node(S.tlas_nodes, 0, &rlo, &rhi);
+    float te;
+    if (!slab(rlo, rhi, w, tmin, tmax, &te)) return false;
+    LaneStack st;
+    st.lds = lds_lane; st.sp = 0; st.overflow = false;
+    st.push(TLAS_BIT | 0u);
+    Ray o = w;
+    bool hit = false;
+    while (st.sp > 0 && !hit) {
+        uint32_t e = st.pop();
+        if (e & TLAS_BIT) {
+            uint32_t ni = e & ~TLAS_BIT;
+            const Node nd = S.tlas_nodes[ni];
+            if (COUNT) cnt->tlas++;
+            if (nd.a != 0u) {
+                uint32_t i1 = nd.a & 0xFFFFu, i2 = (nd.a >> 16) & 0xFFFFu;
+                float4 l1, h1, l2, h2;
+                load_node(S.tlas_nodes, i1, &l1, &h1);
+                load_node(S.tlas_nodes, i2, &l2, &h2);
+                float c1, c2;
+                if (slab(l1, h1, w, tmin, tmax, &c1)) st.push(TLAS_BIT | i1);
+                if (slab(l2, h2, w, tmin, tmax, &c2)) st.push(TLAS_BIT | i2);
+            } else {
+                if (COUNT) cnt->entries++;
+                const InstTrav it = S.inst_trav[nd.b];
+                o = object_ray(it, w);
+                if (it.root >= 0) st.push((uint32_t)it.root);
+            }
+        } else {
+            float4 lo, hi;
+            load_node(S.blas_nodes, e, &lo, &hi);
+            uint32_t a = __builtin_bit_cast(uint32_t, lo.w), b = __builtin_bit_cast(uint32_t, hi.w);
+            if (b == 0u) {
+                if (COUNT) cnt->interior++;
+                float4 l1, h1, l2, h2;
+                load_node(S.blas_nodes, a, &l1, &h1);
+                load_node(S.blas_nodes, a + 1, &l2, &h2);
+                float c1, c2;
+                if (slab(l1, h1, o, tmin, tmax, &c1)) st.push(a);
+                if (slab(l2, h2, o, tmin, tmax, &c2)) st.push(a + 1);
+            } else {
+                if (COUNT) cnt->leaf++;
+                for (uint32_t k = 0; k < b; ++k) {
+                    if (COUNT) cnt->tri++;
+                    const TriPos tp = S.tris[a + k];
+                    float t, u, v;
+                    if (tri_test(tp, o, tmin, tmax, &t, &u, &v)) { hit = true; break; }
+                }
+            }
+        }
+        if (st.overflow) break;
+    }
+    *overflow = st.overflow;
+    return hit;
+}
+
+// XCD-aware workgroup swizzle: the dispatcher hands consecutive workgroup ids
+// to the 8 XCDs round-robin; remap so each XCD owns a contiguous run of tiles
+// (neighbouring tiles share BVH nodes in that XCD's L2).
+KD uint32_t xcd_swizzle(uint32_t bid, uint32_t nblocks) {
+    const uint32_t X = 8;
+    uint32_t per = nblocks / X;
+    if (per == 0 || bid >= per * X) return bid;
+    return (bid % X) * per + (bid / X);
+}
+
+}  // namespace kpt

@@ -1,0 +1,67 @@
+/* kpt_oracle.h — C interface of the CPU oracle (test infrastructure only).
+ *
+ * The oracle restates the reference's host builders and device routines as
+ * plain single-threaded-per-pixel C++ in IEEE semantics.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, as the
+ * checker; the product never links it.
+ */
+#ifndef KPT_ORACLE_H
+#define KPT_ORACLE_H
+#include "../include/kpt_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_scene orc_scene;
+
+orc_scene* orc_scene_create(const kpt_scene_desc* desc);
+void orc_scene_destroy(orc_scene* s);
+/* [0]=triangles [1]=bvh nodes (incl. unused tail) [2]=tlas nodes [3]=instances
+ * [4]=max blas depth [5]=max tlas depth [6]=lights [7]=materials */
+void orc_scene_info(const orc_scene* s, int64_t out[8]);
+/* BLAS of geometry mesh m in the reference layout (32-B BVH2Node), node
+ * indices relative to the BLAS root, and its tri_index slice. */
+int64_t orc_get_blas(const orc_scene* s, int32_t mesh, void* nodes32, int64_t cap,
+                     int32_t* tri_index, int64_t cap_tris);
+int64_t orc_get_tlas(const orc_scene* s, void* nodes32, int64_t cap);
+
+/* Traversal of host rays.  mode 0 = BVH closest hit (TLAS::intersect),
+ * 1 = BVH any hit (TLAS::intersectP), 2 = brute-force closest
+ * (INTERSECT_DEBUG, integrator.cuh:64-95), 3 = brute-force any hit. */
+void orc_trace(const orc_scene* s, const kpt_ray* rays, int64_t n, int32_t mode,
+               kpt_hit* hits, int32_t* occluded, kpt_trace_counters* counters,
+               int32_t* max_stack, int32_t threads);
+
+/* Per-pixel megakernel semantics (kernels.cu:385-451): renders samples
+ * [frame0, frame0+spp) for the listed pixels with static accumulation.
+ * double_precision selects the FP64 instantiation (libm transcendentals).
+ * out_rgb[3*i] is the estimate after the last sample (sum / count).
+ * out_first: optional [n][4] float: last sample's gbuffer bits
+ * (enc nx, enc ny, albedo u32, depth) and out_v: [n][4] vbuffer. */
+void orc_render_pixels(const orc_scene* s, const kpt_settings* settings, const kpt_camera* cam,
+                       int32_t width, int32_t height, const int32_t* pixels, int64_t n,
+                       int32_t frame0, int32_t spp, int32_t double_precision,
+                       float* out_rgb, uint32_t* out_gbuffer, float* out_vbuffer,
+                       int32_t threads);
+
+/* One sample (radiance before clamp/accumulation) per pixel for frame `frame`.
+ * Used to check the accumulation kernels independently. */
+void orc_sample_pixels(const orc_scene* s, const kpt_settings* settings, const kpt_camera* cam,
+                       int32_t width, int32_t height, const int32_t* pixels, int64_t n,
+                       int32_t frame, float* out_rgb, int32_t threads);
+
+/* Deterministic math exported for tests. fn: 0 exp 1 exp2 2 log2 3 pow 4 sin
+ * 5 cos 6 tan 7 acos 8 log */
+void orc_dmath(int32_t fn, const float* x, const float* y, float* out, int64_t n);
+/* pcg_hash (samplers.cuh:12-17). */
+uint32_t orc_pcg_hash(uint32_t v);
+/* glm::inverse (vendored glm 1.0.0 compute_inverse<4,4>) and the reference's
+ * Matrix4x4::inverse (matrix_maths.cuh:304-354). */
+void orc_glm_inverse(const float m[16], float out[16]);
+void orc_mat4_inverse(const float m[16], float out[16]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

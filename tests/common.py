@@ -1,0 +1,74 @@
+"""Shared set-up for the tests: settings, cameras and ray batches."""
+import math
+
+import numpy as np
+
+from kittlespt_amd import abi, scenes
+from kittlespt_amd.context import Context
+
+
+def settings(max_depth=4, seed_mode=abi.SEED_XOR, integrator=abi.INTEGRATOR_MEGAKERNEL,
+             sun_theta=math.radians(30.0)):
+    s = abi.KptSettings()
+    s.max_depth = max_depth
+    s.use_temporal_accumulation = 0
+    s.bloom_final_blend = 0.175
+    s.bloom_internal_blend = 0.65
+    s.sun_angular_diameter_rad = 0.0087
+    s.sun_phi_rad = 4.18879
+    s.sun_theta_rad = sun_theta
+    s.sun_emission_factor = 1.0
+    s.seed_mode = seed_mode
+    s.integrator = integrator
+    return s
+
+
+def view_proj(scene, width, height):
+    proj = scenes.perspective_fov_lh(scene.camera_fov, width, height, 1.0, 100.0)
+    view = scenes.look_at_view(scene.camera_eye, scene.camera_target)
+    return proj, view
+
+
+def host_camera(scene, width, height):
+    """kpt_camera as Renderer::setView computes it, from a host-only context."""
+    c = Context(-2)
+    p, v = view_proj(scene, width, height)
+    c.set_view(p, v)
+    cam = c.camera()
+    c.close()
+    return cam
+
+
+def camera_rays(cam, width, height):
+    """Primary rays of every pixel through the same inverse matrices
+    (float64 here; only used as test inputs)."""
+    ip = np.array(cam.inv_proj, np.float64).reshape(4, 4).T
+    iv = np.array(cam.inv_view, np.float64).reshape(4, 4).T
+    ys, xs = np.mgrid[0:height, 0:width]
+    u = (xs + 0.5) / width * 2 - 1
+    v = (ys + 0.5) / height * 2 - 1
+    p = np.stack([u, v, np.ones_like(u), np.ones_like(u)], -1).reshape(-1, 4) @ ip.T
+    d = p[:, :3] / p[:, 3:4]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    dw = np.concatenate([d, np.zeros((len(d), 1))], 1) @ iv.T
+    dw = dw[:, :3] / np.linalg.norm(dw[:, :3], axis=1, keepdims=True)
+    rays = np.zeros(len(dw), abi.RAY_DTYPE)
+    rays["org"] = iv[:3, 3]
+    rays["dir"] = dw
+    rays["tmin"] = 0.0
+    rays["tmax"] = np.inf
+    return rays
+
+
+def random_rays(n, lo, hi, target_center, target_spread, seed=0, tmax=np.inf):
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(lo, hi, (n, 3))
+    tgt = rng.normal(size=(n, 3)) * target_spread + np.asarray(target_center)
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros(n, abi.RAY_DTYPE)
+    rays["org"] = o
+    rays["dir"] = d
+    rays["tmin"] = 0.0
+    rays["tmax"] = tmax
+    return rays

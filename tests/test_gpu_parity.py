@@ -1,0 +1,176 @@
+"""GPU parity: the gfx950 kernels against the CPU oracle on identical inputs.
+
+Bar: traversal bit-exact (instance id, primitive id, t, barycentrics, occlusion);
+radiance of every pixel equal to the oracle's (asserted as mean relative error
+<= 1e-4, and reported as the count of bit-different pixels, expected 0);
+G-buffer words bit-exact.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from kittlespt_amd import abi, scenes
+from kittlespt_amd.context import Context
+
+from common import camera_rays, host_camera, random_rays, settings, view_proj
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def plain():
+    sc = scenes.scene_plain()
+    ctx = Context(0)
+    ctx.upload(sc)
+    orc = oracle.OracleScene(sc)
+    yield sc, ctx, orc
+    ctx.close()
+    orc.close()
+
+
+@pytest.fixture(scope="module")
+def inst_small():
+    sc = scenes.scene_instanced_small()
+    ctx = Context(0)
+    ctx.upload(sc)
+    orc = oracle.OracleScene(sc)
+    yield sc, ctx, orc
+    ctx.close()
+    orc.close()
+
+
+def _assert_hits_equal(g, o):
+    assert np.array_equal(g["prim_id"], o["prim_id"]), f"prim mismatches: {(g['prim_id'] != o['prim_id']).sum()}"
+    assert np.array_equal(g["inst_id"], o["inst_id"])
+    hit = o["prim_id"] >= 0
+    assert np.array_equal(g["t"][hit].view(np.uint32), o["t"][hit].view(np.uint32))
+    assert np.array_equal(g["bary"][hit].view(np.uint32), o["bary"][hit].view(np.uint32))
+    assert np.all(np.isinf(g["t"][~hit]))
+
+
+def test_trace_closest_plain_camera(plain):
+    sc, ctx, orc = plain
+    cam = host_camera(sc, 128, 128)
+    rays = camera_rays(cam, 128, 128)
+    print("case: plain scene, 128x128 camera rays", flush=True)
+    g = ctx.trace_closest(rays)
+    o, _, _ = orc.trace(rays, oracle.MODE_CLOSEST)
+    _assert_hits_equal(g, o)
+    assert (o["prim_id"] >= 0).mean() > 0.3
+
+
+def test_trace_closest_plain_random(plain):
+    sc, ctx, orc = plain
+    rays = random_rays(50_000, (-3, 0.05, -3), (3, 3, 3), (0, 1, 0), 0.8, seed=7)
+    print("case: plain scene, 50k random rays", flush=True)
+    g = ctx.trace_closest(rays)
+    o, _, _ = orc.trace(rays, oracle.MODE_CLOSEST)
+    _assert_hits_equal(g, o)
+
+
+def test_trace_any_plain(plain):
+    sc, ctx, orc = plain
+    rays = random_rays(50_000, (-3, 0.05, -3), (3, 3, 3), (0, 1, 0), 0.8, seed=8)
+    rays["tmax"] = np.random.default_rng(9).uniform(0.1, 6.0, len(rays)).astype(np.float32)
+    print("case: plain scene, 50k any-hit rays with finite tmax", flush=True)
+    g = ctx.trace_any(rays)
+    o, _, _ = orc.trace(rays, oracle.MODE_ANY)
+    assert np.array_equal(g, o)
+    assert 0.05 < o.mean() < 0.95
+
+
+def test_trace_instanced(inst_small):
+    sc, ctx, orc = inst_small
+    rays = random_rays(40_000, (-10, -10, -10), (110, 110, 110), (50, 50, 50), 30.0, seed=11)
+    print("case: small instanced scene, 40k random rays", flush=True)
+    g = ctx.trace_closest(rays)
+    o, _, _ = orc.trace(rays, oracle.MODE_CLOSEST)
+    _assert_hits_equal(g, o)
+    ga = ctx.trace_any(rays)
+    oa, _, _ = orc.trace(rays, oracle.MODE_ANY)
+    assert np.array_equal(ga, oa)
+
+
+def test_trace_counts_bounded_by_reference(plain):
+    """The GPU traversal does no more work than the reference's (it enters
+    each BLAS with the closest hit so far as tmax)."""
+    sc, ctx, orc = plain
+    cam = host_camera(sc, 64, 64)
+    rays = camera_rays(cam, 64, 64)
+    g, gc = ctx.trace_closest(rays, counted=True)
+    o, oc, _ = orc.trace(rays, oracle.MODE_CLOSEST)
+    _assert_hits_equal(g, o)
+    for k in ("blas_interior_pops", "blas_leaf_pops", "tri_tests", "blas_entries", "tlas_pops"):
+        assert 0 < gc[k] <= oc[k], (k, gc[k], oc[k])
+
+
+def _render_both(sc, ctx, orc, W, H, spp, integrator, max_depth=4, seed_mode=abi.SEED_XOR):
+    st = settings(max_depth=max_depth, seed_mode=seed_mode, integrator=integrator,
+                  sun_theta=sc.sun_theta)
+    ctx.set_settings(st)
+    ctx.resize(W, H)
+    p, v = view_proj(sc, W, H)
+    ctx.set_view(p, v)
+    cam = ctx.camera()
+    ctx.render(spp)
+    img = ctx.readback(0)[..., :3].reshape(-1, 3)
+    gbuf = ctx.readback(2).reshape(-1, 4)
+    vbuf = ctx.readback(3).reshape(-1, 4)
+    if spp > 1:
+        cam.prev_inv_view[:] = cam.inv_view[:]
+        cam.prev_inv_proj[:] = cam.inv_proj[:]
+    pix = np.arange(W * H, dtype=np.int32)
+    orgb, og, ov = orc.render_pixels(st, cam, W, H, pix, 0, spp, want_buffers=True)
+    return img, gbuf, vbuf, orgb, og, ov
+
+
+def _rel_err(a, b):
+    return np.abs(a.astype(np.float64) - b) / np.maximum(np.abs(b.astype(np.float64)), 1e-6)
+
+
+@pytest.mark.parametrize("integrator", [abi.INTEGRATOR_MEGAKERNEL, abi.INTEGRATOR_WAVEFRONT])
+def test_render_plain_parity(plain, integrator):
+    sc, ctx, orc = plain
+    print(f"case: plain scene render 64x64 x 2 spp, integrator {integrator}", flush=True)
+    img, gbuf, vbuf, orgb, og, ov = _render_both(sc, ctx, orc, 64, 64, 2, integrator)
+    err = _rel_err(img, orgb)
+    diff = int((img.view(np.uint32) != orgb.view(np.uint32)).any(axis=1).sum())
+    print(f"mean rel err {err.mean():.3e}, max {err.max():.3e}, bit-different pixels {diff}/{len(img)}")
+    assert err.mean() <= 1e-4
+    assert np.array_equal(gbuf[:, 2:4], og[:, 2:4])                       # albedo bytes + depth bits
+    finite = np.isfinite(gbuf[:, :2].view(np.float32)).all(axis=1)
+    assert np.array_equal(gbuf[finite, :2], og[finite, :2])               # octahedral normal bits
+    assert np.array_equal(vbuf[:, 2:], ov[:, 2:])                         # primitive / instance ids
+    assert np.allclose(vbuf[:, :2], ov[:, :2], atol=0, rtol=0, equal_nan=True)
+
+
+def test_render_instanced_parity(inst_small):
+    sc, ctx, orc = inst_small
+    print("case: instanced scene render 48x32 x 1 spp wavefront, depth 8", flush=True)
+    img, gbuf, vbuf, orgb, og, ov = _render_both(sc, ctx, orc, 48, 32, 1, abi.INTEGRATOR_WAVEFRONT, max_depth=8)
+    err = _rel_err(img, orgb)
+    print(f"mean rel err {err.mean():.3e}, bit-different pixels "
+          f"{int((img.view(np.uint32) != orgb.view(np.uint32)).any(axis=1).sum())}")
+    assert err.mean() <= 1e-4
+
+
+def test_wavefront_equals_megakernel(plain):
+    sc, ctx, orc = plain
+    W, H = 96, 64
+    out = []
+    for integ in (abi.INTEGRATOR_MEGAKERNEL, abi.INTEGRATOR_WAVEFRONT):
+        ctx.set_settings(settings(max_depth=4, integrator=integ, sun_theta=sc.sun_theta))
+        ctx.resize(W, H)
+        p, v = view_proj(sc, W, H)
+        ctx.set_view(p, v)
+        ctx.render(3)
+        out.append((ctx.readback(0), ctx.readback(2)))
+    assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_reference_seed_mode(plain):
+    sc, ctx, orc = plain
+    img, gbuf, vbuf, orgb, og, ov = _render_both(sc, ctx, orc, 32, 32, 3, abi.INTEGRATOR_WAVEFRONT,
+                                                 seed_mode=abi.SEED_REFERENCE)
+    assert _rel_err(img, orgb).mean() <= 1e-4
