@@ -95,7 +95,17 @@ def main():
     ap.add_argument("--instances", type=int, default=1000)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="only run one device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        from kittlespt_amd.context import DeviceBuffer, hip
+        a, b = DeviceBuffer(args.calibrate_copy), DeviceBuffer(args.calibrate_copy)
+        rc = hip().hipMemcpy(b.ptr, a.ptr, args.calibrate_copy, 3)   # device to device
+        hip().hipDeviceSynchronize()
+        print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy, "rc": rc}), flush=True)
+        return
 
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build measures one GPU "

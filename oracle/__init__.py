@@ -157,3 +157,69 @@ def mat4_inverse(m16):
     out = np.zeros(16, np.float32)
     lib().orc_mat4_inverse(_ptr(m), _ptr(out))
     return out
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def bsdf_eval(params, wo, wi, double=False):
+    p, wo, wi = _f32(params), _f32(wo), _f32(wi)
+    n = len(wo)
+    f = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    L = lib()
+    L.orc_bsdf_eval.argtypes = [C.c_void_p] * 3 + [C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]
+    L.orc_bsdf_eval(_ptr(p), _ptr(wo), _ptr(wi), n, 1 if double else 0, _ptr(f), _ptr(pdf))
+    return f, pdf
+
+
+def bsdf_sample(params, wo, u, double=False):
+    p, wo, u = _f32(params), _f32(wo), _f32(u)
+    n = len(wo)
+    wi = np.zeros((n, 3), np.float32)
+    f = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    fl = np.zeros(n, np.int32)
+    L = lib()
+    L.orc_bsdf_sample.argtypes = [C.c_void_p] * 3 + [C.c_int64, C.c_int32] + [C.c_void_p] * 4
+    L.orc_bsdf_sample(_ptr(p), _ptr(wo), _ptr(u), n, 1 if double else 0, _ptr(wi), _ptr(f), _ptr(pdf), _ptr(fl))
+    return wi, f, pdf, fl
+
+
+def ggx_d(alpha, wm, double=False):
+    wm = _f32(wm)
+    out = np.zeros(len(wm), np.float32)
+    L = lib()
+    L.orc_ggx_d.argtypes = [C.c_float, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]
+    L.orc_ggx_d(alpha, _ptr(wm), len(wm), 1 if double else 0, _ptr(out))
+    return out
+
+
+def fresnel_dielectric(cos_t, ior, double=False):
+    c, i = _f32(cos_t), _f32(ior)
+    out = np.zeros(len(c), np.float32)
+    L = lib()
+    L.orc_fresnel_dielectric.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]
+    L.orc_fresnel_dielectric(_ptr(c), _ptr(i), len(c), 1 if double else 0, _ptr(out))
+    return out
+
+
+def sky_li(o, d, sun, emission=1.0, double=False):
+    o, d, s = _f32(o), _f32(d), _f32(sun)
+    out = np.zeros((len(o), 3), np.float32)
+    L = lib()
+    L.orc_sky_li.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_float, C.c_int32, C.c_void_p]
+    L.orc_sky_li(_ptr(o), _ptr(d), len(o), _ptr(s), emission, 1 if double else 0, _ptr(out))
+    return out
+
+
+def camera_rays(cam, width, height, pixels):
+    px = np.ascontiguousarray(pixels, np.int32)
+    o = np.zeros((len(px), 3), np.float32)
+    d = np.zeros((len(px), 3), np.float32)
+    L = lib()
+    L.orc_camera_rays.argtypes = [C.POINTER(abi.KptCamera), C.c_int32, C.c_int32, C.c_void_p, C.c_int64,
+                                  C.c_void_p, C.c_void_p]
+    L.orc_camera_rays(C.byref(cam), width, height, _ptr(px), len(px), _ptr(o), _ptr(d))
+    return o, d

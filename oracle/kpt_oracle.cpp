@@ -1788,6 +1788,98 @@ void orc_dmath(int32_t fn, const float* x, const float* y, float* out, int64_t n
 
 uint32_t orc_pcg_hash(uint32_t v) { return PcgHash(v); }
 
+}  // extern "C"
+
+namespace orc {
+template <class R> BSDF<R> ProbeBSDF(const float* p) {
+    BSDF<R> b;
+    b.basis = M3<R>();
+    b.albedo = V3<R>(R(p[0]), R(p[1]), R(p[2]));
+    b.metallic = R(p[3]);
+    b.roughness = R(p[4]);
+    b.transmission = R(p[5]);
+    b.ior = R(p[6]);
+    b.backface = p[7] != 0.0f;
+    b.ggx.alpha = Sqr(b.roughness);
+    return b;
+}
+template <class R> void BsdfEval(const float* params, const float* wo, const float* wi, int64_t n, float* of, float* op) {
+    BSDF<R> b = ProbeBSDF<R>(params);
+    for (int64_t i = 0; i < n; ++i) {
+        V3<R> o(R(wo[3 * i]), R(wo[3 * i + 1]), R(wo[3 * i + 2])), w(R(wi[3 * i]), R(wi[3 * i + 1]), R(wi[3 * i + 2]));
+        V3<R> f = b.f(o, w);
+        of[3 * i] = (float)f.x; of[3 * i + 1] = (float)f.y; of[3 * i + 2] = (float)f.z;
+        op[i] = (float)b.pdf(o, w);
+    }
+}
+template <class R> void BsdfSample(const float* params, const float* wo, const float* u, int64_t n, float* owi, float* of,
+                                   float* op, int32_t* ofl) {
+    BSDF<R> b = ProbeBSDF<R>(params);
+    for (int64_t i = 0; i < n; ++i) {
+        V3<R> o(R(wo[3 * i]), R(wo[3 * i + 1]), R(wo[3 * i + 2]));
+        V2<R> u2, x2;
+        u2.x = R(u[4 * i]); u2.y = R(u[4 * i + 1]); x2.x = R(u[4 * i + 2]); x2.y = R(u[4 * i + 3]);
+        BSDFSample<R> s = b.sampleF(o, u2, x2);
+        owi[3 * i] = (float)s.wi.x; owi[3 * i + 1] = (float)s.wi.y; owi[3 * i + 2] = (float)s.wi.z;
+        of[3 * i] = (float)s.f.x; of[3 * i + 1] = (float)s.f.y; of[3 * i + 2] = (float)s.f.z;
+        op[i] = (float)s.pdf;
+        ofl[i] = s.scatter;
+    }
+}
+}  // namespace orc
+
+extern "C" {
+
+void orc_bsdf_eval(const float* params, const float* wo, const float* wi, int64_t n, int32_t dp, float* out_f,
+                   float* out_pdf) {
+    if (dp) BsdfEval<double>(params, wo, wi, n, out_f, out_pdf);
+    else BsdfEval<float>(params, wo, wi, n, out_f, out_pdf);
+}
+
+void orc_bsdf_sample(const float* params, const float* wo, const float* u, int64_t n, int32_t dp, float* out_wi,
+                     float* out_f, float* out_pdf, int32_t* out_flags) {
+    if (dp) BsdfSample<double>(params, wo, u, n, out_wi, out_f, out_pdf, out_flags);
+    else BsdfSample<float>(params, wo, u, n, out_wi, out_f, out_pdf, out_flags);
+}
+
+void orc_ggx_d(float alpha, const float* wm, int64_t n, int32_t dp, float* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        if (dp) { GGX<double> g; g.alpha = alpha; out[i] = (float)g.D(V3<double>(wm[3 * i], wm[3 * i + 1], wm[3 * i + 2])); }
+        else { GGX<float> g; g.alpha = alpha; out[i] = g.D(V3<float>(wm[3 * i], wm[3 * i + 1], wm[3 * i + 2])); }
+    }
+}
+
+void orc_fresnel_dielectric(const float* c, const float* ior, int64_t n, int32_t dp, float* out) {
+    for (int64_t i = 0; i < n; ++i)
+        out[i] = dp ? (float)FresnelDielectric<double>(c[i], ior[i]) : FresnelDielectric<float>(c[i], ior[i]);
+}
+
+void orc_sky_li(const float* o, const float* d, int64_t n, const float sun[3], float emission, int32_t dp, float* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        if (dp) {
+            Ray<double> r(V3<double>(o[3 * i], o[3 * i + 1], o[3 * i + 2]), V3<double>(d[3 * i], d[3 * i + 1], d[3 * i + 2]));
+            V3<double> L = SkyLi(r, V3<double>(sun[0], sun[1], sun[2]), (double)emission);
+            out[3 * i] = (float)L.x; out[3 * i + 1] = (float)L.y; out[3 * i + 2] = (float)L.z;
+        } else {
+            Ray<float> r(V3<float>(o[3 * i], o[3 * i + 1], o[3 * i + 2]), V3<float>(d[3 * i], d[3 * i + 1], d[3 * i + 2]));
+            V3<float> L = SkyLi(r, V3<float>(sun[0], sun[1], sun[2]), emission);
+            out[3 * i] = L.x; out[3 * i + 1] = L.y; out[3 * i + 2] = L.z;
+        }
+    }
+}
+
+void orc_camera_rays(const kpt_camera* cam, int32_t W, int32_t H, const int32_t* pixels, int64_t n, float* org,
+                     float* dir) {
+    kpt_settings st;
+    std::memset(&st, 0, sizeof(st));
+    FrameConst fc = MakeFrame(nullptr, &st, cam);
+    for (int64_t i = 0; i < n; ++i) {
+        Ray<float> r = CameraRay<float>(fc, pixels[i] % W, pixels[i] / W, W, H);
+        org[3 * i] = r.o.x; org[3 * i + 1] = r.o.y; org[3 * i + 2] = r.o.z;
+        dir[3 * i] = r.d.x; dir[3 * i + 1] = r.d.y; dir[3 * i + 2] = r.d.z;
+    }
+}
+
 void orc_glm_inverse(const float m[16], float out[16]) { StoreM4(GlmInverse(LoadM4(m)), out); }
 void orc_mat4_inverse(const float m[16], float out[16]) { StoreM4(Inverse(LoadM4(m)), out); }
 

@@ -51,6 +51,26 @@ void orc_sample_pixels(const orc_scene* s, const kpt_settings* settings, const k
                        int32_t width, int32_t height, const int32_t* pixels, int64_t n,
                        int32_t frame, float* out_rgb, int32_t threads);
 
+/* BSDF probes (UnifiedBSDF, bsdf.cuh:166-624) in the shading frame
+ * (basis = identity, normal +z).  params: [albedo r g b, metallic, roughness,
+ * transmission, ior, backface].  dp selects the double instantiation.
+ * f: out_f[3*i]; pdf: out_pdf[i]. */
+void orc_bsdf_eval(const float* params, const float* wo, const float* wi, int64_t n, int32_t dp,
+                   float* out_f, float* out_pdf);
+/* sampleF with uniform numbers u[4*i..] = (u.x, u.y, X.x, X.y); outputs wi,
+ * f, pdf and the scatter flags. */
+void orc_bsdf_sample(const float* params, const float* wo, const float* u, int64_t n, int32_t dp,
+                     float* out_wi, float* out_f, float* out_pdf, int32_t* out_flags);
+/* GGX::D for a given alpha and microfacet normal; fresnelDielectric. */
+void orc_ggx_d(float alpha, const float* wm, int64_t n, int32_t dp, float* out);
+void orc_fresnel_dielectric(const float* cos_theta, const float* ior, int64_t n, int32_t dp, float* out);
+/* NishitaAtmosphereModel::sampleLi for rays (o, d) with sun direction sun. */
+void orc_sky_li(const float* o, const float* d, int64_t n, const float sun[3], float emission, int32_t dp,
+                float* out_rgb);
+/* Camera::generateRay for pixels (x, y). */
+void orc_camera_rays(const kpt_camera* cam, int32_t width, int32_t height, const int32_t* pixels, int64_t n,
+                     float* out_org, float* out_dir);
+
 /* Deterministic math exported for tests. fn: 0 exp 1 exp2 2 log2 3 pow 4 sin
  * 5 cos 6 tan 7 acos 8 log */
 void orc_dmath(int32_t fn, const float* x, const float* y, float* out, int64_t n);
