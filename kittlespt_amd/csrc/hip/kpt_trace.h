@@ -6,8 +6,12 @@
 //   * one loop and one stack for both levels; a stack entry carries a level
 //     bit, so the BLAS of a TLAS leaf is traversed to completion before the
 //     next TLAS entry is popped (LIFO), exactly as the nested reference calls;
+//   * the nearer child is kept in a register ("current entry") instead of
+//     being pushed and popped at once, and only the farther sibling goes on
+//     the stack; the visiting order is the reference's (it pushes the farther
+//     child first and pops the nearer one next);
 //   * the per-lane stack lives in LDS ([entry][lane] layout: lanes of a wave
-//     hit 64 distinct banks) with a private-memory (scratch) spill region for
+//     hit distinct banks) with a private-memory (scratch) spill region for
 //     entries beyond the LDS depth, instead of the reference's fixed 16-entry
 //     register array that overflows on deep trees;
 //   * the BLAS is entered with tmax = min(tmax, closest-so-far).  The reference
@@ -19,6 +23,8 @@
 //     same;
 //   * any-hit needs no ordering: its result is "some reachable triangle is
 //     hit", which does not depend on the visiting order.
+// Traversal is a state machine (init / step) so persistent kernels can refill
+// a lane with a new ray as soon as its ray finishes.
 #pragma once
 #include "kpt_vec.h"
 #include "../common/kpt_device_types.h"
@@ -75,7 +81,7 @@ KD Ray object_ray(const InstTrav& it, const Ray& w) {
     return make_ray(o, d);
 }
 
-// Triangle::intersect (triangle.cuh:66-102).  Returns t (> tmin, < tmax) or -1.
+// Triangle::intersect (triangle.cuh:66-102) on a slot's precomputed edges.
 KD bool tri_test(const TriPos& tp, const Ray& r, float tmin, float tmax, float* t_out, float* u_out, float* v_out) {
     const float EPS = 1.0e-5f;
     v3 v0 = mk3(tp.v0[0], tp.v0[1], tp.v0[2]);
@@ -117,152 +123,206 @@ struct LaneStack {
     }
 };
 
-// Closest hit.  COUNT adds traversal event counts.
+// Closest-hit traversal state of one lane.
+struct ClosestTrav {
+    Ray w, o;
+    float tmin, tmax;
+    Hit h;
+    int32_t cur_inst;
+    uint32_t cur;
+    LaneStack st;
+
+    // TLAS::intersect entry: empty TLAS or a miss of the root box ends the
+    // query at once (tlas.cu:62-69).  Returns false when nothing remains.
+    KD bool init(const SceneView& S, const Ray& ray, float t0, float t1, uint32_t* lds) {
+        w = ray; o = ray; tmin = t0; tmax = t1;
+        h.t = __builtin_inff(); h.slot = -1; h.inst = -1; h.u = 0.0f; h.v = 0.0f;
+        cur_inst = -1;
+        st.lds = lds; st.sp = 0; st.overflow = false;
+        if (S.tlas_root < 0) return false;
+        float4 rlo, rhi;
+        load_node(S.tlas_nodes, 0, &rlo, &rhi);
+        float te;
+        if (!slab(rlo, rhi, w, tmin, tmax, &te)) return false;
+        cur = TLAS_BIT | 0u;
+        return true;
+    }
+
+    // Continues with the stack top; false when the stack is empty.
+    KD bool next() {
+        if (st.sp == 0 || st.overflow) return false;
+        cur = st.pop();
+        return true;
+    }
+
+    // Processes the current entry; returns false when the query is finished.
+    template <bool COUNT>
+    KD bool step(const SceneView& S, TraceCounts* cnt) {
+        const uint32_t e = cur;
+        if (e & TLAS_BIT) {
+            const uint32_t ni = e & ~TLAS_BIT;
+            float4 lo, hi;
+            load_node(S.tlas_nodes, ni, &lo, &hi);
+            const uint32_t a = __builtin_bit_cast(uint32_t, lo.w), b = __builtin_bit_cast(uint32_t, hi.w);
+            if (COUNT) cnt->tlas++;
+            if (a != 0u) {
+                const uint32_t i1 = a & 0xFFFFu, i2 = (a >> 16) & 0xFFFFu;
+                float4 l1, h1, l2, h2;
+                load_node(S.tlas_nodes, i1, &l1, &h1);
+                load_node(S.tlas_nodes, i2, &l2, &h2);
+                float c1, c2;
+                const bool k1 = slab(l1, h1, w, tmin, tmax, &c1) && c1 < h.t;
+                const bool k2 = slab(l2, h2, w, tmin, tmax, &c2) && c2 < h.t;
+                if (k1 && k2) {
+                    if (c1 > c2) { st.push(TLAS_BIT | i1); cur = TLAS_BIT | i2; }
+                    else { st.push(TLAS_BIT | i2); cur = TLAS_BIT | i1; }
+                    return !st.overflow;
+                }
+                if (k1) { cur = TLAS_BIT | i1; return true; }
+                if (k2) { cur = TLAS_BIT | i2; return true; }
+                return next();
+            }
+            if (COUNT) cnt->entries++;
+            cur_inst = (int32_t)b;
+            const InstTrav it = S.inst_trav[cur_inst];
+            o = object_ray(it, w);
+            if (it.root >= 0) { cur = (uint32_t)it.root; return true; }
+            return next();
+        }
+        float4 lo, hi;
+        load_node(S.blas_nodes, e, &lo, &hi);
+        const uint32_t a = __builtin_bit_cast(uint32_t, lo.w), b = __builtin_bit_cast(uint32_t, hi.w);
+        const float bt = fminf_(tmax, h.t);
+        if (b == 0u) {
+            if (COUNT) cnt->interior++;
+            float4 l1, h1, l2, h2;
+            load_node(S.blas_nodes, a, &l1, &h1);
+            load_node(S.blas_nodes, a + 1, &l2, &h2);
+            float c1, c2;
+            const bool k1 = slab(l1, h1, o, tmin, bt, &c1) && c1 < h.t;
+            const bool k2 = slab(l2, h2, o, tmin, bt, &c2) && c2 < h.t;
+            if (k1 && k2) {
+                if (c1 > c2) { st.push(a); cur = a + 1; }
+                else { st.push(a + 1); cur = a; }
+                return !st.overflow;
+            }
+            if (k1) { cur = a; return true; }
+            if (k2) { cur = a + 1; return true; }
+            return next();
+        }
+        if (COUNT) cnt->leaf++;
+        for (uint32_t k = 0; k < b; ++k) {
+            if (COUNT) cnt->tri++;
+            const TriPos tp = S.tris[a + k];
+            float t, u, v;
+            if (tri_test(tp, o, tmin, fminf_(tmax, h.t), &t, &u, &v) && t < h.t) {
+                h.t = t; h.slot = (int32_t)(a + k); h.inst = cur_inst; h.u = u; h.v = v;
+            }
+        }
+        return next();
+    }
+};
+
+// Any-hit traversal state of one lane.
+struct AnyTrav {
+    Ray w, o;
+    float tmin, tmax;
+    bool hit;
+    uint32_t cur;
+    LaneStack st;
+
+    KD bool init(const SceneView& S, const Ray& ray, float t0, float t1, uint32_t* lds) {
+        w = ray; o = ray; tmin = t0; tmax = t1; hit = false;
+        st.lds = lds; st.sp = 0; st.overflow = false;
+        if (S.tlas_root < 0) return false;
+        float4 rlo, rhi;
+        load_node(S.tlas_nodes, 0, &rlo, &rhi);
+        float te;
+        if (!slab(rlo, rhi, w, tmin, tmax, &te)) return false;
+        cur = TLAS_BIT | 0u;
+        return true;
+    }
+    KD bool next() {
+        if (st.sp == 0 || st.overflow) return false;
+        cur = st.pop();
+        return true;
+    }
+    template <bool COUNT>
+    KD bool step(const SceneView& S, TraceCounts* cnt) {
+        const uint32_t e = cur;
+        if (e & TLAS_BIT) {
+            const uint32_t ni = e & ~TLAS_BIT;
+            float4 lo, hi;
+            load_node(S.tlas_nodes, ni, &lo, &hi);
+            const uint32_t a = __builtin_bit_cast(uint32_t, lo.w), b = __builtin_bit_cast(uint32_t, hi.w);
+            if (COUNT) cnt->tlas++;
+            if (a != 0u) {
+                const uint32_t i1 = a & 0xFFFFu, i2 = (a >> 16) & 0xFFFFu;
+                float4 l1, h1, l2, h2;
+                load_node(S.tlas_nodes, i1, &l1, &h1);
+                load_node(S.tlas_nodes, i2, &l2, &h2);
+                float c1, c2;
+                const bool k1 = slab(l1, h1, w, tmin, tmax, &c1);
+                const bool k2 = slab(l2, h2, w, tmin, tmax, &c2);
+                if (k1 && k2) { st.push(TLAS_BIT | i1); cur = TLAS_BIT | i2; return !st.overflow; }
+                if (k1) { cur = TLAS_BIT | i1; return true; }
+                if (k2) { cur = TLAS_BIT | i2; return true; }
+                return next();
+            }
+            if (COUNT) cnt->entries++;
+            const InstTrav it = S.inst_trav[b];
+            o = object_ray(it, w);
+            if (it.root >= 0) { cur = (uint32_t)it.root; return true; }
+            return next();
+        }
+        float4 lo, hi;
+        load_node(S.blas_nodes, e, &lo, &hi);
+        const uint32_t a = __builtin_bit_cast(uint32_t, lo.w), b = __builtin_bit_cast(uint32_t, hi.w);
+        if (b == 0u) {
+            if (COUNT) cnt->interior++;
+            float4 l1, h1, l2, h2;
+            load_node(S.blas_nodes, a, &l1, &h1);
+            load_node(S.blas_nodes, a + 1, &l2, &h2);
+            float c1, c2;
+            const bool k1 = slab(l1, h1, o, tmin, tmax, &c1);
+            const bool k2 = slab(l2, h2, o, tmin, tmax, &c2);
+            if (k1 && k2) { st.push(a); cur = a + 1; return !st.overflow; }
+            if (k1) { cur = a; return true; }
+            if (k2) { cur = a + 1; return true; }
+            return next();
+        }
+        if (COUNT) cnt->leaf++;
+        for (uint32_t k = 0; k < b; ++k) {
+            if (COUNT) cnt->tri++;
+            const TriPos tp = S.tris[a + k];
+            float t, u, v;
+            if (tri_test(tp, o, tmin, tmax, &t, &u, &v)) { hit = true; return false; }
+        }
+        return next();
+    }
+};
+
+// Whole-query forms (megakernel and trace entry points).
 template <bool COUNT>
 KD Hit trace_closest(const SceneView& S, const Ray& w, float tmin, float tmax, uint32_t* lds_lane,
                      bool* overflow, TraceCounts* cnt) {
-    Hit h;
-    h.t = __builtin_inff(); h.slot = -1; h.inst = -1; h.u = 0.0f; h.v = 0.0f;
-    if (S.tlas_root < 0) return h;
-    float4 rlo, rhi;
-    load_node(S.tlas_nodes, 0, &rlo, &rhi);
-    float te;
-    if (!slab(rlo, rhi, w, tmin, tmax, &te)) return h;
-    LaneStack st;
-    st.lds = lds_lane; st.sp = 0; st.overflow = false;
-    st.push(TLAS_BIT | 0u);
-    Ray o = w;
-    int32_t cur_inst = -1;
-    while (st.sp > 0) {
-        uint32_t e = st.pop();
-        if (e & TLAS_BIT) {
-            uint32_t ni = e & ~TLAS_BIT;
-            const Node nd = S.tlas_nodes[ni];
-            if (COUNT) cnt->tlas++;
-            if (nd.a != 0u) {
-                uint32_t i1 = nd.a & 0xFFFFu, i2 = (nd.a >> 16) & 0xFFFFu;
-                float4 l1, h1, l2, h2;
-                load_node(S.tlas_nodes, i1, &l1, &h1);
-                load_node(S.tlas_nodes, i2, &l2, &h2);
-                float c1, c2;
-                bool k1 = slab(l1, h1, w, tmin, tmax, &c1);
-                bool k2 = slab(l2, h2, w, tmin, tmax, &c2);
-                k1 = k1 && c1 < h.t;
-                k2 = k2 && c2 < h.t;
-                if (k1 && k2) {
-                    if (c1 > c2) { st.push(TLAS_BIT | i1); st.push(TLAS_BIT | i2); }
-                    else { st.push(TLAS_BIT | i2); st.push(TLAS_BIT | i1); }
-                } else if (k1) {
-                    st.push(TLAS_BIT | i1);
-                } else if (k2) {
-                    st.push(TLAS_BIT | i2);
-                }
-            } else {
-                if (COUNT) cnt->entries++;
-                cur_inst = (int32_t)nd.b;
-                const InstTrav it = S.inst_trav[cur_inst];
-                o = object_ray(it, w);
-                if (it.root >= 0) st.push((uint32_t)it.root);
-            }
-        } else {
-            const float bt = fminf_(tmax, h.t);
-            float4 lo, hi;
-            load_node(S.blas_nodes, e, &lo, &hi);
-            uint32_t a = __builtin_bit_cast(uint32_t, lo.w), b = __builtin_bit_cast(uint32_t, hi.w);
-            if (b == 0u) {
-                if (COUNT) cnt->interior++;
-                float4 l1, h1, l2, h2;
-                load_node(S.blas_nodes, a, &l1, &h1);
-                load_node(S.blas_nodes, a + 1, &l2, &h2);
-                float c1, c2;
-                bool k1 = slab(l1, h1, o, tmin, bt, &c1) && c1 < h.t;
-                bool k2 = slab(l2, h2, o, tmin, bt, &c2) && c2 < h.t;
-                if (k1 && k2) {
-                    if (c1 > c2) { st.push(a); st.push(a + 1); }
-                    else { st.push(a + 1); st.push(a); }
-                } else if (k1) {
-                    st.push(a);
-                } else if (k2) {
-                    st.push(a + 1);
-                }
-            } else {
-                if (COUNT) cnt->leaf++;
-                for (uint32_t k = 0; k < b; ++k) {
-                    if (COUNT) cnt->tri++;
-                    const TriPos tp = S.tris[a + k];
-                    float t, u, v;
-                    if (tri_test(tp, o, tmin, fminf_(tmax, h.t), &t, &u, &v) && t < h.t) {
-                        h.t = t; h.slot = (int32_t)(a + k); h.inst = cur_inst; h.u = u; h.v = v;
-                    }
-                }
-            }
-        }
-        if (st.overflow) break;
+    ClosestTrav tr;
+    if (tr.init(S, w, tmin, tmax, lds_lane)) {
+        while (tr.template step<COUNT>(S, cnt)) {}
     }
-    *overflow = st.overflow;
-    return h;
+    *overflow = tr.st.overflow;
+    return tr.h;
 }
 
-// Any hit in (tmin, tmax).
 template <bool COUNT>
 KD bool trace_any(const SceneView& S, const Ray& w, float tmin, float tmax, uint32_t* lds_lane, bool* overflow,
                   TraceCounts* cnt) {
-    if (S.tlas_root < 0) return false;
-    float4 rlo, rhi;
-    load_node(S.tlas_nodes, 0, &rlo, &rhi);
-    float te;
-    if (!slab(rlo, rhi, w, tmin, tmax, &te)) return false;
-    LaneStack st;
-    st.lds = lds_lane; st.sp = 0; st.overflow = false;
-    st.push(TLAS_BIT | 0u);
-    Ray o = w;
-    bool hit = false;
-    while (st.sp > 0 && !hit) {
-        uint32_t e = st.pop();
-        if (e & TLAS_BIT) {
-            uint32_t ni = e & ~TLAS_BIT;
-            const Node nd = S.tlas_nodes[ni];
-            if (COUNT) cnt->tlas++;
-            if (nd.a != 0u) {
-                uint32_t i1 = nd.a & 0xFFFFu, i2 = (nd.a >> 16) & 0xFFFFu;
-                float4 l1, h1, l2, h2;
-                load_node(S.tlas_nodes, i1, &l1, &h1);
-                load_node(S.tlas_nodes, i2, &l2, &h2);
-                float c1, c2;
-                if (slab(l1, h1, w, tmin, tmax, &c1)) st.push(TLAS_BIT | i1);
-                if (slab(l2, h2, w, tmin, tmax, &c2)) st.push(TLAS_BIT | i2);
-            } else {
-                if (COUNT) cnt->entries++;
-                const InstTrav it = S.inst_trav[nd.b];
-                o = object_ray(it, w);
-                if (it.root >= 0) st.push((uint32_t)it.root);
-            }
-        } else {
-            float4 lo, hi;
-            load_node(S.blas_nodes, e, &lo, &hi);
-            uint32_t a = __builtin_bit_cast(uint32_t, lo.w), b = __builtin_bit_cast(uint32_t, hi.w);
-            if (b == 0u) {
-                if (COUNT) cnt->interior++;
-                float4 l1, h1, l2, h2;
-                load_node(S.blas_nodes, a, &l1, &h1);
-                load_node(S.blas_nodes, a + 1, &l2, &h2);
-                float c1, c2;
-                if (slab(l1, h1, o, tmin, tmax, &c1)) st.push(a);
-                if (slab(l2, h2, o, tmin, tmax, &c2)) st.push(a + 1);
-            } else {
-                if (COUNT) cnt->leaf++;
-                for (uint32_t k = 0; k < b; ++k) {
-                    if (COUNT) cnt->tri++;
-                    const TriPos tp = S.tris[a + k];
-                    float t, u, v;
-                    if (tri_test(tp, o, tmin, tmax, &t, &u, &v)) { hit = true; break; }
-                }
-            }
-        }
-        if (st.overflow) break;
+    AnyTrav tr;
+    if (tr.init(S, w, tmin, tmax, lds_lane)) {
+        while (tr.template step<COUNT>(S, cnt)) {}
     }
-    *overflow = st.overflow;
-    return hit;
+    *overflow = tr.st.overflow;
+    return tr.hit;
 }
 
 // XCD-aware workgroup swizzle: the dispatcher hands consecutive workgroup ids

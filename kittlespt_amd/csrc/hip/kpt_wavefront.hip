@@ -28,14 +28,44 @@ constexpr int MAX_BOUNCE_SLOTS = 40;
 KD uint32_t fbits(float f) { return __builtin_bit_cast(uint32_t, f); }
 KD float bitsf(uint32_t u) { return __builtin_bit_cast(float, u); }
 
-// Grabs the next chunk of `count` for this workgroup; uniform across the block.
-__device__ __forceinline__ int pull_chunk(int32_t* ctr, int* s_base) {
-    if (threadIdx.x == 0) *s_base = atomicAdd(ctr, WF_BLOCK);
-    __syncthreads();
-    int b = *s_base;
-    __syncthreads();
-    return b;
+constexpr int WAVE = 64;
+constexpr int TRACE_CHUNK = 128;   // queue entries a wave claims per atomic
+constexpr int REFILL_MIN = 16;     // idle lanes that trigger a refill
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
+__device__ __forceinline__ uint64_t lanes_below() { return (1ull << lane_id()) - 1ull; }
+
+// One atomic per wave: claims `n` consecutive queue entries (wave-uniform result).
+__device__ __forceinline__ int wave_claim(int32_t* ctr, int n) {
+    int base = 0;
+    if (lane_id() == 0) base = atomicAdd(ctr, n);
+    return __shfl(base, 0, WAVE);
 }
+
+// Wave-local work source over a global queue of `count` entries: a wave claims
+// TRACE_CHUNK entries at a time and hands them to its idle lanes, so a lane
+// whose ray finishes starts another without waiting for the rest of the wave.
+struct WaveFeeder {
+    int pos, end;        // wave-uniform claimed range [pos, end)
+    bool drained;
+    __device__ void init() { pos = 0; end = 0; drained = false; }
+    // Returns the queue position for this lane (when `idle`) or -1.
+    __device__ int feed(int32_t* ctr, int count, bool idle) {
+        const uint64_t m = __ballot(idle);
+        const int n = __popcll(m);
+        if (n == 0 || (n < REFILL_MIN && n < WAVE)) return -1;
+        if (pos >= end && !drained) {
+            int b = wave_claim(ctr, TRACE_CHUNK);
+            if (b >= count) drained = true;
+            else { pos = b; end = min(b + TRACE_CHUNK, count); }
+        }
+        if (pos >= end) return -1;
+        const int k = pos + __popcll(m & lanes_below());
+        pos = min(pos + n, end);
+        return (idle && k < end) ? k : -1;
+    }
+    __device__ bool exhausted() const { return drained && pos >= end; }
+};
 
 __global__ void k_fill_i32(int32_t* p, int32_t v, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -66,24 +96,40 @@ template <bool COUNT>
 __global__ __launch_bounds__(WF_BLOCK) void k_trace(SceneView S, WavefrontState W, int parity, int pull_slot,
                                                     int32_t* overflow, unsigned long long* counters) {
     __shared__ uint32_t s_stack[LDS_STACK][TRACE_BLOCK];
-    __shared__ int s_base;
     const int count = W.counters[C_QCOUNT + parity];
     const int32_t* q = W.queue[parity];
+    int32_t* ctr = &W.counters[pull_slot];
+    uint32_t* lds = &s_stack[0][threadIdx.x];
     bool ovf = false;
     TraceCounts c = {0, 0, 0, 0, 0};
+    WaveFeeder feeder;
+    feeder.init();
+    ClosestTrav tr;
+    int p = -1;
+    bool busy = false;
     for (;;) {
-        int base = pull_chunk(&W.counters[pull_slot], &s_base);
-        if (base >= count) break;
-        int k = base + (int)threadIdx.x;
-        if (k < count) {
-            int p = q[k];
-            float4 o = W.ray_o[p], d = W.ray_d[p];
-            Ray r = make_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z));
-            bool of = false;
-            Hit h = trace_closest<COUNT>(S, r, 0.0f, __builtin_inff(), &s_stack[0][threadIdx.x], &of, &c);
-            ovf = ovf || of;
-            W.hit[p] = make_float4(h.t, bitsf((uint32_t)h.slot), bitsf((uint32_t)h.inst), h.u);
-            W.hit_v[p] = h.v;
+        const int k = feeder.feed(ctr, count, !busy);
+        if (k >= 0) {
+            p = q[k];
+            const float4 o = W.ray_o[p], d = W.ray_d[p];
+            busy = tr.init(S, make_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z)), 0.0f, __builtin_inff(), lds);
+            if (!busy) {   // empty scene or root box missed
+                W.hit[p] = make_float4(__builtin_inff(), bitsf(0xFFFFFFFFu), bitsf(0xFFFFFFFFu), 0.0f);
+                W.hit_v[p] = 0.0f;
+            }
+        }
+        if (!__any(busy)) {
+            if (feeder.exhausted()) break;
+            continue;
+        }
+        if (busy) {
+            busy = tr.template step<COUNT>(S, &c);
+            if (!busy) {
+                ovf = ovf || tr.st.overflow;
+                const Hit& h = tr.h;
+                W.hit[p] = make_float4(h.t, bitsf((uint32_t)h.slot), bitsf((uint32_t)h.inst), h.u);
+                W.hit_v[p] = h.v;
+            }
         }
     }
     if (ovf) atomicOr(overflow, 1);
@@ -106,88 +152,60 @@ __device__ __forceinline__ int path_key(const SceneView& S, const WavefrontState
     return m % (nkeys - 1);
 }
 
-// Counting sort of the live queue by material key: per-block histograms,
-// one exclusive scan (key-major), stable-within-key scatter.
+// Counting sort of the live queue by material key (misses form their own
+// key), so a shading wave runs one material's code: block-aggregated key
+// totals, an exclusive scan of the (few) keys, then a scatter in which each
+// block reserves its range per key with one atomic.  Order inside a key is
+// free: paths are independent.
+// W.hist layout: [0, MAX_KEYS) totals, [MAX_KEYS, 2*MAX_KEYS) running cursors.
 __global__ __launch_bounds__(WF_BLOCK) void k_sort_hist(SceneView S, WavefrontState W, int parity, int nkeys) {
     __shared__ int h[MAX_KEYS];
     const int count = W.counters[C_QCOUNT + parity];
-    const int base = blockIdx.x * WF_BLOCK;
-    if (base >= count) return;
     for (int k = threadIdx.x; k < nkeys; k += WF_BLOCK) h[k] = 0;
     __syncthreads();
-    int i = base + threadIdx.x;
-    if (i < count) atomicAdd(&h[path_key(S, W, W.queue[parity][i], nkeys)], 1);
+    for (int i = blockIdx.x * WF_BLOCK + threadIdx.x; i < count; i += gridDim.x * WF_BLOCK)
+        atomicAdd(&h[path_key(S, W, W.queue[parity][i], nkeys)], 1);
     __syncthreads();
-    for (int k = threadIdx.x; k < nkeys; k += WF_BLOCK) W.hist[(size_t)k * gridDim.x + blockIdx.x] = h[k];
+    for (int k = threadIdx.x; k < nkeys; k += WF_BLOCK)
+        if (h[k]) atomicAdd(&W.hist[k], h[k]);
 }
 
-__global__ __launch_bounds__(1024) void k_sort_scan(WavefrontState W, int parity, int nkeys, int nblocks) {
-    // exclusive scan over hist[key][block] (key-major), in place
-    __shared__ int s_part[1024];
-    const int count = W.counters[C_QCOUNT + parity];
-    const int used_blocks = (count + WF_BLOCK - 1) / WF_BLOCK;
-    const int n = nkeys * nblocks;
-    const int per = (n + 1023) / 1024;
-    const int t = threadIdx.x;
-    int beg = t * per, end = min(n, beg + per);
-    int sum = 0;
-    for (int j = beg; j < end; ++j) {
-        int b = j % nblocks;
-        int v = (b < used_blocks) ? W.hist[j] : 0;
-        sum += v;
-    }
-    s_part[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int v = (t >= off) ? s_part[t - off] : 0;
-        __syncthreads();
-        s_part[t] += v;
-        __syncthreads();
-    }
-    int run = (t == 0) ? 0 : s_part[t - 1];
-    for (int j = beg; j < end; ++j) {
-        int b = j % nblocks;
-        int v = (b < used_blocks) ? W.hist[j] : 0;
-        W.hist[j] = run;
-        run += v;
+__global__ void k_sort_offsets(WavefrontState W, int nkeys) {
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int k = 0; k < nkeys; ++k) { W.hist[MAX_KEYS + k] = run; run += W.hist[k]; }
     }
 }
 
 __global__ __launch_bounds__(WF_BLOCK) void k_sort_scatter(SceneView S, WavefrontState W, int parity, int nkeys) {
-    __shared__ int keys[WF_BLOCK];
+    __shared__ int h[MAX_KEYS];
+    __shared__ int base[MAX_KEYS];
     const int count = W.counters[C_QCOUNT + parity];
-    const int base = blockIdx.x * WF_BLOCK;
-    if (base >= count) return;
-    int i = base + threadIdx.x;
-    int p = (i < count) ? W.queue[parity][i] : -1;
-    int key = (p >= 0) ? path_key(S, W, p, nkeys) : -1;
-    keys[threadIdx.x] = key;
-    __syncthreads();
-    if (p >= 0) {
-        int rank = 0;
-        for (int j = 0; j < (int)threadIdx.x; ++j) rank += (keys[j] == key) ? 1 : 0;
-        int dst = W.hist[(size_t)key * gridDim.x + blockIdx.x] + rank;
-        W.sort_tmp[dst] = p;
+    for (int b0 = blockIdx.x * WF_BLOCK; b0 < count; b0 += gridDim.x * WF_BLOCK) {
+        for (int k = threadIdx.x; k < nkeys; k += WF_BLOCK) h[k] = 0;
+        __syncthreads();
+        const int i = b0 + threadIdx.x;
+        const int p = (i < count) ? W.queue[parity][i] : -1;
+        const int key = (p >= 0) ? path_key(S, W, p, nkeys) : 0;
+        const int rank = (p >= 0) ? atomicAdd(&h[key], 1) : 0;
+        __syncthreads();
+        for (int k = threadIdx.x; k < nkeys; k += WF_BLOCK)
+            base[k] = h[k] ? atomicAdd(&W.hist[MAX_KEYS + k], h[k]) : 0;
+        __syncthreads();
+        if (p >= 0) W.sort_tmp[base[key] + rank] = p;
+        __syncthreads();
     }
-}
-
-__global__ void k_copy_queue(WavefrontState W, int parity) {
-    const int count = W.counters[C_QCOUNT + parity];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) W.queue[parity][i] = W.sort_tmp[i];
 }
 
 // One bounce of Li for every live path (integrator.cuh:266-398).
 __global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, WavefrontState W, int parity,
-                                                    int pull_slot) {
-    __shared__ int s_base;
+                                                    int pull_slot, const int32_t* q) {
     const int count = W.counters[C_QCOUNT + parity];
-    const int32_t* q = W.queue[parity];
     int32_t* nq = W.queue[parity ^ 1];
     for (;;) {
-        int base = pull_chunk(&W.counters[pull_slot], &s_base);
+        const int base = wave_claim(&W.counters[pull_slot], WAVE);
         if (base >= count) break;
-        int k = base + (int)threadIdx.x;
+        const int k = base + lane_id();
         if (k >= count) continue;
         const int p = q[k];
         float4 o4 = W.ray_o[p], d4 = W.ray_d[p], b4 = W.beta[p], l4 = W.L[p];
@@ -280,22 +298,37 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, 
 // Visibility of the deferred next-event estimates; adds their term to L.
 __global__ __launch_bounds__(WF_BLOCK) void k_shadow(SceneView S, WavefrontState W, int pull_slot, int32_t* overflow) {
     __shared__ uint32_t s_stack[LDS_STACK][TRACE_BLOCK];
-    __shared__ int s_base;
     const int count = W.counters[C_SHADOW];
+    int32_t* ctr = &W.counters[pull_slot];
+    uint32_t* lds = &s_stack[0][threadIdx.x];
     bool ovf = false;
+    WaveFeeder feeder;
+    feeder.init();
+    AnyTrav tr;
+    int k = -1;
+    bool busy = false;
     for (;;) {
-        int base = pull_chunk(&W.counters[pull_slot], &s_base);
-        if (base >= count) break;
-        int k = base + (int)threadIdx.x;
-        if (k < count) {
-            float4 o = W.shadow_o[k], d = W.shadow_d[k];
-            Ray r = make_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z));
-            bool of = false;
-            bool occluded = trace_any<false>(S, r, 0.0f, o.w, &s_stack[0][threadIdx.x], &of, nullptr);
-            ovf = ovf || of;
-            int p = W.shadow_queue[k];
-            float4 c = occluded ? W.shadow_z[k] : W.shadow_c[k];
-            float4 l = W.L[p];
+        const int kk = feeder.feed(ctr, count, !busy);
+        bool done_now = false;
+        if (kk >= 0) {
+            k = kk;
+            const float4 o = W.shadow_o[k], d = W.shadow_d[k];
+            busy = tr.init(S, make_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z)), 0.0f, o.w, lds);
+            done_now = !busy;
+        }
+        if (!done_now && !__any(busy)) {
+            if (feeder.exhausted()) break;
+            continue;
+        }
+        if (busy) {
+            busy = tr.template step<false>(S, nullptr);
+            done_now = !busy;
+        }
+        if (done_now) {
+            ovf = ovf || tr.st.overflow;
+            const int p = W.shadow_queue[k];
+            const float4 c = tr.hit ? W.shadow_z[k] : W.shadow_c[k];
+            const float4 l = W.L[p];
             W.L[p] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
         }
     }
@@ -361,13 +394,16 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
                                overflow, (unsigned long long*)nullptr);
         }
         if (timed) (void)hipEventRecord(ins->end[ins->launches++], s);
+        const int32_t* shade_q = W.queue[parity];
         if (sort_by_material && nkeys > 1) {
-            hipLaunchKernelGGL(k_sort_hist, dim3(gblocks), dim3(WF_BLOCK), 0, s, S, W, parity, nkeys);
-            hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, W, parity, nkeys, (int)gblocks);
-            hipLaunchKernelGGL(k_sort_scatter, dim3(gblocks), dim3(WF_BLOCK), 0, s, S, W, parity, nkeys);
-            hipLaunchKernelGGL(k_copy_queue, dim3(gblocks), dim3(WF_BLOCK), 0, s, W, parity);
+            hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(2 * MAX_KEYS), 0, s, W.hist, 0, 2 * MAX_KEYS);
+            hipLaunchKernelGGL(k_sort_hist, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, W, parity, nkeys);
+            hipLaunchKernelGGL(k_sort_offsets, dim3(1), dim3(64), 0, s, W, nkeys);
+            hipLaunchKernelGGL(k_sort_scatter, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, W, parity, nkeys);
+            shade_q = W.sort_tmp;
         }
-        hipLaunchKernelGGL(k_shade, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity, C_PULL + 3 * bounce + 1);
+        hipLaunchKernelGGL(k_shade, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity, C_PULL + 3 * bounce + 1,
+                           shade_q);
         hipLaunchKernelGGL(k_shadow, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, W, C_PULL + 3 * bounce + 2, overflow);
     }
     hipLaunchKernelGGL(k_finish, dim3(gblocks), dim3(WF_BLOCK), 0, s, S, fc, frame, W, fb);
