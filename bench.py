@@ -25,7 +25,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from kittlespt_amd import abi, scenes  # noqa: E402
+from kittlespt_amd import abi, autotune, scenes  # noqa: E402
 from kittlespt_amd.context import Context  # noqa: E402
 
 METRIC = "Msamples/s @1080p + time-to-target-RMSE vs FP64, cached vs plain, 1 GPU"
@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--instances", type=int, default=1000)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", default="", help="fixed knobs, e.g. refill=0,blocks_per_cu=8,sort=1")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -129,6 +131,15 @@ def main():
     view = scenes.look_at_view(scene.camera_eye, scene.camera_target)
     ctx.set_view(proj, view)
     cam = ctx.camera()
+
+    if integ == abi.INTEGRATOR_WAVEFRONT and args.tuning:
+        kv = dict(x.split("=") for x in args.tuning.split(","))
+        ctx.set_tuning(**{k: int(v) for k, v in kv.items()})
+    elif integ == abi.INTEGRATOR_WAVEFRONT and not args.no_autotune:
+        t0 = time.time()
+        best, ms, _ = autotune.tune(ctx, spp=1, log=log)
+        log(f"[bench] autotuned in {time.time() - t0:.1f}s: {best} ({ms:.2f} ms per 1-spp frame)")
+    tuning = ctx.tuning() if integ == abi.INTEGRATOR_WAVEFRONT else None
 
     for i in range(args.warmup):
         ctx.render(args.spp)
@@ -186,7 +197,7 @@ def main():
                                f"{scene.num_instances} instances of 8 BLAS, 16 textured materials, Nishita sky",
                    "integrator": args.integrator, "width": W, "height": H, "spp": args.spp,
                    "max_depth": args.depth, "triangles": int(len(scene.positions)),
-                   "instances": int(scene.num_instances), "parallelism": "single GPU"},
+                   "instances": int(scene.num_instances), "parallelism": "single GPU", "tuning": tuning},
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

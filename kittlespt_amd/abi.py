@@ -172,6 +172,8 @@ SIGNATURES = [
     ("kpt_last_timing", C.c_int, [_P, C.POINTER(C.c_double)]),
     ("kpt_set_trace_counting", C.c_int, [_P, C.c_int32]),
     ("kpt_get_render_counters", C.c_int, [_P, C.POINTER(KptTraceCounters)]),
+    ("kpt_set_tuning", C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32]),
+    ("kpt_get_tuning", C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32]),
 ]
 
 

@@ -68,6 +68,8 @@ struct kpt_ctx {
     WavefrontState ws{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, tev0 = nullptr, tev1 = nullptr;
     WfInstrument ins{};
+    TraceTuning tune{0, 128, 16, 8};
+    int32_t sort_by_material = 1;
     bool count_trace = false;
     unsigned long long* d_counts = nullptr;
     kpt_trace_counters render_counts{};
@@ -698,7 +700,7 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
         if (ctx->settings.integrator == KPT_INTEGRATOR_WAVEFRONT) {
             ctx->ins.counters = ctx->count_trace ? ctx->d_counts : nullptr;
             HIPCHK(wavefront_frame(ctx->stream, S, fc, ctx->frame_index, ctx->fb, ctx->ws, ctx->d_overflow,
-                                   (int32_t)ctx->materials.size(), 1, &ctx->ins));
+                                   (int32_t)ctx->materials.size(), ctx->sort_by_material, ctx->tune, &ctx->ins));
             // per-launch trace times of this frame (events already recorded)
             HIPCHK(hipEventSynchronize(ctx->ins.end[ctx->ins.launches - 1]));
             for (int j = 0; j < ctx->ins.launches; ++j) {
@@ -775,6 +777,29 @@ int kpt_sync(kpt_ctx* ctx) {
     int32_t ovf = 0;
     HIPCHK(hipMemcpy(&ovf, ctx->d_overflow, sizeof(int32_t), hipMemcpyDeviceToHost));
     if (ovf) return fail(ctx, KPT_ERR_OVERFLOW, "traversal stack overflow (depth > 128)");
+    return KPT_OK;
+}
+
+int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n) {
+    if (!ctx || !knobs || n < 5) return KPT_ERR_INVALID_ARG;
+    if (knobs[0] < 0 || knobs[0] > 1 || knobs[1] < 64 || knobs[1] > 65536 || knobs[2] < 1 || knobs[2] > 64 ||
+        knobs[3] < 1 || knobs[3] > 64 || knobs[4] < 0 || knobs[4] > 1)
+        return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
+    ctx->tune.refill = knobs[0];
+    ctx->tune.chunk = knobs[1];
+    ctx->tune.refill_min = knobs[2];
+    ctx->tune.blocks_per_cu = knobs[3];
+    ctx->sort_by_material = knobs[4];
+    return KPT_OK;
+}
+
+int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n) {
+    if (!ctx || !knobs || n < 5) return KPT_ERR_INVALID_ARG;
+    knobs[0] = ctx->tune.refill;
+    knobs[1] = ctx->tune.chunk;
+    knobs[2] = ctx->tune.refill_min;
+    knobs[3] = ctx->tune.blocks_per_cu;
+    knobs[4] = ctx->sort_by_material;
     return KPT_OK;
 }
 

@@ -55,6 +55,14 @@ hipError_t launch_megakernel(hipStream_t s, const SceneView& S, const FrameConst
                              const FrameBuffers& fb, int32_t* overflow);
 hipError_t launch_clear(hipStream_t s, float4* p, int64_t n);
 
+// Launch-shape knobs of the trace stages (chosen by the autotuner).
+struct TraceTuning {
+    int refill;        // 0: a lane traces one ray to completion; 1: idle lanes are refilled
+    int chunk;         // queue entries claimed per wave atomic (refill mode)
+    int refill_min;    // idle lanes that trigger a refill
+    int blocks_per_cu; // persistent grid size per compute unit
+};
+
 // Events bracketing every closest-hit trace launch of a frame (recorded on the
 // context stream), and optional traversal event counters.
 constexpr int WF_MAX_TRACE_LAUNCHES = 40;
@@ -68,6 +76,6 @@ struct WfInstrument {
 // Wavefront path tracer: one reference frame (one sample per pixel).
 hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& fc, int32_t frame,
                            const FrameBuffers& fb, WavefrontState& W, int32_t* overflow, int32_t num_materials,
-                           int32_t sort_by_material, WfInstrument* ins);
+                           int32_t sort_by_material, const TraceTuning& tune, WfInstrument* ins);
 
 }  // namespace kpt

@@ -105,10 +105,12 @@ KD bool tri_test(const TriPos& tp, const Ray& r, float tmin, float tmax, float* 
     return false;
 }
 
-// Per-lane stack: LDS part [LDS_STACK][TRACE_BLOCK] + private spill.
+// Per-lane stack: LDS part [LDS_STACK][TRACE_BLOCK] + a private spill array
+// owned by the caller (kept outside the traversal state so the state itself
+// stays in registers).
 struct LaneStack {
     uint32_t* lds;      // &s_stack[0][lane]
-    uint32_t spill[SPILL_STACK];
+    uint32_t* spill;    // private array of SPILL_STACK entries
     int sp;
     bool overflow;
     KD void push(uint32_t e) {
@@ -134,11 +136,11 @@ struct ClosestTrav {
 
     // TLAS::intersect entry: empty TLAS or a miss of the root box ends the
     // query at once (tlas.cu:62-69).  Returns false when nothing remains.
-    KD bool init(const SceneView& S, const Ray& ray, float t0, float t1, uint32_t* lds) {
+    KD bool init(const SceneView& S, const Ray& ray, float t0, float t1, uint32_t* lds, uint32_t* spill) {
         w = ray; o = ray; tmin = t0; tmax = t1;
         h.t = __builtin_inff(); h.slot = -1; h.inst = -1; h.u = 0.0f; h.v = 0.0f;
         cur_inst = -1;
-        st.lds = lds; st.sp = 0; st.overflow = false;
+        st.lds = lds; st.spill = spill; st.sp = 0; st.overflow = false;
         if (S.tlas_root < 0) return false;
         float4 rlo, rhi;
         load_node(S.tlas_nodes, 0, &rlo, &rhi);
@@ -231,9 +233,9 @@ struct AnyTrav {
     uint32_t cur;
     LaneStack st;
 
-    KD bool init(const SceneView& S, const Ray& ray, float t0, float t1, uint32_t* lds) {
+    KD bool init(const SceneView& S, const Ray& ray, float t0, float t1, uint32_t* lds, uint32_t* spill) {
         w = ray; o = ray; tmin = t0; tmax = t1; hit = false;
-        st.lds = lds; st.sp = 0; st.overflow = false;
+        st.lds = lds; st.spill = spill; st.sp = 0; st.overflow = false;
         if (S.tlas_root < 0) return false;
         float4 rlo, rhi;
         load_node(S.tlas_nodes, 0, &rlo, &rhi);
@@ -306,8 +308,9 @@ struct AnyTrav {
 template <bool COUNT>
 KD Hit trace_closest(const SceneView& S, const Ray& w, float tmin, float tmax, uint32_t* lds_lane,
                      bool* overflow, TraceCounts* cnt) {
+    uint32_t spill[SPILL_STACK];
     ClosestTrav tr;
-    if (tr.init(S, w, tmin, tmax, lds_lane)) {
+    if (tr.init(S, w, tmin, tmax, lds_lane, spill)) {
         while (tr.template step<COUNT>(S, cnt)) {}
     }
     *overflow = tr.st.overflow;
@@ -317,8 +320,9 @@ KD Hit trace_closest(const SceneView& S, const Ray& w, float tmin, float tmax, u
 template <bool COUNT>
 KD bool trace_any(const SceneView& S, const Ray& w, float tmin, float tmax, uint32_t* lds_lane, bool* overflow,
                   TraceCounts* cnt) {
+    uint32_t spill[SPILL_STACK];
     AnyTrav tr;
-    if (tr.init(S, w, tmin, tmax, lds_lane)) {
+    if (tr.init(S, w, tmin, tmax, lds_lane, spill)) {
         while (tr.template step<COUNT>(S, cnt)) {}
     }
     *overflow = tr.st.overflow;

@@ -29,8 +29,6 @@ KD uint32_t fbits(float f) { return __builtin_bit_cast(uint32_t, f); }
 KD float bitsf(uint32_t u) { return __builtin_bit_cast(float, u); }
 
 constexpr int WAVE = 64;
-constexpr int TRACE_CHUNK = 128;   // queue entries a wave claims per atomic
-constexpr int REFILL_MIN = 16;     // idle lanes that trigger a refill
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
 __device__ __forceinline__ uint64_t lanes_below() { return (1ull << lane_id()) - 1ull; }
@@ -43,21 +41,25 @@ __device__ __forceinline__ int wave_claim(int32_t* ctr, int n) {
 }
 
 // Wave-local work source over a global queue of `count` entries: a wave claims
-// TRACE_CHUNK entries at a time and hands them to its idle lanes, so a lane
-// whose ray finishes starts another without waiting for the rest of the wave.
+// `chunk` entries at a time and hands them to its idle lanes (once at least
+// `refill_min` are idle), so a lane whose ray finishes starts another without
+// waiting for the rest of the wave.
 struct WaveFeeder {
     int pos, end;        // wave-uniform claimed range [pos, end)
     bool drained;
-    __device__ void init() { pos = 0; end = 0; drained = false; }
+    int chunk, refill_min;
+    __device__ void init(const TraceTuning& t) {
+        pos = 0; end = 0; drained = false; chunk = t.chunk; refill_min = t.refill_min;
+    }
     // Returns the queue position for this lane (when `idle`) or -1.
     __device__ int feed(int32_t* ctr, int count, bool idle) {
         const uint64_t m = __ballot(idle);
         const int n = __popcll(m);
-        if (n == 0 || (n < REFILL_MIN && n < WAVE)) return -1;
+        if (n == 0 || (n < refill_min && n < WAVE)) return -1;
         if (pos >= end && !drained) {
-            int b = wave_claim(ctr, TRACE_CHUNK);
+            int b = wave_claim(ctr, chunk);
             if (b >= count) drained = true;
-            else { pos = b; end = min(b + TRACE_CHUNK, count); }
+            else { pos = b; end = min(b + chunk, count); }
         }
         if (pos >= end) return -1;
         const int k = pos + __popcll(m & lanes_below());
@@ -94,41 +96,61 @@ __global__ __launch_bounds__(WF_BLOCK) void k_generate(FrameConst fc, int32_t fr
 // Closest hit of every live path.  COUNT accumulates traversal events.
 template <bool COUNT>
 __global__ __launch_bounds__(WF_BLOCK) void k_trace(SceneView S, WavefrontState W, int parity, int pull_slot,
-                                                    int32_t* overflow, unsigned long long* counters) {
+                                                    int32_t* overflow, unsigned long long* counters,
+                                                    TraceTuning tune) {
     __shared__ uint32_t s_stack[LDS_STACK][TRACE_BLOCK];
+    uint32_t spill[SPILL_STACK];
     const int count = W.counters[C_QCOUNT + parity];
     const int32_t* q = W.queue[parity];
     int32_t* ctr = &W.counters[pull_slot];
     uint32_t* lds = &s_stack[0][threadIdx.x];
     bool ovf = false;
     TraceCounts c = {0, 0, 0, 0, 0};
-    WaveFeeder feeder;
-    feeder.init();
-    ClosestTrav tr;
-    int p = -1;
-    bool busy = false;
-    for (;;) {
-        const int k = feeder.feed(ctr, count, !busy);
-        if (k >= 0) {
-            p = q[k];
+    if (!tune.refill) {
+        // one ray per lane to completion; a wave claims 64 entries at a time
+        for (;;) {
+            const int base = wave_claim(ctr, WAVE);
+            if (base >= count) break;
+            const int k = base + lane_id();
+            if (k >= count) continue;
+            const int p = q[k];
             const float4 o = W.ray_o[p], d = W.ray_d[p];
-            busy = tr.init(S, make_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z)), 0.0f, __builtin_inff(), lds);
-            if (!busy) {   // empty scene or root box missed
-                W.hit[p] = make_float4(__builtin_inff(), bitsf(0xFFFFFFFFu), bitsf(0xFFFFFFFFu), 0.0f);
-                W.hit_v[p] = 0.0f;
+            ClosestTrav tr;
+            if (tr.init(S, make_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z)), 0.0f, __builtin_inff(), lds, spill)) {
+                while (tr.template step<COUNT>(S, &c)) {}
             }
+            ovf = ovf || tr.st.overflow;
+            W.hit[p] = make_float4(tr.h.t, bitsf((uint32_t)tr.h.slot), bitsf((uint32_t)tr.h.inst), tr.h.u);
+            W.hit_v[p] = tr.h.v;
         }
-        if (!__any(busy)) {
-            if (feeder.exhausted()) break;
-            continue;
-        }
-        if (busy) {
-            busy = tr.template step<COUNT>(S, &c);
-            if (!busy) {
+    } else {
+        WaveFeeder feeder;
+        feeder.init(tune);
+        ClosestTrav tr;
+        int p = -1;
+        bool busy = false;
+        for (;;) {
+            const int k = feeder.feed(ctr, count, !busy);
+            bool done_now = false;
+            if (k >= 0) {
+                p = q[k];
+                const float4 o = W.ray_o[p], d = W.ray_d[p];
+                busy = tr.init(S, make_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z)), 0.0f, __builtin_inff(), lds,
+                               spill);
+                done_now = !busy;   // empty scene or root box missed
+            }
+            if (!__any(busy || done_now)) {   // wave-uniform
+                if (feeder.exhausted()) break;
+                continue;
+            }
+            if (busy) {
+                busy = tr.template step<COUNT>(S, &c);
+                done_now = !busy;
+            }
+            if (done_now) {
                 ovf = ovf || tr.st.overflow;
-                const Hit& h = tr.h;
-                W.hit[p] = make_float4(h.t, bitsf((uint32_t)h.slot), bitsf((uint32_t)h.inst), h.u);
-                W.hit_v[p] = h.v;
+                W.hit[p] = make_float4(tr.h.t, bitsf((uint32_t)tr.h.slot), bitsf((uint32_t)tr.h.inst), tr.h.u);
+                W.hit_v[p] = tr.h.v;
             }
         }
     }
@@ -296,40 +318,62 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, 
 }
 
 // Visibility of the deferred next-event estimates; adds their term to L.
-__global__ __launch_bounds__(WF_BLOCK) void k_shadow(SceneView S, WavefrontState W, int pull_slot, int32_t* overflow) {
+__device__ __forceinline__ void shadow_resolve(WavefrontState& W, int k, bool occluded) {
+    const int p = W.shadow_queue[k];
+    const float4 c = occluded ? W.shadow_z[k] : W.shadow_c[k];
+    const float4 l = W.L[p];
+    W.L[p] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
+}
+
+__global__ __launch_bounds__(WF_BLOCK) void k_shadow(SceneView S, WavefrontState W, int pull_slot, int32_t* overflow,
+                                                     TraceTuning tune) {
     __shared__ uint32_t s_stack[LDS_STACK][TRACE_BLOCK];
+    uint32_t spill[SPILL_STACK];
     const int count = W.counters[C_SHADOW];
     int32_t* ctr = &W.counters[pull_slot];
     uint32_t* lds = &s_stack[0][threadIdx.x];
     bool ovf = false;
-    WaveFeeder feeder;
-    feeder.init();
-    AnyTrav tr;
-    int k = -1;
-    bool busy = false;
-    for (;;) {
-        const int kk = feeder.feed(ctr, count, !busy);
-        bool done_now = false;
-        if (kk >= 0) {
-            k = kk;
+    if (!tune.refill) {
+        for (;;) {
+            const int base = wave_claim(ctr, WAVE);
+            if (base >= count) break;
+            const int k = base + lane_id();
+            if (k >= count) continue;
             const float4 o = W.shadow_o[k], d = W.shadow_d[k];
-            busy = tr.init(S, make_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z)), 0.0f, o.w, lds);
-            done_now = !busy;
-        }
-        if (!done_now && !__any(busy)) {
-            if (feeder.exhausted()) break;
-            continue;
-        }
-        if (busy) {
-            busy = tr.template step<false>(S, nullptr);
-            done_now = !busy;
-        }
-        if (done_now) {
+            AnyTrav tr;
+            if (tr.init(S, make_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z)), 0.0f, o.w, lds, spill)) {
+                while (tr.template step<false>(S, nullptr)) {}
+            }
             ovf = ovf || tr.st.overflow;
-            const int p = W.shadow_queue[k];
-            const float4 c = tr.hit ? W.shadow_z[k] : W.shadow_c[k];
-            const float4 l = W.L[p];
-            W.L[p] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
+            shadow_resolve(W, k, tr.hit);
+        }
+    } else {
+        WaveFeeder feeder;
+        feeder.init(tune);
+        AnyTrav tr;
+        int k = -1;
+        bool busy = false;
+        for (;;) {
+            const int kk = feeder.feed(ctr, count, !busy);
+            bool done_now = false;
+            if (kk >= 0) {
+                k = kk;
+                const float4 o = W.shadow_o[k], d = W.shadow_d[k];
+                busy = tr.init(S, make_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z)), 0.0f, o.w, lds, spill);
+                done_now = !busy;
+            }
+            if (!__any(busy || done_now)) {   // wave-uniform
+                if (feeder.exhausted()) break;
+                continue;
+            }
+            if (busy) {
+                busy = tr.template step<false>(S, nullptr);
+                done_now = !busy;
+            }
+            if (done_now) {
+                ovf = ovf || tr.st.overflow;
+                shadow_resolve(W, k, tr.hit);
+            }
         }
     }
     if (ovf) atomicOr(overflow, 1);
@@ -362,7 +406,7 @@ int g_num_cus = 0;
 
 hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& fc, int32_t frame,
                            const FrameBuffers& fb, WavefrontState& W, int32_t* overflow, int32_t num_materials,
-                           int32_t sort_by_material, WfInstrument* ins) {
+                           int32_t sort_by_material, const TraceTuning& tune, WfInstrument* ins) {
     if (ins) ins->launches = 0;
     if (g_num_cus == 0) {
         int dev = 0;
@@ -374,7 +418,8 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
     const int64_t npx = (int64_t)fc.width * fc.height;
     const unsigned gblocks = (unsigned)((npx + WF_BLOCK - 1) / WF_BLOCK);
     // persistent grids: 8 workgroups of 256 lanes per CU
-    const unsigned pgrid = (unsigned)std::min<int64_t>((int64_t)g_num_cus * 8, gblocks);
+    const unsigned pgrid = (unsigned)std::max<int64_t>(
+        1, std::min<int64_t>((int64_t)g_num_cus * std::max(1, tune.blocks_per_cu), gblocks));
     const int nkeys = std::min(MAX_KEYS, num_materials + 1);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, W.counters, 0, C_PULL + 3 * MAX_BOUNCE_SLOTS);
     hipLaunchKernelGGL(k_generate, dim3(gblocks), dim3(WF_BLOCK), 0, s, fc, frame, W);
@@ -388,10 +433,10 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
         if (timed) (void)hipEventRecord(ins->begin[ins->launches], s);
         if (ins && ins->counters) {
             hipLaunchKernelGGL(k_trace<true>, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, W, parity, C_PULL + 3 * bounce,
-                               overflow, ins->counters);
+                               overflow, ins->counters, tune);
         } else {
             hipLaunchKernelGGL(k_trace<false>, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, W, parity, C_PULL + 3 * bounce,
-                               overflow, (unsigned long long*)nullptr);
+                               overflow, (unsigned long long*)nullptr, tune);
         }
         if (timed) (void)hipEventRecord(ins->end[ins->launches++], s);
         const int32_t* shade_q = W.queue[parity];
@@ -404,7 +449,8 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
         }
         hipLaunchKernelGGL(k_shade, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity, C_PULL + 3 * bounce + 1,
                            shade_q);
-        hipLaunchKernelGGL(k_shadow, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, W, C_PULL + 3 * bounce + 2, overflow);
+        hipLaunchKernelGGL(k_shadow, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, W, C_PULL + 3 * bounce + 2, overflow,
+                           tune);
     }
     hipLaunchKernelGGL(k_finish, dim3(gblocks), dim3(WF_BLOCK), 0, s, S, fc, frame, W, fb);
     return hipGetLastError();
