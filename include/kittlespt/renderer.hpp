@@ -1,0 +1,261 @@
+// kittlespt/renderer.hpp — KittlesPT::Renderer (KittlesPT/src/renderer.hpp:13-58)
+// as a header-only C++ class over the C-ABI of libkpt_hip.so (kpt_abi.h).
+//
+// Same method names, arguments and side effects as the reference (setters
+// reset accumulation as renderer.cu:223, 264-266, 293, 304, 351-353 do).
+// GL textures are replaced by host readback (readOutput / readDebug).  Errors:
+// methods that the reference declares void throw KittlesPT::RendererError
+// (the reference calls exit(99), error_check.cu:4-13); bool setters return
+// false on a bad index as the reference does.
+#pragma once
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../kpt_abi.h"
+#include "pod_types.hpp"
+
+namespace KittlesPT {
+
+class RendererError : public std::runtime_error {
+public:
+    RendererError(int code, const std::string& msg) : std::runtime_error(msg), code(code) {}
+    int code;
+};
+
+class Renderer {
+public:
+    // renderer.cu:22-46
+    void initialize(int device = -1) {
+        if (m_ctx) return;
+        check(kpt_ctx_create(device, &m_ctx), "kpt_ctx_create");
+        kpt_get_settings(m_ctx, &m_settings);
+        m_settings.integrator = KPT_INTEGRATOR_WAVEFRONT;
+        check(kpt_set_settings(m_ctx, &m_settings), "kpt_set_settings");
+    }
+    // renderer.cu:48-53
+    void shutdown() {
+        if (m_ctx) kpt_ctx_destroy(m_ctx);
+        m_ctx = nullptr;
+    }
+    ~Renderer() { shutdown(); }
+
+    // renderer.cu:55-101 (no upscaler: render and output resolutions agree)
+    void resizeResolution(uint32_t width, uint32_t height) {
+        if (width == m_width && height == m_height) return;
+        check(kpt_resize(m_ctx, (int32_t)width, (int32_t)height), "kpt_resize");
+        m_width = width;
+        m_height = height;
+    }
+
+    // renderer.cu:103-195: one path-traced sample per pixel per call
+    void executeRendering(float /*delta_time_ms*/) { check(kpt_render(m_ctx, 1), "kpt_render"); }
+
+    // replaces getRenderTargetTexture(GLuint) (renderer.cu:197-200): RGBA32F
+    // estimate of the accumulated radiance, row-major, width*height*4 floats
+    std::vector<float> readOutput() const {
+        std::vector<float> out((size_t)m_width * m_height * 4);
+        check(kpt_readback(m_ctx, 0, out.data(), (int64_t)(out.size() * sizeof(float))), "kpt_readback");
+        return out;
+    }
+    // replaces getDebugRenderTargetTexture (renderer.cu:202-205): the V-buffer
+    std::vector<float> readDebug() const {
+        std::vector<float> out((size_t)m_width * m_height * 4);
+        check(kpt_readback(m_ctx, 3, out.data(), (int64_t)(out.size() * sizeof(float))), "kpt_readback");
+        return out;
+    }
+
+    size_t getMaterialsCount() const { return m_materials.size(); }
+    size_t getMeshCount() const { return m_models.size(); }
+
+    // renderer.cu:356-436 (+ setMeshTransform for every instance)
+    void loadScene(const BasicScene& s) {
+        std::vector<float> pos, nrm, uv;
+        std::vector<int32_t> mids, first, count;
+        std::vector<float> mesh_model;
+        for (const MeshSceneEntity& m : s.mesh_entities) {
+            first.push_back((int32_t)mids.size());
+            count.push_back((int32_t)m.shape_entities.size());
+            mesh_model.insert(mesh_model.end(), m.model_matrix.data(), m.model_matrix.data() + 16);
+            for (const TriangleSceneEntity& t : m.shape_entities) {
+                const vec3* P[3] = {&t.p0, &t.p1, &t.p2};
+                const vec3* N[3] = {&t.n0, &t.n1, &t.n2};
+                const vec2* T[3] = {&t.t0, &t.t1, &t.t2};
+                for (int v = 0; v < 3; ++v) {
+                    pos.insert(pos.end(), {P[v]->x, P[v]->y, P[v]->z});
+                    nrm.insert(nrm.end(), {N[v]->x, N[v]->y, N[v]->z});
+                    uv.insert(uv.end(), {T[v]->x, T[v]->y});
+                }
+                mids.push_back(t.material_id);
+            }
+        }
+        std::vector<kpt_material> mats;
+        for (const MaterialSceneEntity& m : s.material_entities) mats.push_back(toAbi(m));
+        std::vector<kpt_texture_desc> texs;
+        std::vector<uint8_t> pixels;
+        for (const TextureSceneEntity& t : s.texture_entities) {
+            kpt_texture_desc d{t.width, t.height, t.channels_count, 0, (uint64_t)pixels.size()};
+            texs.push_back(d);
+            pixels.insert(pixels.end(), t.pixels_data.begin(), t.pixels_data.end());
+        }
+        std::vector<int32_t> inst_mesh;
+        std::vector<float> inst_model;
+        for (const InstanceSceneEntity& i : s.instance_entities) {
+            inst_mesh.push_back(i.mesh_index);
+            inst_model.insert(inst_model.end(), i.model_matrix.data(), i.model_matrix.data() + 16);
+        }
+        kpt_scene_desc d{};
+        d.num_triangles = (int32_t)mids.size();
+        d.positions = pos.data();
+        d.normals = nrm.data();
+        d.uvs = uv.data();
+        d.material_ids = mids.data();
+        d.num_meshes = (int32_t)first.size();
+        d.mesh_first = first.data();
+        d.mesh_count = count.data();
+        d.mesh_model = mesh_model.data();
+        d.num_instances = (int32_t)inst_mesh.size();
+        d.instance_mesh = inst_mesh.data();
+        d.instance_model = inst_model.data();
+        d.num_materials = (int32_t)mats.size();
+        d.materials = mats.data();
+        d.num_textures = (int32_t)texs.size();
+        d.textures = texs.data();
+        d.pixels = pixels.data();
+        d.num_pixel_bytes = pixels.size();
+        check(kpt_upload_scene(m_ctx, &d), "kpt_upload_scene");
+        m_materials = s.material_entities;
+        m_models.clear();
+        if (s.instance_entities.empty())
+            for (const MeshSceneEntity& m : s.mesh_entities) m_models.push_back(m.model_matrix);
+        else
+            for (const InstanceSceneEntity& i : s.instance_entities) m_models.push_back(i.model_matrix);
+    }
+
+    // renderer.cu:207-226
+    bool setMaterial(uint32_t index, const MaterialSceneEntity& material) {
+        if (index >= m_materials.size()) return false;
+        kpt_material m = toAbi(material);
+        if (kpt_set_material(m_ctx, (int32_t)index, &m) != KPT_OK) return false;
+        m_materials[index] = material;
+        return true;
+    }
+    // renderer.cu:228-248 (bounds-checked: throws instead of reading garbage)
+    MaterialSceneEntity getMaterial(uint32_t index) const {
+        if (index >= m_materials.size()) throw RendererError(KPT_ERR_INVALID_ARG, "material index out of range");
+        MaterialSceneEntity m = m_materials[index];
+        m.name = "unpreserved_name";
+        return m;
+    }
+
+    // renderer.cu:250-269.  Matrices: KittlesPT::mat4 or any column-major
+    // 4x4 type indexed m[column][row] over contiguous floats (glm::mat4).
+    template <class M>
+    bool setMeshTransform(uint32_t index, const M& model) {
+        if (index >= m_models.size()) return false;
+        mat4 m = toMat4(model);
+        if (kpt_set_instance_transform(m_ctx, (int32_t)index, m.data()) != KPT_OK) return false;
+        m_models[index] = m;
+        return true;
+    }
+    template <class M = mat4>
+    M getMeshTransform(uint32_t index) const {
+        if (index >= m_models.size()) throw RendererError(KPT_ERR_INVALID_ARG, "mesh index out of range");
+        M out;
+        std::memcpy(&out[0][0], m_models[index].data(), 16 * sizeof(float));
+        return out;
+    }
+
+    // renderer.cu:290-299
+    void setProceduralEnvironmentData(const ProceduralEnvironmentSettings& e) {
+        m_env = e;
+        push();
+    }
+    ProceduralEnvironmentSettings getProceduralEnvironmentData() const { return m_env; }
+
+    // renderer.cu:301-310
+    void setRendererSettings(const RendererSettings& s) {
+        m_rs = s;
+        push();
+    }
+    RendererSettings getRendererSettings() const { return m_rs; }
+
+    // renderer.cu:342-354
+    template <class M>
+    void setView(const M& projection, const M& view) {
+        mat4 p = toMat4(projection), v = toMat4(view);
+        check(kpt_set_view(m_ctx, p.data(), v.data()), "kpt_set_view");
+    }
+
+    // renderer.cu:312-334: stored; consumed by the post chain
+    void setExposure(const ExposureValues& v, float ev_comp, float white_point_ev, float black_point_ev) {
+        m_exposure = v;
+        m_ev_comp = ev_comp;
+        m_white = white_point_ev;
+        m_black = black_point_ev;
+    }
+    ExposureValues getExposure() const { return m_exposure; }
+
+    // renderer.cu:336-340
+    void resetAccumulation() { check(kpt_reset_accumulation(m_ctx), "kpt_reset_accumulation"); }
+
+    // extensions of this library
+    kpt_ctx* context() const { return m_ctx; }
+    void setSeedMode(int mode) { m_settings.seed_mode = mode; push(); }
+
+private:
+    template <class M>
+    static mat4 toMat4(const M& src) {
+        mat4 m;
+        std::memcpy(m.data(), &src[0][0], 16 * sizeof(float));
+        return m;
+    }
+    static kpt_material toAbi(const MaterialSceneEntity& m) {
+        kpt_material k{};
+        k.albedo_texture_id = m.albedo_texture_id;
+        k.albedo[0] = m.albedo_factor.x; k.albedo[1] = m.albedo_factor.y; k.albedo[2] = m.albedo_factor.z;
+        k.orm_texture_id = m.ORM_texture_id;
+        k.metallic = m.metallic_factor;
+        k.roughness = m.roughness_factor;
+        k.transmission_texture_id = m.transmission_texture_id;
+        k.transmission = m.transmission_factor;
+        k.ior = m.ior;
+        k.emission_texture_id = m.emission_texture_id;
+        k.emission_factor[0] = m.emission_factor.x; k.emission_factor[1] = m.emission_factor.y;
+        k.emission_factor[2] = m.emission_factor.z;
+        k.emission_scale_nits = m.emission_scale_nits;
+        k.normal_texture_id = m.normal_texture_id;
+        k.normal_scale = m.normal_scale;
+        return k;
+    }
+    void push() {
+        m_settings.max_depth = m_rs.integrator_max_ray_depth;
+        m_settings.use_temporal_accumulation = m_rs.integrator_use_temporal_accumulation ? 1 : 0;
+        m_settings.bloom_generate_bloom = m_rs.bloom_generate_bloom ? 1 : 0;
+        m_settings.bloom_use_karis_average = m_rs.bloom_use_karis_average ? 1 : 0;
+        m_settings.bloom_final_blend = m_rs.bloom_final_blend;
+        m_settings.bloom_internal_blend = m_rs.bloom_internal_blend;
+        m_settings.enable_auto_exposure = m_rs.tonemapper_enable_auto_exposure ? 1 : 0;
+        m_settings.sun_angular_diameter_rad = m_env.sun_angular_diameter_rad;
+        m_settings.sun_phi_rad = m_env.sun_phi_rad;
+        m_settings.sun_theta_rad = m_env.sun_theta_rad;
+        m_settings.sun_emission_factor = m_env.sun_emission_factor;
+        check(kpt_set_settings(m_ctx, &m_settings), "kpt_set_settings");
+    }
+    void check(int rc, const char* what) const {
+        if (rc != KPT_OK) throw RendererError(rc, std::string(what) + ": " + kpt_last_error(m_ctx));
+    }
+
+    kpt_ctx* m_ctx = nullptr;
+    kpt_settings m_settings{};
+    RendererSettings m_rs;
+    ProceduralEnvironmentSettings m_env;
+    ExposureValues m_exposure;
+    float m_ev_comp = 0.0f, m_white = 6.5f, m_black = -10.0f;
+    uint32_t m_width = 0, m_height = 0;
+    std::vector<MaterialSceneEntity> m_materials;
+    std::vector<mat4> m_models;
+};
+
+}  // namespace KittlesPT

@@ -48,5 +48,22 @@ def build(verbose=False):
     return OUT
 
 
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+
+def compile_cpp_app(sources, out, extra=()):
+    """Compiles and links a host C++ application against include/ and the
+    in-tree libkpt_hip.so (rpath set, so it runs without LD_LIBRARY_PATH)."""
+    os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
+    libdir = os.path.dirname(OUT)
+    cmd = (["g++", "-std=c++17", "-O2", "-Wall", "-I", INCLUDE] + list(sources) + list(extra) +
+           ["-L", libdir, "-lkpt_hip", f"-Wl,-rpath,{libdir}", "-o", out])
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"g++ failed building {out}")
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
