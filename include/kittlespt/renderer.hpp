@@ -49,22 +49,21 @@ public:
         m_height = height;
     }
 
-    // renderer.cu:103-195: one path-traced sample per pixel per call
-    void executeRendering(float /*delta_time_ms*/) { check(kpt_render(m_ctx, 1), "kpt_render"); }
+    // renderer.cu:103-195: one path-traced sample per pixel per call, then
+    // modulation, auto exposure, bloom and the AgX post pass
+    void executeRendering(float delta_time_ms) {
+        check(kpt_set_frame_delta(m_ctx, delta_time_ms), "kpt_set_frame_delta");
+        check(kpt_render(m_ctx, 1), "kpt_render");
+    }
 
-    // replaces getRenderTargetTexture(GLuint) (renderer.cu:197-200): RGBA32F
-    // estimate of the accumulated radiance, row-major, width*height*4 floats
-    std::vector<float> readOutput() const {
-        std::vector<float> out((size_t)m_width * m_height * 4);
-        check(kpt_readback(m_ctx, 0, out.data(), (int64_t)(out.size() * sizeof(float))), "kpt_readback");
-        return out;
-    }
+    // replaces getRenderTargetTexture(GLuint) (renderer.cu:197-200): the
+    // post-processed output (non-linear sRGB, RGBA32F), row-major,
+    // width*height*4 floats
+    std::vector<float> readOutput() const { return read(4); }
+    // the modulated linear render buffer (render_texture)
+    std::vector<float> readRender() const { return read(0); }
     // replaces getDebugRenderTargetTexture (renderer.cu:202-205): the V-buffer
-    std::vector<float> readDebug() const {
-        std::vector<float> out((size_t)m_width * m_height * 4);
-        check(kpt_readback(m_ctx, 3, out.data(), (int64_t)(out.size() * sizeof(float))), "kpt_readback");
-        return out;
-    }
+    std::vector<float> readDebug() const { return read(3); }
 
     size_t getMaterialsCount() const { return m_materials.size(); }
     size_t getMeshCount() const { return m_models.size(); }
@@ -188,14 +187,26 @@ public:
         check(kpt_set_view(m_ctx, p.data(), v.data()), "kpt_set_view");
     }
 
-    // renderer.cu:312-334: stored; consumed by the post chain
+    // renderer.cu:312-329
     void setExposure(const ExposureValues& v, float ev_comp, float white_point_ev, float black_point_ev) {
-        m_exposure = v;
-        m_ev_comp = ev_comp;
-        m_white = white_point_ev;
-        m_black = black_point_ev;
+        kpt_exposure e{v.aperture_f_num, v.ISO, v.shutter_speed_secs, v.MAX_ISO, v.MIN_ISO,
+                       v.MIN_SHUTTER_SECS, v.MAX_SHUTTER_SECS, ev_comp, white_point_ev, black_point_ev};
+        check(kpt_set_exposure(m_ctx, &e), "kpt_set_exposure");
     }
-    ExposureValues getExposure() const { return m_exposure; }
+    // renderer.cu:331-334: the auto-exposure program's current values
+    ExposureValues getExposure() const {
+        kpt_exposure e{};
+        check(kpt_get_exposure(m_ctx, &e), "kpt_get_exposure");
+        ExposureValues v;
+        v.aperture_f_num = e.aperture_f_num;
+        v.ISO = e.iso;
+        v.shutter_speed_secs = e.shutter_speed_secs;
+        v.MAX_ISO = e.max_iso;
+        v.MIN_ISO = e.min_iso;
+        v.MIN_SHUTTER_SECS = e.min_shutter_secs;
+        v.MAX_SHUTTER_SECS = e.max_shutter_secs;
+        return v;
+    }
 
     // renderer.cu:336-340
     void resetAccumulation() { check(kpt_reset_accumulation(m_ctx), "kpt_reset_accumulation"); }
@@ -205,6 +216,11 @@ public:
     void setSeedMode(int mode) { m_settings.seed_mode = mode; push(); }
 
 private:
+    std::vector<float> read(int which) const {
+        std::vector<float> out((size_t)m_width * m_height * 4);
+        check(kpt_readback(m_ctx, which, out.data(), (int64_t)(out.size() * sizeof(float))), "kpt_readback");
+        return out;
+    }
     template <class M>
     static mat4 toMat4(const M& src) {
         mat4 m;
@@ -251,8 +267,6 @@ private:
     kpt_settings m_settings{};
     RendererSettings m_rs;
     ProceduralEnvironmentSettings m_env;
-    ExposureValues m_exposure;
-    float m_ev_comp = 0.0f, m_white = 6.5f, m_black = -10.0f;
     uint32_t m_width = 0, m_height = 0;
     std::vector<MaterialSceneEntity> m_materials;
     std::vector<mat4> m_models;

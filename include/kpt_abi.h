@@ -154,6 +154,31 @@ typedef struct kpt_settings {
     int32_t reserved[3];
 } kpt_settings;
 
+/* Camera exposure as Renderer::setExposure takes it (ExposureValues,
+ * pod_types.hpp:9-33, plus the EV compensation and the film's white and
+ * black points, renderer.cu:312-329). */
+typedef struct kpt_exposure {
+    float aperture_f_num;
+    float iso;
+    float shutter_speed_secs;
+    int32_t max_iso;
+    int32_t min_iso;
+    float min_shutter_secs;
+    float max_shutter_secs;
+    float ev_comp;
+    float white_point_ev;
+    float black_point_ev;
+} kpt_exposure;
+
+/* Film state the post pass reads (Film, film.cuh:186-191) and the adapted
+ * scene luminance of the auto-exposure loop (histogram.cu:147-151). */
+typedef struct kpt_film_state {
+    float luminance_exposure_scalar;
+    float white_point_ev;
+    float black_point_ev;
+    float scene_average_luminance;
+} kpt_film_state;
+
 /* ---- context ---- */
 int kpt_get_abi_version(void);
 /* Renderer::initialize (renderer.cu:22-46). device < 0 picks the current one. */
@@ -239,6 +264,23 @@ int kpt_set_trace_counting(kpt_ctx* ctx, int32_t enable);
 int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n);
 int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n);
 int kpt_get_render_counters(const kpt_ctx* ctx, kpt_trace_counters* out);
+
+/* ---- post chain (executeRendering after the path trace, renderer.cu:117-186) ---- */
+/* Renderer::setExposure (renderer.cu:312-329): records the values for the
+ * auto-exposure program and sets the film's exposure scalar
+ * (getStandardOutputBasedExposure, autoexposure_program.hpp:38-47) and
+ * white/black points.  Does not reset accumulation. */
+int kpt_set_exposure(kpt_ctx* ctx, const kpt_exposure* e);
+/* Renderer::getExposure (renderer.cu:331-334): the auto-exposure program's
+ * current values (updated on the device every frame when auto exposure is
+ * on; this call synchronises the context stream). */
+int kpt_get_exposure(kpt_ctx* ctx, kpt_exposure* out);
+/* Film scalar, white/black points and the adapted scene luminance
+ * (synchronises the context stream). */
+int kpt_get_film(kpt_ctx* ctx, kpt_film_state* out);
+/* executeRendering's delta_time_ms (renderer.cu:103-105): the frame time the
+ * auto-exposure adaptation uses for the following kpt_render frames. */
+int kpt_set_frame_delta(kpt_ctx* ctx, float delta_time_ms);
 
 #ifdef __cplusplus
 }

@@ -130,6 +130,43 @@ class KptSettings(C.Structure):
     ]
 
 
+class KptExposure(C.Structure):
+    _fields_ = [
+        ("aperture_f_num", C.c_float),
+        ("iso", C.c_float),
+        ("shutter_speed_secs", C.c_float),
+        ("max_iso", C.c_int32),
+        ("min_iso", C.c_int32),
+        ("min_shutter_secs", C.c_float),
+        ("max_shutter_secs", C.c_float),
+        ("ev_comp", C.c_float),
+        ("white_point_ev", C.c_float),
+        ("black_point_ev", C.c_float),
+    ]
+
+    @classmethod
+    def camera(cls, aperture=2.8, iso=100, shutter=1.0 / 60.0, ev_comp=0.0, white=6.5, black=-10.0,
+               max_iso=6400, min_iso=100, max_shutter=0.0333, min_shutter=0.00025):
+        """ExposureValues(aperture, iso, shutter) with the reference's limits
+        (pod_types.hpp:11-33) plus the setExposure extras."""
+        return cls(aperture, float(iso), shutter, max_iso, min_iso, min_shutter, max_shutter, ev_comp, white, black)
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class KptFilmState(C.Structure):
+    _fields_ = [
+        ("luminance_exposure_scalar", C.c_float),
+        ("white_point_ev", C.c_float),
+        ("black_point_ev", C.c_float),
+        ("scene_average_luminance", C.c_float),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # dtype matching kpt_ray for zero-copy numpy arrays of rays
 RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("prim_id", "<i4"), ("inst_id", "<i4"), ("bary", "<f4", 3)])
@@ -174,6 +211,10 @@ SIGNATURES = [
     ("kpt_get_render_counters", C.c_int, [_P, C.POINTER(KptTraceCounters)]),
     ("kpt_set_tuning", C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32]),
     ("kpt_get_tuning", C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32]),
+    ("kpt_set_exposure", C.c_int, [_P, C.POINTER(KptExposure)]),
+    ("kpt_get_exposure", C.c_int, [_P, C.POINTER(KptExposure)]),
+    ("kpt_get_film", C.c_int, [_P, C.POINTER(KptFilmState)]),
+    ("kpt_set_frame_delta", C.c_int, [_P, C.c_float]),
 ]
 
 

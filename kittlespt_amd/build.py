@@ -17,6 +17,7 @@ OUT = os.path.join(HERE, "_lib", "libkpt_hip.so")
 SOURCES = [
     os.path.join(CSRC, "hip", "kpt_kernels.hip"),
     os.path.join(CSRC, "hip", "kpt_wavefront.hip"),
+    os.path.join(CSRC, "hip", "kpt_post.hip"),
     os.path.join(CSRC, "hip", "kpt_abi.hip"),
     os.path.join(CSRC, "host", "kpt_build.cpp"),
 ]

@@ -225,3 +225,28 @@ class Context:
         c = abi.KptTraceCounters()
         self._ck(self.L.kpt_get_render_counters(self.h, C.byref(c)))
         return c.as_dict()
+
+    # ---- post chain
+    def set_exposure(self, exposure):
+        """Renderer::setExposure: a KptExposure (abi.KptExposure.camera(...))."""
+        self._ck(self.L.kpt_set_exposure(self.h, C.byref(exposure)))
+
+    def exposure(self):
+        e = abi.KptExposure()
+        self._ck(self.L.kpt_get_exposure(self.h, C.byref(e)))
+        return e
+
+    def film(self):
+        f = abi.KptFilmState()
+        self._ck(self.L.kpt_get_film(self.h, C.byref(f)))
+        return f
+
+    def set_frame_delta(self, ms):
+        self._ck(self.L.kpt_set_frame_delta(self.h, float(ms)))
+
+    def estimate(self):
+        """Demodulated radiance estimate of static accumulation: the running
+        sum divided by the frame count, as the trace kernels compute it before
+        the post chain modulates the render buffer in place."""
+        acc = self.readback(1)[..., :3]
+        return acc / np.float32(max(1, self.frame_index()))

@@ -1,6 +1,7 @@
 // kpt_abi.hip — implementation of the C-ABI in include/kpt_abi.h: context,
 // scene upload (host builders + HBM layout), render orchestration.
 #include <algorithm>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -74,6 +75,11 @@ struct kpt_ctx {
     unsigned long long* d_counts = nullptr;
     kpt_trace_counters render_counts{};
     double timing[4] = {0, 0, 0, 0};
+    // post chain
+    PostBuffers pb{};
+    ExposureState h_expo{};
+    float frame_delta_ms = 0.0f;
+    int num_cus = 256;
 };
 
 namespace {
@@ -203,6 +209,23 @@ FrameConst make_frame(const kpt_ctx* c, int32_t /*frame*/) {
 void free_frame(kpt_ctx* c) {
     dev_free(&c->fb.accum); dev_free(&c->fb.render); dev_free(&c->fb.gbuffer);
     dev_free(&c->fb.prev_gbuffer); dev_free(&c->fb.vbuffer); dev_free(&c->fb.output);
+    for (int k = 0; k < MAX_MIPS; ++k) dev_free(&c->pb.mip[k]);
+    c->pb.levels = 0;
+}
+
+// Renderer state before any setExposure call: AutoExposureProgram members
+// (autoexposure_program.hpp:75-78, ExposureValues defaults pod_types.hpp:26-32)
+// and the Film defaults (film.cuh:188-190).
+void default_exposure(ExposureState* e) {
+    std::memset(e, 0, sizeof(*e));
+    e->ae.max_iso = 6400;
+    e->ae.min_iso = 100;
+    e->ae.min_shutter_secs = 0.00025f;
+    e->ae.max_shutter_secs = 0.0333f;
+    e->scalar = 1.0f;
+    e->white_ev = 6.5f;
+    e->black_ev = -10.0f;
+    e->avg_lum = 0.0f;
 }
 
 void free_wavefront(kpt_ctx* c) {
@@ -248,6 +271,7 @@ int kpt_ctx_create(int device, kpt_ctx** out) {
     *out = nullptr;
     kpt_ctx* ctx = new kpt_ctx();
     default_settings(&ctx->settings);
+    default_exposure(&ctx->h_expo);
     hm::Mat4 I = hm::identity();
     std::memcpy(ctx->cam.inv_view, I.m, 64); std::memcpy(ctx->cam.inv_proj, I.m, 64);
     std::memcpy(ctx->cam.prev_inv_view, I.m, 64); std::memcpy(ctx->cam.prev_inv_proj, I.m, 64);
@@ -269,7 +293,17 @@ int kpt_ctx_create(int device, kpt_ctx** out) {
         hipMemset(ctx->d_overflow, 0, sizeof(int32_t)) != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||
         hipEventCreate(&ctx->ev1) != hipSuccess || hipEventCreate(&ctx->tev0) != hipSuccess ||
         hipEventCreate(&ctx->tev1) != hipSuccess ||
-        hipMalloc((void**)&ctx->d_counts, 8 * sizeof(unsigned long long)) != hipSuccess) {
+        hipMalloc((void**)&ctx->d_counts, 8 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc((void**)&ctx->pb.expo, sizeof(ExposureState)) != hipSuccess ||
+        hipMalloc((void**)&ctx->pb.hist, 256 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(ctx->pb.hist, 0, 256 * sizeof(unsigned long long)) != hipSuccess) {
+        delete ctx;
+        return KPT_ERR_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        ctx->num_cus = prop.multiProcessorCount;
+    if (hipMemcpy(ctx->pb.expo, &ctx->h_expo, sizeof(ExposureState), hipMemcpyHostToDevice) != hipSuccess) {
         delete ctx;
         return KPT_ERR_HIP;
     }
@@ -302,6 +336,8 @@ int kpt_ctx_destroy(kpt_ctx* ctx) {
             if (ctx->ins.end[k]) (void)hipEventDestroy(ctx->ins.end[k]);
         }
         dev_free(&ctx->d_counts);
+        dev_free(&ctx->pb.expo);
+        dev_free(&ctx->pb.hist);
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     }
     delete ctx;
@@ -659,6 +695,13 @@ int kpt_resize(kpt_ctx* ctx, int32_t width, int32_t height) {
     HIPCHK(hipMalloc((void**)&ctx->fb.prev_gbuffer, n * sizeof(uint4)));
     HIPCHK(hipMalloc((void**)&ctx->fb.vbuffer, n * sizeof(float4)));
     HIPCHK(hipMalloc((void**)&ctx->fb.output, n * sizeof(float4)));
+    // bloom mip chain at output resolution (MipChain::resize, mipchain.cuh:17-35)
+    ctx->pb.levels = bloom_levels(width, height);
+    for (int k = 0; k <= ctx->pb.levels; ++k) {
+        const size_t mn = (size_t)(width >> k) * (size_t)(height >> k);
+        HIPCHK(hipMalloc((void**)&ctx->pb.mip[k], mn * sizeof(float4)));
+        HIPCHK(launch_clear(ctx->stream, ctx->pb.mip[k], (int64_t)mn));
+    }
     ctx->width = width;
     ctx->height = height;
     HIPCHK(launch_clear(ctx->stream, (float4*)ctx->fb.prev_gbuffer, (int64_t)n));
@@ -713,6 +756,18 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
             HIPCHK(launch_megakernel(ctx->stream, S, fc, ctx->frame_index, ctx->fb, ctx->d_overflow));
             trace_launches += 1;
         }
+        // modulate, auto exposure, bloom, post (renderer.cu:117-170)
+        PostParams pp{};
+        pp.width = ctx->width;
+        pp.height = ctx->height;
+        pp.temporal = ctx->settings.use_temporal_accumulation ? 1 : 0;
+        pp.auto_exposure = ctx->settings.enable_auto_exposure ? 1 : 0;
+        pp.bloom = ctx->settings.bloom_generate_bloom ? 1 : 0;
+        pp.karis = ctx->settings.bloom_use_karis_average ? 1 : 0;
+        pp.bloom_final_blend = ctx->settings.bloom_final_blend;
+        pp.bloom_internal_blend = ctx->settings.bloom_internal_blend;
+        pp.frame_delta_ms = ctx->frame_delta_ms;
+        HIPCHK(launch_post_frame(ctx->stream, ctx->fb, ctx->pb, pp, ctx->num_cus));
         // end of executeRendering: gbuffer -> prev_gbuffer, camera prev <- curr,
         // frame_index++ (renderer.cu:186-194)
         HIPCHK(hipMemcpyAsync(ctx->fb.prev_gbuffer, ctx->fb.gbuffer, (size_t)npx * sizeof(uint4),
@@ -741,6 +796,55 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
     int32_t ovf = 0;
     HIPCHK(hipMemcpy(&ovf, ctx->d_overflow, sizeof(int32_t), hipMemcpyDeviceToHost));
     if (ovf) return fail(ctx, KPT_ERR_OVERFLOW, "traversal stack overflow (depth > 128)");
+    return KPT_OK;
+}
+
+int kpt_set_exposure(kpt_ctx* ctx, const kpt_exposure* e) {
+    if (!ctx || !e) return KPT_ERR_INVALID_ARG;
+    // AutoExposureProgram::recordValues + getStandardOutputBasedExposure + Film
+    ctx->h_expo.ae = *e;
+    const float q = 0.65f;
+    const float l_avg = (10.0f / q) * ((e->aperture_f_num * e->aperture_f_num) / (e->iso * e->shutter_speed_secs));
+    ctx->h_expo.scalar = 0.18f / l_avg;
+    ctx->h_expo.white_ev = e->white_point_ev;
+    ctx->h_expo.black_ev = e->black_point_ev;
+    if (ctx->host_only) return KPT_OK;
+    // the adapted luminance stays as the device left it
+    HIPCHK(hipMemcpyAsync(ctx->pb.expo, &ctx->h_expo, offsetof(ExposureState, avg_lum), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return KPT_OK;
+}
+
+static int read_exposure(kpt_ctx* ctx) {
+    if (ctx->host_only) return KPT_OK;
+    HIPCHK(hipMemcpyAsync(&ctx->h_expo, ctx->pb.expo, sizeof(ExposureState), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return KPT_OK;
+}
+
+int kpt_get_exposure(kpt_ctx* ctx, kpt_exposure* out) {
+    if (!ctx || !out) return KPT_ERR_INVALID_ARG;
+    int rc = read_exposure(ctx);
+    if (rc != KPT_OK) return rc;
+    *out = ctx->h_expo.ae;
+    return KPT_OK;
+}
+
+int kpt_get_film(kpt_ctx* ctx, kpt_film_state* out) {
+    if (!ctx || !out) return KPT_ERR_INVALID_ARG;
+    int rc = read_exposure(ctx);
+    if (rc != KPT_OK) return rc;
+    out->luminance_exposure_scalar = ctx->h_expo.scalar;
+    out->white_point_ev = ctx->h_expo.white_ev;
+    out->black_point_ev = ctx->h_expo.black_ev;
+    out->scene_average_luminance = ctx->h_expo.avg_lum;
+    return KPT_OK;
+}
+
+int kpt_set_frame_delta(kpt_ctx* ctx, float delta_time_ms) {
+    if (!ctx || !(delta_time_ms >= 0.0f)) return KPT_ERR_INVALID_ARG;
+    ctx->frame_delta_ms = delta_time_ms;
     return KPT_OK;
 }
 

@@ -47,6 +47,40 @@ struct WavefrontState {
     int32_t* hist;        // material histogram for the shading sort
 };
 
+// Exposure state resident in device memory: the auto-exposure program's
+// values, the film the post pass reads and the adapted scene luminance.  The
+// histogram kernels update it every frame without a host round trip.
+struct ExposureState {
+    kpt_exposure ae;          // AutoExposureProgram::m_exposure_values + ev comp, white/black
+    float scalar;             // Film::luminance_exposure_scalar
+    float white_ev;           // Film::white_point_ev
+    float black_ev;           // Film::black_point_ev
+    float avg_lum;            // ShaderData::scene_average_luminance
+};
+constexpr int MAX_MIPS = 7;   // MipChain::max_mip_count
+
+// Buffers of the post chain (bloom mips at output resolution, histogram bins).
+struct PostBuffers {
+    float4* mip[MAX_MIPS];
+    int32_t levels;                 // mips beyond mip 0
+    unsigned long long* hist;       // 256 fixed-point bins (weight * 2^23)
+    ExposureState* expo;
+};
+struct PostParams {
+    int32_t width, height;
+    int32_t temporal;               // copy render -> accumulation before modulation
+    int32_t auto_exposure;
+    int32_t bloom, karis;
+    float bloom_final_blend, bloom_internal_blend;
+    float frame_delta_ms;
+};
+// executeRendering after the path trace (renderer.cu:117-170): modulate,
+// auto exposure, bloom, post.  Everything stays on the stream.
+hipError_t launch_post_frame(hipStream_t s, const FrameBuffers& fb, const PostBuffers& pb, const PostParams& p,
+                             int num_cus);
+// Mip levels of the bloom chain (mipchain.cuh:17-25, 44-50).
+int bloom_levels(int width, int height);
+
 hipError_t launch_trace_closest(hipStream_t s, const SceneView& S, const kpt_ray* rays, kpt_hit* hits, int64_t n,
                                 int32_t* overflow, kpt_trace_counters* counters);
 hipError_t launch_trace_any(hipStream_t s, const SceneView& S, const kpt_ray* rays, int32_t* occluded, int64_t n,

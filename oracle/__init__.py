@@ -223,3 +223,138 @@ def camera_rays(cam, width, height, pixels):
                                   C.c_void_p, C.c_void_p]
     L.orc_camera_rays(C.byref(cam), width, height, _ptr(px), len(px), _ptr(o), _ptr(d))
     return o, d
+
+
+# ---------------------------------------------------------------- post chain
+def _sig(name, res, args):
+    f = getattr(lib(), name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+def post_frame(estimate, gbuffer, settings, exposure, film, frame_delta_ms=16.0):
+    """One frame of the post chain (modulate, auto exposure, bloom, post) over
+    the demodulated estimate [H,W,4] and gbuffer [H,W,4] u32.  exposure and
+    film (KptExposure / KptFilmState) are updated in place.  Returns
+    (render, output, bloom_mip0, histogram)."""
+    H, W = estimate.shape[:2]
+    est = _f32(estimate)
+    gb = np.ascontiguousarray(gbuffer, np.uint32)
+    render = np.zeros((H, W, 4), np.float32)
+    out = np.zeros((H, W, 4), np.float32)
+    bloom = np.zeros((H, W, 4), np.float32)
+    hist = np.zeros(256, np.uint64)
+    f = _sig("orc_post_frame", None, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(abi.KptSettings),
+                                      C.c_float, C.POINTER(abi.KptExposure), C.POINTER(abi.KptFilmState)] +
+             [C.c_void_p] * 4)
+    f(_ptr(est), _ptr(gb), W, H, C.byref(settings), frame_delta_ms, C.byref(exposure), C.byref(film),
+      _ptr(render), _ptr(out), _ptr(bloom), _ptr(hist))
+    return render, out, bloom, hist
+
+
+def set_exposure(exposure):
+    film = abi.KptFilmState()
+    _sig("orc_set_exposure", None, [C.POINTER(abi.KptExposure), C.POINTER(abi.KptFilmState)])(
+        C.byref(exposure), C.byref(film))
+    return film
+
+
+def exposure_program(exposure, avg_lum):
+    _sig("orc_exposure_program", None, [C.POINTER(abi.KptExposure), C.c_float])(C.byref(exposure), avg_lum)
+    return exposure
+
+
+def lum_to_bin(lum, black_ev=-10.0, white_ev=6.5):
+    x = _f32(lum)
+    out = np.zeros(len(x), np.uint32)
+    _sig("orc_lum_to_bin", None, [C.c_void_p, C.c_int64, C.c_float, C.c_float, C.c_void_p])(
+        _ptr(x), len(x), black_ev, white_ev, _ptr(out))
+    return out
+
+
+def meter_weight(width, height, xy):
+    xy = np.ascontiguousarray(xy, np.int32)
+    out = np.zeros(len(xy), np.float32)
+    _sig("orc_meter_weight", None, [C.c_int32, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p])(
+        width, height, _ptr(xy), len(xy), _ptr(out))
+    return out
+
+
+def histogram(rgba, white_ev=6.5, black_ev=-10.0):
+    img = _f32(rgba)
+    H, W = img.shape[:2]
+    bins = np.zeros(256, np.uint64)
+    _sig("orc_histogram", None, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_void_p])(
+        _ptr(img), W, H, white_ev, black_ev, _ptr(bins))
+    return bins
+
+
+def histogram_average(bins, white_ev=6.5, black_ev=-10.0, last=0.0, dt_ms=16.0):
+    b = np.ascontiguousarray(bins, np.uint64)
+    return _sig("orc_histogram_average", C.c_float, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float])(
+        _ptr(b), white_ev, black_ev, last, dt_ms)
+
+
+def film(rgb, scalar=1.0, white_ev=6.5, black_ev=-10.0):
+    x = _f32(rgb).reshape(-1, 3)
+    out = np.zeros_like(x)
+    _sig("orc_film", None, [C.c_void_p, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_void_p])(
+        _ptr(x), len(x), scalar, white_ev, black_ev, _ptr(out))
+    return out
+
+
+def agx_curve(x):
+    x = _f32(x)
+    out = np.zeros_like(x)
+    _sig("orc_agx_curve", None, [C.c_void_p, C.c_int64, C.c_void_p])(_ptr(x), x.size, _ptr(out))
+    return out
+
+
+def srgb_encode(x):
+    x = _f32(x)
+    out = np.zeros_like(x)
+    _sig("orc_srgb_encode", None, [C.c_void_p, C.c_int64, C.c_void_p])(_ptr(x), x.size, _ptr(out))
+    return out
+
+
+def yxy_roundtrip(rgb, scalar=1.0):
+    x = _f32(rgb).reshape(-1, 3)
+    yxy = np.zeros_like(x)
+    back = np.zeros_like(x)
+    _sig("orc_yxy_roundtrip", None, [C.c_void_p, C.c_int64, C.c_float, C.c_void_p, C.c_void_p])(
+        _ptr(x), len(x), scalar, _ptr(yxy), _ptr(back))
+    return yxy, back
+
+
+def bilinear(rgba, xy, filter_alpha=True):
+    img = _f32(rgba)
+    H, W = img.shape[:2]
+    xy = _f32(xy).reshape(-1, 2)
+    out = np.zeros((len(xy), 4), np.float32)
+    _sig("orc_bilinear", None, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p])(
+        _ptr(img), W, H, _ptr(xy), len(xy), 1 if filter_alpha else 0, _ptr(out))
+    return out
+
+
+def bloom_down(src, karis=False):
+    s = _f32(src)
+    sh, sw = s.shape[:2]
+    dst = np.zeros((sh >> 1, sw >> 1, 4), np.float32)
+    _sig("orc_bloom_down", None, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_int32])(
+        _ptr(s), sw, sh, _ptr(dst), sw >> 1, sh >> 1, 1 if karis else 0)
+    return dst
+
+
+def bloom_up(src, dst, blend=0.65):
+    s = _f32(src)
+    d = np.array(dst, np.float32, copy=True, order="C")
+    sh, sw = s.shape[:2]
+    dh, dw = d.shape[:2]
+    _sig("orc_bloom_up", None, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_float])(
+        _ptr(s), sw, sh, _ptr(d), dw, dh, blend)
+    return d
+
+
+def bloom_levels(width, height):
+    return int(_sig("orc_bloom_levels", C.c_int32, [C.c_int32, C.c_int32])(width, height))

@@ -81,6 +81,33 @@ uint32_t orc_pcg_hash(uint32_t v);
 void orc_glm_inverse(const float m[16], float out[16]);
 void orc_mat4_inverse(const float m[16], float out[16]);
 
+/* ---- post chain (kpt_oracle_post.cpp) ---- */
+/* One frame of executeRendering after the path trace (renderer.cu:122-170):
+ * estimate [W*H][4] (demodulated radiance) and gbuffer [W*H][4] in; modulated
+ * render, post-processed output and (bloom on) bloom mip 0 out, each
+ * [W*H][4] or NULL; exposure and film are updated in place when auto
+ * exposure is on; out_hist (optional) receives the 256 fixed-point bins. */
+void orc_post_frame(const float* estimate, const uint32_t* gbuffer, int32_t width, int32_t height,
+                    const kpt_settings* settings, float frame_delta_ms, kpt_exposure* exposure,
+                    kpt_film_state* film, float* out_render, float* out_output, float* out_bloom,
+                    uint64_t* out_hist);
+void orc_set_exposure(const kpt_exposure* e, kpt_film_state* film);
+void orc_exposure_program(kpt_exposure* e, float avg_lum);
+void orc_lum_to_bin(const float* lum, int64_t n, float black_ev, float white_ev, uint32_t* out);
+void orc_meter_weight(int32_t width, int32_t height, const int32_t* xy, int64_t n, float* out);
+void orc_histogram(const float* rgba, int32_t width, int32_t height, float white_ev, float black_ev,
+                   uint64_t* bins);
+float orc_histogram_average(const uint64_t* bins, float white_ev, float black_ev, float last, float dt_ms);
+void orc_film(const float* rgb, int64_t n, float scalar, float white_ev, float black_ev, float* out);
+void orc_agx_curve(const float* x, int64_t n, float* out);
+void orc_srgb_encode(const float* x, int64_t n, float* out);
+void orc_yxy_roundtrip(const float* rgb, int64_t n, float scalar, float* out_yxy, float* out_rgb);
+void orc_bilinear(const float* rgba, int32_t width, int32_t height, const float* xy, int64_t n,
+                  int32_t filter_alpha, float* out);
+void orc_bloom_down(const float* src, int32_t sw, int32_t sh, float* dst, int32_t dw, int32_t dh, int32_t karis);
+void orc_bloom_up(const float* src, int32_t sw, int32_t sh, float* dst, int32_t dw, int32_t dh, float blend);
+int32_t orc_bloom_levels(int32_t width, int32_t height);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,25 @@ def test_host_only_context_errors():
     c.close()
 
 
+def test_host_only_set_exposure_matches_oracle():
+    """Renderer::setExposure's film scalar (host side of the C-ABI) equals the
+    oracle's getStandardOutputBasedExposure, bit for bit; defaults match the
+    reference's Film before any setExposure."""
+    c = Context(-2)
+    f0 = c.film()
+    assert (f0.luminance_exposure_scalar, f0.white_point_ev, f0.black_point_ev) == (1.0, 6.5, -10.0)
+    for e in (abi.KptExposure.camera(), abi.KptExposure.camera(4.0, 800, 1 / 250, 1.0, 5.0, -8.0)):
+        c.set_exposure(e)
+        got = c.film()
+        want = oracle.set_exposure(e)
+        assert got.luminance_exposure_scalar == want.luminance_exposure_scalar
+        assert (got.white_point_ev, got.black_point_ev) == (want.white_point_ev, want.black_point_ev)
+        assert c.exposure().as_dict() == e.as_dict()
+    with pytest.raises(abi.KptError):
+        c.set_frame_delta(-1.0)
+    c.close()
+
+
 def _compare_trees(sc):
     ctx = Context(-2)
     ctx.upload(sc)

@@ -92,8 +92,9 @@ def test_cpp_renderer_matches_binding(tmp_path, which):
         ctx.resize(W, H)
         p, v = view_proj(sc, W, H)
         ctx.set_view(p, v)
+        ctx.set_frame_delta(16.0)
         ctx.render(spp)
-        img, vb = ctx.readback(0), ctx.readback(3)
+        img, vb = ctx.readback(4), ctx.readback(3)
     finally:
         ctx.close()
     assert np.array_equal(got[0].view(np.uint32), img.view(np.uint32))

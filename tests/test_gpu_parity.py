@@ -113,7 +113,7 @@ def _render_both(sc, ctx, orc, W, H, spp, integrator, max_depth=4, seed_mode=abi
     ctx.set_view(p, v)
     cam = ctx.camera()
     ctx.render(spp)
-    img = ctx.readback(0)[..., :3].reshape(-1, 3)
+    img = ctx.estimate().reshape(-1, 3)
     gbuf = ctx.readback(2).reshape(-1, 4)
     vbuf = ctx.readback(3).reshape(-1, 4)
     if spp > 1:
