@@ -193,9 +193,15 @@ const char* kpt_last_error(const kpt_ctx* ctx);
  * 250-269): BLAS binned-SAH and TLAS agglomerative builds (as_builder.cuh),
  * upload of nodes, triangles, instances, materials and textures to HBM. */
 int kpt_upload_scene(kpt_ctx* ctx, const kpt_scene_desc* scene);
-/* Renderer::setMeshTransform (renderer.cu:250-269) + TLAS rebuild
- * (renderer.cu:115): updates one instance and rebuilds the TLAS on the host. */
+/* Renderer::setMeshTransform (renderer.cu:250-269): updates one instance
+ * (previous inverse kept for motion vectors); the TLAS is rebuilt on the host
+ * before the next frame or trace (updateTLAS, renderer.cu:115). */
 int kpt_set_instance_transform(kpt_ctx* ctx, int32_t instance, const float model[16]);
+/* setMeshTransform for instances [first, first+count) in one call (models:
+ * count x 16 floats, column-major); the TLAS is rebuilt once, before the next
+ * frame or trace, as updateTLAS does every frame (renderer.cu:115).  No
+ * reference counterpart for the batching itself. */
+int kpt_set_instance_transforms(kpt_ctx* ctx, int32_t first, int32_t count, const float* models);
 /* Renderer::setMaterial (renderer.cu:207-226). */
 int kpt_set_material(kpt_ctx* ctx, int32_t index, const kpt_material* m);
 /* Scene statistics: [0]=triangles [1]=blas nodes [2]=tlas nodes [3]=instances

@@ -188,6 +188,7 @@ SIGNATURES = [
     ("kpt_last_error", C.c_char_p, [_P]),
     ("kpt_upload_scene", C.c_int, [_P, C.POINTER(KptSceneDesc)]),
     ("kpt_set_instance_transform", C.c_int, [_P, C.c_int32, C.POINTER(C.c_float)]),
+    ("kpt_set_instance_transforms", C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_float)]),
     ("kpt_set_material", C.c_int, [_P, C.c_int32, C.POINTER(KptMaterial)]),
     ("kpt_scene_info", C.c_int, [_P, C.POINTER(C.c_int64)]),
     ("kpt_get_blas", C.c_int, [_P, C.c_int32, _P, C.c_int64, C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int64)]),

@@ -136,6 +136,10 @@ class Context:
         m = np.ascontiguousarray(model16, np.float32)
         self._ck(self.L.kpt_set_instance_transform(self.h, i, abi.fptr(m)))
 
+    def set_instance_transforms(self, first, models):
+        m = np.ascontiguousarray(models, np.float32).reshape(-1, 16)
+        self._ck(self.L.kpt_set_instance_transforms(self.h, first, len(m), abi.fptr(m)))
+
     # ---- traversal (host arrays in, host arrays out)
     def trace_closest(self, rays, counted=False):
         rays = np.ascontiguousarray(rays, abi.RAY_DTYPE)

@@ -128,6 +128,8 @@ struct alignas(16) FrameConst {
     int32_t num_lights;
     int32_t num_instances;
     int32_t tlas_root;
+    int32_t temporal;      // integrator_use_temporal_accumulation
+    float prev_cam_pos[3]; // prev_inv_view column 3 (testReprojection, kernels.cu:215)
 };
 
 // Scene pointers passed by value to kernels.

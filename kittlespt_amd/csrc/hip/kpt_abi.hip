@@ -75,6 +75,7 @@ struct kpt_ctx {
     unsigned long long* d_counts = nullptr;
     kpt_trace_counters render_counts{};
     double timing[4] = {0, 0, 0, 0};
+    bool tlas_dirty = false;
     // post chain
     PostBuffers pb{};
     ExposureState h_expo{};
@@ -161,13 +162,27 @@ void make_instance(kpt_ctx* c, int32_t i, const float* model, bool keep_prev) {
 
 int upload_tlas(kpt_ctx* ctx) {
     ctx->tlas = build_tlas(ctx->inst_bounds);
+    ctx->tlas_dirty = false;
     if (ctx->tlas.nodes.size() > 65535) return fail(ctx, KPT_ERR_UNSUPPORTED, "TLAS exceeds 65535 nodes (16-bit child fields)");
     if (ctx->host_only) return KPT_OK;
+    // the previous frame may still read the old arrays
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(dev_upload(&ctx->d_tlas, ctx->tlas.nodes));
     HIPCHK(dev_upload(&ctx->d_itrav, ctx->itrav));
     HIPCHK(dev_upload(&ctx->d_ishade, ctx->ishade));
     return KPT_OK;
 }
+
+// Rebuilds the TLAS after instance edits, before the next use.
+int ensure_tlas(kpt_ctx* ctx) {
+    if (!ctx->tlas_dirty) return KPT_OK;
+    return upload_tlas(ctx);
+}
+#define ENSURE_TLAS(c)                      \
+    do {                                    \
+        int rc_ = ensure_tlas(c);           \
+        if (rc_ != KPT_OK) return rc_;      \
+    } while (0)
 
 FrameConst make_frame(const kpt_ctx* c, int32_t /*frame*/) {
     FrameConst fc;
@@ -181,6 +196,10 @@ FrameConst make_frame(const kpt_ctx* c, int32_t /*frame*/) {
     fc.cam_pos[0] = c->cam.inv_view[12];
     fc.cam_pos[1] = c->cam.inv_view[13];
     fc.cam_pos[2] = c->cam.inv_view[14];
+    fc.prev_cam_pos[0] = c->cam.prev_inv_view[12];
+    fc.prev_cam_pos[1] = c->cam.prev_inv_view[13];
+    fc.prev_cam_pos[2] = c->cam.prev_inv_view[14];
+    fc.temporal = c->settings.use_temporal_accumulation ? 1 : 0;
     const kpt_settings& s = c->settings;
     // getSunDirection (integrator.cuh:222-230)
     float phi = s.sun_phi_rad, th = s.sun_theta_rad;
@@ -528,19 +547,29 @@ int kpt_upload_scene(kpt_ctx* ctx, const kpt_scene_desc* d) {
     return KPT_OK;
 }
 
-int kpt_set_instance_transform(kpt_ctx* ctx, int32_t instance, const float model[16]) {
-    if (!ctx || !model) return KPT_ERR_INVALID_ARG;
+int kpt_set_instance_transforms(kpt_ctx* ctx, int32_t first, int32_t count, const float* models) {
+    if (!ctx || !models || count < 0) return KPT_ERR_INVALID_ARG;
     if (!ctx->scene_ready) return fail(ctx, KPT_ERR_STATE, "no scene");
-    if (instance < 0 || instance >= (int32_t)ctx->itrav.size()) return fail(ctx, KPT_ERR_INVALID_ARG, "instance out of range");
-    std::memcpy(&ctx->inst_model[16 * (size_t)instance], model, 64);
-    make_instance(ctx, instance, model, true);
-    // setMeshTransform resets accumulation unless temporal (renderer.cu:264-266)
-    if (!ctx->settings.use_temporal_accumulation) ctx->frame_index = 0;
-    if (!ctx->host_only && ctx->fb.accum && !ctx->settings.use_temporal_accumulation) {
-        HIPCHK(launch_clear(ctx->stream, ctx->fb.accum, (int64_t)ctx->width * ctx->height));
+    if (first < 0 || (int64_t)first + count > (int64_t)ctx->itrav.size())
+        return fail(ctx, KPT_ERR_INVALID_ARG, "instance out of range");
+    for (int32_t k = 0; k < count; ++k) {
+        const float* m = models + 16 * (size_t)k;
+        std::memcpy(&ctx->inst_model[16 * (size_t)(first + k)], m, 64);
+        make_instance(ctx, first + k, m, true);
     }
-    if (!ctx->host_only) HIPCHK(hipStreamSynchronize(ctx->stream));
-    return upload_tlas(ctx);
+    // setMeshTransform resets accumulation unless temporal (renderer.cu:264-266)
+    if (!ctx->settings.use_temporal_accumulation) {
+        ctx->frame_index = 0;
+        if (!ctx->host_only && ctx->fb.accum)
+            HIPCHK(launch_clear(ctx->stream, ctx->fb.accum, (int64_t)ctx->width * ctx->height));
+    }
+    // the TLAS is rebuilt before the next frame or trace (updateTLAS, renderer.cu:115)
+    ctx->tlas_dirty = true;
+    return KPT_OK;
+}
+
+int kpt_set_instance_transform(kpt_ctx* ctx, int32_t instance, const float model[16]) {
+    return kpt_set_instance_transforms(ctx, instance, 1, model);
 }
 
 int kpt_set_material(kpt_ctx* ctx, int32_t index, const kpt_material* s) {
@@ -598,6 +627,10 @@ int kpt_get_blas(const kpt_ctx* ctx, int32_t mesh, void* nodes32, int64_t cap_no
 
 int kpt_get_tlas(const kpt_ctx* ctx, void* nodes32, int64_t cap_nodes, int64_t* out_count) {
     if (!ctx || !nodes32 || !out_count) return KPT_ERR_INVALID_ARG;
+    if (ctx->tlas_dirty) {
+        int rc = ensure_tlas(const_cast<kpt_ctx*>(ctx));
+        if (rc != KPT_OK) return rc;
+    }
     if ((int64_t)ctx->tlas.nodes.size() > cap_nodes) return KPT_ERR_INVALID_ARG;
     struct RefT { float pmin[3]; float pmax[3]; uint32_t lr; int32_t blas; };
     RefT* out = (RefT*)nodes32;
@@ -615,6 +648,7 @@ int kpt_trace_closest(kpt_ctx* ctx, const kpt_ray* rays, kpt_hit* hits, int64_t 
     if (!ctx || (n > 0 && (!rays || !hits)) || n < 0) return KPT_ERR_INVALID_ARG;
     if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
     if (!ctx->scene_ready) return fail(ctx, KPT_ERR_STATE, "no scene");
+    ENSURE_TLAS(ctx);
     HIPCHK(launch_trace_closest(ctx->stream, scene_view(ctx), rays, hits, n, ctx->d_overflow, nullptr));
     return KPT_OK;
 }
@@ -623,6 +657,7 @@ int kpt_trace_any(kpt_ctx* ctx, const kpt_ray* rays, int32_t* occluded, int64_t 
     if (!ctx || (n > 0 && (!rays || !occluded)) || n < 0) return KPT_ERR_INVALID_ARG;
     if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
     if (!ctx->scene_ready) return fail(ctx, KPT_ERR_STATE, "no scene");
+    ENSURE_TLAS(ctx);
     HIPCHK(launch_trace_any(ctx->stream, scene_view(ctx), rays, occluded, n, ctx->d_overflow));
     return KPT_OK;
 }
@@ -632,6 +667,7 @@ int kpt_trace_closest_counted(kpt_ctx* ctx, const kpt_ray* rays, kpt_hit* hits, 
     if (!ctx || !host_counters || n < 0 || (n > 0 && (!rays || !hits))) return KPT_ERR_INVALID_ARG;
     if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
     if (!ctx->scene_ready) return fail(ctx, KPT_ERR_STATE, "no scene");
+    ENSURE_TLAS(ctx);
     kpt_trace_counters* d = nullptr;
     HIPCHK(hipMalloc((void**)&d, sizeof(kpt_trace_counters)));
     hipError_t e = hipMemsetAsync(d, 0, sizeof(kpt_trace_counters), ctx->stream);
@@ -725,7 +761,7 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
     if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
     if (!ctx->scene_ready) return fail(ctx, KPT_ERR_STATE, "no scene");
     if (!ctx->fb.accum) return fail(ctx, KPT_ERR_STATE, "no frame buffers (kpt_resize first)");
-    if (ctx->settings.use_temporal_accumulation) return fail(ctx, KPT_ERR_UNSUPPORTED, "temporal accumulation not built");
+    ENSURE_TLAS(ctx);
     SceneView S = scene_view(ctx);
     const int64_t npx = (int64_t)ctx->width * ctx->height;
     if (ctx->settings.integrator == KPT_INTEGRATOR_WAVEFRONT) {

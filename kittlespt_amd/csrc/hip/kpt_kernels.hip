@@ -2,6 +2,7 @@
 // with their host launch wrappers.
 #include "kpt_integrator.h"
 #include "kpt_launch.h"
+#include "kpt_temporal.h"
 
 namespace kpt {
 
@@ -82,7 +83,8 @@ hipError_t launch_trace_any(hipStream_t s, const SceneView& S, const kpt_ray* ra
 
 // ------------------------------------------------------------------ megakernel
 // computePathTraceSamplesMegaKernel (kernels.cu:385-451) with static
-// accumulation (Integrator::addSample, integrator.cuh:406-416).  16x16 pixel
+// accumulation (Integrator::addSample, integrator.cuh:406-416) or temporal
+// reprojection (reprojectAccumulate, kernels.cu:299-338).  16x16 pixel
 // tiles, XCD-swizzled tile order.
 constexpr int TILE = 16;
 
@@ -108,14 +110,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_megakernel(SceneView S, FrameCo
     v3 L = li_path(S, fc, ray, seed, &s_stack[0][threadIdx.x], &ovf, &vs);
     L = (mk3(1.0f, 1.0f, 1.0f) * 1.0f) * L;
     L = clamp_output(L);
-    fb.gbuffer[idx] = pack_gbuffer(vs.albedo, vs.normal, vs.depth);
-    v2 vel = velocity(S, fc, vs.inst, vs.pos);
-    fb.vbuffer[idx] = make_float4(vel.x, vel.y, (float)vs.prim, (float)vs.inst);
-    const float4 a = fb.accum[idx];
-    v3 acc = mk3(a.x, a.y, a.z) + L;
-    fb.accum[idx] = make_float4(acc.x, acc.y, acc.z, 1.0f);
-    v3 est = acc / (float)(frame + 1);
-    fb.render[idx] = make_float4(est.x, est.y, est.z, 1.0f);
+    write_pixel(S, fc, fb, frame, idx, px, py, L, vs.albedo, vs.normal, vs.depth, vs.inst, vs.prim, vs.pos);
     if (ovf) atomicOr(overflow, 1);
 }
 

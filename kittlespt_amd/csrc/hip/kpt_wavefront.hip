@@ -12,6 +12,7 @@
 // 256-path chunks from a per-stage atomic counter until the queue is drained.
 #include "kpt_integrator.h"
 #include "kpt_launch.h"
+#include "kpt_temporal.h"
 
 namespace kpt {
 
@@ -390,14 +391,8 @@ __global__ __launch_bounds__(WF_BLOCK) void k_finish(SceneView S, FrameConst fc,
     L = clamp_output(L);
     float4 va = W.vis_a[i], vn = W.vis_n[i], vp = W.vis_p[i];
     int inst = (int)fbits(vn.w), prim = (int)fbits(vp.w);
-    fb.gbuffer[i] = pack_gbuffer(mk3(va.x, va.y, va.z), mk3(vn.x, vn.y, vn.z), va.w);
-    v2 vel = velocity(S, fc, inst, mk3(vp.x, vp.y, vp.z));
-    fb.vbuffer[i] = make_float4(vel.x, vel.y, (float)prim, (float)inst);
-    const float4 a = fb.accum[i];
-    v3 acc = mk3(a.x, a.y, a.z) + L;
-    fb.accum[i] = make_float4(acc.x, acc.y, acc.z, 1.0f);
-    v3 est = acc / (float)(frame + 1);
-    fb.render[i] = make_float4(est.x, est.y, est.z, 1.0f);
+    write_pixel(S, fc, fb, frame, i, (int)(i % fc.width), (int)(i / fc.width), L, mk3(va.x, va.y, va.z),
+                mk3(vn.x, vn.y, vn.z), va.w, inst, prim, mk3(vp.x, vp.y, vp.z));
 }
 
 int g_num_cus = 0;

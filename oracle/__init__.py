@@ -125,6 +125,26 @@ class OracleScene:
                                 _ptr(v) if v is not None else None, threads or threads_default())
         return (rgb, g, v) if want_buffers else rgb
 
+    def set_instance_transform(self, i, model16):
+        m = np.ascontiguousarray(model16, np.float32)
+        L = lib()
+        L.orc_set_instance_transform.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+        L.orc_set_instance_transform(self.h, i, _ptr(m))
+
+    def render_temporal(self, settings, camera, width, height, frame, prev_accum, prev_gbuffer, threads=None):
+        """One temporal frame; returns (render [H,W,4], gbuffer, vbuffer)."""
+        pa = np.ascontiguousarray(prev_accum, np.float32)
+        pg = np.ascontiguousarray(prev_gbuffer, np.uint32)
+        r = np.zeros((height, width, 4), np.float32)
+        g = np.zeros((height, width, 4), np.uint32)
+        v = np.zeros((height, width, 4), np.float32)
+        L = lib()
+        L.orc_render_temporal.argtypes = [C.c_void_p, C.POINTER(abi.KptSettings), C.POINTER(abi.KptCamera),
+                                          C.c_int32, C.c_int32, C.c_int32] + [C.c_void_p] * 5 + [C.c_int32]
+        L.orc_render_temporal(self.h, C.byref(settings), C.byref(camera), width, height, frame, _ptr(pa), _ptr(pg),
+                              _ptr(r), _ptr(g), _ptr(v), threads or threads_default())
+        return r, g, v
+
     def sample_pixels(self, settings, camera, width, height, pixels, frame, threads=None):
         pixels = np.ascontiguousarray(pixels, np.int32)
         rgb = np.zeros((len(pixels), 3), np.float32)

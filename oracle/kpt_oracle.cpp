@@ -1527,6 +1527,51 @@ void RenderPixel(const FrameConst& fc, int W, int H, int pidx, int frame, V3<R>*
     *sample = ClampOutput(L);
 }
 
+// testReprojection / testReprojectedDepth (kernels.cu:196-222)
+inline bool TestReprojection(const FrameConst& fc, const VisSurf<float>& vs, float ppx, float ppy,
+                             const uint32_t* prev_g, int W, int H, V3<float> prev_cam) {
+    const TriangleMesh& m = fc.S->meshes[vs.inst];
+    V3<float> local = xyz(Mul(m.curr_inv_model, V4<float>(vs.position, 1.0f)));
+    M4<float> prev_model = Inverse(m.prev_inv_model);
+    int ix = (int)ppx, iy = (int)ppy;
+    ix = ix < 0 ? 0 : (ix > W - 1 ? W - 1 : ix);
+    iy = iy < 0 ? 0 : (iy > H - 1 ? H - 1 : iy);
+    float prev_depth = kpt::dm::u2f(prev_g[4 * ((size_t)iy * W + ix) + 3]);
+    V3<float> prev_world = xyz(Mul(prev_model, V4<float>(local, 1.0f)));
+    float d = Length(prev_cam - prev_world);
+    return std::fabs(d - prev_depth) < (prev_depth * 0.045f);
+}
+
+// reprojectAccumulate (kernels.cu:299-338) without SUPER_SAMPLE
+inline void ReprojectAccumulate(const FrameConst& fc, int W, int H, int px, int py, const VisSurf<float>& vs,
+                                V2<float> vel, V3<float> curr, const float* prev_acc, const uint32_t* prev_g,
+                                V3<float> prev_cam, float out[4]) {
+    auto pass = [&](float w) { out[0] = curr.x; out[1] = curr.y; out[2] = curr.z; out[3] = w; };
+    if (vs.inst < 0) return pass(0.0f);
+    float ppx = (float)px - vel.x * (float)W, ppy = (float)py - vel.y * (float)H;
+    if (ppx < 0 || ppx >= (float)W || ppy < 0 || ppy >= (float)H) return pass(0.0f);
+    if (!TestReprojection(fc, vs, ppx, ppy, prev_g, W, H, prev_cam)) return pass(0.0f);
+    // DeviceTextureBuffer::textureReadBilinear(p, false): alpha of the first tap
+    int dx = (int)ppx, dy = (int)ppy;
+    float ws = ppx - (float)dx, wt = ppy - (float)dy;
+    auto tap = [&](int x, int y, int c) {
+        x = x < 0 ? 0 : (x > W - 1 ? W - 1 : x);
+        y = y < 0 ? 0 : (y > H - 1 ? H - 1 : y);
+        return prev_acc[4 * ((size_t)y * W + x) + c];
+    };
+    float prev[3];
+    for (int c = 0; c < 3; ++c) {
+        float t0 = tap(dx, dy, c) + ws * (tap(dx + 1, dy, c) - tap(dx, dy, c));
+        float t1 = tap(dx, dy + 1, c) + ws * (tap(dx + 1, dy + 1, c) - tap(dx, dy + 1, c));
+        prev[c] = t0 + wt * (t1 - t0);
+    }
+    float history = tap(dx, dy, 3);
+    float alpha = 1.0f / std::fmin(history + 1, 16.0f);
+    const float cur[3] = {curr.x, curr.y, curr.z};
+    for (int c = 0; c < 3; ++c) out[c] = prev[c] + alpha * (cur[c] - prev[c]);
+    out[3] = history + 1;
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -1755,6 +1800,43 @@ void orc_render_pixels(const orc_scene* s, const kpt_settings* settings, const k
             out_v[4 * i] = vel.x; out_v[4 * i + 1] = vel.y;
             out_v[4 * i + 2] = (float)vs.prim; out_v[4 * i + 3] = (float)vs.inst;
         }
+    });
+}
+
+// Renderer::setMeshTransform (renderer.cu:250-269): TriangleMesh::setTransform
+// keeps the previous inverse (triangle.cuh:142-146), BLAS2::setTransform
+// (blas.cuh:190-200); then the per-frame TLAS rebuild (renderer.cu:115).
+void orc_set_instance_transform(orc_scene* s, int32_t i, const float model[16]) {
+    M4<float> M = LoadM4(model);
+    BlasSetTransform(s->blas[i], M);
+    TriangleMesh& tm = s->meshes[i];
+    tm.prev_inv_model = tm.curr_inv_model;
+    tm.curr_inv_model = GlmInverse(M);
+    BuildTlas(s);
+}
+
+// One frame with integrator_use_temporal_accumulation (kernels.cu:385-451):
+// every pixel's sample, G/V-buffers and the reprojected render value from the
+// previous frame's accumulation and G-buffer.
+void orc_render_temporal(const orc_scene* s, const kpt_settings* settings, const kpt_camera* cam, int32_t W,
+                         int32_t H, int32_t frame, const float* prev_accum, const uint32_t* prev_gbuffer,
+                         float* out_render, uint32_t* out_g, float* out_v, int32_t threads) {
+    FrameConst fc = MakeFrame(s, settings, cam);
+    V3<float> prev_cam = xyz(LoadM4(cam->prev_inv_view).c[3]);
+    ParallelFor((int64_t)W * H, threads, [&](int64_t i) {
+        V3<float> smp;
+        VisSurf<float> vs;
+        RenderPixel<float>(fc, W, H, (int)i, frame, &smp, &vs);
+        float ex, ey;
+        EncodeUnit(vs.normal, &ex, &ey);
+        uint32_t alb = FloatToByte(vs.albedo.x) | (FloatToByte(vs.albedo.y) << 8) | (FloatToByte(vs.albedo.z) << 16);
+        out_g[4 * i] = kpt::dm::f2u(ex); out_g[4 * i + 1] = kpt::dm::f2u(ey);
+        out_g[4 * i + 2] = alb; out_g[4 * i + 3] = kpt::dm::f2u(vs.depth);
+        V2<float> vel = Velocity(fc, vs);
+        out_v[4 * i] = vel.x; out_v[4 * i + 1] = vel.y;
+        out_v[4 * i + 2] = (float)vs.prim; out_v[4 * i + 3] = (float)vs.inst;
+        ReprojectAccumulate(fc, W, H, (int)(i % W), (int)(i / W), vs, vel, smp, prev_accum, prev_gbuffer, prev_cam,
+                            out_render + 4 * i);
     });
 }
 

@@ -45,6 +45,17 @@ void orc_render_pixels(const orc_scene* s, const kpt_settings* settings, const k
                        float* out_rgb, uint32_t* out_gbuffer, float* out_vbuffer,
                        int32_t threads);
 
+/* Renderer::setMeshTransform for instance i (previous inverse kept) and the
+ * TLAS rebuild. */
+void orc_set_instance_transform(orc_scene* s, int32_t instance, const float model[16]);
+/* One temporal-accumulation frame over the whole image: render [W*H][4]
+ * (reprojected colour, history length), gbuffer [W*H][4], vbuffer [W*H][4];
+ * prev_accum / prev_gbuffer are the previous frame's buffers. */
+void orc_render_temporal(const orc_scene* s, const kpt_settings* settings, const kpt_camera* cam,
+                         int32_t width, int32_t height, int32_t frame, const float* prev_accum,
+                         const uint32_t* prev_gbuffer, float* out_render, uint32_t* out_gbuffer,
+                         float* out_vbuffer, int32_t threads);
+
 /* One sample (radiance before clamp/accumulation) per pixel for frame `frame`.
  * Used to check the accumulation kernels independently. */
 void orc_sample_pixels(const orc_scene* s, const kpt_settings* settings, const kpt_camera* cam,
