@@ -179,6 +179,26 @@ typedef struct kpt_film_state {
     float scene_average_luminance;
 } kpt_film_state;
 
+/* Radiance cache (no reference counterpart: the learned part of this
+ * renderer).  A 6-layer, 64-wide ReLU MLP over a 32-feature frequency
+ * encoding of (position, direction, roughness); BF16 weights on the matrix
+ * cores, FP32 accumulation; trained online with Adam (beta 0.9 / 0.99,
+ * eps 1e-15) on a relative L2 loss over the tracer's own path suffixes. */
+typedef struct kpt_cache_settings {
+    int32_t enabled;          /* non-training paths end in the cache at the bounce-1 vertex */
+    int32_t train;            /* record suffixes and train every frame */
+    int32_t train_samples;    /* training paths per frame (65536) */
+    int32_t train_steps;      /* Adam steps per frame; batch = train_samples / train_steps */
+    float learning_rate;      /* 1e-3 */
+    float radiance_scale;     /* radiance = radiance_scale * max(output, 0) */
+    uint32_t seed;            /* weight initialisation (He uniform from a PCG hash) */
+    int32_t reserved;
+    float bounds_min[3];      /* position normalisation box; an empty box means */
+    float bounds_max[3];      /* the scene's TLAS bounds */
+} kpt_cache_settings;
+
+#define KPT_CACHE_PARAMS 18624
+
 /* ---- context ---- */
 int kpt_get_abi_version(void);
 /* Renderer::initialize (renderer.cu:22-46). device < 0 picks the current one. */
@@ -287,6 +307,31 @@ int kpt_get_film(kpt_ctx* ctx, kpt_film_state* out);
 /* executeRendering's delta_time_ms (renderer.cu:103-105): the frame time the
  * auto-exposure adaptation uses for the following kpt_render frames. */
 int kpt_set_frame_delta(kpt_ctx* ctx, float delta_time_ms);
+
+/* ---- radiance cache (no reference counterpart) ---- */
+/* Applies the settings; the first call (and any call with a new seed)
+ * initialises the weights and clears the Adam state.  The cache runs in the
+ * wavefront integrator only. */
+int kpt_cache_configure(kpt_ctx* ctx, const kpt_cache_settings* s);
+int kpt_cache_get_settings(const kpt_ctx* ctx, kpt_cache_settings* out);
+/* FP32 master weights, KPT_CACHE_PARAMS floats: per layer row-major
+ * [out][in], layers 32->64, 4 x 64->64, 64->3.  Setting them clears Adam. */
+int kpt_cache_get_weights(kpt_ctx* ctx, float* host_out, int64_t n);
+int kpt_cache_set_weights(kpt_ctx* ctx, const float* host_w, int64_t n);
+/* Network outputs for n queries (DEVICE pointers): positions n x 4 floats
+ * (xyz, roughness), directions n x 4 (outgoing direction xyz, unused); out
+ * n x 4 floats (the 3 raw outputs, 0). */
+int kpt_cache_infer(kpt_ctx* ctx, const float* positions, const float* directions, int64_t n, float* out);
+/* `steps` Adam steps over n samples (DEVICE pointers; queries as above,
+ * targets n x 4 in output units), batches of ceil(n / steps) in order.
+ * host_loss (optional) receives the loss of the last step. */
+int kpt_cache_train(kpt_ctx* ctx, const float* positions, const float* directions, const float* targets,
+                    int64_t n, int32_t steps, float* host_loss);
+/* Gradient of the last Adam step (before the update), KPT_CACHE_PARAMS floats. */
+int kpt_cache_get_gradient(kpt_ctx* ctx, float* host_out, int64_t n);
+/* out[0] loss of the last step, [1] samples in it, [2] queries of the last
+ * frame, [3] training records of the last frame, [4] Adam steps so far. */
+int kpt_cache_stats(kpt_ctx* ctx, double out[5]);
 
 #ifdef __cplusplus
 }

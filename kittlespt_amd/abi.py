@@ -167,6 +167,26 @@ class KptFilmState(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class KptCacheSettings(C.Structure):
+    _fields_ = [
+        ("enabled", C.c_int32),
+        ("train", C.c_int32),
+        ("train_samples", C.c_int32),
+        ("train_steps", C.c_int32),
+        ("learning_rate", C.c_float),
+        ("radiance_scale", C.c_float),
+        ("seed", C.c_uint32),
+        ("reserved", C.c_int32),
+        ("bounds_min", C.c_float * 3),
+        ("bounds_max", C.c_float * 3),
+    ]
+
+
+CACHE_PARAMS = 18624
+# (out, in) of the cache layers; weights are stored per layer row-major [out][in]
+CACHE_LAYERS = [(64, 32), (64, 64), (64, 64), (64, 64), (64, 64), (3, 64)]
+
+
 # dtype matching kpt_ray for zero-copy numpy arrays of rays
 RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("prim_id", "<i4"), ("inst_id", "<i4"), ("bary", "<f4", 3)])
@@ -216,6 +236,14 @@ SIGNATURES = [
     ("kpt_get_exposure", C.c_int, [_P, C.POINTER(KptExposure)]),
     ("kpt_get_film", C.c_int, [_P, C.POINTER(KptFilmState)]),
     ("kpt_set_frame_delta", C.c_int, [_P, C.c_float]),
+    ("kpt_cache_configure", C.c_int, [_P, C.POINTER(KptCacheSettings)]),
+    ("kpt_cache_get_settings", C.c_int, [_P, C.POINTER(KptCacheSettings)]),
+    ("kpt_cache_get_weights", C.c_int, [_P, C.POINTER(C.c_float), C.c_int64]),
+    ("kpt_cache_set_weights", C.c_int, [_P, C.POINTER(C.c_float), C.c_int64]),
+    ("kpt_cache_infer", C.c_int, [_P, _P, _P, C.c_int64, _P]),
+    ("kpt_cache_train", C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_int32, C.POINTER(C.c_float)]),
+    ("kpt_cache_get_gradient", C.c_int, [_P, C.POINTER(C.c_float), C.c_int64]),
+    ("kpt_cache_stats", C.c_int, [_P, C.POINTER(C.c_double)]),
 ]
 
 

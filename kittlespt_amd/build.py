@@ -18,6 +18,7 @@ SOURCES = [
     os.path.join(CSRC, "hip", "kpt_kernels.hip"),
     os.path.join(CSRC, "hip", "kpt_wavefront.hip"),
     os.path.join(CSRC, "hip", "kpt_post.hip"),
+    os.path.join(CSRC, "hip", "kpt_cache.hip"),
     os.path.join(CSRC, "hip", "kpt_abi.hip"),
     os.path.join(CSRC, "host", "kpt_build.cpp"),
 ]

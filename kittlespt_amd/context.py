@@ -248,6 +248,57 @@ class Context:
     def set_frame_delta(self, ms):
         self._ck(self.L.kpt_set_frame_delta(self.h, float(ms)))
 
+    # ---- radiance cache
+    def cache_configure(self, enabled=False, train=False, train_samples=65536, train_steps=4, learning_rate=1e-3,
+                        radiance_scale=1e4, seed=1, bounds=None):
+        s = abi.KptCacheSettings()
+        s.enabled, s.train = int(enabled), int(train)
+        s.train_samples, s.train_steps = int(train_samples), int(train_steps)
+        s.learning_rate, s.radiance_scale, s.seed = float(learning_rate), float(radiance_scale), int(seed)
+        if bounds is not None:
+            s.bounds_min[:] = [float(x) for x in bounds[0]]
+            s.bounds_max[:] = [float(x) for x in bounds[1]]
+        self._ck(self.L.kpt_cache_configure(self.h, C.byref(s)))
+        return s
+
+    def cache_weights(self):
+        w = np.zeros(abi.CACHE_PARAMS, np.float32)
+        self._ck(self.L.kpt_cache_get_weights(self.h, abi.fptr(w), len(w)))
+        return w
+
+    def cache_set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        self._ck(self.L.kpt_cache_set_weights(self.h, abi.fptr(w), len(w)))
+
+    def cache_gradient(self):
+        g = np.zeros(abi.CACHE_PARAMS, np.float32)
+        self._ck(self.L.kpt_cache_get_gradient(self.h, abi.fptr(g), len(g)))
+        return g
+
+    def cache_infer(self, pos, dirs):
+        pa = np.ascontiguousarray(pos, np.float32).reshape(-1, 4)
+        p = DeviceBuffer.from_array(pa)
+        d = DeviceBuffer.from_array(np.ascontiguousarray(dirs, np.float32).reshape(-1, 4))
+        n = len(pa)
+        out = DeviceBuffer(max(16, n * 16))
+        self._ck(self.L.kpt_cache_infer(self.h, p.ptr, d.ptr, n, out.ptr))
+        self._ck(self.L.kpt_sync(self.h))
+        return out.download(np.zeros((n, 4), np.float32))
+
+    def cache_train(self, pos, dirs, targets, steps=1):
+        pa = np.ascontiguousarray(pos, np.float32).reshape(-1, 4)
+        p = DeviceBuffer.from_array(pa)
+        d = DeviceBuffer.from_array(np.ascontiguousarray(dirs, np.float32).reshape(-1, 4))
+        t = DeviceBuffer.from_array(np.ascontiguousarray(targets, np.float32).reshape(-1, 4))
+        loss = C.c_float(0.0)
+        self._ck(self.L.kpt_cache_train(self.h, p.ptr, d.ptr, t.ptr, len(pa), steps, C.byref(loss)))
+        return loss.value
+
+    def cache_stats(self):
+        out = (C.c_double * 5)()
+        self._ck(self.L.kpt_cache_stats(self.h, out))
+        return dict(zip(("loss", "samples", "queries", "train_records", "steps"), list(out)))
+
     def estimate(self):
         """Demodulated radiance estimate of static accumulation: the running
         sum divided by the frame count, as the trace kernels compute it before

@@ -81,6 +81,12 @@ struct kpt_ctx {
     ExposureState h_expo{};
     float frame_delta_ms = 0.0f;
     int num_cus = 256;
+    // radiance cache
+    kpt_cache_settings cache_set{};
+    CacheBuffers cb{};
+    bool cache_weights = false;      // weights initialised
+    int64_t cache_steps = 0;         // Adam steps taken
+    double cache_frame_counts[2] = {0, 0};
 };
 
 namespace {
@@ -235,6 +241,127 @@ void free_frame(kpt_ctx* c) {
 // Renderer state before any setExposure call: AutoExposureProgram members
 // (autoexposure_program.hpp:75-78, ExposureValues defaults pod_types.hpp:26-32)
 // and the Film defaults (film.cuh:188-190).
+// ---------------------------------------------------------------- radiance cache
+void free_cache(kpt_ctx* c) {
+    CacheBuffers& b = c->cb;
+    dev_free(&b.w); dev_free(&b.adam_m); dev_free(&b.adam_v); dev_free(&b.grad); dev_free(&b.wb); dev_free(&b.wbt);
+    dev_free(&b.partial); dev_free(&b.loss_partial); dev_free(&b.stats); dev_free(&b.q_pos); dev_free(&b.q_dir);
+    dev_free(&b.q_beta); dev_free(&b.t_pos); dev_free(&b.t_dir); dev_free(&b.t_beta); dev_free(&b.t_lbefore);
+    dev_free(&b.t_target); dev_free(&b.counters);
+    b = CacheBuffers{};
+}
+
+void default_cache_settings(kpt_cache_settings* s) {
+    std::memset(s, 0, sizeof(*s));
+    s->train_samples = 65536;
+    s->train_steps = 4;
+    s->learning_rate = 1e-3f;
+    s->radiance_scale = 1e4f;
+    s->seed = 1;
+}
+
+// The parameter buffers (allocated once) and the training workspace for
+// batches of up to `batch` samples.
+int cache_alloc_params(kpt_ctx* ctx, int32_t batch) {
+    CacheBuffers& b = ctx->cb;
+    if (!b.w) {
+        HIPCHK(hipMalloc((void**)&b.w, RC_PARAMS * sizeof(float)));
+        HIPCHK(hipMalloc((void**)&b.adam_m, RC_PARAMS * sizeof(float)));
+        HIPCHK(hipMalloc((void**)&b.adam_v, RC_PARAMS * sizeof(float)));
+        HIPCHK(hipMalloc((void**)&b.grad, RC_PARAMS * sizeof(float)));
+        HIPCHK(hipMalloc((void**)&b.wb, RC_PADDED * sizeof(uint16_t)));
+        HIPCHK(hipMalloc((void**)&b.wbt, RC_PADDED * sizeof(uint16_t)));
+        HIPCHK(hipMalloc((void**)&b.stats, 4 * sizeof(float)));
+        HIPCHK(hipMalloc((void**)&b.counters, 4 * sizeof(int32_t)));
+        HIPCHK(hipMemsetAsync(b.grad, 0, RC_PARAMS * sizeof(float), ctx->stream));
+        HIPCHK(hipMemsetAsync(b.stats, 0, 4 * sizeof(float), ctx->stream));
+        HIPCHK(hipMemsetAsync(b.counters, 0, 4 * sizeof(int32_t), ctx->stream));
+    }
+    const int need = cache_train_workgroups(std::max(batch, 1));
+    if (need > b.max_wg) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        dev_free(&b.partial);
+        dev_free(&b.loss_partial);
+        HIPCHK(hipMalloc((void**)&b.partial, (size_t)need * RC_PADDED * sizeof(float)));
+        HIPCHK(hipMalloc((void**)&b.loss_partial, (size_t)need * sizeof(float)));
+        b.max_wg = need;
+    }
+    return KPT_OK;
+}
+
+// Per-frame records: one query per path, `train_cap` training records.
+int cache_alloc_records(kpt_ctx* ctx, int64_t npx, int64_t train_cap) {
+    CacheBuffers& b = ctx->cb;
+    if (b.query_cap < npx) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        dev_free(&b.q_pos); dev_free(&b.q_dir); dev_free(&b.q_beta);
+        HIPCHK(hipMalloc((void**)&b.q_pos, (size_t)npx * sizeof(float4)));
+        HIPCHK(hipMalloc((void**)&b.q_dir, (size_t)npx * sizeof(float4)));
+        HIPCHK(hipMalloc((void**)&b.q_beta, (size_t)npx * sizeof(float4)));
+        b.query_cap = npx;
+    }
+    if (b.train_cap < train_cap) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        dev_free(&b.t_pos); dev_free(&b.t_dir); dev_free(&b.t_beta); dev_free(&b.t_lbefore); dev_free(&b.t_target);
+        HIPCHK(hipMalloc((void**)&b.t_pos, (size_t)train_cap * sizeof(float4)));
+        HIPCHK(hipMalloc((void**)&b.t_dir, (size_t)train_cap * sizeof(float4)));
+        HIPCHK(hipMalloc((void**)&b.t_beta, (size_t)train_cap * sizeof(float4)));
+        HIPCHK(hipMalloc((void**)&b.t_lbefore, (size_t)train_cap * sizeof(float4)));
+        HIPCHK(hipMalloc((void**)&b.t_target, (size_t)train_cap * sizeof(float4)));
+        b.train_cap = train_cap;
+    }
+    return KPT_OK;
+}
+
+// Uploads FP32 master weights, clears the Adam state, republishes BF16.
+int cache_upload_weights(kpt_ctx* ctx, const float* w) {
+    CacheBuffers& b = ctx->cb;
+    HIPCHK(hipMemcpyAsync(b.w, w, RC_PARAMS * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemsetAsync(b.adam_m, 0, RC_PARAMS * sizeof(float), ctx->stream));
+    HIPCHK(hipMemsetAsync(b.adam_v, 0, RC_PARAMS * sizeof(float), ctx->stream));
+    HIPCHK(launch_cache_publish(ctx->stream, b));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->cache_steps = 0;
+    ctx->cache_weights = true;
+    return KPT_OK;
+}
+
+// He-uniform initialisation: w = (2u - 1) * sqrt(6 / fan_in), u from a PCG hash
+// of (seed, parameter index).
+std::vector<float> cache_init_weights(uint32_t seed) {
+    std::vector<float> w(RC_PARAMS);
+    for (int L = 0; L < RC_LAYERS; ++L) {
+        const float a = std::sqrt(6.0f / (float)rc_in(L));
+        const int n = rc_out(L) * rc_in(L);
+        for (int k = 0; k < n; ++k) {
+            const int i = rc_off(L) + k;
+            const uint32_t h = pcg_hash(seed ^ pcg_hash((uint32_t)i + 0x9E3779B9u));
+            const float u = (float)(h >> 8) * (1.0f / 16777216.0f);
+            w[i] = (2.0f * u - 1.0f) * a;
+        }
+    }
+    return w;
+}
+
+// Position box and output unit of the encoding.
+CacheEncode cache_encode(const kpt_ctx* ctx) {
+    CacheEncode e{};
+    const kpt_cache_settings& s = ctx->cache_set;
+    float lo[3], hi[3];
+    bool box = true;
+    for (int k = 0; k < 3; ++k) box = box && (s.bounds_max[k] > s.bounds_min[k]);
+    for (int k = 0; k < 3; ++k) {
+        if (box) { lo[k] = s.bounds_min[k]; hi[k] = s.bounds_max[k]; }
+        else if (!ctx->tlas.nodes.empty()) { lo[k] = ctx->tlas.nodes[0].lo[k]; hi[k] = ctx->tlas.nodes[0].hi[k]; }
+        else { lo[k] = 0.0f; hi[k] = 1.0f; }
+        const float ext = hi[k] - lo[k];
+        e.bmin[k] = lo[k];
+        e.binv[k] = ext > 0.0f ? 1.0f / ext : 1.0f;
+    }
+    e.scale = s.radiance_scale;
+    return e;
+}
+
 void default_exposure(ExposureState* e) {
     std::memset(e, 0, sizeof(*e));
     e->ae.max_iso = 6400;
@@ -291,6 +418,7 @@ int kpt_ctx_create(int device, kpt_ctx** out) {
     kpt_ctx* ctx = new kpt_ctx();
     default_settings(&ctx->settings);
     default_exposure(&ctx->h_expo);
+    default_cache_settings(&ctx->cache_set);
     hm::Mat4 I = hm::identity();
     std::memcpy(ctx->cam.inv_view, I.m, 64); std::memcpy(ctx->cam.inv_proj, I.m, 64);
     std::memcpy(ctx->cam.prev_inv_view, I.m, 64); std::memcpy(ctx->cam.prev_inv_proj, I.m, 64);
@@ -357,6 +485,7 @@ int kpt_ctx_destroy(kpt_ctx* ctx) {
         dev_free(&ctx->d_counts);
         dev_free(&ctx->pb.expo);
         dev_free(&ctx->pb.hist);
+        free_cache(ctx);
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     }
     delete ctx;
@@ -768,6 +897,26 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
         int rc = alloc_wavefront(ctx, npx);
         if (rc != KPT_OK) return rc;
     }
+    // radiance cache: wavefront only
+    const kpt_cache_settings& cs = ctx->cache_set;
+    const bool use_cache = cs.enabled || cs.train;
+    if (use_cache && ctx->settings.integrator != KPT_INTEGRATOR_WAVEFRONT)
+        return fail(ctx, KPT_ERR_UNSUPPORTED, "the radiance cache runs in the wavefront integrator");
+    CacheHooks hooks{};
+    const int32_t batch = std::max(1, (cs.train_samples + std::max(1, cs.train_steps) - 1) / std::max(1, cs.train_steps));
+    if (use_cache) {
+        if (!ctx->cache_weights) return fail(ctx, KPT_ERR_STATE, "radiance cache not configured");
+        int rc = cache_alloc_params(ctx, batch);
+        if (rc == KPT_OK) rc = cache_alloc_records(ctx, npx, std::max(1, cs.train_samples));
+        if (rc != KPT_OK) return rc;
+        hooks.cb = &ctx->cb;
+        hooks.cf.enabled = cs.enabled;
+        hooks.cf.train = cs.train;
+        const double prob = std::min(1.0, (double)cs.train_samples / (double)std::max<int64_t>(1, npx));
+        hooks.cf.train_threshold = (uint32_t)std::min(16777216.0, std::ceil(prob * 16777216.0));
+        hooks.enc = cache_encode(ctx);
+        hooks.num_cus = ctx->num_cus;
+    }
     if (ctx->count_trace) {
         HIPCHK(hipMemsetAsync(ctx->d_counts, 0, 8 * sizeof(unsigned long long), ctx->stream));
     }
@@ -778,8 +927,19 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
         FrameConst fc = make_frame(ctx, ctx->frame_index);
         if (ctx->settings.integrator == KPT_INTEGRATOR_WAVEFRONT) {
             ctx->ins.counters = ctx->count_trace ? ctx->d_counts : nullptr;
+            hooks.cf.frame = (uint32_t)ctx->frame_index;
             HIPCHK(wavefront_frame(ctx->stream, S, fc, ctx->frame_index, ctx->fb, ctx->ws, ctx->d_overflow,
-                                   (int32_t)ctx->materials.size(), ctx->sort_by_material, ctx->tune, &ctx->ins));
+                                   (int32_t)ctx->materials.size(), ctx->sort_by_material, ctx->tune, &ctx->ins,
+                                   use_cache ? &hooks : nullptr));
+            if (use_cache && cs.train) {
+                // Adam steps over this frame's suffixes (they shape the next frames)
+                for (int32_t st = 0; st < std::max(1, cs.train_steps); ++st) {
+                    HIPCHK(launch_cache_train_step(ctx->stream, ctx->cb, hooks.enc, ctx->cb.t_pos, ctx->cb.t_dir,
+                                                   ctx->cb.t_target, ctx->cb.counters + 1, 0, ctx->cb.train_cap,
+                                                   (int64_t)st * batch, batch, cs.learning_rate,
+                                                   ++ctx->cache_steps));
+                }
+            }
             // per-launch trace times of this frame (events already recorded)
             HIPCHK(hipEventSynchronize(ctx->ins.end[ctx->ins.launches - 1]));
             for (int j = 0; j < ctx->ins.launches; ++j) {
@@ -881,6 +1041,107 @@ int kpt_get_film(kpt_ctx* ctx, kpt_film_state* out) {
 int kpt_set_frame_delta(kpt_ctx* ctx, float delta_time_ms) {
     if (!ctx || !(delta_time_ms >= 0.0f)) return KPT_ERR_INVALID_ARG;
     ctx->frame_delta_ms = delta_time_ms;
+    return KPT_OK;
+}
+
+// ---------------------------------------------------------------- radiance cache
+int kpt_cache_configure(kpt_ctx* ctx, const kpt_cache_settings* s) {
+    if (!ctx || !s) return KPT_ERR_INVALID_ARG;
+    if (s->train_samples < 0 || s->train_samples > (1 << 22) || s->train_steps < 0 || s->train_steps > 64 ||
+        !(s->learning_rate >= 0.0f) || !(s->radiance_scale > 0.0f))
+        return fail(ctx, KPT_ERR_INVALID_ARG, "cache settings out of range");
+    if ((s->enabled || s->train) && s->train_samples > 0 && s->train_steps == 0)
+        return fail(ctx, KPT_ERR_INVALID_ARG, "training needs at least one step per frame");
+    if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
+    const bool reseed = !ctx->cache_weights || s->seed != ctx->cache_set.seed;
+    ctx->cache_set = *s;
+    const int32_t steps = std::max(1, s->train_steps);
+    int rc = cache_alloc_params(ctx, (std::max(1, s->train_samples) + steps - 1) / steps);
+    if (rc != KPT_OK) return rc;
+    if (reseed) {
+        std::vector<float> w = cache_init_weights(s->seed);
+        rc = cache_upload_weights(ctx, w.data());
+        if (rc != KPT_OK) return rc;
+    }
+    return KPT_OK;
+}
+
+int kpt_cache_get_settings(const kpt_ctx* ctx, kpt_cache_settings* out) {
+    if (!ctx || !out) return KPT_ERR_INVALID_ARG;
+    *out = ctx->cache_set;
+    return KPT_OK;
+}
+
+int kpt_cache_get_weights(kpt_ctx* ctx, float* host_out, int64_t n) {
+    if (!ctx || !host_out || n < RC_PARAMS) return KPT_ERR_INVALID_ARG;
+    if (!ctx->cache_weights) return fail(ctx, KPT_ERR_STATE, "radiance cache not configured");
+    HIPCHK(hipMemcpyAsync(host_out, ctx->cb.w, RC_PARAMS * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return KPT_OK;
+}
+
+int kpt_cache_set_weights(kpt_ctx* ctx, const float* host_w, int64_t n) {
+    if (!ctx || !host_w || n < RC_PARAMS) return KPT_ERR_INVALID_ARG;
+    if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
+    int rc = cache_alloc_params(ctx, 1);
+    if (rc != KPT_OK) return rc;
+    return cache_upload_weights(ctx, host_w);
+}
+
+int kpt_cache_infer(kpt_ctx* ctx, const float* positions, const float* directions, int64_t n, float* out) {
+    if (!ctx || n < 0 || (n > 0 && (!positions || !directions || !out))) return KPT_ERR_INVALID_ARG;
+    if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
+    if (!ctx->cache_weights) return fail(ctx, KPT_ERR_STATE, "radiance cache not configured");
+    HIPCHK(launch_cache_infer_raw(ctx->stream, ctx->cb, cache_encode(ctx), (const float4*)positions,
+                                  (const float4*)directions, n, (float4*)out, ctx->num_cus));
+    return KPT_OK;
+}
+
+int kpt_cache_train(kpt_ctx* ctx, const float* positions, const float* directions, const float* targets, int64_t n,
+                    int32_t steps, float* host_loss) {
+    if (!ctx || n < 0 || steps <= 0 || (n > 0 && (!positions || !directions || !targets)))
+        return KPT_ERR_INVALID_ARG;
+    if (n > ((int64_t)1 << 26)) return fail(ctx, KPT_ERR_INVALID_ARG, "too many training samples");
+    if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
+    if (!ctx->cache_weights) return fail(ctx, KPT_ERR_STATE, "radiance cache not configured");
+    const int32_t batch = (int32_t)std::max<int64_t>(1, (n + steps - 1) / steps);
+    int rc = cache_alloc_params(ctx, batch);
+    if (rc != KPT_OK) return rc;
+    const CacheEncode enc = cache_encode(ctx);
+    for (int32_t st = 0; st < steps; ++st) {
+        HIPCHK(launch_cache_train_step(ctx->stream, ctx->cb, enc, (const float4*)positions, (const float4*)directions,
+                                       (const float4*)targets, nullptr, n, n, (int64_t)st * batch, batch,
+                                       ctx->cache_set.learning_rate, ++ctx->cache_steps));
+    }
+    if (host_loss) {
+        HIPCHK(hipMemcpyAsync(host_loss, ctx->cb.stats, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    return KPT_OK;
+}
+
+int kpt_cache_get_gradient(kpt_ctx* ctx, float* host_out, int64_t n) {
+    if (!ctx || !host_out || n < RC_PARAMS) return KPT_ERR_INVALID_ARG;
+    if (!ctx->cache_weights) return fail(ctx, KPT_ERR_STATE, "radiance cache not configured");
+    HIPCHK(hipMemcpyAsync(host_out, ctx->cb.grad, RC_PARAMS * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return KPT_OK;
+}
+
+int kpt_cache_stats(kpt_ctx* ctx, double out[5]) {
+    if (!ctx || !out) return KPT_ERR_INVALID_ARG;
+    for (int k = 0; k < 5; ++k) out[k] = 0.0;
+    out[4] = (double)ctx->cache_steps;
+    if (ctx->host_only || !ctx->cache_weights) return KPT_OK;
+    float st[2];
+    int32_t cnt[2];
+    HIPCHK(hipMemcpyAsync(st, ctx->cb.stats, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(cnt, ctx->cb.counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    out[0] = st[0];
+    out[1] = st[1];
+    out[2] = cnt[0];
+    out[3] = cnt[1];
     return KPT_OK;
 }
 

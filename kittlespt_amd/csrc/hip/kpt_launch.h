@@ -6,6 +6,7 @@
 
 #include "../../../include/kpt_abi.h"
 #include "../common/kpt_device_types.h"
+#include "kpt_cache.h"
 
 namespace kpt {
 
@@ -107,9 +108,33 @@ struct WfInstrument {
     unsigned long long* counters;          // device [5] or nullptr (counted traversal)
 };
 
-// Wavefront path tracer: one reference frame (one sample per pixel).
+// The radiance cache of the wavefront tracer (null when unused).
+struct CacheHooks {
+    const CacheBuffers* cb;
+    CacheFrame cf;
+    CacheEncode enc;
+    int num_cus;
+};
+
+// Wavefront path tracer: one reference frame (one sample per pixel).  With
+// a cache, non-training paths end in it at the bounce-1 vertex and training
+// paths leave suffix targets for the caller's training steps.
 hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& fc, int32_t frame,
                            const FrameBuffers& fb, WavefrontState& W, int32_t* overflow, int32_t num_materials,
-                           int32_t sort_by_material, const TraceTuning& tune, WfInstrument* ins);
+                           int32_t sort_by_material, const TraceTuning& tune, WfInstrument* ins,
+                           const CacheHooks* cache);
+
+// Radiance-cache kernels (kpt_cache.hip).
+int cache_train_workgroups(int32_t batch);
+hipError_t launch_cache_publish(hipStream_t s, const CacheBuffers& cb);
+hipError_t launch_cache_infer_paths(hipStream_t s, const CacheBuffers& cb, const CacheEncode& enc, float4* L,
+                                    int64_t max_queries, int num_cus);
+hipError_t launch_cache_infer_raw(hipStream_t s, const CacheBuffers& cb, const CacheEncode& enc, const float4* pos,
+                                  const float4* dir, int64_t n, float4* out, int num_cus);
+hipError_t launch_cache_targets(hipStream_t s, const CacheBuffers& cb, const float4* L, float scale);
+hipError_t launch_cache_train_step(hipStream_t s, const CacheBuffers& cb, const CacheEncode& enc, const float4* pos,
+                                   const float4* dir, const float4* target, const int32_t* count_dev,
+                                   int64_t count_host, int64_t cap, int64_t first, int32_t batch, float lr,
+                                   int64_t step);
 
 }  // namespace kpt

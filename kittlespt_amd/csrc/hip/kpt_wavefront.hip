@@ -222,7 +222,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_sort_scatter(SceneView S, Wavefron
 
 // One bounce of Li for every live path (integrator.cuh:266-398).
 __global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, WavefrontState W, int parity,
-                                                    int pull_slot, const int32_t* q) {
+                                                    int pull_slot, const int32_t* q, CacheBuffers cb, CacheFrame cf) {
     const int count = W.counters[C_QCOUNT + parity];
     int32_t* nq = W.queue[parity ^ 1];
     for (;;) {
@@ -267,8 +267,34 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, 
                 W.vis_p[p] = make_float4(s.pos.x, s.pos.y, s.pos.z, bitsf((uint32_t)s.prim));
                 b.albedo = mk3(1.0f, 1.0f, 1.0f);
             }
+            // radiance cache at the bounce-1 vertex: training paths record the
+            // radiance so far (their suffix becomes the target), the others
+            // end here and are completed by the network
+            bool to_cache = false;
+            if (depth == 1 && (cf.enabled || cf.train)) {
+                const uint32_t hs = pcg_hash((uint32_t)p ^ (cf.frame * 0x9E3779B9u)) & 0xFFFFFFu;
+                const float4 qpos = make_float4(s.pos.x, s.pos.y, s.pos.z, b.roughness);
+                const float4 qdir = make_float4(-ray.d.x, -ray.d.y, -ray.d.z, bitsf((uint32_t)p));
+                if (cf.train && hs < cf.train_threshold) {
+                    const int r = atomicAdd(&cb.counters[1], 1);
+                    if (r < cb.train_cap) {
+                        cb.t_pos[r] = qpos;
+                        cb.t_dir[r] = qdir;
+                        cb.t_beta[r] = make_float4(beta.x, beta.y, beta.z, 0.0f);
+                        cb.t_lbefore[r] = make_float4(L.x, L.y, L.z, 0.0f);
+                    }
+                } else if (cf.enabled) {
+                    const int r = atomicAdd(&cb.counters[0], 1);
+                    if (r < cb.query_cap) {
+                        cb.q_pos[r] = qpos;
+                        cb.q_dir[r] = qdir;
+                        cb.q_beta[r] = make_float4(beta.x, beta.y, beta.z, 0.0f);
+                        to_cache = true;
+                    }
+                }
+            }
             if (any_nonspec) bsdf_regularize(b);
-            if (!(b.alpha < 0.001f)) {
+            if (!to_cache && !(b.alpha < 0.001f)) {
                 NeeResult ne = next_event(S, fc, s, b, ray.d, beta, seed);
                 if (ne.trace) {
                     int sidx = atomicAdd(&W.counters[C_SHADOW], 1);
@@ -283,7 +309,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, 
             }
             const bool at_max = (depth == fc.max_depth);
             depth++;
-            if (!at_max) {
+            if (!at_max && !to_cache) {
                 v3 wo = -ray.d;
                 v2 u2, X2;
                 u2.x = rng1(seed); u2.y = rng1(seed);
@@ -401,8 +427,16 @@ int g_num_cus = 0;
 
 hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& fc, int32_t frame,
                            const FrameBuffers& fb, WavefrontState& W, int32_t* overflow, int32_t num_materials,
-                           int32_t sort_by_material, const TraceTuning& tune, WfInstrument* ins) {
+                           int32_t sort_by_material, const TraceTuning& tune, WfInstrument* ins,
+                           const CacheHooks* cache) {
     if (ins) ins->launches = 0;
+    CacheBuffers cb{};
+    CacheFrame cf{};
+    if (cache && cache->cb && (cache->cf.enabled || cache->cf.train)) {
+        cb = *cache->cb;
+        cf = cache->cf;
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, cb.counters, 0, 2);
+    }
     if (g_num_cus == 0) {
         int dev = 0;
         (void)hipGetDevice(&dev);
@@ -443,9 +477,17 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
             shade_q = W.sort_tmp;
         }
         hipLaunchKernelGGL(k_shade, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity, C_PULL + 3 * bounce + 1,
-                           shade_q);
+                           shade_q, cb, cf);
         hipLaunchKernelGGL(k_shadow, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, W, C_PULL + 3 * bounce + 2, overflow,
                            tune);
+    }
+    if (cf.enabled) {
+        hipError_t e = launch_cache_infer_paths(s, cb, cache->enc, W.L, npx, cache->num_cus);
+        if (e != hipSuccess) return e;
+    }
+    if (cf.train) {
+        hipError_t e = launch_cache_targets(s, cb, W.L, cache->enc.scale);
+        if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_finish, dim3(gblocks), dim3(WF_BLOCK), 0, s, S, fc, frame, W, fb);
     return hipGetLastError();

@@ -378,3 +378,48 @@ def bloom_up(src, dst, blend=0.65):
 
 def bloom_levels(width, height):
     return int(_sig("orc_bloom_levels", C.c_int32, [C.c_int32, C.c_int32])(width, height))
+
+
+# ---------------------------------------------------------------- radiance cache network
+def _enc(bmin, bmax, scale=1.0):
+    bmin = np.asarray(bmin, np.float32)
+    ext = np.asarray(bmax, np.float32) - bmin
+    binv = np.float32(1.0) / ext
+    return np.concatenate([bmin, binv, [np.float32(scale)]]).astype(np.float32)
+
+
+def cache_encode(pos, dirs, bmin, bmax):
+    p, d = _f32(pos).reshape(-1, 4), _f32(dirs).reshape(-1, 4)
+    out = np.zeros((len(p), 32), np.float32)
+    e = _enc(bmin, bmax)
+    _sig("orc_cache_encode", None, [C.c_void_p] * 2 + [C.c_int64, C.c_void_p, C.c_void_p])(
+        _ptr(p), _ptr(d), len(p), _ptr(e), _ptr(out))
+    return out
+
+
+def cache_forward(w, pos, dirs, bmin, bmax, mode=0):
+    w, p, d = _f32(w), _f32(pos).reshape(-1, 4), _f32(dirs).reshape(-1, 4)
+    out = np.zeros((len(p), 3), np.float64)
+    e = _enc(bmin, bmax)
+    _sig("orc_cache_forward", None, [C.c_void_p] * 3 + [C.c_int64, C.c_void_p, C.c_int32, C.c_void_p])(
+        _ptr(w), _ptr(p), _ptr(d), len(p), _ptr(e), mode, _ptr(out))
+    return out
+
+
+def cache_loss_grad(w, pos, dirs, targets, bmin, bmax, mode=0, w_den=None, want_grad=True):
+    w, p, d, t = _f32(w), _f32(pos).reshape(-1, 4), _f32(dirs).reshape(-1, 4), _f32(targets).reshape(-1, 4)
+    wd = _f32(w_den) if w_den is not None else None
+    g = np.zeros(len(w), np.float64) if want_grad else None
+    e = _enc(bmin, bmax)
+    f = _sig("orc_cache_loss_grad", C.c_double, [C.c_void_p] * 5 + [C.c_int64, C.c_void_p, C.c_int32, C.c_void_p])
+    loss = f(_ptr(w), _ptr(wd) if wd is not None else None, _ptr(p), _ptr(d), _ptr(t), len(p), _ptr(e), mode,
+             _ptr(g) if g is not None else None)
+    return loss, g
+
+
+def cache_adam(w, m, v, grad, step, lr):
+    w, m, v = (np.array(a, np.float32, copy=True) for a in (w, m, v))
+    g = _f32(grad)
+    _sig("orc_cache_adam", None, [C.c_void_p] * 4 + [C.c_int64, C.c_float])(
+        _ptr(w), _ptr(m), _ptr(v), _ptr(g), step, lr)
+    return w, m, v

@@ -119,6 +119,20 @@ void orc_bloom_down(const float* src, int32_t sw, int32_t sh, float* dst, int32_
 void orc_bloom_up(const float* src, int32_t sw, int32_t sh, float* dst, int32_t dw, int32_t dh, float blend);
 int32_t orc_bloom_levels(int32_t width, int32_t height);
 
+/* ---- radiance-cache network (kpt_oracle_cache.cpp) ---- */
+/* enc = {bmin xyz, binv xyz, scale}; pos/dir n x 4 floats; out n x 32 */
+void orc_cache_encode(const float* pos, const float* dir, int64_t n, const float enc[7], float* out);
+/* mode 0: BF16 rounding points of the GPU, double sums; mode 1: exact FP64.
+ * w: KPT_CACHE_PARAMS floats; out n x 3. */
+void orc_cache_forward(const float* w, const float* pos, const float* dir, int64_t n, const float enc[7],
+                       int32_t mode, double* out);
+/* Mean relative-L2 loss over n samples (targets n x 4) and, when grad is not
+ * NULL, its gradient; w_den (or NULL) gives the denominator outputs. */
+double orc_cache_loss_grad(const float* w, const float* w_den, const float* pos, const float* dir,
+                           const float* targets, int64_t n, const float enc[7], int32_t mode, double* grad);
+/* One Adam step (beta 0.9 / 0.99, eps 1e-15) in float, in place. */
+void orc_cache_adam(float* w, float* m, float* v, const float* grad, int64_t step, float lr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,148 @@
+"""GPU radiance cache (kpt_cache.hip) against the oracle
+(oracle/kpt_oracle_cache.cpp), and what the trained network gains.
+
+* forward: BF16 MFMA inference vs the oracle's BF16-rounding forward;
+* gradient: one training step's weight gradient vs the oracle's backward
+  (same rounding points, double sums) and the loss value;
+* Adam: the updated weights vs the oracle's Adam applied to that gradient;
+* learning: the network fits a smooth function of its inputs;
+* rendering: with the cache trained online, the image is closer to an FP64
+  converged render than the same tracer with the network's output zeroed
+  (paths ending at the same vertex without it).
+
+The MFMA sums in FP32 in an order the hardware chooses, so forward and
+gradient are compared with tolerances; Adam is exact float arithmetic.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from kittlespt_amd import abi, scenes
+from common import host_camera, settings, view_proj
+from test_cache_oracle import BMAX, BMIN, he_weights, queries
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from kittlespt_amd.context import Context
+    c = Context(0)
+    c.cache_configure(bounds=(BMIN, BMAX), seed=3)
+    yield c
+    c.close()
+
+
+def test_cache_forward_matches_oracle(ctx):
+    w = he_weights(11)
+    ctx.cache_set_weights(w)
+    pos, dirs = queries(4096, 12)
+    print("[cache] forward 4096 queries", flush=True)
+    got = ctx.cache_infer(pos, dirs)[:, :3].astype(np.float64)
+    want = oracle.cache_forward(w, pos, dirs, BMIN, BMAX, mode=0)
+    scale = np.abs(want).max()
+    err = np.abs(got - want) / scale
+    print(f"max |out| {scale:.3f}, max err {err.max():.2e}, mean err {err.mean():.2e}")
+    assert err.max() < 2e-2 and err.mean() < 2e-3
+    assert np.abs(got - oracle.cache_forward(w, pos, dirs, BMIN, BMAX, mode=1)).max() < 0.05 * scale
+
+
+def test_cache_gradient_and_adam_match_oracle(ctx):
+    w = he_weights(13)
+    ctx.cache_set_weights(w)
+    n = 1000                                           # a ragged last workgroup (1000 = 31 * 32 + 8)
+    pos, dirs = queries(n, 14)
+    t = np.zeros((n, 4), np.float32)
+    t[:, :3] = np.random.default_rng(15).uniform(0, 2, (n, 3))
+    print("[cache] one training step over 1000 samples", flush=True)
+    loss = ctx.cache_train(pos, dirs, t, steps=1)
+    g = ctx.cache_gradient().astype(np.float64)
+    lo, go = oracle.cache_loss_grad(w, pos, dirs, t, BMIN, BMAX, mode=0)
+    rel = np.linalg.norm(g - go) / np.linalg.norm(go)
+    cos = g @ go / (np.linalg.norm(g) * np.linalg.norm(go))
+    print(f"loss gpu {loss:.6f} oracle {lo:.6f}; gradient rel err {rel:.2e}, cosine {cos:.6f}")
+    assert loss == pytest.approx(lo, rel=1e-3)
+    assert rel < 2e-2 and cos > 0.9995
+    off = 0
+    for out, inp in abi.CACHE_LAYERS:                  # every layer's gradient, not only the total
+        gl, ol = g[off:off + out * inp], go[off:off + out * inp]
+        off += out * inp
+        assert np.linalg.norm(gl - ol) <= 5e-2 * np.linalg.norm(ol) + 1e-12
+    w_new, _, _ = oracle.cache_adam(w, np.zeros_like(w), np.zeros_like(w), g.astype(np.float32), 1, 1e-3)
+    assert np.allclose(ctx.cache_weights(), w_new, rtol=2e-6, atol=1e-9)
+
+
+def test_cache_learns_a_smooth_function(ctx):
+    ctx.cache_configure(bounds=(BMIN, BMAX), seed=21, learning_rate=1e-3)
+    pos, dirs = queries(16384, 22)
+    ph = (pos[:, :3] - BMIN) / (np.array(BMAX) - BMIN)
+
+    def f(ph, d, r):
+        return np.stack([1 + np.sin(3 * ph[:, 0]) * d[:, 1], 0.5 + ph[:, 1] * r, 1 + 0.5 * np.cos(2 * ph[:, 2])], 1)
+
+    t = np.zeros((len(pos), 4), np.float32)
+    t[:, :3] = f(ph, dirs, pos[:, 3])
+    print("[cache] 300 Adam steps on a smooth target", flush=True)
+    first = ctx.cache_train(pos, dirs, t, steps=1)
+    last = first
+    for _ in range(30):
+        last = ctx.cache_train(pos, dirs, t, steps=10)
+    hp, hd = queries(4096, 23)
+    hph = (hp[:, :3] - BMIN) / (np.array(BMAX) - BMIN)
+    pred = ctx.cache_infer(hp, hd)[:, :3]
+    truth = f(hph, hd, hp[:, 3])
+    mse = ((pred - truth) ** 2).mean()
+    base = ((truth - truth.mean(0)) ** 2).mean()
+    print(f"loss {first:.4f} -> {last:.5f}; held-out mse {mse:.5f} vs constant predictor {base:.5f}")
+    assert last < 0.1 * first
+    assert mse < 0.1 * base
+
+
+def _rel_mse(est, ref):
+    ref = ref.astype(np.float64)
+    m = np.mean(ref)
+    return float(np.mean((est - ref) ** 2 / (ref ** 2 + (0.1 * m) ** 2)))
+
+
+def test_trained_cache_beats_the_same_tracer_without_it():
+    from kittlespt_amd.context import Context
+    sc = scenes.scene_plain(subdiv=3)
+    W, H, frames = 64, 48, 32
+    st = settings(max_depth=6, integrator=abi.INTEGRATOR_WAVEFRONT, sun_theta=sc.sun_theta)
+    cam = host_camera(sc, W, H)
+    orc = oracle.OracleScene(sc)
+    pix = np.arange(W * H, dtype=np.int32)
+    print("[cache] FP64 reference 64x48 x 256 spp on the CPU", flush=True)
+    ref = orc.render_pixels(st, cam, W, H, pix, 0, 256, double=True).reshape(H, W, 3)
+    orc.close()
+    ctx = Context(0)
+    try:
+        ctx.upload(sc)
+        ctx.set_settings(st)
+        ctx.resize(W, H)
+        p, v = view_proj(sc, W, H)
+        ctx.set_view(p, v)
+        # warm-up: train online on the scene's own suffixes
+        ctx.cache_configure(enabled=False, train=True, train_samples=1024, train_steps=4, seed=5)
+        print("[cache] online training 200 frames", flush=True)
+        ctx.render(200)
+        trained = ctx.cache_weights()
+        stats = ctx.cache_stats()
+        print(f"[cache] after warm-up: {stats}")
+        res = {}
+        for name, w in (("cache", trained), ("zero", np.zeros_like(trained))):
+            ctx.cache_configure(enabled=True, train=False, seed=5)
+            ctx.cache_set_weights(w)
+            ctx.reset()
+            ctx.render(frames)
+            res[name] = _rel_mse(ctx.estimate(), ref)
+        ctx.cache_configure(enabled=False, train=False, seed=5)
+        ctx.reset()
+        ctx.render(frames)
+        res["plain"] = _rel_mse(ctx.estimate(), ref)
+    finally:
+        ctx.close()
+    print(f"[cache] relative MSE vs FP64 at {frames} spp: {res}")
+    assert res["cache"] < 0.5 * res["zero"]
