@@ -8,8 +8,12 @@ max depth 8, 1,000 instances of 8 seeded BLAS (50k-200k triangles each),
 buffers resident in HBM before timing starts.
 
 Prints ONE JSON line with the metric, a roofline object for the dominant
-kernel (closest-hit traversal, HBM-bound) and a CPU baseline (the oracle on
-the host cores, bounded pixel sample).
+kernel (closest-hit traversal, HBM-bound), a CPU baseline (the oracle on
+the host cores, bounded pixel sample) and the time-to-target-RMSE record:
+the plain wavefront tracer's RMSE at --ttt-spp against an FP64 converged
+render of a random pixel subset is the target, and the wall-clock (HIP event
+frame times) each method needs to reach it is reported, the cached method
+training its network online from random weights with its warm-up counted.
 """
 from __future__ import annotations
 
@@ -55,12 +59,23 @@ def make_settings(depth, integrator):
     return s
 
 
-def cpu_baseline(scene, st, cam, W, H, budget_s):
+_ORACLE = {}
+
+
+def oracle_scene(scene):
+    """The host oracle's copy of the scene (built once, shared by the CPU
+    baseline and the FP64 reference)."""
     import oracle
+    if id(scene) not in _ORACLE:
+        t0 = time.time()
+        _ORACLE[id(scene)] = oracle.OracleScene(scene)
+        log(f"[cpu] oracle scene built in {time.time() - t0:.1f}s")
+    return _ORACLE[id(scene)]
+
+
+def cpu_baseline(scene, st, cam, W, H, budget_s):
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    t0 = time.time()
-    orc = oracle.OracleScene(scene)
-    log(f"[cpu] oracle scene built in {time.time() - t0:.1f}s")
+    orc = oracle_scene(scene)
     rng = np.random.default_rng(1234)
     n = 2048
     spent = 0.0
@@ -76,10 +91,63 @@ def cpu_baseline(scene, st, cam, W, H, budget_s):
         if spent >= budget_s * 0.5 or n >= 1 << 20:
             break
         n = int(min(1 << 20, n * max(2.0, min(8.0, (budget_s * 0.5) / max(dt, 1e-3)))))
-    orc.close()
     return {"value": total_samples / spent / 1e6, "unit": "Msamples/s", "cores": cores, "kind": "port",
             "sample": f"{total_samples} random pixels of the 1920x1080 frame, 1 sample each, depth "
                       f"{st.max_depth}, same scene/camera/seeds, oracle C++ (IEEE float) on {cores} threads"}
+
+
+def fp64_reference(scene, st, cam, W, H, pixels, spp):
+    """Converged FP64 render of a pixel subset on the host cores (the oracle's
+    double instantiation): the judge of the RMSE, never the thing measured."""
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    orc = oracle_scene(scene)
+    t0 = time.time()
+    ref = orc.render_pixels(st, cam, W, H, pixels, 0, spp, double=True, threads=cores)
+    log(f"[ref] FP64 reference: {len(pixels)} pixels x {spp} spp in {time.time() - t0:.1f}s on {cores} threads")
+    return ref.astype(np.float64)
+
+
+def rel_rmse(est, ref):
+    """RMSE relative to the reference, per channel, with a floor of 10 % of the
+    mean so dark pixels do not dominate."""
+    floor = 0.1 * ref.mean()
+    return float(np.sqrt(np.mean((est - ref) ** 2 / (ref ** 2 + floor ** 2))))
+
+
+def run_curve(ctx, pixels, ref, frames, checkpoints, warmup_frames=0, label=""):
+    """Renders `frames` frames of 1 spp (after `warmup_frames` frames whose
+    time counts but whose samples are discarded); returns [(seconds, spp,
+    rmse)] at the checkpoints.  Seconds are HIP-event frame times."""
+    seconds = 0.0
+    for _ in range(warmup_frames):
+        ctx.render(1)
+        seconds += ctx.timing()[0] / 1e3
+    if warmup_frames:
+        ctx.reset()
+    out = []
+    for f in range(1, frames + 1):
+        ctx.render(1)
+        seconds += ctx.timing()[0] / 1e3
+        if f in checkpoints:
+            est = ctx.estimate().reshape(-1, 3)[pixels].astype(np.float64)
+            out.append((seconds, f, rel_rmse(est, ref)))
+            log(f"[ttt] {label} {f:4d} spp  {seconds:7.3f} s  rel RMSE {out[-1][2]:.4f}")
+    return out
+
+
+def time_at(curve, target):
+    """First time the curve reaches the target RMSE (log-log interpolation
+    between checkpoints), or None."""
+    prev = None
+    for s, spp, e in curve:
+        if e <= target:
+            if prev is None or prev[2] <= e:
+                return s
+            ps, _, pe = prev
+            t = (math.log(pe) - math.log(target)) / (math.log(pe) - math.log(e))
+            return math.exp(math.log(ps) + t * (math.log(s) - math.log(ps)))
+        prev = (s, spp, e)
+    return None
 
 
 def main():
@@ -97,6 +165,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", default="", help="fixed knobs, e.g. refill=0,blocks_per_cu=8,sort=1")
+    ap.add_argument("--ttt-spp", type=int, default=64,
+                    help="time-to-target run: the plain tracer's RMSE at this many spp is the target (0: skip)")
+    ap.add_argument("--ttt-pixels", type=int, default=1024)
+    ap.add_argument("--ref-spp", type=int, default=1024, help="spp of the FP64 reference pixels")
+    ap.add_argument("--cache-warmup", type=int, default=8, help="cache training frames before accumulating")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -188,6 +261,38 @@ def main():
         except Exception as e:  # the baseline is reported, never fatal
             log(f"[cpu] baseline failed: {e}")
 
+    # time to a target RMSE against an FP64 converged render: plain wavefront
+    # vs wavefront + radiance cache (trained online from scratch, its warm-up
+    # frames counted)
+    ttt = None
+    if integ == abi.INTEGRATOR_WAVEFRONT and args.ttt_spp > 0:
+        rng = np.random.default_rng(77)
+        pix = np.sort(rng.choice(W * H, size=args.ttt_pixels, replace=False)).astype(np.int32)
+        ref = fp64_reference(scene, st, cam, W, H, pix, args.ref_spp)
+        checkpoints = sorted({1, 2, 4, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256} | {args.ttt_spp})
+        plain_cp = [c for c in checkpoints if c <= args.ttt_spp]
+        ctx.cache_configure(enabled=False, train=False)
+        ctx.reset()
+        plain = run_curve(ctx, pix, ref, args.ttt_spp, set(plain_cp), label="plain ")
+        target = plain[-1][2]
+        t_plain = plain[-1][0]
+        ctx.cache_configure(enabled=True, train=True, train_samples=65536, train_steps=4, learning_rate=1e-3,
+                            radiance_scale=1e4, seed=11)
+        cached = run_curve(ctx, pix, ref, args.ttt_spp, set(plain_cp), warmup_frames=args.cache_warmup,
+                           label="cached")
+        t_cached = time_at(cached, target)
+        ctx.cache_configure(enabled=False, train=False, seed=11)
+        ttt = {"target_rel_rmse": round(target, 5), "target": f"plain wavefront at {args.ttt_spp} spp",
+               "plain_s": round(t_plain, 4), "cached_s": None if t_cached is None else round(t_cached, 4),
+               "speedup": None if t_cached is None else round(t_plain / t_cached, 3),
+               "cached_rmse_at_equal_spp": round(cached[-1][2], 5),
+               "cache_warmup_frames": args.cache_warmup,
+               "reference": f"FP64 oracle render of {args.ttt_pixels} random pixels x {args.ref_spp} spp "
+                            f"(host cores)",
+               "curve_plain": [[round(s, 4), n, round(e, 5)] for s, n, e in plain],
+               "curve_cached": [[round(s, 4), n, round(e, 5)] for s, n, e in cached]}
+        log(f"[ttt] {ttt}")
+
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
@@ -198,7 +303,7 @@ def main():
                    "integrator": args.integrator, "width": W, "height": H, "spp": args.spp,
                    "max_depth": args.depth, "triangles": int(len(scene.positions)),
                    "instances": int(scene.num_instances), "parallelism": "single GPU", "tuning": tuning},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "time_to_target": ttt,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -84,11 +84,11 @@ def test_cache_learns_a_smooth_function(ctx):
 
     t = np.zeros((len(pos), 4), np.float32)
     t[:, :3] = f(ph, dirs, pos[:, 3])
-    print("[cache] 300 Adam steps on a smooth target", flush=True)
+    print("[cache] 1200 Adam steps (batches of 2048) on a smooth target", flush=True)
     first = ctx.cache_train(pos, dirs, t, steps=1)
     last = first
-    for _ in range(30):
-        last = ctx.cache_train(pos, dirs, t, steps=10)
+    for _ in range(150):
+        last = ctx.cache_train(pos, dirs, t, steps=8)
     hp, hd = queries(4096, 23)
     hph = (hp[:, :3] - BMIN) / (np.array(BMAX) - BMIN)
     pred = ctx.cache_infer(hp, hd)[:, :3]
@@ -96,8 +96,8 @@ def test_cache_learns_a_smooth_function(ctx):
     mse = ((pred - truth) ** 2).mean()
     base = ((truth - truth.mean(0)) ** 2).mean()
     print(f"loss {first:.4f} -> {last:.5f}; held-out mse {mse:.5f} vs constant predictor {base:.5f}")
-    assert last < 0.1 * first
-    assert mse < 0.1 * base
+    assert last < 0.02 * first
+    assert mse < 0.25 * base
 
 
 def _rel_mse(est, ref):
