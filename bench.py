@@ -135,6 +135,29 @@ def run_curve(ctx, pixels, ref, frames, checkpoints, warmup_frames=0, label=""):
     return out
 
 
+def batched_views(ctx, scene, n_views, spp, seed=64):
+    """The multi-view workload: n_views cameras around the instance box with
+    square resolutions log-uniform in [128, 1024], all traced as one set of
+    paths per sample (kpt_render_views)."""
+    rng = np.random.default_rng(seed)
+    views = []
+    for _ in range(n_views):
+        side = int(round(math.exp(rng.uniform(math.log(128), math.log(1024)))))
+        az, el = rng.uniform(0, 2 * math.pi), rng.uniform(0.1, 0.9)
+        eye = (50 + 110 * math.cos(el) * math.cos(az), 45 + 110 * math.sin(el), 50 + 110 * math.cos(el) * math.sin(az))
+        proj = scenes.perspective_fov_lh(scene.camera_fov, side, side, 1.0, 100.0)
+        views.append((proj, scenes.look_at_view(eye, (50.0, 45.0, 50.0)), side, side))
+    pixels = sum(w * h for _, _, w, h in views)
+    ctx.render_views(views, spp=1)                      # allocation and warm-up
+    ctx.render_views(views, spp=spp)
+    ms = ctx.timing()[0]
+    out = {"views": n_views, "pixels": int(pixels), "spp": spp, "ms": round(ms, 3),
+           "Msamples_per_s": round(pixels * spp / (ms / 1e3) / 1e6, 3),
+           "resolutions": "square, side log-uniform in [128, 1024]"}
+    log(f"[views] {out}")
+    return out
+
+
 def time_at(curve, target):
     """First time the curve reaches the target RMSE (log-log interpolation
     between checkpoints), or None."""
@@ -165,6 +188,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", default="", help="fixed knobs, e.g. refill=0,blocks_per_cu=8,sort=1")
+    ap.add_argument("--views", type=int, default=64, help="cameras of the batched multi-view leg (0: skip)")
     ap.add_argument("--ttt-spp", type=int, default=64,
                     help="time-to-target run: the plain tracer's RMSE at this many spp is the target (0: skip)")
     ap.add_argument("--ttt-pixels", type=int, default=1024)
@@ -261,6 +285,10 @@ def main():
         except Exception as e:  # the baseline is reported, never fatal
             log(f"[cpu] baseline failed: {e}")
 
+    views = None
+    if integ == abi.INTEGRATOR_WAVEFRONT and args.views > 0:
+        views = batched_views(ctx, scene, args.views, 4)
+
     # time to a target RMSE against an FP64 converged render: plain wavefront
     # vs wavefront + radiance cache (trained online from scratch, its warm-up
     # frames counted)
@@ -303,7 +331,7 @@ def main():
                    "integrator": args.integrator, "width": W, "height": H, "spp": args.spp,
                    "max_depth": args.depth, "triangles": int(len(scene.positions)),
                    "instances": int(scene.num_instances), "parallelism": "single GPU", "tuning": tuning},
-        "roofline": roof, "cpu_baseline": cpu, "time_to_target": ttt,
+        "roofline": roof, "cpu_baseline": cpu, "time_to_target": ttt, "batched_views": views,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

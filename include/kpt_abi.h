@@ -199,6 +199,15 @@ typedef struct kpt_cache_settings {
 
 #define KPT_CACHE_PARAMS 18624
 
+/* One camera of a multi-view batch: the matrices Renderer::setView takes
+ * (column-major) and the view's resolution. */
+typedef struct kpt_view {
+    float projection[16];
+    float view[16];
+    int32_t width;
+    int32_t height;
+} kpt_view;
+
 /* ---- context ---- */
 int kpt_get_abi_version(void);
 /* Renderer::initialize (renderer.cu:22-46). device < 0 picks the current one. */
@@ -307,6 +316,18 @@ int kpt_get_film(kpt_ctx* ctx, kpt_film_state* out);
 /* executeRendering's delta_time_ms (renderer.cu:103-105): the frame time the
  * auto-exposure adaptation uses for the following kpt_render frames. */
 int kpt_set_frame_delta(kpt_ctx* ctx, float delta_time_ms);
+
+/* ---- batched views (no reference counterpart: the reference renders one
+ * camera per executeRendering) ---- */
+/* Renders `spp` samples per pixel of n views of any resolutions as ONE set of
+ * paths: every wavefront stage's persistent kernels pull work from a single
+ * queue over all views' pixels.  Sample k of a pixel uses frame index k and
+ * the pixel's own seed, so each view's result equals rendering that view
+ * alone with kpt_render from frame 0.  out: DEVICE RGBA32F, views back to
+ * back (view v starts at the sum of the earlier views' width*height), the
+ * static-accumulation estimate after spp samples.  No G-buffer, post chain
+ * or cache; the context's frame buffers are untouched. */
+int kpt_render_views(kpt_ctx* ctx, const kpt_view* views, int32_t n, int32_t spp, float* out);
 
 /* ---- radiance cache (no reference counterpart) ---- */
 /* Applies the settings; the first call (and any call with a new seed)

@@ -182,6 +182,15 @@ class KptCacheSettings(C.Structure):
     ]
 
 
+class KptView(C.Structure):
+    _fields_ = [
+        ("projection", C.c_float * 16),
+        ("view", C.c_float * 16),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+    ]
+
+
 CACHE_PARAMS = 18624
 # (out, in) of the cache layers; weights are stored per layer row-major [out][in]
 CACHE_LAYERS = [(64, 32), (64, 64), (64, 64), (64, 64), (64, 64), (3, 64)]
@@ -236,6 +245,7 @@ SIGNATURES = [
     ("kpt_get_exposure", C.c_int, [_P, C.POINTER(KptExposure)]),
     ("kpt_get_film", C.c_int, [_P, C.POINTER(KptFilmState)]),
     ("kpt_set_frame_delta", C.c_int, [_P, C.c_float]),
+    ("kpt_render_views", C.c_int, [_P, C.POINTER(KptView), C.c_int32, C.c_int32, _P]),
     ("kpt_cache_configure", C.c_int, [_P, C.POINTER(KptCacheSettings)]),
     ("kpt_cache_get_settings", C.c_int, [_P, C.POINTER(KptCacheSettings)]),
     ("kpt_cache_get_weights", C.c_int, [_P, C.POINTER(C.c_float), C.c_int64]),

@@ -248,6 +248,26 @@ class Context:
     def set_frame_delta(self, ms):
         self._ck(self.L.kpt_set_frame_delta(self.h, float(ms)))
 
+    # ---- batched views
+    def render_views(self, views, spp=1):
+        """views: list of (projection16, view16, width, height).  Returns a
+        list of [H, W, 4] estimates (one wavefront over all views' pixels)."""
+        arr = (abi.KptView * len(views))()
+        total = 0
+        for k, (proj, view, w, h) in enumerate(views):
+            arr[k].projection[:] = [float(x) for x in np.asarray(proj, np.float32).reshape(-1)]
+            arr[k].view[:] = [float(x) for x in np.asarray(view, np.float32).reshape(-1)]
+            arr[k].width, arr[k].height = int(w), int(h)
+            total += int(w) * int(h)
+        out = DeviceBuffer(total * 16)
+        self._ck(self.L.kpt_render_views(self.h, arr, len(views), spp, out.ptr))
+        flat = out.download(np.zeros((total, 4), np.float32))
+        res, off = [], 0
+        for (_, _, w, h) in views:
+            res.append(flat[off:off + w * h].reshape(h, w, 4))
+            off += w * h
+        return res
+
     # ---- radiance cache
     def cache_configure(self, enabled=False, train=False, train_samples=65536, train_steps=4, learning_rate=1e-3,
                         radiance_scale=1e4, seed=1, bounds=None):

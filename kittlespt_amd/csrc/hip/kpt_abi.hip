@@ -76,6 +76,7 @@ struct kpt_ctx {
     kpt_trace_counters render_counts{};
     double timing[4] = {0, 0, 0, 0};
     bool tlas_dirty = false;
+    ViewConst* d_views = nullptr;    // multi-view batch cameras
     // post chain
     PostBuffers pb{};
     ExposureState h_expo{};
@@ -485,6 +486,7 @@ int kpt_ctx_destroy(kpt_ctx* ctx) {
         dev_free(&ctx->d_counts);
         dev_free(&ctx->pb.expo);
         dev_free(&ctx->pb.hist);
+        dev_free(&ctx->d_views);
         free_cache(ctx);
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     }
@@ -832,6 +834,63 @@ int kpt_set_view(kpt_ctx* ctx, const float projection[16], const float view[16])
     std::memcpy(ctx->cam.inv_proj, ip.m, 64);
     std::memcpy(ctx->cam.inv_view, iv.m, 64);
     if (!ctx->settings.use_temporal_accumulation) return kpt_reset_accumulation(ctx);
+    return KPT_OK;
+}
+
+int kpt_render_views(kpt_ctx* ctx, const kpt_view* views, int32_t n, int32_t spp, float* out) {
+    if (!ctx || !views || n <= 0 || spp < 0 || !out) return KPT_ERR_INVALID_ARG;
+    if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
+    if (!ctx->scene_ready) return fail(ctx, KPT_ERR_STATE, "no scene");
+    if (n > 65536) return fail(ctx, KPT_ERR_INVALID_ARG, "too many views");
+    std::vector<ViewConst> vc((size_t)n);
+    int64_t total = 0;
+    for (int32_t v = 0; v < n; ++v) {
+        const kpt_view& kv = views[v];
+        if (kv.width <= 0 || kv.height <= 0) return fail(ctx, KPT_ERR_INVALID_ARG, "view with an empty resolution");
+        // Renderer::setView's inverses (renderer.cu:342-349)
+        hm::Mat4 ip = hm::glm_inverse(hm::load(kv.projection));
+        hm::Mat4 iv = hm::glm_inverse(hm::load(kv.view));
+        ViewConst& c = vc[(size_t)v];
+        std::memset(&c, 0, sizeof(c));
+        std::memcpy(c.inv_view, iv.m, 64);
+        std::memcpy(c.inv_proj, ip.m, 64);
+        c.cam_pos[0] = iv.m[12]; c.cam_pos[1] = iv.m[13]; c.cam_pos[2] = iv.m[14];
+        c.width = kv.width;
+        c.height = kv.height;
+        c.offset = total;
+        total += (int64_t)kv.width * kv.height;
+    }
+    if (total >= ((int64_t)1 << 31)) return fail(ctx, KPT_ERR_INVALID_ARG, "batch exceeds 2^31 pixels");
+    ENSURE_TLAS(ctx);
+    int rc = alloc_wavefront(ctx, total);
+    if (rc != KPT_OK) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(dev_upload(&ctx->d_views, vc));
+    ViewBatch vb;
+    vb.views = ctx->d_views;
+    vb.nviews = n;
+    vb.total = total;
+    vb.out = (float4*)out;
+    HIPCHK(launch_clear(ctx->stream, vb.out, total));
+    SceneView S = scene_view(ctx);
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    for (int32_t k = 0; k < spp; ++k) {
+        FrameConst fc = make_frame(ctx, k);
+        HIPCHK(wavefront_frame(ctx->stream, S, fc, k, ctx->fb, ctx->ws, ctx->d_overflow,
+                               (int32_t)ctx->materials.size(), ctx->sort_by_material, ctx->tune, nullptr, nullptr,
+                               &vb));
+    }
+    HIPCHK(launch_views_mean(ctx->stream, vb.out, total, spp));
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->timing[0] = ms;
+    ctx->timing[1] = 0.0;
+    ctx->timing[2] = 0.0;
+    int32_t ovf = 0;
+    HIPCHK(hipMemcpy(&ovf, ctx->d_overflow, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (ovf) return fail(ctx, KPT_ERR_OVERFLOW, "traversal stack overflow (depth > 128)");
     return KPT_OK;
 }
 

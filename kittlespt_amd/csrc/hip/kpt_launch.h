@@ -108,6 +108,25 @@ struct WfInstrument {
     unsigned long long* counters;          // device [5] or nullptr (counted traversal)
 };
 
+// One camera of a multi-view batch (device array).
+struct alignas(16) ViewConst {
+    float inv_view[16];
+    float inv_proj[16];
+    float cam_pos[3];
+    int32_t width;
+    int32_t height;
+    int32_t pad;
+    int64_t offset;      // first path (pixel) of this view in the batch
+};
+// Many views of uneven resolution traced as one set of paths.
+struct ViewBatch {
+    const ViewConst* views;
+    int32_t nviews;
+    int64_t total;       // pixels of all views
+    float4* out;         // running sum per pixel, views back to back
+};
+hipError_t launch_views_mean(hipStream_t s, float4* out, int64_t n, int32_t spp);
+
 // The radiance cache of the wavefront tracer (null when unused).
 struct CacheHooks {
     const CacheBuffers* cb;
@@ -122,7 +141,7 @@ struct CacheHooks {
 hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& fc, int32_t frame,
                            const FrameBuffers& fb, WavefrontState& W, int32_t* overflow, int32_t num_materials,
                            int32_t sort_by_material, const TraceTuning& tune, WfInstrument* ins,
-                           const CacheHooks* cache);
+                           const CacheHooks* cache, const ViewBatch* views = nullptr);
 
 // Radiance-cache kernels (kpt_cache.hip).
 int cache_train_workgroups(int32_t batch);

@@ -496,14 +496,18 @@ KD v3 clamp_output(v3 v) {
 
 // Camera::generateRay with getShadingJob's pixel-centre mapping
 // (camera.cuh:16-26, shading_kernel.cuh:15-27).
-KD Ray camera_ray(const FrameConst& fc, int px, int py) {
-    float u = ((float)px + 0.5f) / (float)fc.width;
-    float v = ((float)py + 0.5f) / (float)fc.height;
+KD Ray camera_ray_m(const float* inv_view, const float* inv_proj, const float* cam_pos, int width, int height,
+                    int px, int py) {
+    float u = ((float)px + 0.5f) / (float)width;
+    float v = ((float)py + 0.5f) / (float)height;
     float nx = 2.0f * u - 1.0f, ny = 2.0f * v - 1.0f;
-    v4 tcs = mul4(fc.inv_proj, mk4(nx, ny, 1.0f, 1.0f));
-    v4 tws = mul4(fc.inv_view, mk4(normalize(xyz(tcs) / tcs.w), 0.0f));
+    v4 tcs = mul4(inv_proj, mk4(nx, ny, 1.0f, 1.0f));
+    v4 tws = mul4(inv_view, mk4(normalize(xyz(tcs) / tcs.w), 0.0f));
     v3 dir = normalize(xyz(tws));
-    return make_ray(mk3(fc.cam_pos[0], fc.cam_pos[1], fc.cam_pos[2]), dir);
+    return make_ray(mk3(cam_pos[0], cam_pos[1], cam_pos[2]), dir);
+}
+KD Ray camera_ray(const FrameConst& fc, int px, int py) {
+    return camera_ray_m(fc.inv_view, fc.inv_proj, fc.cam_pos, fc.width, fc.height, px, py);
 }
 
 // GBuffer pack (packing.cuh:13-17, 105-125, 178-197)

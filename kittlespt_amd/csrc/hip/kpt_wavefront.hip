@@ -94,6 +94,54 @@ __global__ __launch_bounds__(WF_BLOCK) void k_generate(FrameConst fc, int32_t fr
     if (i == 0) W.counters[C_QCOUNT] = (int32_t)npx;
 }
 
+// Multi-view generation: path i belongs to the view whose pixel range holds
+// it (binary search over the views' offsets); the seed and the camera ray are
+// those of that view's own pixel, so a view renders exactly as it would alone.
+__global__ __launch_bounds__(WF_BLOCK) void k_generate_views(FrameConst fc, int32_t frame, WavefrontState W,
+                                                             ViewBatch vb) {
+    const int64_t i = (int64_t)blockIdx.x * WF_BLOCK + threadIdx.x;
+    if (i >= vb.total) return;
+    int lo = 0, hi = vb.nviews - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (vb.views[mid].offset <= i) lo = mid; else hi = mid - 1;
+    }
+    const ViewConst& v = vb.views[lo];
+    const int64_t j = i - v.offset;
+    const int px = (int)(j % v.width), py = (int)(j / v.width);
+    uint32_t seed = init_seed(px, py, v.width, frame, fc.seed_mode);
+    (void)rng1(seed);
+    (void)rng1(seed);
+    Ray r = camera_ray_m(v.inv_view, v.inv_proj, v.cam_pos, v.width, v.height, px, py);
+    W.ray_o[i] = make_float4(r.o.x, r.o.y, r.o.z, bitsf((uint32_t)i));
+    W.ray_d[i] = make_float4(r.d.x, r.d.y, r.d.z, bitsf(seed));
+    W.beta[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    W.L[i] = make_float4(0.0f, 0.0f, 0.0f, bitsf(0u));
+    W.prev_pos[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    W.queue[0][i] = (int32_t)i;
+    if (i == 0) W.counters[C_QCOUNT] = (int32_t)vb.total;
+}
+
+// Multi-view finish: clampOutput and the running sum of the samples.
+__global__ __launch_bounds__(WF_BLOCK) void k_finish_views(WavefrontState W, ViewBatch vb) {
+    const int64_t i = (int64_t)blockIdx.x * WF_BLOCK + threadIdx.x;
+    if (i >= vb.total) return;
+    const float4 l = W.L[i];
+    v3 L = (mk3(1.0f, 1.0f, 1.0f) * 1.0f) * mk3(l.x, l.y, l.z);
+    L = clamp_output(L);
+    const float4 a = vb.out[i];
+    v3 acc = mk3(a.x, a.y, a.z) + L;
+    vb.out[i] = make_float4(acc.x, acc.y, acc.z, 1.0f);
+}
+
+__global__ __launch_bounds__(WF_BLOCK) void k_views_mean(float4* out, int64_t n, int32_t spp) {
+    const int64_t i = (int64_t)blockIdx.x * WF_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = out[i];
+    v3 est = mk3(a.x, a.y, a.z) / (float)spp;
+    out[i] = make_float4(est.x, est.y, est.z, 1.0f);
+}
+
 // Closest hit of every live path.  COUNT accumulates traversal events.
 template <bool COUNT>
 __global__ __launch_bounds__(WF_BLOCK) void k_trace(SceneView S, WavefrontState W, int parity, int pull_slot,
@@ -428,7 +476,7 @@ int g_num_cus = 0;
 hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& fc, int32_t frame,
                            const FrameBuffers& fb, WavefrontState& W, int32_t* overflow, int32_t num_materials,
                            int32_t sort_by_material, const TraceTuning& tune, WfInstrument* ins,
-                           const CacheHooks* cache) {
+                           const CacheHooks* cache, const ViewBatch* views) {
     if (ins) ins->launches = 0;
     CacheBuffers cb{};
     CacheFrame cf{};
@@ -444,14 +492,17 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess) g_num_cus = prop.multiProcessorCount;
         if (g_num_cus <= 0) g_num_cus = 256;
     }
-    const int64_t npx = (int64_t)fc.width * fc.height;
+    const int64_t npx = views ? views->total : (int64_t)fc.width * fc.height;
     const unsigned gblocks = (unsigned)((npx + WF_BLOCK - 1) / WF_BLOCK);
     // persistent grids: 8 workgroups of 256 lanes per CU
     const unsigned pgrid = (unsigned)std::max<int64_t>(
         1, std::min<int64_t>((int64_t)g_num_cus * std::max(1, tune.blocks_per_cu), gblocks));
     const int nkeys = std::min(MAX_KEYS, num_materials + 1);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, W.counters, 0, C_PULL + 3 * MAX_BOUNCE_SLOTS);
-    hipLaunchKernelGGL(k_generate, dim3(gblocks), dim3(WF_BLOCK), 0, s, fc, frame, W);
+    if (views)
+        hipLaunchKernelGGL(k_generate_views, dim3(gblocks), dim3(WF_BLOCK), 0, s, fc, frame, W, *views);
+    else
+        hipLaunchKernelGGL(k_generate, dim3(gblocks), dim3(WF_BLOCK), 0, s, fc, frame, W);
     const int max_bounces = std::min(fc.max_depth + 1, MAX_BOUNCE_SLOTS);
     for (int bounce = 0; bounce < max_bounces; ++bounce) {
         const int parity = bounce & 1;
@@ -489,7 +540,17 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
         hipError_t e = launch_cache_targets(s, cb, W.L, cache->enc.scale);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_finish, dim3(gblocks), dim3(WF_BLOCK), 0, s, S, fc, frame, W, fb);
+    if (views)
+        hipLaunchKernelGGL(k_finish_views, dim3(gblocks), dim3(WF_BLOCK), 0, s, W, *views);
+    else
+        hipLaunchKernelGGL(k_finish, dim3(gblocks), dim3(WF_BLOCK), 0, s, S, fc, frame, W, fb);
+    return hipGetLastError();
+}
+
+hipError_t launch_views_mean(hipStream_t s, float4* out, int64_t n, int32_t spp) {
+    if (n <= 0 || spp <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_views_mean, dim3((unsigned)((n + WF_BLOCK - 1) / WF_BLOCK)), dim3(WF_BLOCK), 0, s, out, n,
+                       spp);
     return hipGetLastError();
 }
 
