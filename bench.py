@@ -234,7 +234,9 @@ def main():
         ctx.set_tuning(**{k: int(v) for k, v in kv.items()})
     elif integ == abi.INTEGRATOR_WAVEFRONT and not args.no_autotune:
         t0 = time.time()
-        best, ms, _ = autotune.tune(ctx, spp=1, log=log)
+        # per-launch trace events need the eager launch path: graph replay is
+        # measured separately below
+        best, ms, _ = autotune.tune(ctx, spp=1, log=log, skip=("graph",))
         log(f"[bench] autotuned in {time.time() - t0:.1f}s: {best} ({ms:.2f} ms per 1-spp frame)")
     tuning = ctx.tuning() if integ == abi.INTEGRATOR_WAVEFRONT else None
 
@@ -285,6 +287,21 @@ def main():
         except Exception as e:  # the baseline is reported, never fatal
             log(f"[cpu] baseline failed: {e}")
 
+    # the same steps with every frame replayed as a captured hipGraph
+    graph_rec = None
+    if integ == abi.INTEGRATOR_WAVEFRONT:
+        ctx.set_tuning(graph=1)
+        ctx.render(args.spp)
+        ctx.reset()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            ctx.render(args.spp)
+        gwall = time.perf_counter() - t0
+        ctx.set_tuning(graph=0)
+        graph_rec = {"value": round(samples / gwall / 1e6, 3), "ms_per_step": round(gwall * 1e3 / args.steps, 3),
+                     "note": "same workload, each 1-spp frame captured as one hipGraph and replayed"}
+        log(f"[bench] graph replay: {graph_rec}")
+
     views = None
     if integ == abi.INTEGRATOR_WAVEFRONT and args.views > 0:
         views = batched_views(ctx, scene, args.views, 4)
@@ -332,6 +349,7 @@ def main():
                    "max_depth": args.depth, "triangles": int(len(scene.positions)),
                    "instances": int(scene.num_instances), "parallelism": "single GPU", "tuning": tuning},
         "roofline": roof, "cpu_baseline": cpu, "time_to_target": ttt, "batched_views": views,
+        "graph_replay": graph_rec,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

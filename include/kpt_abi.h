@@ -295,7 +295,9 @@ int kpt_set_trace_counting(kpt_ctx* ctx, int32_t enable);
  * reference counterpart): knobs[0] lane refill (0/1), [1] queue entries a
  * wave claims per atomic (64..65536), [2] idle lanes that trigger a refill
  * (1..64), [3] persistent workgroups per compute unit (1..64), [4] sort
- * shading by material (0/1).  n >= 5. */
+ * shading by material (0/1), [5] (when n >= 6) capture every frame as a
+ * hipGraph and replay it (0/1; the per-launch trace times of kpt_last_timing
+ * are then not measured).  n >= 5. */
 int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n);
 int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n);
 int kpt_get_render_counters(const kpt_ctx* ctx, kpt_trace_counters* out);

@@ -2,7 +2,7 @@
 
 Knobs (kpt_set_tuning): lane refill on/off, queue entries claimed per wave
 atomic, refill threshold, persistent workgroups per compute unit, material
-sort on/off.  The tuner renders real frames of the caller's scene and camera
+sort on/off, per-frame hipGraph replay on/off.  The tuner renders real frames of the caller's scene and camera
 and keeps the fastest setting by coordinate descent (each knob swept with the
 others fixed, two passes).  Results are cached in a JSON file keyed by the
 workload so later runs reuse them.
@@ -19,8 +19,9 @@ SPACE = {
     "chunk": [64, 128, 256, 512],
     "refill_min": [8, 16, 32],
     "sort": [0, 1],
+    "graph": [0, 1],
 }
-ORDER = ["refill", "blocks_per_cu", "sort", "chunk", "refill_min"]
+ORDER = ["refill", "blocks_per_cu", "sort", "graph", "chunk", "refill_min"]
 
 
 def measure(ctx, cfg, spp=1, repeats=2):
@@ -33,14 +34,14 @@ def measure(ctx, cfg, spp=1, repeats=2):
     return best
 
 
-def tune(ctx, spp=1, repeats=2, passes=2, log=None):
+def tune(ctx, spp=1, repeats=2, passes=2, log=None, skip=()):
     cur = ctx.tuning()
     cur_t = measure(ctx, cur, spp, repeats)
     history = [(dict(cur), cur_t)]
     for _ in range(passes):
         improved = False
         for knob in ORDER:
-            if knob in ("chunk", "refill_min") and not cur["refill"]:
+            if knob in skip or (knob in ("chunk", "refill_min") and not cur["refill"]):
                 continue
             for v in SPACE[knob]:
                 if v == cur[knob]:

@@ -209,18 +209,20 @@ class Context:
         self._ck(self.L.kpt_last_timing(self.h, out))
         return list(out)
 
-    TUNING_KEYS = ("refill", "chunk", "refill_min", "blocks_per_cu", "sort")
+    TUNING_KEYS = ("refill", "chunk", "refill_min", "blocks_per_cu", "sort", "graph")
 
     def tuning(self):
-        k = (C.c_int32 * 5)()
-        self._ck(self.L.kpt_get_tuning(self.h, k, 5))
+        n = len(self.TUNING_KEYS)
+        k = (C.c_int32 * n)()
+        self._ck(self.L.kpt_get_tuning(self.h, k, n))
         return dict(zip(self.TUNING_KEYS, list(k)))
 
     def set_tuning(self, **kw):
         cur = self.tuning()
         cur.update(kw)
-        k = (C.c_int32 * 5)(*[int(cur[n]) for n in self.TUNING_KEYS])
-        self._ck(self.L.kpt_set_tuning(self.h, k, 5))
+        n = len(self.TUNING_KEYS)
+        k = (C.c_int32 * n)(*[int(cur[x]) for x in self.TUNING_KEYS])
+        self._ck(self.L.kpt_set_tuning(self.h, k, n))
 
     def set_trace_counting(self, enable):
         self._ck(self.L.kpt_set_trace_counting(self.h, 1 if enable else 0))

@@ -68,7 +68,10 @@ struct kpt_ctx {
     FrameBuffers fb{};
     WavefrontState ws{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, tev0 = nullptr, tev1 = nullptr;
-    WfInstrument ins{};
+    static constexpr int INS_RING = 4;   // frames whose trace events may be in flight
+    WfInstrument ins[INS_RING]{};
+    int32_t use_graph = 0;               // capture each frame as a hipGraph and replay it
+    hipGraphExec_t graph_exec = nullptr;
     TraceTuning tune{0, 128, 16, 8};
     int32_t sort_by_material = 1;
     bool count_trace = false;
@@ -455,10 +458,13 @@ int kpt_ctx_create(int device, kpt_ctx** out) {
         delete ctx;
         return KPT_ERR_HIP;
     }
-    for (int k = 0; k < WF_MAX_TRACE_LAUNCHES; ++k) {
-        if (hipEventCreate(&ctx->ins.begin[k]) != hipSuccess || hipEventCreate(&ctx->ins.end[k]) != hipSuccess) {
-            delete ctx;
-            return KPT_ERR_HIP;
+    for (int r = 0; r < kpt_ctx::INS_RING; ++r) {
+        for (int k = 0; k < WF_MAX_TRACE_LAUNCHES; ++k) {
+            if (hipEventCreate(&ctx->ins[r].begin[k]) != hipSuccess ||
+                hipEventCreate(&ctx->ins[r].end[k]) != hipSuccess) {
+                delete ctx;
+                return KPT_ERR_HIP;
+            }
         }
     }
     *out = ctx;
@@ -479,10 +485,13 @@ int kpt_ctx_destroy(kpt_ctx* ctx) {
         if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
         if (ctx->tev0) (void)hipEventDestroy(ctx->tev0);
         if (ctx->tev1) (void)hipEventDestroy(ctx->tev1);
-        for (int k = 0; k < WF_MAX_TRACE_LAUNCHES; ++k) {
-            if (ctx->ins.begin[k]) (void)hipEventDestroy(ctx->ins.begin[k]);
-            if (ctx->ins.end[k]) (void)hipEventDestroy(ctx->ins.end[k]);
+        for (int r = 0; r < kpt_ctx::INS_RING; ++r) {
+            for (int k = 0; k < WF_MAX_TRACE_LAUNCHES; ++k) {
+                if (ctx->ins[r].begin[k]) (void)hipEventDestroy(ctx->ins[r].begin[k]);
+                if (ctx->ins[r].end[k]) (void)hipEventDestroy(ctx->ins[r].end[k]);
+            }
         }
+        if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
         dev_free(&ctx->d_counts);
         dev_free(&ctx->pb.expo);
         dev_free(&ctx->pb.hist);
@@ -982,61 +991,108 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     double trace_ms = 0.0;
     int trace_launches = 0;
+    const bool wavefront = ctx->settings.integrator == KPT_INTEGRATOR_WAVEFRONT;
+    const bool graph = ctx->use_graph && !ctx->count_trace;
+    // trace times of a frame whose ring slot is about to be reused (or at the end)
+    auto harvest = [&](WfInstrument& ins) -> hipError_t {
+        if (ins.launches <= 0) return hipSuccess;
+        hipError_t e = hipEventSynchronize(ins.end[ins.launches - 1]);
+        for (int j = 0; e == hipSuccess && j < ins.launches; ++j) {
+            float t = 0.0f;
+            e = hipEventElapsedTime(&t, ins.begin[j], ins.end[j]);
+            trace_ms += t;
+        }
+        trace_launches += ins.launches;
+        ins.launches = 0;
+        return e;
+    };
+    for (int r = 0; r < kpt_ctx::INS_RING; ++r) ctx->ins[r].launches = 0;
     for (int32_t k = 0; k < spp; ++k) {
         FrameConst fc = make_frame(ctx, ctx->frame_index);
-        if (ctx->settings.integrator == KPT_INTEGRATOR_WAVEFRONT) {
-            ctx->ins.counters = ctx->count_trace ? ctx->d_counts : nullptr;
-            hooks.cf.frame = (uint32_t)ctx->frame_index;
-            HIPCHK(wavefront_frame(ctx->stream, S, fc, ctx->frame_index, ctx->fb, ctx->ws, ctx->d_overflow,
-                                   (int32_t)ctx->materials.size(), ctx->sort_by_material, ctx->tune, &ctx->ins,
-                                   use_cache ? &hooks : nullptr));
-            if (use_cache && cs.train) {
-                // Adam steps over this frame's suffixes (they shape the next frames)
-                for (int32_t st = 0; st < std::max(1, cs.train_steps); ++st) {
-                    HIPCHK(launch_cache_train_step(ctx->stream, ctx->cb, hooks.enc, ctx->cb.t_pos, ctx->cb.t_dir,
-                                                   ctx->cb.t_target, ctx->cb.counters + 1, 0, ctx->cb.train_cap,
-                                                   (int64_t)st * batch, batch, cs.learning_rate,
-                                                   ++ctx->cache_steps));
+        WfInstrument* ins = nullptr;
+        if (wavefront && !graph) {
+            ins = &ctx->ins[k % kpt_ctx::INS_RING];
+            HIPCHK(harvest(*ins));
+            ins->counters = ctx->count_trace ? ctx->d_counts : nullptr;
+        }
+        hooks.cf.frame = (uint32_t)ctx->frame_index;
+        // one executeRendering frame, enqueued without a host synchronisation
+        auto enqueue = [&]() -> hipError_t {
+            hipError_t e;
+            if (wavefront) {
+                e = wavefront_frame(ctx->stream, S, fc, ctx->frame_index, ctx->fb, ctx->ws, ctx->d_overflow,
+                                    (int32_t)ctx->materials.size(), ctx->sort_by_material, ctx->tune, ins,
+                                    use_cache ? &hooks : nullptr);
+                if (e != hipSuccess) return e;
+                if (use_cache && cs.train) {
+                    // Adam steps over this frame's suffixes (they shape the next frames)
+                    for (int32_t st = 0; st < std::max(1, cs.train_steps); ++st) {
+                        e = launch_cache_train_step(ctx->stream, ctx->cb, hooks.enc, ctx->cb.t_pos, ctx->cb.t_dir,
+                                                    ctx->cb.t_target, ctx->cb.counters + 1, 0, ctx->cb.train_cap,
+                                                    (int64_t)st * batch, batch, cs.learning_rate,
+                                                    ++ctx->cache_steps);
+                        if (e != hipSuccess) return e;
+                    }
+                }
+            } else {
+                e = launch_megakernel(ctx->stream, S, fc, ctx->frame_index, ctx->fb, ctx->d_overflow);
+                if (e != hipSuccess) return e;
+            }
+            // modulate, auto exposure, bloom, post (renderer.cu:117-170)
+            PostParams pp{};
+            pp.width = ctx->width;
+            pp.height = ctx->height;
+            pp.temporal = ctx->settings.use_temporal_accumulation ? 1 : 0;
+            pp.auto_exposure = ctx->settings.enable_auto_exposure ? 1 : 0;
+            pp.bloom = ctx->settings.bloom_generate_bloom ? 1 : 0;
+            pp.karis = ctx->settings.bloom_use_karis_average ? 1 : 0;
+            pp.bloom_final_blend = ctx->settings.bloom_final_blend;
+            pp.bloom_internal_blend = ctx->settings.bloom_internal_blend;
+            pp.frame_delta_ms = ctx->frame_delta_ms;
+            e = launch_post_frame(ctx->stream, ctx->fb, ctx->pb, pp, ctx->num_cus);
+            if (e != hipSuccess) return e;
+            // end of executeRendering: gbuffer -> prev_gbuffer (renderer.cu:186)
+            return hipMemcpyAsync(ctx->fb.prev_gbuffer, ctx->fb.gbuffer, (size_t)npx * sizeof(uint4),
+                                  hipMemcpyDeviceToDevice, ctx->stream);
+        };
+        if (graph) {
+            // the frame's ~80 launches captured once and replayed as one graph;
+            // later frames update the executable graph in place
+            HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+            hipError_t e = enqueue();
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+            if (e == hipSuccess) e = e2;
+            if (e == hipSuccess && ctx->graph_exec) {
+                hipGraphNode_t bad = nullptr;
+                hipGraphExecUpdateResult res;
+                if (hipGraphExecUpdate(ctx->graph_exec, g, &bad, &res) != hipSuccess ||
+                    res != hipGraphExecUpdateSuccess) {
+                    (void)hipGetLastError();
+                    (void)hipGraphExecDestroy(ctx->graph_exec);
+                    ctx->graph_exec = nullptr;
                 }
             }
-            // per-launch trace times of this frame (events already recorded)
-            HIPCHK(hipEventSynchronize(ctx->ins.end[ctx->ins.launches - 1]));
-            for (int j = 0; j < ctx->ins.launches; ++j) {
-                float t = 0.0f;
-                HIPCHK(hipEventElapsedTime(&t, ctx->ins.begin[j], ctx->ins.end[j]));
-                trace_ms += t;
-            }
-            trace_launches += ctx->ins.launches;
+            if (e == hipSuccess && !ctx->graph_exec) e = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
+            if (e == hipSuccess) e = hipGraphLaunch(ctx->graph_exec, ctx->stream);
+            if (g) (void)hipGraphDestroy(g);
+            HIPCHK(e);
         } else {
-            HIPCHK(launch_megakernel(ctx->stream, S, fc, ctx->frame_index, ctx->fb, ctx->d_overflow));
-            trace_launches += 1;
+            HIPCHK(enqueue());
         }
-        // modulate, auto exposure, bloom, post (renderer.cu:117-170)
-        PostParams pp{};
-        pp.width = ctx->width;
-        pp.height = ctx->height;
-        pp.temporal = ctx->settings.use_temporal_accumulation ? 1 : 0;
-        pp.auto_exposure = ctx->settings.enable_auto_exposure ? 1 : 0;
-        pp.bloom = ctx->settings.bloom_generate_bloom ? 1 : 0;
-        pp.karis = ctx->settings.bloom_use_karis_average ? 1 : 0;
-        pp.bloom_final_blend = ctx->settings.bloom_final_blend;
-        pp.bloom_internal_blend = ctx->settings.bloom_internal_blend;
-        pp.frame_delta_ms = ctx->frame_delta_ms;
-        HIPCHK(launch_post_frame(ctx->stream, ctx->fb, ctx->pb, pp, ctx->num_cus));
-        // end of executeRendering: gbuffer -> prev_gbuffer, camera prev <- curr,
-        // frame_index++ (renderer.cu:186-194)
-        HIPCHK(hipMemcpyAsync(ctx->fb.prev_gbuffer, ctx->fb.gbuffer, (size_t)npx * sizeof(uint4),
-                              hipMemcpyDeviceToDevice, ctx->stream));
+        if (!wavefront) trace_launches += 1;
+        // camera prev <- curr, frame_index++ (renderer.cu:188-194)
         std::memcpy(ctx->cam.prev_inv_view, ctx->cam.inv_view, 64);
         std::memcpy(ctx->cam.prev_inv_proj, ctx->cam.inv_proj, 64);
         ctx->frame_index++;
     }
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     HIPCHK(hipEventSynchronize(ctx->ev1));
+    for (int r = 0; r < kpt_ctx::INS_RING; ++r) HIPCHK(harvest(ctx->ins[r]));
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->timing[0] = ms;
-    ctx->timing[1] = (ctx->settings.integrator == KPT_INTEGRATOR_WAVEFRONT) ? trace_ms : ms;
+    ctx->timing[1] = (wavefront && !graph) ? trace_ms : (wavefront ? 0.0 : ms);
     ctx->timing[2] = trace_launches;
     if (ctx->count_trace) {
         unsigned long long h[8];
@@ -1250,6 +1306,10 @@ int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n) {
     ctx->tune.refill_min = knobs[2];
     ctx->tune.blocks_per_cu = knobs[3];
     ctx->sort_by_material = knobs[4];
+    if (n >= 6) {
+        if (knobs[5] < 0 || knobs[5] > 1) return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
+        ctx->use_graph = knobs[5];
+    }
     return KPT_OK;
 }
 
@@ -1260,6 +1320,7 @@ int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n) {
     knobs[2] = ctx->tune.refill_min;
     knobs[3] = ctx->tune.blocks_per_cu;
     knobs[4] = ctx->sort_by_material;
+    if (n >= 6) knobs[5] = ctx->use_graph;
     return KPT_OK;
 }
 
