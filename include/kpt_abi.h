@@ -297,7 +297,9 @@ int kpt_set_trace_counting(kpt_ctx* ctx, int32_t enable);
  * (1..64), [3] persistent workgroups per compute unit (1..64), [4] sort
  * shading by material (0/1), [5] (when n >= 6) capture every frame as a
  * hipGraph and replay it (0/1; the per-launch trace times of kpt_last_timing
- * are then not measured).  n >= 5. */
+ * are then not measured), [6] (when n >= 7) sort continuation and shadow
+ * rays by direction octant and Morton cell of the origin before tracing
+ * (0/1; not applied inside captured graphs).  n >= 5. */
 int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n);
 int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n);
 int kpt_get_render_counters(const kpt_ctx* ctx, kpt_trace_counters* out);

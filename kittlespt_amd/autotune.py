@@ -20,8 +20,9 @@ SPACE = {
     "refill_min": [8, 16, 32],
     "sort": [0, 1],
     "graph": [0, 1],
+    "ray_sort": [0, 1],
 }
-ORDER = ["refill", "blocks_per_cu", "sort", "graph", "chunk", "refill_min"]
+ORDER = ["ray_sort", "refill", "blocks_per_cu", "sort", "graph", "chunk", "refill_min"]
 
 
 def measure(ctx, cfg, spp=1, repeats=2):

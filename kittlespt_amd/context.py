@@ -209,7 +209,7 @@ class Context:
         self._ck(self.L.kpt_last_timing(self.h, out))
         return list(out)
 
-    TUNING_KEYS = ("refill", "chunk", "refill_min", "blocks_per_cu", "sort", "graph")
+    TUNING_KEYS = ("refill", "chunk", "refill_min", "blocks_per_cu", "sort", "graph", "ray_sort")
 
     def tuning(self):
         n = len(self.TUNING_KEYS)

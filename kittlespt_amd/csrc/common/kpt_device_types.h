@@ -130,6 +130,8 @@ struct alignas(16) FrameConst {
     int32_t tlas_root;
     int32_t temporal;      // integrator_use_temporal_accumulation
     float prev_cam_pos[3]; // prev_inv_view column 3 (testReprojection, kernels.cu:215)
+    float scene_lo[3];     // TLAS root box (spatial ray-sort keys)
+    float scene_inv[3];    // 1 / extent per axis
 };
 
 // Scene pointers passed by value to kernels.

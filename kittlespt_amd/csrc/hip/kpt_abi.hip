@@ -72,7 +72,7 @@ struct kpt_ctx {
     WfInstrument ins[INS_RING]{};
     int32_t use_graph = 0;               // capture each frame as a hipGraph and replay it
     hipGraphExec_t graph_exec = nullptr;
-    TraceTuning tune{0, 128, 16, 8};
+    TraceTuning tune{0, 128, 16, 8, 0};
     int32_t sort_by_material = 1;
     bool count_trace = false;
     unsigned long long* d_counts = nullptr;
@@ -210,6 +210,12 @@ FrameConst make_frame(const kpt_ctx* c, int32_t /*frame*/) {
     fc.prev_cam_pos[1] = c->cam.prev_inv_view[13];
     fc.prev_cam_pos[2] = c->cam.prev_inv_view[14];
     fc.temporal = c->settings.use_temporal_accumulation ? 1 : 0;
+    for (int k = 0; k < 3; ++k) {
+        const float lo = c->tlas.nodes.empty() ? 0.0f : c->tlas.nodes[0].lo[k];
+        const float hi = c->tlas.nodes.empty() ? 1.0f : c->tlas.nodes[0].hi[k];
+        fc.scene_lo[k] = lo;
+        fc.scene_inv[k] = (hi > lo) ? 1.0f / (hi - lo) : 1.0f;
+    }
     const kpt_settings& s = c->settings;
     // getSunDirection (integrator.cuh:222-230)
     float phi = s.sun_phi_rad, th = s.sun_theta_rad;
@@ -385,6 +391,9 @@ void free_wavefront(kpt_ctx* c) {
     dev_free(&w.queue[0]); dev_free(&w.queue[1]); dev_free(&w.shadow_queue); dev_free(&w.shadow_o);
     dev_free(&w.shadow_d); dev_free(&w.shadow_c); dev_free(&w.shadow_z); dev_free(&w.sort_keys);
     dev_free(&w.sort_tmp); dev_free(&w.counters); dev_free(&w.hist);
+    for (int k = 0; k < 2; ++k) { dev_free(&w.rkeys[k]); dev_free(&w.rvals[k]); }
+    if (w.rtemp) { (void)hipFree(w.rtemp); w.rtemp = nullptr; }
+    w.rtemp_bytes = 0;
     w.capacity = 0;
 }
 
@@ -406,6 +415,12 @@ int alloc_wavefront(kpt_ctx* ctx, int64_t n) {
     HIPCHK(hipMalloc((void**)&w.sort_keys, i4)); HIPCHK(hipMalloc((void**)&w.sort_tmp, i4));
     HIPCHK(hipMalloc((void**)&w.counters, 256 * sizeof(int32_t)));
     HIPCHK(hipMalloc((void**)&w.hist, (size_t)64 * ((n + 255) / 256) * sizeof(int32_t)));
+    for (int k = 0; k < 2; ++k) {
+        HIPCHK(hipMalloc((void**)&w.rkeys[k], i4));
+        HIPCHK(hipMalloc((void**)&w.rvals[k], i4));
+    }
+    w.rtemp_bytes = ray_sort_temp_bytes(n);
+    HIPCHK(hipMalloc(&w.rtemp, std::max<size_t>(w.rtemp_bytes, 256)));
     w.capacity = n;
     return KPT_OK;
 }
@@ -1020,8 +1035,11 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
         auto enqueue = [&]() -> hipError_t {
             hipError_t e;
             if (wavefront) {
+                // the library radix sort stays out of captured graphs
+                TraceTuning tune = ctx->tune;
+                if (graph) tune.ray_sort = 0;
                 e = wavefront_frame(ctx->stream, S, fc, ctx->frame_index, ctx->fb, ctx->ws, ctx->d_overflow,
-                                    (int32_t)ctx->materials.size(), ctx->sort_by_material, ctx->tune, ins,
+                                    (int32_t)ctx->materials.size(), ctx->sort_by_material, tune, ins,
                                     use_cache ? &hooks : nullptr);
                 if (e != hipSuccess) return e;
                 if (use_cache && cs.train) {
@@ -1310,6 +1328,10 @@ int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n) {
         if (knobs[5] < 0 || knobs[5] > 1) return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
         ctx->use_graph = knobs[5];
     }
+    if (n >= 7) {
+        if (knobs[6] < 0 || knobs[6] > 1) return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
+        ctx->tune.ray_sort = knobs[6];
+    }
     return KPT_OK;
 }
 
@@ -1321,6 +1343,7 @@ int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n) {
     knobs[3] = ctx->tune.blocks_per_cu;
     knobs[4] = ctx->sort_by_material;
     if (n >= 6) knobs[5] = ctx->use_graph;
+    if (n >= 7) knobs[6] = ctx->tune.ray_sort;
     return KPT_OK;
 }
 

@@ -46,6 +46,11 @@ struct WavefrontState {
     int32_t* sort_tmp;
     int32_t* counters;    // [0] = next queue size, [1] = shadow count, [2] = overflow
     int32_t* hist;        // material histogram for the shading sort
+    // spatial ray sort: keys and path/shadow indices (in, out), sort workspace
+    uint32_t* rkeys[2];
+    int32_t* rvals[2];
+    void* rtemp;
+    size_t rtemp_bytes;
 };
 
 // Exposure state resident in device memory: the auto-exposure program's
@@ -96,6 +101,7 @@ struct TraceTuning {
     int chunk;         // queue entries claimed per wave atomic (refill mode)
     int refill_min;    // idle lanes that trigger a refill
     int blocks_per_cu; // persistent grid size per compute unit
+    int ray_sort;      // sort secondary and shadow rays by direction octant + Morton cell of the origin
 };
 
 // Events bracketing every closest-hit trace launch of a frame (recorded on the
@@ -126,6 +132,8 @@ struct ViewBatch {
     float4* out;         // running sum per pixel, views back to back
 };
 hipError_t launch_views_mean(hipStream_t s, float4* out, int64_t n, int32_t spp);
+// Workspace bytes of the spatial ray sort over n rays.
+size_t ray_sort_temp_bytes(int64_t n);
 
 // The radiance cache of the wavefront tracer (null when unused).
 struct CacheHooks {

@@ -10,6 +10,8 @@
 // shades), so the radiance of every pixel is bit-identical to the megakernel.
 // Trace and shade stages are persistent: a grid sized to the device pulls
 // 256-path chunks from a per-stage atomic counter until the queue is drained.
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "kpt_integrator.h"
 #include "kpt_launch.h"
 #include "kpt_temporal.h"
@@ -142,15 +144,62 @@ __global__ __launch_bounds__(WF_BLOCK) void k_views_mean(float4* out, int64_t n,
     out[i] = make_float4(est.x, est.y, est.z, 1.0f);
 }
 
+// ---- spatial ray sort: rays with nearby origins (and, for continuation rays,
+// the same direction octant) end up in the same wave, so a wave's traversals
+// share nodes and cache lines.  Results do not depend on the order.
+constexpr uint32_t RKEY_INVALID = 1u << 15;   // sorts after every valid 15-bit key
+constexpr int RKEY_BITS = 16;
+
+KD uint32_t spread4(uint32_t v) {   // bits b3..b0 -> b3 _ _ b2 _ _ b1 _ _ b0
+    return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4) | ((v & 8u) << 6);
+}
+KD uint32_t ray_key(const FrameConst& fc, float4 o, float4 d, bool with_dir) {
+    auto cell = [&](float x, int a) {
+        float t = (x - fc.scene_lo[a]) * fc.scene_inv[a];
+        t = fminf_(fmaxf_(t, 0.0f), 0.9999f);
+        return (uint32_t)(t * 16.0f);
+    };
+    uint32_t m = spread4(cell(o.x, 0)) | (spread4(cell(o.y, 1)) << 1) | (spread4(cell(o.z, 2)) << 2);
+    if (with_dir) m |= ((d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u)) << 12;
+    return m;
+}
+
+__global__ __launch_bounds__(WF_BLOCK) void k_ray_keys(FrameConst fc, WavefrontState W, int parity, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * WF_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const int count = W.counters[C_QCOUNT + parity];
+    if (i < count) {
+        const int p = W.queue[parity][i];
+        W.rkeys[0][i] = ray_key(fc, W.ray_o[p], W.ray_d[p], true);
+        W.rvals[0][i] = p;
+    } else {
+        W.rkeys[0][i] = RKEY_INVALID;
+        W.rvals[0][i] = -1;
+    }
+}
+
+__global__ __launch_bounds__(WF_BLOCK) void k_shadow_keys(FrameConst fc, WavefrontState W, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * WF_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const int count = W.counters[C_SHADOW];
+    if (i < count) {
+        W.rkeys[0][i] = ray_key(fc, W.shadow_o[i], W.shadow_d[i], true);
+        W.rvals[0][i] = (int32_t)i;
+    } else {
+        W.rkeys[0][i] = RKEY_INVALID;
+        W.rvals[0][i] = -1;
+    }
+}
+
 // Closest hit of every live path.  COUNT accumulates traversal events.
 template <bool COUNT>
 __global__ __launch_bounds__(WF_BLOCK) void k_trace(SceneView S, WavefrontState W, int parity, int pull_slot,
                                                     int32_t* overflow, unsigned long long* counters,
-                                                    TraceTuning tune) {
+                                                    TraceTuning tune, const int32_t* sorted) {
     __shared__ uint32_t s_stack[LDS_STACK][TRACE_BLOCK];
     uint32_t spill[SPILL_STACK];
     const int count = W.counters[C_QCOUNT + parity];
-    const int32_t* q = W.queue[parity];
+    const int32_t* q = sorted ? sorted : W.queue[parity];
     int32_t* ctr = &W.counters[pull_slot];
     uint32_t* lds = &s_stack[0][threadIdx.x];
     bool ovf = false;
@@ -401,7 +450,7 @@ __device__ __forceinline__ void shadow_resolve(WavefrontState& W, int k, bool oc
 }
 
 __global__ __launch_bounds__(WF_BLOCK) void k_shadow(SceneView S, WavefrontState W, int pull_slot, int32_t* overflow,
-                                                     TraceTuning tune) {
+                                                     TraceTuning tune, const int32_t* order) {
     __shared__ uint32_t s_stack[LDS_STACK][TRACE_BLOCK];
     uint32_t spill[SPILL_STACK];
     const int count = W.counters[C_SHADOW];
@@ -412,8 +461,9 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shadow(SceneView S, WavefrontState
         for (;;) {
             const int base = wave_claim(ctr, WAVE);
             if (base >= count) break;
-            const int k = base + lane_id();
-            if (k >= count) continue;
+            const int kq = base + lane_id();
+            if (kq >= count) continue;
+            const int k = order ? order[kq] : kq;
             const float4 o = W.shadow_o[k], d = W.shadow_d[k];
             AnyTrav tr;
             if (tr.init(S, make_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z)), 0.0f, o.w, lds, spill)) {
@@ -432,7 +482,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shadow(SceneView S, WavefrontState
             const int kk = feeder.feed(ctr, count, !busy);
             bool done_now = false;
             if (kk >= 0) {
-                k = kk;
+                k = order ? order[kk] : kk;
                 const float4 o = W.shadow_o[k], d = W.shadow_d[k];
                 busy = tr.init(S, make_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z)), 0.0f, o.w, lds, spill);
                 done_now = !busy;
@@ -509,14 +559,24 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
         // next queue and shadow count start at zero
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, W.counters + C_QCOUNT + (parity ^ 1), 0, 1);
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, W.counters + C_SHADOW, 0, 1);
+        // continuation rays (not the pixel-ordered camera rays) in spatial order
+        const int32_t* trace_q = nullptr;
+        if (tune.ray_sort && bounce > 0 && W.rtemp) {
+            hipLaunchKernelGGL(k_ray_keys, dim3(gblocks), dim3(WF_BLOCK), 0, s, fc, W, parity, npx);
+            size_t bytes = W.rtemp_bytes;
+            hipError_t e = rocprim::radix_sort_pairs(W.rtemp, bytes, W.rkeys[0], W.rkeys[1], W.rvals[0], W.rvals[1],
+                                                     (size_t)npx, 0, RKEY_BITS, s);
+            if (e != hipSuccess) return e;
+            trace_q = W.rvals[1];
+        }
         const bool timed = ins && ins->launches < WF_MAX_TRACE_LAUNCHES;
         if (timed) (void)hipEventRecord(ins->begin[ins->launches], s);
         if (ins && ins->counters) {
             hipLaunchKernelGGL(k_trace<true>, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, W, parity, C_PULL + 3 * bounce,
-                               overflow, ins->counters, tune);
+                               overflow, ins->counters, tune, trace_q);
         } else {
             hipLaunchKernelGGL(k_trace<false>, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, W, parity, C_PULL + 3 * bounce,
-                               overflow, (unsigned long long*)nullptr, tune);
+                               overflow, (unsigned long long*)nullptr, tune, trace_q);
         }
         if (timed) (void)hipEventRecord(ins->end[ins->launches++], s);
         const int32_t* shade_q = W.queue[parity];
@@ -529,8 +589,17 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
         }
         hipLaunchKernelGGL(k_shade, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity, C_PULL + 3 * bounce + 1,
                            shade_q, cb, cf);
+        const int32_t* shadow_order = nullptr;
+        if (tune.ray_sort && W.rtemp) {
+            hipLaunchKernelGGL(k_shadow_keys, dim3(gblocks), dim3(WF_BLOCK), 0, s, fc, W, npx);
+            size_t bytes = W.rtemp_bytes;
+            hipError_t e = rocprim::radix_sort_pairs(W.rtemp, bytes, W.rkeys[0], W.rkeys[1], W.rvals[0], W.rvals[1],
+                                                     (size_t)npx, 0, RKEY_BITS, s);
+            if (e != hipSuccess) return e;
+            shadow_order = W.rvals[1];
+        }
         hipLaunchKernelGGL(k_shadow, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, W, C_PULL + 3 * bounce + 2, overflow,
-                           tune);
+                           tune, shadow_order);
     }
     if (cf.enabled) {
         hipError_t e = launch_cache_infer_paths(s, cb, cache->enc, W.L, npx, cache->num_cus);
@@ -545,6 +614,14 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
     else
         hipLaunchKernelGGL(k_finish, dim3(gblocks), dim3(WF_BLOCK), 0, s, S, fc, frame, W, fb);
     return hipGetLastError();
+}
+
+size_t ray_sort_temp_bytes(int64_t n) {
+    size_t bytes = 0;
+    if (rocprim::radix_sort_pairs(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (int32_t*)nullptr,
+                                  (int32_t*)nullptr, (size_t)n, 0, RKEY_BITS) != hipSuccess)
+        return 0;
+    return bytes;
 }
 
 hipError_t launch_views_mean(hipStream_t s, float4* out, int64_t n, int32_t spp) {
