@@ -181,19 +181,35 @@ TlasTree build_tlas(const std::vector<Aabb>& inst) {
         blas[ptr] = i;
         lr[ptr++] = 0u;
     }
+    // Active boxes in list order (structure of arrays, mirroring ids[]) so the
+    // best-match scan streams through memory and vectorises; the arithmetic per
+    // candidate is the reference's, and the scan keeps its first minimum.
+    std::vector<float> slo[3], shi[3], sa_buf(N);
+    for (int k = 0; k < 3; ++k) {
+        slo[k].resize(N);
+        shi[k].resize(N);
+        for (int32_t i = 0; i < N; ++i) { slo[k][i] = box[ids[i]].lo[k]; shi[k][i] = box[ids[i]].hi[k]; }
+    }
+    auto fmaxf_ = [](float a, float b) { return a > b ? a : b; };   // finite inputs: fmax up to the sign of 0
+    auto fminf_ = [](float a, float b) { return a < b ? a : b; };
     // findBestMatch: smallest half area of the merged box, first minimum wins
     auto best_match = [&](int32_t n, int32_t A) {
+        const float ax0 = slo[0][A], ay0 = slo[1][A], az0 = slo[2][A];
+        const float ax1 = shi[0][A], ay1 = shi[1][A], az1 = shi[2][A];
+        const float *l0 = slo[0].data(), *l1 = slo[1].data(), *l2 = slo[2].data();
+        const float *h0 = shi[0].data(), *h1 = shi[1].data(), *h2 = shi[2].data();
+        float* sa = sa_buf.data();
+        for (int32_t B = 0; B < n; ++B) {
+            const float ex = fmaxf_(ax1, h0[B]) - fminf_(ax0, l0[B]);
+            const float ey = fmaxf_(ay1, h1[B]) - fminf_(ay0, l1[B]);
+            const float ez = fmaxf_(az1, h2[B]) - fminf_(az0, l2[B]);
+            sa[B] = ex * ey + ey * ez + ez * ex;
+        }
         float smallest = INFINITY;
         int32_t best = -1;
-        const Box& a = box[ids[A]];
         for (int32_t B = 0; B < n; ++B) {
             if (B == A) continue;
-            const Box& b = box[ids[B]];
-            float ex = std::fmax(a.hi[0], b.hi[0]) - std::fmin(a.lo[0], b.lo[0]);
-            float ey = std::fmax(a.hi[1], b.hi[1]) - std::fmin(a.lo[1], b.lo[1]);
-            float ez = std::fmax(a.hi[2], b.hi[2]) - std::fmin(a.lo[2], b.lo[2]);
-            float sa = ex * ey + ey * ez + ez * ex;
-            if (sa < smallest) { smallest = sa; best = B; }
+            if (sa[B] < smallest) { smallest = sa[B]; best = B; }
         }
         return best;
     };
@@ -209,6 +225,10 @@ TlasTree build_tlas(const std::vector<Aabb>& inst) {
             lr[ptr] = (uint32_t)na + ((uint32_t)nb << 16);
             ids[A] = ptr++;
             ids[B] = ids[left - 1];
+            for (int k = 0; k < 3; ++k) {
+                slo[k][A] = m.lo[k]; shi[k][A] = m.hi[k];
+                slo[k][B] = slo[k][left - 1]; shi[k][B] = shi[k][left - 1];
+            }
             B = best_match(--left, A);
         } else {
             A = B;

@@ -95,6 +95,26 @@ def test_product_trees_equal_oracle_instanced():
     _compare_trees(scenes.scene_instanced_small())
 
 
+def test_tlas_of_1000_moving_instances_equals_oracle():
+    """The TLAS of the 1,000-instance layout, rebuilt after moving 100
+    instances (the per-frame update of the animated workload), node for node."""
+    sc = scenes.scene_instanced(num_instances=1000, tex_size=8, tris_range=(60, 120), num_blas=8, num_materials=4)
+    ctx = Context(-2)
+    ctx.upload(sc)
+    orc = oracle.OracleScene(sc)
+    rng = np.random.default_rng(5)
+    models = np.array(sc.instance_model[:100], np.float32).reshape(100, 16).copy()
+    models[:, 12:15] += rng.uniform(-3, 3, (100, 3)).astype(np.float32)
+    ctx.set_instance_transforms(0, models)
+    for i in range(100):
+        orc.set_instance_transform(i, models[i])
+    pt, ot = ctx.tlas(), orc.tlas()
+    assert len(pt) == len(ot) == 2000
+    assert np.array_equal(pt.view(np.uint32), ot.view(np.uint32))
+    ctx.close()
+    orc.close()
+
+
 def _leaves(nodes, root=0):
     out = []
     st = [root]
