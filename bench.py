@@ -246,6 +246,7 @@ def main():
     ap.add_argument("--tuning", default="", help="fixed knobs, e.g. refill=0,blocks_per_cu=8,sort=1")
     ap.add_argument("--views", type=int, default=64, help="cameras of the batched multi-view leg (0: skip)")
     ap.add_argument("--full4k", type=int, default=12, help="frames of the animated 4K full-frame leg (0: skip)")
+    ap.add_argument("--no-graph-leg", action="store_true", help="skip the hipGraph replay leg")
     ap.add_argument("--ttt-spp", type=int, default=64,
                     help="time-to-target run: the plain tracer's RMSE at this many spp is the target (0: skip)")
     ap.add_argument("--ttt-pixels", type=int, default=1024)
@@ -346,7 +347,7 @@ def main():
 
     # the same steps with every frame replayed as a captured hipGraph
     graph_rec = None
-    if integ == abi.INTEGRATOR_WAVEFRONT:
+    if integ == abi.INTEGRATOR_WAVEFRONT and not args.no_graph_leg:
         ctx.set_tuning(graph=1)
         ctx.render(args.spp)
         ctx.reset()

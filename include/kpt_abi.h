@@ -357,6 +357,11 @@ int kpt_cache_get_gradient(kpt_ctx* ctx, float* host_out, int64_t n);
 /* out[0] loss of the last step, [1] samples in it, [2] queries of the last
  * frame, [3] training records of the last frame, [4] Adam steps so far. */
 int kpt_cache_stats(kpt_ctx* ctx, double out[5]);
+/* Host copies of the last frame's cache queries (in slot order): positions
+ * n x 4 (xyz, roughness), directions n x 4 (outgoing xyz, bit-cast path
+ * index), throughputs n x 4.  *count receives the number of queries; at most
+ * cap are copied. */
+int kpt_cache_get_queries(kpt_ctx* ctx, float* pos, float* dir, float* beta, int64_t cap, int64_t* count);
 
 #ifdef __cplusplus
 }

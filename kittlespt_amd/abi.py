@@ -254,6 +254,7 @@ SIGNATURES = [
     ("kpt_cache_train", C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_int32, C.POINTER(C.c_float)]),
     ("kpt_cache_get_gradient", C.c_int, [_P, C.POINTER(C.c_float), C.c_int64]),
     ("kpt_cache_stats", C.c_int, [_P, C.POINTER(C.c_double)]),
+    ("kpt_cache_get_queries", C.c_int, [_P, _P, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
 ]
 
 

@@ -316,6 +316,17 @@ class Context:
         self._ck(self.L.kpt_cache_train(self.h, p.ptr, d.ptr, t.ptr, len(pa), steps, C.byref(loss)))
         return loss.value
 
+    def cache_queries(self):
+        """(positions [n,4], directions [n,4] with bit-cast path index in w,
+        throughputs [n,4]) of the last frame, in slot order."""
+        cnt = C.c_int64(0)
+        self._ck(self.L.kpt_cache_get_queries(self.h, None, None, None, 0, C.byref(cnt)))
+        n = cnt.value
+        p, d, b = (np.zeros((n, 4), np.float32) for _ in range(3))
+        self._ck(self.L.kpt_cache_get_queries(self.h, p.ctypes.data_as(C.c_void_p), d.ctypes.data_as(C.c_void_p),
+                                              b.ctypes.data_as(C.c_void_p), n, C.byref(cnt)))
+        return p, d, b
+
     def cache_stats(self):
         out = (C.c_double * 5)()
         self._ck(self.L.kpt_cache_stats(self.h, out))

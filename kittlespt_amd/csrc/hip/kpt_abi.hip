@@ -1261,6 +1261,25 @@ int kpt_cache_get_gradient(kpt_ctx* ctx, float* host_out, int64_t n) {
     return KPT_OK;
 }
 
+int kpt_cache_get_queries(kpt_ctx* ctx, float* pos, float* dir, float* beta, int64_t cap, int64_t* count) {
+    if (!ctx || !count || cap < 0 || (cap > 0 && (!pos || !dir || !beta))) return KPT_ERR_INVALID_ARG;
+    *count = 0;
+    if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
+    if (!ctx->cb.counters || !ctx->cb.q_pos) return KPT_OK;
+    int32_t c = 0;
+    HIPCHK(hipMemcpyAsync(&c, ctx->cb.counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int64_t n = std::min<int64_t>(std::min<int64_t>(c, ctx->cb.query_cap), cap);
+    *count = std::min<int64_t>(c, ctx->cb.query_cap);
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(pos, ctx->cb.q_pos, (size_t)n * 16, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(dir, ctx->cb.q_dir, (size_t)n * 16, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(beta, ctx->cb.q_beta, (size_t)n * 16, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    return KPT_OK;
+}
+
 int kpt_cache_stats(kpt_ctx* ctx, double out[5]) {
     if (!ctx || !out) return KPT_ERR_INVALID_ARG;
     for (int k = 0; k < 5; ++k) out[k] = 0.0;

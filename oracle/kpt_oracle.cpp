@@ -1285,7 +1285,16 @@ template <class R> V3<R> SunDir(const kpt_settings& st) {  // integrator.cuh:222
 }
 
 // Integrator::Li, integrator.cuh:241-402
-template <class R> V3<R> Li(const FrameConst& fc, Ray<R> ray, Sampler<R>& sampler, VisSurf<R>* vs) {
+// A path that ends in the radiance cache at its bounce-1 vertex: the query
+// the renderer evaluates there (position, roughness, outgoing direction,
+// throughput); `hit` is false for paths that never reach that vertex.
+struct CacheQuery {
+    bool hit = false;
+    float pos[3], rough, dir[3], beta[3];
+};
+
+template <class R> V3<R> Li(const FrameConst& fc, Ray<R> ray, Sampler<R>& sampler, VisSurf<R>* vs,
+                            CacheQuery* cq = nullptr) {
     const orc_scene* S = fc.S;
     const int32_t max_depth = fc.st.max_depth;
     V3<R> L(0, 0, 0), beta(1, 1, 1);
@@ -1337,6 +1346,15 @@ template <class R> V3<R> Li(const FrameConst& fc, Ray<R> ray, Sampler<R>& sample
             vs->albedo = alb; vs->depth = si.distance; vs->normal = si.n; vs->position = si.pos;
             vs->inst = si.inst; vs->prim = si.prim;
             bsdf.albedo = V3<R>(1, 1, 1);
+        }
+        if (cq && depth == 1) {
+            // the cache completes this path: no NEE, no continuation
+            cq->hit = true;
+            cq->pos[0] = (float)si.pos.x; cq->pos[1] = (float)si.pos.y; cq->pos[2] = (float)si.pos.z;
+            cq->rough = (float)bsdf.roughness;
+            cq->dir[0] = (float)-ray.d.x; cq->dir[1] = (float)-ray.d.y; cq->dir[2] = (float)-ray.d.z;
+            cq->beta[0] = (float)beta.x; cq->beta[1] = (float)beta.y; cq->beta[2] = (float)beta.z;
+            break;
         }
         if (any_nonspec) bsdf.regularize();
         if (bsdf.nonSpecular()) {
@@ -1800,6 +1818,35 @@ void orc_render_pixels(const orc_scene* s, const kpt_settings* settings, const k
             out_v[4 * i] = vel.x; out_v[4 * i + 1] = vel.y;
             out_v[4 * i + 2] = (float)vs.prim; out_v[4 * i + 3] = (float)vs.inst;
         }
+    });
+}
+
+// One frame in which every path ends in the radiance cache at its bounce-1
+// vertex (no training paths): per pixel the radiance gathered before that
+// vertex's suffix (unclamped; out_L n x 3) and the cache query
+// (out_q n x 12: hit, position xyz, roughness, direction xyz, throughput xyz,
+// unused).
+void orc_render_cache_queries(const orc_scene* s, const kpt_settings* settings, const kpt_camera* cam, int32_t W,
+                              int32_t H, int32_t frame, float* out_L, float* out_q, int32_t threads) {
+    FrameConst fc = MakeFrame(s, settings, cam);
+    ParallelFor((int64_t)W * H, threads, [&](int64_t i) {
+        int px = (int)(i % W), py = (int)(i / W);
+        Sampler<float> sm;
+        sm.initPixelSeed(px, py, W, frame, fc.st.seed_mode);
+        (void)sm.get2D();
+        Ray<float> ray = CameraRay<float>(fc, px, py, W, H);
+        VisSurf<float> vs;
+        CacheQuery cq;
+        V3<float> L = Li<float>(fc, ray, sm, &vs, &cq);
+        V3<float> w(1.0f, 1.0f, 1.0f);
+        L = (w * 1.0f) * L;
+        out_L[3 * i] = L.x; out_L[3 * i + 1] = L.y; out_L[3 * i + 2] = L.z;
+        float* q = out_q + 12 * i;
+        q[0] = cq.hit ? 1.0f : 0.0f;
+        for (int k = 0; k < 3; ++k) { q[1 + k] = cq.pos[k]; q[5 + k] = cq.dir[k]; q[8 + k] = cq.beta[k]; }
+        q[4] = cq.rough;
+        q[11] = 0.0f;
+        if (!cq.hit) for (int k = 1; k < 12; ++k) q[k] = 0.0f;
     });
 }
 

@@ -100,6 +100,57 @@ def test_cache_learns_a_smooth_function(ctx):
     assert mse < 0.25 * base
 
 
+def test_cached_render_matches_oracle():
+    """One frame in which every path ends in the cache at the bounce-1 vertex:
+    the GPU's queries (position, roughness, direction, throughput of each
+    path) equal the oracle's bit for bit, paths without a query are
+    bit-exact, and the cached pixels equal radiance-so-far + throughput *
+    scale * max(net(query), 0) within the network's tolerance."""
+    from kittlespt_amd.context import Context
+    sc = scenes.scene_plain(subdiv=3)
+    W, H = 48, 32
+    box = ((-20.0, -1.0, -20.0), (20.0, 3.0, 20.0))
+    st = settings(max_depth=4, integrator=abi.INTEGRATOR_WAVEFRONT, sun_theta=sc.sun_theta)
+    w = he_weights(31) * np.float32(0.5)
+    ctx = Context(0)
+    orc = oracle.OracleScene(sc)
+    try:
+        ctx.upload(sc)
+        ctx.set_settings(st)
+        ctx.resize(W, H)
+        p, v = view_proj(sc, W, H)
+        ctx.set_view(p, v)
+        cam = ctx.camera()
+        ctx.cache_configure(enabled=True, train=False, seed=7, bounds=box, radiance_scale=1e4)
+        ctx.cache_set_weights(w)
+        print("[cache] one cached frame 48x32", flush=True)
+        ctx.render(1)
+        gp, gd, gb = ctx.cache_queries()
+        est = ctx.estimate().reshape(-1, 3)
+    finally:
+        ctx.close()
+    Lb, q = orc.render_cache_queries(st, cam, W, H, 0)
+    orc.close()
+    hit = np.nonzero(q[:, 0] > 0)[0]
+    paths = gd[:, 3].view(np.uint32).astype(np.int64)
+    assert len(hit) > 100 and np.array_equal(np.sort(paths), hit)
+    order = np.argsort(paths)
+    gp, gd, gb = gp[order], gd[order], gb[order]
+    u = np.uint32
+    assert np.array_equal(gp[:, :3].view(u), q[hit, 1:4].view(u)) and np.array_equal(gp[:, 3].view(u), q[hit, 4].view(u))
+    assert np.array_equal(gd[:, :3].view(u), q[hit, 5:8].view(u)) and np.array_equal(gb[:, :3].view(u), q[hit, 8:11].view(u))
+    o = oracle.cache_forward(w, gp, gd, box[0], box[1], mode=0).astype(np.float32)
+    rad = np.float32(1e4) * np.maximum(o, np.float32(0))
+    want = Lb.copy()
+    want[hit] = Lb[hit] + gb[:, :3] * rad
+    want = np.where(np.isfinite(want).all(1, keepdims=True), np.clip(want, 0, 1e8), 0).astype(np.float32)
+    miss = np.setdiff1d(np.arange(W * H), hit)
+    assert np.array_equal(est[miss].view(u), want[miss].view(u))
+    rel = np.abs(est[hit] - want[hit]) / (np.abs(want[hit]) + 1e-3 * np.abs(want).mean())
+    print(f"cached pixels: max rel err {rel.max():.2e}, mean {rel.mean():.2e}")
+    assert rel.max() < 5e-3 and rel.mean() < 1e-4
+
+
 def _rel_mse(est, ref):
     ref = ref.astype(np.float64)
     m = np.mean(ref)
