@@ -88,9 +88,9 @@ def cpu_baseline(scene, st, cam, W, H, budget_s):
         spent += dt
         total_samples += n
         log(f"[cpu] {n} pixel-samples in {dt:.2f}s")
-        if spent >= budget_s * 0.5 or n >= 1 << 20:
+        if spent >= budget_s * 0.5:
             break
-        n = int(min(1 << 20, n * max(2.0, min(8.0, (budget_s * 0.5) / max(dt, 1e-3)))))
+        n = int(min(1 << 20, n * max(2.0, min(8.0, (budget_s * 0.5 - spent) / max(dt, 1e-3)))))
     return {"value": total_samples / spent / 1e6, "unit": "Msamples/s", "cores": cores, "kind": "port",
             "sample": f"{total_samples} random pixels of the 1920x1080 frame, 1 sample each, depth "
                       f"{st.max_depth}, same scene/camera/seeds, oracle C++ (IEEE float) on {cores} threads"}
@@ -102,7 +102,8 @@ def fp64_reference(scene, st, cam, W, H, pixels, spp):
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     orc = oracle_scene(scene)
     t0 = time.time()
-    ref = orc.render_pixels(st, cam, W, H, pixels, 0, spp, double=True, threads=cores)
+    # frames from 2^20 on: seeds the GPU's first frames never use
+    ref = orc.render_pixels(st, cam, W, H, pixels, 1 << 20, spp, double=True, threads=cores)
     log(f"[ref] FP64 reference: {len(pixels)} pixels x {spp} spp in {time.time() - t0:.1f}s on {cores} threads")
     return ref.astype(np.float64)
 
@@ -131,7 +132,11 @@ def run_curve(ctx, pixels, ref, frames, checkpoints, warmup_frames=0, label=""):
         if f in checkpoints:
             est = ctx.estimate().reshape(-1, 3)[pixels].astype(np.float64)
             out.append((seconds, f, rel_rmse(est, ref)))
-            log(f"[ttt] {label} {f:4d} spp  {seconds:7.3f} s  rel RMSE {out[-1][2]:.4f}")
+            extra = ""
+            if ctx.cache_settings().enabled:
+                cs = ctx.cache_stats()
+                extra = f"  (cache loss {cs['loss']:.4f}, {int(cs['train_records'])} records)"
+            log(f"[ttt] {label} {f:4d} spp  {seconds:7.3f} s  rel RMSE {out[-1][2]:.4f}{extra}")
     return out
 
 
@@ -203,6 +208,7 @@ def full_frame_4k(ctx, scene, frames, seed=100):
         times.append(ctx.timing()[0])
     out = ctx.readback(4)
     ctx.cache_configure(enabled=False, train=False, seed=13)
+    ctx.set_instance_transforms(0, scene.instance_model[:100])   # the later legs render the scene at rest
     steady = times[2:] if len(times) > 4 else times
     rec = {"width": W, "height": H, "frames": frames, "animated_instances": 100,
            "gpu_ms_per_frame": round(float(np.mean(steady)), 3),
@@ -247,10 +253,10 @@ def main():
     ap.add_argument("--views", type=int, default=64, help="cameras of the batched multi-view leg (0: skip)")
     ap.add_argument("--full4k", type=int, default=12, help="frames of the animated 4K full-frame leg (0: skip)")
     ap.add_argument("--no-graph-leg", action="store_true", help="skip the hipGraph replay leg")
-    ap.add_argument("--ttt-spp", type=int, default=64,
+    ap.add_argument("--ttt-spp", type=int, default=256,
                     help="time-to-target run: the plain tracer's RMSE at this many spp is the target (0: skip)")
-    ap.add_argument("--ttt-pixels", type=int, default=1024)
-    ap.add_argument("--ref-spp", type=int, default=1024, help="spp of the FP64 reference pixels")
+    ap.add_argument("--ttt-pixels", type=int, default=2048)
+    ap.add_argument("--ref-spp", type=int, default=8192, help="spp of the FP64 reference pixels")
     ap.add_argument("--cache-warmup", type=int, default=8, help="cache training frames before accumulating")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (HBM counter calibration)")

@@ -185,7 +185,7 @@ typedef struct kpt_film_state {
  * cores, FP32 accumulation; trained online with Adam (beta 0.9 / 0.99,
  * eps 1e-15) on a relative L2 loss over the tracer's own path suffixes. */
 typedef struct kpt_cache_settings {
-    int32_t enabled;          /* non-training paths end in the cache at the bounce-1 vertex */
+    int32_t enabled;          /* non-training paths end in the cache at the bounce-1 vertex (after its NEE) */
     int32_t train;            /* record suffixes and train every frame */
     int32_t train_samples;    /* training paths per frame (65536) */
     int32_t train_steps;      /* Adam steps per frame; batch = train_samples / train_steps */

@@ -283,6 +283,11 @@ class Context:
         self._ck(self.L.kpt_cache_configure(self.h, C.byref(s)))
         return s
 
+    def cache_settings(self):
+        s = abi.KptCacheSettings()
+        self._ck(self.L.kpt_cache_get_settings(self.h, C.byref(s)))
+        return s
+
     def cache_weights(self):
         w = np.zeros(abi.CACHE_PARAMS, np.float32)
         self._ck(self.L.kpt_cache_get_weights(self.h, abi.fptr(w), len(w)))

@@ -54,7 +54,7 @@ struct CacheBuffers {
     float4* t_pos;            // training records: position, roughness
     float4* t_dir;            // direction, bit-cast path index
     float4* t_beta;           // throughput at the vertex
-    float4* t_lbefore;        // radiance of the path before the vertex's suffix
+    float4* t_lbefore;        // radiance of the path up to the vertex's next-event estimate
     float4* t_target;         // suffix radiance / throughput / scale
     int32_t* counters;        // [0] queries, [1] training records
     int64_t query_cap, train_cap;

@@ -364,11 +364,14 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, 
                 W.vis_p[p] = make_float4(s.pos.x, s.pos.y, s.pos.z, bitsf((uint32_t)s.prim));
                 b.albedo = mk3(1.0f, 1.0f, 1.0f);
             }
-            // radiance cache at the bounce-1 vertex: training paths record the
-            // radiance so far (their suffix becomes the target), the others
-            // end here and are completed by the network
+            if (any_nonspec) bsdf_regularize(b);
+            // radiance cache at the bounce-1 vertex, after its next-event
+            // estimate (which stays exact): training paths record the radiance
+            // so far (their suffix becomes the target), the others end here
+            // and the network completes what their continuation would add
             bool to_cache = false;
-            if (depth == 1 && (cf.enabled || cf.train)) {
+            int32_t trec = -1;
+            if (depth == 1 && fc.max_depth > 1 && (cf.enabled || cf.train)) {
                 const uint32_t hs = pcg_hash((uint32_t)p ^ (cf.frame * 0x9E3779B9u)) & 0xFFFFFFu;
                 const float4 qpos = make_float4(s.pos.x, s.pos.y, s.pos.z, b.roughness);
                 const float4 qdir = make_float4(-ray.d.x, -ray.d.y, -ray.d.z, bitsf((uint32_t)p));
@@ -378,7 +381,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, 
                         cb.t_pos[r] = qpos;
                         cb.t_dir[r] = qdir;
                         cb.t_beta[r] = make_float4(beta.x, beta.y, beta.z, 0.0f);
-                        cb.t_lbefore[r] = make_float4(L.x, L.y, L.z, 0.0f);
+                        trec = r;
                     }
                 } else if (cf.enabled) {
                     const int r = atomicAdd(&cb.counters[0], 1);
@@ -390,20 +393,21 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, 
                     }
                 }
             }
-            if (any_nonspec) bsdf_regularize(b);
-            if (!to_cache && !(b.alpha < 0.001f)) {
+            if (!(b.alpha < 0.001f)) {
                 NeeResult ne = next_event(S, fc, s, b, ray.d, beta, seed);
                 if (ne.trace) {
                     int sidx = atomicAdd(&W.counters[C_SHADOW], 1);
                     W.shadow_queue[sidx] = p;
                     W.shadow_o[sidx] = make_float4(ne.org.x, ne.org.y, ne.org.z, ne.tmax);
-                    W.shadow_d[sidx] = make_float4(ne.dir.x, ne.dir.y, ne.dir.z, 0.0f);
+                    W.shadow_d[sidx] = make_float4(ne.dir.x, ne.dir.y, ne.dir.z, bitsf((uint32_t)trec));
                     W.shadow_c[sidx] = make_float4(ne.unoccl.x, ne.unoccl.y, ne.unoccl.z, 0.0f);
                     W.shadow_z[sidx] = make_float4(ne.occl.x, ne.occl.y, ne.occl.z, 0.0f);
                 } else {
                     L = L + ne.occl;
                 }
             }
+            // a deferred estimate rewrites this once its visibility is known
+            if (trec >= 0) cb.t_lbefore[trec] = make_float4(L.x, L.y, L.z, 0.0f);
             const bool at_max = (depth == fc.max_depth);
             depth++;
             if (!at_max && !to_cache) {
@@ -441,16 +445,21 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, 
     }
 }
 
-// Visibility of the deferred next-event estimates; adds their term to L.
-__device__ __forceinline__ void shadow_resolve(WavefrontState& W, int k, bool occluded) {
+// Visibility of the deferred next-event estimates; adds their term to L.  An
+// estimate made at a cache-training vertex (record index in shadow_d.w, -1
+// for none) also moves that record's radiance-so-far past itself.
+__device__ __forceinline__ void shadow_resolve(WavefrontState& W, float4* t_lbefore, int k, bool occluded) {
     const int p = W.shadow_queue[k];
     const float4 c = occluded ? W.shadow_z[k] : W.shadow_c[k];
     const float4 l = W.L[p];
-    W.L[p] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
+    const float4 nl = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
+    W.L[p] = nl;
+    const int32_t rec = (int32_t)fbits(W.shadow_d[k].w);
+    if (rec >= 0) t_lbefore[rec] = make_float4(nl.x, nl.y, nl.z, 0.0f);
 }
 
 __global__ __launch_bounds__(WF_BLOCK) void k_shadow(SceneView S, WavefrontState W, int pull_slot, int32_t* overflow,
-                                                     TraceTuning tune, const int32_t* order) {
+                                                     TraceTuning tune, const int32_t* order, float4* t_lbefore) {
     __shared__ uint32_t s_stack[LDS_STACK][TRACE_BLOCK];
     uint32_t spill[SPILL_STACK];
     const int count = W.counters[C_SHADOW];
@@ -470,7 +479,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shadow(SceneView S, WavefrontState
                 while (tr.template step<false>(S, nullptr)) {}
             }
             ovf = ovf || tr.st.overflow;
-            shadow_resolve(W, k, tr.hit);
+            shadow_resolve(W, t_lbefore, k, tr.hit);
         }
     } else {
         WaveFeeder feeder;
@@ -497,7 +506,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shadow(SceneView S, WavefrontState
             }
             if (done_now) {
                 ovf = ovf || tr.st.overflow;
-                shadow_resolve(W, k, tr.hit);
+                shadow_resolve(W, t_lbefore, k, tr.hit);
             }
         }
     }
@@ -599,7 +608,7 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
             shadow_order = W.rvals[1];
         }
         hipLaunchKernelGGL(k_shadow, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, W, C_PULL + 3 * bounce + 2, overflow,
-                           tune, shadow_order);
+                           tune, shadow_order, cb.t_lbefore);
     }
     if (cf.enabled) {
         hipError_t e = launch_cache_infer_paths(s, cb, cache->enc, W.L, npx, cache->num_cus);

@@ -1285,9 +1285,10 @@ template <class R> V3<R> SunDir(const kpt_settings& st) {  // integrator.cuh:222
 }
 
 // Integrator::Li, integrator.cuh:241-402
-// A path that ends in the radiance cache at its bounce-1 vertex: the query
-// the renderer evaluates there (position, roughness, outgoing direction,
-// throughput); `hit` is false for paths that never reach that vertex.
+// A path that ends in the radiance cache at its bounce-1 vertex, after that
+// vertex's next-event estimate: the query the renderer evaluates there
+// (position, regularised roughness, outgoing direction, throughput); `hit` is
+// false for paths that never reach that vertex (or whose depth limit is 1).
 struct CacheQuery {
     bool hit = false;
     float pos[3], rough, dir[3], beta[3];
@@ -1347,16 +1348,8 @@ template <class R> V3<R> Li(const FrameConst& fc, Ray<R> ray, Sampler<R>& sample
             vs->inst = si.inst; vs->prim = si.prim;
             bsdf.albedo = V3<R>(1, 1, 1);
         }
-        if (cq && depth == 1) {
-            // the cache completes this path: no NEE, no continuation
-            cq->hit = true;
-            cq->pos[0] = (float)si.pos.x; cq->pos[1] = (float)si.pos.y; cq->pos[2] = (float)si.pos.z;
-            cq->rough = (float)bsdf.roughness;
-            cq->dir[0] = (float)-ray.d.x; cq->dir[1] = (float)-ray.d.y; cq->dir[2] = (float)-ray.d.z;
-            cq->beta[0] = (float)beta.x; cq->beta[1] = (float)beta.y; cq->beta[2] = (float)beta.z;
-            break;
-        }
         if (any_nonspec) bsdf.regularize();
+        const bool to_cache = cq && depth == 1 && max_depth > 1;
         if (bsdf.nonSpecular()) {
             const R sun_p = R(0.5f);
             if (sampler.get1D() < sun_p) {
@@ -1425,6 +1418,15 @@ template <class R> V3<R> Li(const FrameConst& fc, Ray<R> ray, Sampler<R>& sample
                 }
                 L = L + Ld * beta / (R(1) - sun_p);
             }
+        }
+        if (to_cache) {
+            // the cache completes what the continuation of this path would add
+            cq->hit = true;
+            cq->pos[0] = (float)si.pos.x; cq->pos[1] = (float)si.pos.y; cq->pos[2] = (float)si.pos.z;
+            cq->rough = (float)bsdf.roughness;
+            cq->dir[0] = (float)-ray.d.x; cq->dir[1] = (float)-ray.d.y; cq->dir[2] = (float)-ray.d.z;
+            cq->beta[0] = (float)beta.x; cq->beta[1] = (float)beta.y; cq->beta[2] = (float)beta.z;
+            break;
         }
         if (depth++ == max_depth) break;
         V3<R> wo = -ray.d;
@@ -1822,8 +1824,9 @@ void orc_render_pixels(const orc_scene* s, const kpt_settings* settings, const k
 }
 
 // One frame in which every path ends in the radiance cache at its bounce-1
-// vertex (no training paths): per pixel the radiance gathered before that
-// vertex's suffix (unclamped; out_L n x 3) and the cache query
+// vertex (no training paths): per pixel the radiance gathered up to and
+// including that vertex's next-event estimate (unclamped; out_L n x 3) and
+// the cache query
 // (out_q n x 12: hit, position xyz, roughness, direction xyz, throughput xyz,
 // unused).
 void orc_render_cache_queries(const orc_scene* s, const kpt_settings* settings, const kpt_camera* cam, int32_t W,

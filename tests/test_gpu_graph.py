@@ -46,15 +46,16 @@ def test_graph_replay_equals_eager(mode):
     if mode == "cache":
         rng = np.random.default_rng(3)
         w = (rng.uniform(-1, 1, abi.CACHE_PARAMS) * 0.2).astype(np.float32)
-    ctx = Context(0)
-    try:
-        ctx.upload(sc)
-        print(f"[graph] {mode}: eager frames", flush=True)
-        eager = _frames(ctx, sc, 0, 5, st, W, H, w)
-        print(f"[graph] {mode}: graph frames", flush=True)
-        graph = _frames(ctx, sc, 1, 5, st, W, H, w)
-    finally:
-        ctx.close()
+    runs = []
+    for graph in (0, 1):                   # a fresh context each: the adapted scene luminance persists in one
+        ctx = Context(0)
+        try:
+            ctx.upload(sc)
+            print(f"[graph] {mode}: {'graph' if graph else 'eager'} frames", flush=True)
+            runs.append(_frames(ctx, sc, graph, 5, st, W, H, w))
+        finally:
+            ctx.close()
+    eager, graph = runs
     for f, (a, b) in enumerate(zip(eager, graph)):
         for k, name in enumerate(("render", "accum", "gbuffer", "output")):
             assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), f"frame {f}: {name} differs"
