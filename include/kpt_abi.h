@@ -185,14 +185,17 @@ typedef struct kpt_film_state {
  * cores, FP32 accumulation; trained online with Adam (beta 0.9 / 0.99,
  * eps 1e-15) on a relative L2 loss over the tracer's own path suffixes. */
 typedef struct kpt_cache_settings {
-    int32_t enabled;          /* non-training paths end in the cache at the bounce-1 vertex (after its NEE) */
+    int32_t enabled;          /* non-training paths end in the cache at the bounce-1 vertex */
     int32_t train;            /* record suffixes and train every frame */
     int32_t train_samples;    /* training paths per frame (65536) */
     int32_t train_steps;      /* Adam steps per frame; batch = train_samples / train_steps */
     float learning_rate;      /* 1e-3 */
     float radiance_scale;     /* radiance = radiance_scale * max(output, 0) */
     uint32_t seed;            /* weight initialisation (He uniform from a PCG hash) */
-    int32_t reserved;
+    int32_t nee_at_vertex;    /* 0: the network predicts all radiance scattered at the
+                                 vertex (its next-event estimate is not traced);
+                                 1: that estimate is traced exactly and the network
+                                 predicts only what the continuation adds */
     float bounds_min[3];      /* position normalisation box; an empty box means */
     float bounds_max[3];      /* the scene's TLAS bounds */
 } kpt_cache_settings;

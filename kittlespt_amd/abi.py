@@ -176,7 +176,7 @@ class KptCacheSettings(C.Structure):
         ("learning_rate", C.c_float),
         ("radiance_scale", C.c_float),
         ("seed", C.c_uint32),
-        ("reserved", C.c_int32),
+        ("nee_at_vertex", C.c_int32),
         ("bounds_min", C.c_float * 3),
         ("bounds_max", C.c_float * 3),
     ]

@@ -995,6 +995,7 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
         hooks.cb = &ctx->cb;
         hooks.cf.enabled = cs.enabled;
         hooks.cf.train = cs.train;
+        hooks.cf.nee = cs.nee_at_vertex;
         const double prob = std::min(1.0, (double)cs.train_samples / (double)std::max<int64_t>(1, npx));
         hooks.cf.train_threshold = (uint32_t)std::min(16777216.0, std::ceil(prob * 16777216.0));
         hooks.enc = cache_encode(ctx);
@@ -1181,7 +1182,7 @@ int kpt_set_frame_delta(kpt_ctx* ctx, float delta_time_ms) {
 int kpt_cache_configure(kpt_ctx* ctx, const kpt_cache_settings* s) {
     if (!ctx || !s) return KPT_ERR_INVALID_ARG;
     if (s->train_samples < 0 || s->train_samples > (1 << 22) || s->train_steps < 0 || s->train_steps > 64 ||
-        !(s->learning_rate >= 0.0f) || !(s->radiance_scale > 0.0f))
+        !(s->learning_rate >= 0.0f) || !(s->radiance_scale > 0.0f) || (s->nee_at_vertex & ~1))
         return fail(ctx, KPT_ERR_INVALID_ARG, "cache settings out of range");
     if ((s->enabled || s->train) && s->train_samples > 0 && s->train_steps == 0)
         return fail(ctx, KPT_ERR_INVALID_ARG, "training needs at least one step per frame");

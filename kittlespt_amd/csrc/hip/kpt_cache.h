@@ -50,10 +50,10 @@ struct CacheBuffers {
     // per-frame records (filled by the wavefront shade stage)
     float4* q_pos;            // position xyz, roughness
     float4* q_dir;            // outgoing direction xyz, bit-cast path index
-    float4* q_beta;           // path throughput at the vertex
+    float4* q_beta;           // path throughput x reflectance proxy (0.04 + 0.96 albedo)
     float4* t_pos;            // training records: position, roughness
     float4* t_dir;            // direction, bit-cast path index
-    float4* t_beta;           // throughput at the vertex
+    float4* t_beta;           // throughput x reflectance proxy at the vertex
     float4* t_lbefore;        // radiance of the path up to the vertex's next-event estimate
     float4* t_target;         // suffix radiance / throughput / scale
     int32_t* counters;        // [0] queries, [1] training records
@@ -66,6 +66,7 @@ struct CacheFrame {
     int32_t train;            // record training suffixes
     uint32_t train_threshold; // path is a training path when hash < threshold (of 2^24)
     uint32_t frame;
+    int32_t nee;              // trace the vertex's next-event estimate; the network learns the rest
 };
 
 }  // namespace kpt

@@ -365,22 +365,29 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, 
                 b.albedo = mk3(1.0f, 1.0f, 1.0f);
             }
             if (any_nonspec) bsdf_regularize(b);
-            // radiance cache at the bounce-1 vertex, after its next-event
-            // estimate (which stays exact): training paths record the radiance
-            // so far (their suffix becomes the target), the others end here
-            // and the network completes what their continuation would add
+            // radiance cache at the bounce-1 vertex: training paths record the
+            // radiance so far (their suffix becomes the target), the others
+            // end here and the network completes them.  With cf.nee the
+            // vertex's next-event estimate is traced exactly first and the
+            // network stands for the continuation only; without it the
+            // network stands for all radiance scattered at the vertex and no
+            // shadow ray is traced for an ending path.  The network learns
+            // radiance divided by the path throughput and a reflectance proxy
+            // of the vertex (0.04 + 0.96 albedo), so the albedo texture's
+            // detail stays out of what it has to fit.
             bool to_cache = false;
             int32_t trec = -1;
             if (depth == 1 && fc.max_depth > 1 && (cf.enabled || cf.train)) {
                 const uint32_t hs = pcg_hash((uint32_t)p ^ (cf.frame * 0x9E3779B9u)) & 0xFFFFFFu;
                 const float4 qpos = make_float4(s.pos.x, s.pos.y, s.pos.z, b.roughness);
                 const float4 qdir = make_float4(-ray.d.x, -ray.d.y, -ray.d.z, bitsf((uint32_t)p));
+                const v3 bq = beta * (mk3(0.04f, 0.04f, 0.04f) + b.albedo * 0.96f);
                 if (cf.train && hs < cf.train_threshold) {
                     const int r = atomicAdd(&cb.counters[1], 1);
                     if (r < cb.train_cap) {
                         cb.t_pos[r] = qpos;
                         cb.t_dir[r] = qdir;
-                        cb.t_beta[r] = make_float4(beta.x, beta.y, beta.z, 0.0f);
+                        cb.t_beta[r] = make_float4(bq.x, bq.y, bq.z, 0.0f);
                         trec = r;
                     }
                 } else if (cf.enabled) {
@@ -388,12 +395,16 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, 
                     if (r < cb.query_cap) {
                         cb.q_pos[r] = qpos;
                         cb.q_dir[r] = qdir;
-                        cb.q_beta[r] = make_float4(beta.x, beta.y, beta.z, 0.0f);
+                        cb.q_beta[r] = make_float4(bq.x, bq.y, bq.z, 0.0f);
                         to_cache = true;
                     }
                 }
             }
-            if (!(b.alpha < 0.001f)) {
+            if (trec >= 0 && !cf.nee) {
+                cb.t_lbefore[trec] = make_float4(L.x, L.y, L.z, 0.0f);
+                trec = -1;
+            }
+            if (!(to_cache && !cf.nee) && !(b.alpha < 0.001f)) {
                 NeeResult ne = next_event(S, fc, s, b, ray.d, beta, seed);
                 if (ne.trace) {
                     int sidx = atomicAdd(&W.counters[C_SHADOW], 1);

@@ -1285,11 +1285,14 @@ template <class R> V3<R> SunDir(const kpt_settings& st) {  // integrator.cuh:222
 }
 
 // Integrator::Li, integrator.cuh:241-402
-// A path that ends in the radiance cache at its bounce-1 vertex, after that
-// vertex's next-event estimate: the query the renderer evaluates there
-// (position, regularised roughness, outgoing direction, throughput); `hit` is
-// false for paths that never reach that vertex (or whose depth limit is 1).
+// A path that ends in the radiance cache at its bounce-1 vertex: the query
+// the renderer evaluates there (position, regularised roughness, outgoing
+// direction, throughput times the vertex's reflectance proxy 0.04 + 0.96
+// albedo); `hit` is false for paths that never reach that vertex (or whose
+// depth limit is 1).  `nee` (input): the vertex's next-event estimate is made
+// before the path ends; otherwise the cache stands for it too.
 struct CacheQuery {
+    bool nee = false;
     bool hit = false;
     float pos[3], rough, dir[3], beta[3];
 };
@@ -1350,6 +1353,19 @@ template <class R> V3<R> Li(const FrameConst& fc, Ray<R> ray, Sampler<R>& sample
         }
         if (any_nonspec) bsdf.regularize();
         const bool to_cache = cq && depth == 1 && max_depth > 1;
+        auto record_query = [&]() {
+            cq->hit = true;
+            cq->pos[0] = (float)si.pos.x; cq->pos[1] = (float)si.pos.y; cq->pos[2] = (float)si.pos.z;
+            cq->rough = (float)bsdf.roughness;
+            cq->dir[0] = (float)-ray.d.x; cq->dir[1] = (float)-ray.d.y; cq->dir[2] = (float)-ray.d.z;
+            // throughput times the vertex's reflectance proxy (0.04 + 0.96 albedo)
+            V3<R> bq = beta * (V3<R>(R(0.04f), R(0.04f), R(0.04f)) + bsdf.albedo * R(0.96f));
+            cq->beta[0] = (float)bq.x; cq->beta[1] = (float)bq.y; cq->beta[2] = (float)bq.z;
+        };
+        if (to_cache && !cq->nee) {   // the cache stands for all radiance scattered here
+            record_query();
+            break;
+        }
         if (bsdf.nonSpecular()) {
             const R sun_p = R(0.5f);
             if (sampler.get1D() < sun_p) {
@@ -1419,13 +1435,8 @@ template <class R> V3<R> Li(const FrameConst& fc, Ray<R> ray, Sampler<R>& sample
                 L = L + Ld * beta / (R(1) - sun_p);
             }
         }
-        if (to_cache) {
-            // the cache completes what the continuation of this path would add
-            cq->hit = true;
-            cq->pos[0] = (float)si.pos.x; cq->pos[1] = (float)si.pos.y; cq->pos[2] = (float)si.pos.z;
-            cq->rough = (float)bsdf.roughness;
-            cq->dir[0] = (float)-ray.d.x; cq->dir[1] = (float)-ray.d.y; cq->dir[2] = (float)-ray.d.z;
-            cq->beta[0] = (float)beta.x; cq->beta[1] = (float)beta.y; cq->beta[2] = (float)beta.z;
+        if (to_cache) {   // the cache completes what the continuation of this path would add
+            record_query();
             break;
         }
         if (depth++ == max_depth) break;
@@ -1824,13 +1835,13 @@ void orc_render_pixels(const orc_scene* s, const kpt_settings* settings, const k
 }
 
 // One frame in which every path ends in the radiance cache at its bounce-1
-// vertex (no training paths): per pixel the radiance gathered up to and
-// including that vertex's next-event estimate (unclamped; out_L n x 3) and
-// the cache query
+// vertex (no training paths): per pixel the radiance gathered before the
+// cache takes over (up to and including that vertex's next-event estimate
+// when `nee`; unclamped; out_L n x 3) and the cache query
 // (out_q n x 12: hit, position xyz, roughness, direction xyz, throughput xyz,
 // unused).
 void orc_render_cache_queries(const orc_scene* s, const kpt_settings* settings, const kpt_camera* cam, int32_t W,
-                              int32_t H, int32_t frame, float* out_L, float* out_q, int32_t threads) {
+                              int32_t H, int32_t frame, int32_t nee, float* out_L, float* out_q, int32_t threads) {
     FrameConst fc = MakeFrame(s, settings, cam);
     ParallelFor((int64_t)W * H, threads, [&](int64_t i) {
         int px = (int)(i % W), py = (int)(i / W);
@@ -1840,6 +1851,7 @@ void orc_render_cache_queries(const orc_scene* s, const kpt_settings* settings, 
         Ray<float> ray = CameraRay<float>(fc, px, py, W, H);
         VisSurf<float> vs;
         CacheQuery cq;
+        cq.nee = nee != 0;
         V3<float> L = Li<float>(fc, ray, sm, &vs, &cq);
         V3<float> w(1.0f, 1.0f, 1.0f);
         L = (w * 1.0f) * L;

@@ -100,10 +100,12 @@ def test_cache_learns_a_smooth_function(ctx):
     assert mse < 0.25 * base
 
 
-def test_cached_render_matches_oracle():
-    """One frame in which every path ends in the cache at the bounce-1 vertex:
-    the GPU's queries (position, roughness, direction, throughput of each
-    path) equal the oracle's bit for bit, paths without a query are
+@pytest.mark.parametrize("nee", [0, 1])
+def test_cached_render_matches_oracle(nee):
+    """One frame in which every path ends in the cache at the bounce-1 vertex
+    (with and without that vertex's next-event estimate traced first): the
+    GPU's queries (position, roughness, direction, throughput x reflectance
+    of each path) equal the oracle's bit for bit, paths without a query are
     bit-exact, and the cached pixels equal radiance-so-far + throughput *
     scale * max(net(query), 0) within the network's tolerance."""
     from kittlespt_amd.context import Context
@@ -121,15 +123,15 @@ def test_cached_render_matches_oracle():
         p, v = view_proj(sc, W, H)
         ctx.set_view(p, v)
         cam = ctx.camera()
-        ctx.cache_configure(enabled=True, train=False, seed=7, bounds=box, radiance_scale=1e4)
+        ctx.cache_configure(enabled=True, train=False, seed=7, bounds=box, radiance_scale=1e4, nee_at_vertex=nee)
         ctx.cache_set_weights(w)
-        print("[cache] one cached frame 48x32", flush=True)
+        print(f"[cache] one cached frame 48x32, nee_at_vertex={nee}", flush=True)
         ctx.render(1)
         gp, gd, gb = ctx.cache_queries()
         est = ctx.estimate().reshape(-1, 3)
     finally:
         ctx.close()
-    Lb, q = orc.render_cache_queries(st, cam, W, H, 0)
+    Lb, q = orc.render_cache_queries(st, cam, W, H, 0, nee=bool(nee))
     orc.close()
     hit = np.nonzero(q[:, 0] > 0)[0]
     paths = gd[:, 3].view(np.uint32).astype(np.int64)
@@ -157,7 +159,8 @@ def _rel_mse(est, ref):
     return float(np.mean((est - ref) ** 2 / (ref ** 2 + (0.1 * m) ** 2)))
 
 
-def test_trained_cache_beats_the_same_tracer_without_it():
+@pytest.mark.parametrize("nee", [0, 1])
+def test_trained_cache_beats_the_same_tracer_without_it(nee):
     from kittlespt_amd.context import Context
     sc = scenes.scene_plain(subdiv=3)
     W, H, frames = 64, 48, 32
@@ -166,7 +169,7 @@ def test_trained_cache_beats_the_same_tracer_without_it():
     orc = oracle.OracleScene(sc)
     pix = np.arange(W * H, dtype=np.int32)
     print("[cache] FP64 reference 64x48 x 256 spp on the CPU", flush=True)
-    ref = orc.render_pixels(st, cam, W, H, pix, 0, 256, double=True).reshape(H, W, 3)
+    ref = orc.render_pixels(st, cam, W, H, pix, 1 << 20, 256, double=True).reshape(H, W, 3)   # seeds unused below
     orc.close()
     ctx = Context(0)
     try:
@@ -175,16 +178,23 @@ def test_trained_cache_beats_the_same_tracer_without_it():
         ctx.resize(W, H)
         p, v = view_proj(sc, W, H)
         ctx.set_view(p, v)
-        # warm-up: train online on the scene's own suffixes
-        ctx.cache_configure(enabled=False, train=True, train_samples=1024, train_steps=4, seed=5)
-        print("[cache] online training 200 frames", flush=True)
+        # warm-up: train online on the scene's own suffixes, from a 256x192
+        # view of the same camera (the cache is indexed by world position,
+        # so any resolution trains it; this one gives 16x the suffixes)
+        ctx.resize(4 * W, 4 * H)
+        p4, v4 = view_proj(sc, 4 * W, 4 * H)
+        ctx.set_view(p4, v4)
+        ctx.cache_configure(enabled=False, train=True, train_samples=16384, train_steps=4, seed=5, nee_at_vertex=nee)
+        print("[cache] online training 200 frames at 256x192", flush=True)
         ctx.render(200)
         trained = ctx.cache_weights()
         stats = ctx.cache_stats()
         print(f"[cache] after warm-up: {stats}")
+        ctx.resize(W, H)
+        ctx.set_view(p, v)
         res = {}
         for name, w in (("cache", trained), ("zero", np.zeros_like(trained))):
-            ctx.cache_configure(enabled=True, train=False, seed=5)
+            ctx.cache_configure(enabled=True, train=False, seed=5, nee_at_vertex=nee)
             ctx.cache_set_weights(w)
             ctx.reset()
             ctx.render(frames)
@@ -195,5 +205,5 @@ def test_trained_cache_beats_the_same_tracer_without_it():
         res["plain"] = _rel_mse(ctx.estimate(), ref)
     finally:
         ctx.close()
-    print(f"[cache] relative MSE vs FP64 at {frames} spp: {res}")
+    print(f"[cache] nee_at_vertex={nee}: relative MSE vs FP64 at {frames} spp: {res}")
     assert res["cache"] < 0.5 * res["zero"]
