@@ -258,8 +258,11 @@ def main():
     ap.add_argument("--ttt-pixels", type=int, default=2048)
     ap.add_argument("--ref-spp", type=int, default=8192, help="spp of the FP64 reference pixels")
     ap.add_argument("--cache-warmup", type=int, default=8, help="cache training frames before accumulating")
-    ap.add_argument("--cache-nee", type=int, default=0, choices=[0, 1],
+    ap.add_argument("--cache-nee", type=int, default=1, choices=[0, 1],
                     help="1: trace the cache vertex's next-event estimate, the network learns the rest")
+    ap.add_argument("--cache-wide", type=float, default=0.0,
+                    help="paths end in the cache at the first vertex reached through a diffuse sample or a lobe "
+                         "at least this rough (0: always at the bounce-1 vertex)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -396,7 +399,8 @@ def main():
         target = plain[-1][2]
         t_plain = plain[-1][0]
         ctx.cache_configure(enabled=True, train=True, train_samples=65536, train_steps=4, learning_rate=1e-3,
-                            radiance_scale=1e4, seed=11, nee_at_vertex=args.cache_nee)
+                            radiance_scale=1e4, seed=11, nee_at_vertex=args.cache_nee,
+                            wide_roughness=args.cache_wide)
         cached = run_curve(ctx, pix, ref, args.ttt_spp, set(plain_cp), warmup_frames=args.cache_warmup,
                            label="cached")
         t_cached = time_at(cached, target)
@@ -406,6 +410,7 @@ def main():
                "speedup": None if t_cached is None else round(t_plain / t_cached, 3),
                "cached_rmse_at_equal_spp": round(cached[-1][2], 5),
                "cache_warmup_frames": args.cache_warmup, "cache_nee_at_vertex": args.cache_nee,
+               "cache_wide_roughness": args.cache_wide,
                "reference": f"FP64 oracle render of {args.ttt_pixels} random pixels x {args.ref_spp} spp "
                             f"(host cores)",
                "curve_plain": [[round(s, 4), n, round(e, 5)] for s, n, e in plain],

@@ -198,6 +198,9 @@ typedef struct kpt_cache_settings {
                                  predicts only what the continuation adds */
     float bounds_min[3];      /* position normalisation box; an empty box means */
     float bounds_max[3];      /* the scene's TLAS bounds */
+    float wide_roughness;     /* a path's cache vertex is the first one after the primary hit
+                                 reached through a diffuse sample or a lobe of at least this
+                                 roughness (0: always the bounce-1 vertex) */
 } kpt_cache_settings;
 
 #define KPT_CACHE_PARAMS 18624

@@ -179,6 +179,7 @@ class KptCacheSettings(C.Structure):
         ("nee_at_vertex", C.c_int32),
         ("bounds_min", C.c_float * 3),
         ("bounds_max", C.c_float * 3),
+        ("wide_roughness", C.c_float),
     ]
 
 

@@ -272,9 +272,10 @@ class Context:
 
     # ---- radiance cache
     def cache_configure(self, enabled=False, train=False, train_samples=65536, train_steps=4, learning_rate=1e-3,
-                        radiance_scale=1e4, seed=1, bounds=None, nee_at_vertex=0):
+                        radiance_scale=1e4, seed=1, bounds=None, nee_at_vertex=1, wide_roughness=0.0):
         s = abi.KptCacheSettings()
         s.enabled, s.train, s.nee_at_vertex = int(enabled), int(train), int(nee_at_vertex)
+        s.wide_roughness = float(wide_roughness)
         s.train_samples, s.train_steps = int(train_samples), int(train_steps)
         s.learning_rate, s.radiance_scale, s.seed = float(learning_rate), float(radiance_scale), int(seed)
         if bounds is not None:

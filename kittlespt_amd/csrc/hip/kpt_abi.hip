@@ -268,6 +268,8 @@ void default_cache_settings(kpt_cache_settings* s) {
     s->learning_rate = 1e-3f;
     s->radiance_scale = 1e4f;
     s->seed = 1;
+    s->nee_at_vertex = 1;
+    s->wide_roughness = 0.0f;
 }
 
 // The parameter buffers (allocated once) and the training workspace for
@@ -996,6 +998,7 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
         hooks.cf.enabled = cs.enabled;
         hooks.cf.train = cs.train;
         hooks.cf.nee = cs.nee_at_vertex;
+        hooks.cf.wide_roughness = cs.wide_roughness;
         const double prob = std::min(1.0, (double)cs.train_samples / (double)std::max<int64_t>(1, npx));
         hooks.cf.train_threshold = (uint32_t)std::min(16777216.0, std::ceil(prob * 16777216.0));
         hooks.enc = cache_encode(ctx);
@@ -1182,7 +1185,8 @@ int kpt_set_frame_delta(kpt_ctx* ctx, float delta_time_ms) {
 int kpt_cache_configure(kpt_ctx* ctx, const kpt_cache_settings* s) {
     if (!ctx || !s) return KPT_ERR_INVALID_ARG;
     if (s->train_samples < 0 || s->train_samples > (1 << 22) || s->train_steps < 0 || s->train_steps > 64 ||
-        !(s->learning_rate >= 0.0f) || !(s->radiance_scale > 0.0f) || (s->nee_at_vertex & ~1))
+        !(s->learning_rate >= 0.0f) || !(s->radiance_scale > 0.0f) || (s->nee_at_vertex & ~1) ||
+        !(s->wide_roughness >= 0.0f))
         return fail(ctx, KPT_ERR_INVALID_ARG, "cache settings out of range");
     if ((s->enabled || s->train) && s->train_samples > 0 && s->train_steps == 0)
         return fail(ctx, KPT_ERR_INVALID_ARG, "training needs at least one step per frame");

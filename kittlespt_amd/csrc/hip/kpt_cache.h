@@ -67,6 +67,7 @@ struct CacheFrame {
     uint32_t train_threshold; // path is a training path when hash < threshold (of 2^24)
     uint32_t frame;
     int32_t nee;              // trace the vertex's next-event estimate; the network learns the rest
+    float wide_roughness;     // lobes at least this rough (or diffuse samples) lead to a cache vertex
 };
 
 }  // namespace kpt

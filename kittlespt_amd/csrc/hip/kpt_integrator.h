@@ -46,12 +46,14 @@ struct NeeResult {
 
 // sampleLdSun / sampleLd and the L += Ld*beta/p of integrator.cuh:365-375,
 // with the visibility test left to the caller.  Consumes the same random
-// numbers, in the same order, as the reference.
+// numbers, in the same order, as the reference.  `sun_p` is the probability
+// of sampling the sun rather than an area light: the reference's 0.5 on every
+// path vertex; the radiance cache's vertex passes 1 when the scene has no area
+// lights (same random numbers: the selection draw is < 1 always).
 KD NeeResult next_event(const SceneView& S, const FrameConst& fc, const Surf& s, const BSDF& b, v3 ray_d, v3 beta,
-                        uint32_t& seed) {
+                        uint32_t& seed, float sun_p = 0.5f) {
     NeeResult r;
     r.trace = false;
-    const float sun_p = 0.5f;
     const v3 zero = mk3(0.0f, 0.0f, 0.0f);
     if (rng1(seed) < sun_p) {
         // sampleLdSun, integrator.cuh:147-186

@@ -337,6 +337,8 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, 
         uint32_t st = fbits(l4.w);
         int32_t depth = (int32_t)(st & 0xFFFFu);
         bool any_nonspec = (st >> 16) & 1u;
+        bool wide_in = (st >> 17) & 1u;      // this vertex was reached through a wide lobe
+        bool cached = (st >> 18) & 1u;       // the path has met its cache vertex (training record made)
         seed = seed + (uint32_t)depth;
         const bool first = (depth == 0);
         float4 h4 = W.hit[p];
@@ -365,24 +367,33 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, 
                 b.albedo = mk3(1.0f, 1.0f, 1.0f);
             }
             if (any_nonspec) bsdf_regularize(b);
-            // radiance cache at the bounce-1 vertex: training paths record the
-            // radiance so far (their suffix becomes the target), the others
-            // end here and the network completes them.  With cf.nee the
-            // vertex's next-event estimate is traced exactly first and the
-            // network stands for the continuation only; without it the
-            // network stands for all radiance scattered at the vertex and no
-            // shadow ray is traced for an ending path.  The network learns
-            // radiance divided by the path throughput and a reflectance proxy
-            // of the vertex (0.04 + 0.96 albedo), so the albedo texture's
-            // detail stays out of what it has to fit.
+            // radiance cache at the path's cache vertex: the first vertex after
+            // the primary hit that was reached through a wide lobe (a diffuse
+            // sample, or a lobe of roughness >= cf.wide_roughness), where the
+            // network's errors are blurred by that lobe; narrower lobes let
+            // the path continue.  Training paths record the radiance so far
+            // there (their suffix becomes the target), the others end there
+            // and the network completes them.  With cf.nee the vertex's
+            // next-event estimate is traced exactly first and the network
+            // stands for the continuation only; without it the network stands
+            // for all radiance scattered at the vertex and no shadow ray is
+            // traced for an ending path.  The network learns radiance divided
+            // by the path throughput and a reflectance proxy of the vertex
+            // (0.04 + 0.96 albedo), so the albedo texture's detail stays out
+            // of what it has to fit.  While the cache renders, a training
+            // path ends at its next cache vertex after the recorded one, in
+            // the network (self-training: short suffixes, and the target's
+            // tail is the network's own estimate instead of a long noisy one).
             bool to_cache = false;
             int32_t trec = -1;
-            if (depth == 1 && fc.max_depth > 1 && (cf.enabled || cf.train)) {
+            if (depth >= 1 && wide_in && depth < fc.max_depth && (cf.enabled || cf.train) &&
+                !(cached && !cf.enabled)) {
                 const uint32_t hs = pcg_hash((uint32_t)p ^ (cf.frame * 0x9E3779B9u)) & 0xFFFFFFu;
                 const float4 qpos = make_float4(s.pos.x, s.pos.y, s.pos.z, b.roughness);
                 const float4 qdir = make_float4(-ray.d.x, -ray.d.y, -ray.d.z, bitsf((uint32_t)p));
                 const v3 bq = beta * (mk3(0.04f, 0.04f, 0.04f) + b.albedo * 0.96f);
-                if (cf.train && hs < cf.train_threshold) {
+                if (!cached && cf.train && hs < cf.train_threshold) {
+                    cached = true;
                     const int r = atomicAdd(&cb.counters[1], 1);
                     if (r < cb.train_cap) {
                         cb.t_pos[r] = qpos;
@@ -400,12 +411,16 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, 
                     }
                 }
             }
+            // the traced estimate at a cache vertex samples only the sun when
+            // the scene has no area lights (the reference's 50 % area-light
+            // draw would be a wasted, zero sample there)
+            const float sun_p = (cf.nee && (to_cache || trec >= 0) && fc.num_lights == 0) ? 1.0f : 0.5f;
             if (trec >= 0 && !cf.nee) {
                 cb.t_lbefore[trec] = make_float4(L.x, L.y, L.z, 0.0f);
                 trec = -1;
             }
             if (!(to_cache && !cf.nee) && !(b.alpha < 0.001f)) {
-                NeeResult ne = next_event(S, fc, s, b, ray.d, beta, seed);
+                NeeResult ne = next_event(S, fc, s, b, ray.d, beta, seed, sun_p);
                 if (ne.trace) {
                     int sidx = atomicAdd(&W.counters[C_SHADOW], 1);
                     W.shadow_queue[sidx] = p;
@@ -431,6 +446,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, 
                     beta = beta * (bs.f * absdot(s.n, bs.wi) / bs.pdf);
                     p_b = bsdf_pdf(b, wo, bs.wi);
                     any_nonspec = any_nonspec || !((bs.scatter & SC_SPECULAR) != 0);
+                    wide_in = ((bs.scatter & SC_DIFFUSE) != 0) || b.roughness >= cf.wide_roughness;
                     v3 org = (bs.scatter & SC_TRANSMITTED) ? s.pos - (s.n * KPT_HIT_EPS) : s.pos + (s.n * KPT_HIT_EPS);
                     ray = make_ray(org, bs.wi);
                     alive = true;
@@ -445,7 +461,8 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, 
                 }
             }
         }
-        W.L[p] = make_float4(L.x, L.y, L.z, bitsf((uint32_t)depth | (any_nonspec ? 0x10000u : 0u)));
+        W.L[p] = make_float4(L.x, L.y, L.z, bitsf((uint32_t)depth | (any_nonspec ? 0x10000u : 0u) |
+                                                  (wide_in ? 0x20000u : 0u) | (cached ? 0x40000u : 0u)));
         W.ray_d[p] = make_float4(ray.d.x, ray.d.y, ray.d.z, bitsf(seed));
         if (alive) {
             W.ray_o[p] = make_float4(ray.o.x, ray.o.y, ray.o.z, o4.w);

@@ -125,16 +125,18 @@ class OracleScene:
                                 _ptr(v) if v is not None else None, threads or threads_default())
         return (rgb, g, v) if want_buffers else rgb
 
-    def render_cache_queries(self, settings, camera, width, height, frame, nee=False, threads=None):
+    def render_cache_queries(self, settings, camera, width, height, frame, nee=False, wide_roughness=0.0,
+                             threads=None):
         """Paths ending in the cache at bounce 1: (L_before [n,3], queries [n,12])."""
         n = width * height
         Lb = np.zeros((n, 3), np.float32)
         q = np.zeros((n, 12), np.float32)
         L = lib()
         L.orc_render_cache_queries.argtypes = [C.c_void_p, C.POINTER(abi.KptSettings), C.POINTER(abi.KptCamera),
-                                               C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
-                                               C.c_int32]
-        L.orc_render_cache_queries(self.h, C.byref(settings), C.byref(camera), width, height, frame, int(nee), _ptr(Lb),
+                                               C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_void_p,
+                                               C.c_void_p, C.c_int32]
+        L.orc_render_cache_queries(self.h, C.byref(settings), C.byref(camera), width, height, frame, int(nee),
+                                   float(wide_roughness), _ptr(Lb),
                                    _ptr(q), threads or threads_default())
         return Lb, q
 

@@ -1293,6 +1293,7 @@ template <class R> V3<R> SunDir(const kpt_settings& st) {  // integrator.cuh:222
 // before the path ends; otherwise the cache stands for it too.
 struct CacheQuery {
     bool nee = false;
+    float wide_roughness = 0.0f;   // input: lobes this rough (or diffuse samples) lead to the cache vertex
     bool hit = false;
     float pos[3], rough, dir[3], beta[3];
 };
@@ -1307,6 +1308,7 @@ template <class R> V3<R> Li(const FrameConst& fc, Ray<R> ray, Sampler<R>& sample
     int32_t depth = 0;
     R p_b = R(1);
     V3<R> prev_pos(0, 0, 0);
+    bool wide_in = false;   // the current vertex was reached through a wide lobe (cache queries only)
     const V3<R> sun = SunDir<R>(fc.st);
     const R emis = R(fc.st.sun_emission_factor);
     const V3<R> observer(0, R(EARTH_R_F + 1), 0);
@@ -1352,7 +1354,7 @@ template <class R> V3<R> Li(const FrameConst& fc, Ray<R> ray, Sampler<R>& sample
             bsdf.albedo = V3<R>(1, 1, 1);
         }
         if (any_nonspec) bsdf.regularize();
-        const bool to_cache = cq && depth == 1 && max_depth > 1;
+        const bool to_cache = cq && depth >= 1 && wide_in && depth < max_depth;
         auto record_query = [&]() {
             cq->hit = true;
             cq->pos[0] = (float)si.pos.x; cq->pos[1] = (float)si.pos.y; cq->pos[2] = (float)si.pos.z;
@@ -1367,7 +1369,9 @@ template <class R> V3<R> Li(const FrameConst& fc, Ray<R> ray, Sampler<R>& sample
             break;
         }
         if (bsdf.nonSpecular()) {
-            const R sun_p = R(0.5f);
+            // the renderer's cache vertex samples only the sun when there are
+            // no area lights; every other vertex keeps the reference's 0.5
+            const R sun_p = (to_cache && nlights == 0) ? R(1.0f) : R(0.5f);
             if (sampler.get1D() < sun_p) {
                 // sampleLdSun, integrator.cuh:147-186
                 V3<R> Ld(0, 0, 0);
@@ -1449,6 +1453,7 @@ template <class R> V3<R> Li(const FrameConst& fc, Ray<R> ray, Sampler<R>& sample
         p_b = bsdf.pdf(wo, bs.wi);
         bool spec = (bs.scatter & Specular) != 0;
         any_nonspec |= !spec;
+        wide_in = ((bs.scatter & Diffuse) != 0) || (cq && bsdf.roughness >= R(cq->wide_roughness));
         prev_pos = si.pos;
         // SurfaceInteraction::spawnRay, interaction.cuh:31-41
         V3<R> org = (bs.scatter & Transmitted) ? si.pos - (si.n * R(HIT_EPS_F)) : si.pos + (si.n * R(HIT_EPS_F));
@@ -1841,7 +1846,8 @@ void orc_render_pixels(const orc_scene* s, const kpt_settings* settings, const k
 // (out_q n x 12: hit, position xyz, roughness, direction xyz, throughput xyz,
 // unused).
 void orc_render_cache_queries(const orc_scene* s, const kpt_settings* settings, const kpt_camera* cam, int32_t W,
-                              int32_t H, int32_t frame, int32_t nee, float* out_L, float* out_q, int32_t threads) {
+                              int32_t H, int32_t frame, int32_t nee, float wide_roughness, float* out_L, float* out_q,
+                              int32_t threads) {
     FrameConst fc = MakeFrame(s, settings, cam);
     ParallelFor((int64_t)W * H, threads, [&](int64_t i) {
         int px = (int)(i % W), py = (int)(i / W);
@@ -1852,6 +1858,7 @@ void orc_render_cache_queries(const orc_scene* s, const kpt_settings* settings, 
         VisSurf<float> vs;
         CacheQuery cq;
         cq.nee = nee != 0;
+        cq.wide_roughness = wide_roughness;
         V3<float> L = Li<float>(fc, ray, sm, &vs, &cq);
         V3<float> w(1.0f, 1.0f, 1.0f);
         L = (w * 1.0f) * L;

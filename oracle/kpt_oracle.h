@@ -50,8 +50,8 @@ void orc_render_pixels(const orc_scene* s, const kpt_settings* settings, const k
  * vertex's next-event estimate when nee != 0; unclamped), out_q
  * [W*H][12] (hit, position xyz, roughness, direction xyz, throughput xyz, 0). */
 void orc_render_cache_queries(const orc_scene* s, const kpt_settings* settings, const kpt_camera* cam,
-                              int32_t width, int32_t height, int32_t frame, int32_t nee, float* out_L,
-                              float* out_q, int32_t threads);
+                              int32_t width, int32_t height, int32_t frame, int32_t nee, float wide_roughness,
+                              float* out_L, float* out_q, int32_t threads);
 /* Renderer::setMeshTransform for instance i (previous inverse kept) and the
  * TLAS rebuild. */
 void orc_set_instance_transform(orc_scene* s, int32_t instance, const float model[16]);

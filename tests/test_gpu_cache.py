@@ -100,17 +100,19 @@ def test_cache_learns_a_smooth_function(ctx):
     assert mse < 0.25 * base
 
 
-@pytest.mark.parametrize("nee", [0, 1])
-def test_cached_render_matches_oracle(nee):
-    """One frame in which every path ends in the cache at the bounce-1 vertex
-    (with and without that vertex's next-event estimate traced first): the
+@pytest.mark.parametrize("nee,wide", [(0, 0.0), (1, 0.0), (0, 0.7), (1, 0.7)])
+def test_cached_render_matches_oracle(nee, wide):
+    """One frame in which every path ends in the cache at its cache vertex
+    (the bounce-1 vertex, or with wide > 0 the first vertex reached through a
+    diffuse sample or a lobe at least that rough; with and without that
+    vertex's next-event estimate traced first): the
     GPU's queries (position, roughness, direction, throughput x reflectance
     of each path) equal the oracle's bit for bit, paths without a query are
     bit-exact, and the cached pixels equal radiance-so-far + throughput *
     scale * max(net(query), 0) within the network's tolerance."""
     from kittlespt_amd.context import Context
     sc = scenes.scene_plain(subdiv=3)
-    W, H = 48, 32
+    W, H = 64, 48
     box = ((-20.0, -1.0, -20.0), (20.0, 3.0, 20.0))
     st = settings(max_depth=4, integrator=abi.INTEGRATOR_WAVEFRONT, sun_theta=sc.sun_theta)
     w = he_weights(31) * np.float32(0.5)
@@ -123,19 +125,20 @@ def test_cached_render_matches_oracle(nee):
         p, v = view_proj(sc, W, H)
         ctx.set_view(p, v)
         cam = ctx.camera()
-        ctx.cache_configure(enabled=True, train=False, seed=7, bounds=box, radiance_scale=1e4, nee_at_vertex=nee)
+        ctx.cache_configure(enabled=True, train=False, seed=7, bounds=box, radiance_scale=1e4, nee_at_vertex=nee,
+                            wide_roughness=wide)
         ctx.cache_set_weights(w)
-        print(f"[cache] one cached frame 48x32, nee_at_vertex={nee}", flush=True)
+        print(f"[cache] one cached frame 64x48, nee_at_vertex={nee}, wide_roughness={wide}", flush=True)
         ctx.render(1)
         gp, gd, gb = ctx.cache_queries()
         est = ctx.estimate().reshape(-1, 3)
     finally:
         ctx.close()
-    Lb, q = orc.render_cache_queries(st, cam, W, H, 0, nee=bool(nee))
+    Lb, q = orc.render_cache_queries(st, cam, W, H, 0, nee=bool(nee), wide_roughness=wide)
     orc.close()
     hit = np.nonzero(q[:, 0] > 0)[0]
     paths = gd[:, 3].view(np.uint32).astype(np.int64)
-    assert len(hit) > 100 and np.array_equal(np.sort(paths), hit)
+    assert len(hit) > 50 and np.array_equal(np.sort(paths), hit)
     order = np.argsort(paths)
     gp, gd, gb = gp[order], gd[order], gb[order]
     u = np.uint32
@@ -161,15 +164,21 @@ def _rel_mse(est, ref):
 
 @pytest.mark.parametrize("nee", [0, 1])
 def test_trained_cache_beats_the_same_tracer_without_it(nee):
+    """What the trained network gains: the same tracer with the network's
+    output zeroed (paths end at the same vertex, their continuation lost)
+    misses a share of the image's energy; the trained network restores most
+    of it.  Energy = image mean, 128 spp against a 1024-spp FP64 render, a
+    statistic that per-pixel noise barely moves.  The relative MSE of the
+    plain tracer at the same spp is printed beside it."""
     from kittlespt_amd.context import Context
     sc = scenes.scene_plain(subdiv=3)
-    W, H, frames = 64, 48, 32
+    W, H, frames = 64, 48, 128
     st = settings(max_depth=6, integrator=abi.INTEGRATOR_WAVEFRONT, sun_theta=sc.sun_theta)
     cam = host_camera(sc, W, H)
     orc = oracle.OracleScene(sc)
     pix = np.arange(W * H, dtype=np.int32)
-    print("[cache] FP64 reference 64x48 x 256 spp on the CPU", flush=True)
-    ref = orc.render_pixels(st, cam, W, H, pix, 1 << 20, 256, double=True).reshape(H, W, 3)   # seeds unused below
+    print("[cache] FP64 reference 64x48 x 1024 spp on the CPU", flush=True)
+    ref = orc.render_pixels(st, cam, W, H, pix, 1 << 20, 1024, double=True).reshape(H, W, 3)   # seeds unused below
     orc.close()
     ctx = Context(0)
     try:
@@ -193,17 +202,16 @@ def test_trained_cache_beats_the_same_tracer_without_it(nee):
         ctx.resize(W, H)
         ctx.set_view(p, v)
         res = {}
-        for name, w in (("cache", trained), ("zero", np.zeros_like(trained))):
-            ctx.cache_configure(enabled=True, train=False, seed=5, nee_at_vertex=nee)
-            ctx.cache_set_weights(w)
+        for name, w in (("cache", trained), ("zero", np.zeros_like(trained)), ("plain", None)):
+            ctx.cache_configure(enabled=w is not None, train=False, seed=5, nee_at_vertex=nee)
+            if w is not None:
+                ctx.cache_set_weights(w)
             ctx.reset()
             ctx.render(frames)
-            res[name] = _rel_mse(ctx.estimate(), ref)
-        ctx.cache_configure(enabled=False, train=False, seed=5)
-        ctx.reset()
-        ctx.render(frames)
-        res["plain"] = _rel_mse(ctx.estimate(), ref)
+            est = ctx.estimate().astype(np.float64)
+            res[name] = {"energy_err": abs(est.mean() - ref.mean()) / ref.mean(), "rel_mse": _rel_mse(est, ref)}
     finally:
         ctx.close()
-    print(f"[cache] nee_at_vertex={nee}: relative MSE vs FP64 at {frames} spp: {res}")
-    assert res["cache"] < 0.5 * res["zero"]
+    print(f"[cache] nee_at_vertex={nee}, {frames} spp vs FP64: {res}")
+    assert res["zero"]["energy_err"] > 0.02                 # the continuation carries visible energy here
+    assert res["cache"]["energy_err"] < 0.3 * res["zero"]["energy_err"]
