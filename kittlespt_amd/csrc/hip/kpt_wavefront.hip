@@ -590,7 +590,11 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
         hipLaunchKernelGGL(k_generate_views, dim3(gblocks), dim3(WF_BLOCK), 0, s, fc, frame, W, *views);
     else
         hipLaunchKernelGGL(k_generate, dim3(gblocks), dim3(WF_BLOCK), 0, s, fc, frame, W);
-    const int max_bounces = std::min(fc.max_depth + 1, MAX_BOUNCE_SLOTS);
+    int max_bounces = std::min(fc.max_depth + 1, MAX_BOUNCE_SLOTS);
+    // When the cache renders and every lobe leads to a cache vertex, no path
+    // passes vertex 2 (others end at vertex 1, training paths at their second
+    // cache vertex): the launches for deeper bounces would find empty queues.
+    if (cf.enabled && !(cf.wide_roughness > 0.0f)) max_bounces = std::min(max_bounces, 3);
     for (int bounce = 0; bounce < max_bounces; ++bounce) {
         const int parity = bounce & 1;
         // next queue and shadow count start at zero
