@@ -108,6 +108,23 @@ def fp64_reference(scene, st, cam, W, H, pixels, spp):
     return ref.astype(np.float64)
 
 
+def measured_traffic(W, H, instances, depth, spp):
+    """HBM bytes per k_trace launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes of this workload (scaled by the copy-kernel
+    calibration), or (None, None) for another workload."""
+    path = os.path.join(ROOT, "profiles", "k_trace_instanced1080p_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    w = t.get("workload", {})
+    if (w.get("width"), w.get("height"), w.get("instances"), w.get("depth"), w.get("spp")) != (W, H, instances, depth, spp):
+        return None, None
+    b = 1024 * (t["fetch_kib_per_dispatch"] * t["fetch_scale"] + t["write_kib_per_dispatch"] * t["write_scale"])
+    return int(b), os.path.relpath(path, ROOT)
+
+
 def rel_rmse(est, ref):
     """RMSE relative to the reference, per channel, with a floor of 10 % of the
     mean so dark pixels do not dominate."""
@@ -263,6 +280,8 @@ def main():
     ap.add_argument("--cache-wide", type=float, default=0.0,
                     help="paths end in the cache at the first vertex reached through a diffuse sample or a lobe "
                          "at least this rough (0: always at the bounce-1 vertex)")
+    ap.add_argument("--profile-cached", type=int, default=0,
+                    help="only render this many cached 1-spp frames of the workload (for rocprofv3)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -309,6 +328,20 @@ def main():
         log(f"[bench] autotuned in {time.time() - t0:.1f}s: {best} ({ms:.2f} ms per 1-spp frame)")
     tuning = ctx.tuning() if integ == abi.INTEGRATOR_WAVEFRONT else None
 
+    if args.profile_cached:
+        # only the cached tracer's frames (training online), for a kernel profile
+        ctx.cache_configure(enabled=True, train=True, train_samples=65536, train_steps=4, learning_rate=1e-3,
+                            radiance_scale=1e4, seed=11, nee_at_vertex=args.cache_nee,
+                            wide_roughness=args.cache_wide)
+        times = []
+        for f in range(args.profile_cached):
+            ctx.render(1)
+            times.append(ctx.timing()[0])
+            log(f"[profile] cached frame {f}: {times[-1]:.2f} ms")
+        print(json.dumps({"profile_cached_frames": args.profile_cached,
+                          "ms_per_frame_after_2": round(float(np.mean(times[2:] or times)), 3)}), flush=True)
+        return
+
     for i in range(args.warmup):
         ctx.render(args.spp)
         log(f"[bench] warmup step {i} {ctx.timing()[0]:.1f} ms")
@@ -344,10 +377,13 @@ def main():
         bytes_per_launch = bytes_step / count_launches
         avg_launch_s = (trace_ms / launches) / 1e3
         achieved = bytes_per_launch / avg_launch_s / 1e9
+        traffic, traffic_src = measured_traffic(W, H, args.instances, args.depth, args.spp)
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": "k_trace (wavefront closest hit)",
                 "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(avg_launch_s * 1e3, 4)}
+        if traffic_src:
+            roof["traffic_source"] = traffic_src
 
     cpu = None
     if not args.no_cpu_baseline:
