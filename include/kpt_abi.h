@@ -305,7 +305,10 @@ int kpt_set_trace_counting(kpt_ctx* ctx, int32_t enable);
  * hipGraph and replay it (0/1; the per-launch trace times of kpt_last_timing
  * are then not measured), [6] (when n >= 7) sort continuation and shadow
  * rays by direction octant and Morton cell of the origin before tracing
- * (0/1; not applied inside captured graphs).  n >= 5. */
+ * (0/1; not applied inside captured graphs), [7] (when n >= 8) the bounce
+ * from which the paths still alive finish in one persistent launch, a lane
+ * running its whole remaining path (0: never; 0..64; frames in which the
+ * radiance cache renders or trains keep the stage kernels).  n >= 5. */
 int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n);
 int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n);
 int kpt_get_render_counters(const kpt_ctx* ctx, kpt_trace_counters* out);

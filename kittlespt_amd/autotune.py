@@ -2,7 +2,9 @@
 
 Knobs (kpt_set_tuning): lane refill on/off, queue entries claimed per wave
 atomic, refill threshold, persistent workgroups per compute unit, material
-sort on/off, per-frame hipGraph replay on/off.  The tuner renders real frames of the caller's scene and camera
+sort on/off, per-frame hipGraph replay on/off, spatial ray sort on/off, and
+the bounce from which the remaining paths finish in one path-per-lane launch.
+The tuner renders real frames of the caller's scene and camera
 and keeps the fastest setting by coordinate descent (each knob swept with the
 others fixed, two passes).  Results are cached in a JSON file keyed by the
 workload so later runs reuse them.
@@ -21,8 +23,9 @@ SPACE = {
     "sort": [0, 1],
     "graph": [0, 1],
     "ray_sort": [0, 1],
+    "tail": [0, 1, 2, 3, 4],
 }
-ORDER = ["ray_sort", "refill", "blocks_per_cu", "sort", "graph", "chunk", "refill_min"]
+ORDER = ["tail", "ray_sort", "refill", "blocks_per_cu", "sort", "graph", "chunk", "refill_min"]
 
 
 def measure(ctx, cfg, spp=1, repeats=2):

@@ -209,7 +209,7 @@ class Context:
         self._ck(self.L.kpt_last_timing(self.h, out))
         return list(out)
 
-    TUNING_KEYS = ("refill", "chunk", "refill_min", "blocks_per_cu", "sort", "graph", "ray_sort")
+    TUNING_KEYS = ("refill", "chunk", "refill_min", "blocks_per_cu", "sort", "graph", "ray_sort", "tail")
 
     def tuning(self):
         n = len(self.TUNING_KEYS)

@@ -72,7 +72,7 @@ struct kpt_ctx {
     WfInstrument ins[INS_RING]{};
     int32_t use_graph = 0;               // capture each frame as a hipGraph and replay it
     hipGraphExec_t graph_exec = nullptr;
-    TraceTuning tune{0, 128, 16, 8, 0};
+    TraceTuning tune{0, 128, 16, 8, 0, 0};
     int32_t sort_by_material = 1;
     bool count_trace = false;
     unsigned long long* d_counts = nullptr;
@@ -1356,6 +1356,10 @@ int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n) {
         if (knobs[6] < 0 || knobs[6] > 1) return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
         ctx->tune.ray_sort = knobs[6];
     }
+    if (n >= 8) {
+        if (knobs[7] < 0 || knobs[7] > 64) return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
+        ctx->tune.tail = knobs[7];
+    }
     return KPT_OK;
 }
 
@@ -1368,6 +1372,7 @@ int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n) {
     knobs[4] = ctx->sort_by_material;
     if (n >= 6) knobs[5] = ctx->use_graph;
     if (n >= 7) knobs[6] = ctx->tune.ray_sort;
+    if (n >= 8) knobs[7] = ctx->tune.tail;
     return KPT_OK;
 }
 

@@ -155,14 +155,27 @@ KD bool emitted(const SceneView& S, const FrameConst& fc, const Surf& s, bool fi
     return true;
 }
 
-// Megakernel form of Li: the whole path in one thread.
-KD v3 li_path(const SceneView& S, const FrameConst& fc, Ray ray, uint32_t& seed, uint32_t* lds, bool* ovf, VisOut* vs) {
-    v3 L = mk3(0.0f, 0.0f, 0.0f), beta = mk3(1.0f, 1.0f, 1.0f);
-    bool any_nonspec = false;
+// Li's loop state between vertices: what the wavefront keeps in HBM.
+struct PathState {
+    Ray ray;
+    v3 L, beta;
+    bool any_nonspec;
+    int32_t depth;
+    float p_b;
+    v3 prev_pos;
+};
+
+// Li from a path state to the end of the path, in one thread (the megakernel
+// from depth 0; the wavefront's path-per-lane tail from a later bounce).
+KD v3 li_loop(const SceneView& S, const FrameConst& fc, PathState& ps, uint32_t& seed, uint32_t* lds, bool* ovf,
+              VisOut* vs) {
+    Ray ray = ps.ray;
+    v3 L = ps.L, beta = ps.beta;
+    bool any_nonspec = ps.any_nonspec;
     const float eta_scale = 1.0f;
-    int32_t depth = 0;
-    float p_b = 1.0f;
-    v3 prev_pos = mk3(0.0f, 0.0f, 0.0f);
+    int32_t depth = ps.depth;
+    float p_b = ps.p_b;
+    v3 prev_pos = ps.prev_pos;
     while (!is_zero(beta)) {
         seed = seed + (uint32_t)depth;
         const bool first = (depth == 0);
@@ -220,7 +233,21 @@ KD v3 li_path(const SceneView& S, const FrameConst& fc, Ray ray, uint32_t& seed,
             beta = beta / (1.0f - q);
         }
     }
+    ps.depth = depth;
     return L;
+}
+
+// Megakernel form of Li: the whole path in one thread.
+KD v3 li_path(const SceneView& S, const FrameConst& fc, Ray ray, uint32_t& seed, uint32_t* lds, bool* ovf, VisOut* vs) {
+    PathState ps;
+    ps.ray = ray;
+    ps.L = mk3(0.0f, 0.0f, 0.0f);
+    ps.beta = mk3(1.0f, 1.0f, 1.0f);
+    ps.any_nonspec = false;
+    ps.depth = 0;
+    ps.p_b = 1.0f;
+    ps.prev_pos = mk3(0.0f, 0.0f, 0.0f);
+    return li_loop(S, fc, ps, seed, lds, ovf, vs);
 }
 
 }  // namespace kpt

@@ -102,6 +102,7 @@ struct TraceTuning {
     int refill_min;    // idle lanes that trigger a refill
     int blocks_per_cu; // persistent grid size per compute unit
     int ray_sort;      // sort secondary and shadow rays by direction octant + Morton cell of the origin
+    int tail;          // from this bounce on, live paths finish in one path-per-lane launch (0: never)
 };
 
 // Events bracketing every closest-hit trace launch of a frame (recorded on the

@@ -473,6 +473,43 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, 
     }
 }
 
+// The rest of every path still alive, one path per lane (li_loop from the
+// path's state in HBM): after the first bounces few paths remain, each needing
+// several dependent trace/shade/shadow rounds, and per-bounce launches would
+// be bound by the latency of their few rays.  Same arithmetic and random
+// numbers as the stage kernels, so the radiance is unchanged.
+__global__ __launch_bounds__(WF_BLOCK) void k_tail(SceneView S, FrameConst fc, WavefrontState W, int parity,
+                                                   int pull_slot, int32_t* overflow) {
+    __shared__ uint32_t s_stack[LDS_STACK][WF_BLOCK];
+    const int count = W.counters[C_QCOUNT + parity];
+    const int32_t* q = W.queue[parity];
+    bool ovf = false;
+    for (;;) {
+        const int base = wave_claim(&W.counters[pull_slot], WAVE);
+        if (base >= count) break;
+        const int k = base + lane_id();
+        if (k >= count) continue;
+        const int p = q[k];
+        const float4 o4 = W.ray_o[p], d4 = W.ray_d[p], b4 = W.beta[p], l4 = W.L[p], pp = W.prev_pos[p];
+        PathState ps;
+        ps.ray = make_ray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z));
+        ps.L = mk3(l4.x, l4.y, l4.z);
+        ps.beta = mk3(b4.x, b4.y, b4.z);
+        ps.p_b = b4.w;
+        const uint32_t st = fbits(l4.w);
+        ps.depth = (int32_t)(st & 0xFFFFu);
+        ps.any_nonspec = (st >> 16) & 1u;
+        ps.prev_pos = mk3(pp.x, pp.y, pp.z);
+        uint32_t seed = fbits(d4.w);
+        VisOut vs;   // written only at depth 0, which never reaches this kernel
+        bool o = false;
+        const v3 L = li_loop(S, fc, ps, seed, &s_stack[0][threadIdx.x], &o, &vs);
+        ovf = ovf || o;
+        W.L[p] = make_float4(L.x, L.y, L.z, bitsf((uint32_t)ps.depth | (st & 0x10000u)));
+    }
+    if (ovf) atomicOr(overflow, 1);
+}
+
 // Visibility of the deferred next-event estimates; adds their term to L.  An
 // estimate made at a cache-training vertex (record index in shadow_d.w, -1
 // for none) also moves that record's radiance-so-far past itself.
@@ -597,6 +634,11 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
     if (cf.enabled && !(cf.wide_roughness > 0.0f)) max_bounces = std::min(max_bounces, 3);
     for (int bounce = 0; bounce < max_bounces; ++bounce) {
         const int parity = bounce & 1;
+        if (tune.tail > 0 && bounce >= tune.tail && !(cf.enabled || cf.train)) {
+            hipLaunchKernelGGL(k_tail, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity, C_PULL + 3 * bounce,
+                               overflow);
+            break;
+        }
         // next queue and shadow count start at zero
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, W.counters + C_QCOUNT + (parity ^ 1), 0, 1);
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, W.counters + C_SHADOW, 0, 1);

@@ -169,7 +169,8 @@ def test_wavefront_equals_megakernel(plain):
     assert np.array_equal(out[0][1], out[1][1])
 
 
-@pytest.mark.parametrize("knobs", [{"ray_sort": 1}, {"ray_sort": 1, "refill": 1}, {"sort": 0, "blocks_per_cu": 3}])
+@pytest.mark.parametrize("knobs", [{"ray_sort": 1}, {"ray_sort": 1, "refill": 1}, {"sort": 0, "blocks_per_cu": 3},
+                                   {"tail": 1}, {"tail": 2}, {"tail": 4, "ray_sort": 1}])
 def test_launch_shapes_do_not_change_the_image(inst_small, knobs):
     """Every tuning knob only reorders work: the image and G-buffer are the
     default launch shape's, bit for bit."""
@@ -177,7 +178,7 @@ def test_launch_shapes_do_not_change_the_image(inst_small, knobs):
     W, H = 80, 60
     out = []
     for kn in ({}, knobs):
-        ctx.set_tuning(refill=0, chunk=128, refill_min=16, blocks_per_cu=8, sort=1, graph=0, ray_sort=0)
+        ctx.set_tuning(refill=0, chunk=128, refill_min=16, blocks_per_cu=8, sort=1, graph=0, ray_sort=0, tail=0)
         ctx.set_tuning(**kn)
         ctx.set_settings(settings(max_depth=6, integrator=abi.INTEGRATOR_WAVEFRONT, sun_theta=sc.sun_theta))
         ctx.resize(W, H)
@@ -186,7 +187,7 @@ def test_launch_shapes_do_not_change_the_image(inst_small, knobs):
         print(f"[shapes] {kn}", flush=True)
         ctx.render(3)
         out.append((ctx.readback(1), ctx.readback(2)))
-    ctx.set_tuning(refill=0, chunk=128, refill_min=16, blocks_per_cu=8, sort=1, graph=0, ray_sort=0)
+    ctx.set_tuning(refill=0, chunk=128, refill_min=16, blocks_per_cu=8, sort=1, graph=0, ray_sort=0, tail=0)
     assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
     assert np.array_equal(out[0][1], out[1][1])
 
