@@ -301,7 +301,7 @@ int kpt_set_trace_counting(kpt_ctx* ctx, int32_t enable);
  * reference counterpart): knobs[0] lane refill (0/1), [1] queue entries a
  * wave claims per atomic (64..65536), [2] idle lanes that trigger a refill
  * (1..64), [3] persistent workgroups per compute unit (1..64), [4] sort
- * shading by material (0/1), [5] (when n >= 6) capture every frame as a
+ * shading by material (0 no, 1 every bounce, 2 from bounce 1 on), [5] (when n >= 6) capture every frame as a
  * hipGraph and replay it (0/1; the per-launch trace times of kpt_last_timing
  * are then not measured), [6] (when n >= 7) sort continuation and shadow
  * rays by direction octant and Morton cell of the origin before tracing

@@ -20,7 +20,7 @@ SPACE = {
     "blocks_per_cu": [4, 8, 12, 16],
     "chunk": [64, 128, 256, 512],
     "refill_min": [8, 16, 32],
-    "sort": [0, 1],
+    "sort": [0, 1, 2],
     "graph": [0, 1],
     "ray_sort": [0, 1],
     "tail": [0, 1, 2, 3, 4],

@@ -1341,7 +1341,7 @@ int kpt_sync(kpt_ctx* ctx) {
 int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n) {
     if (!ctx || !knobs || n < 5) return KPT_ERR_INVALID_ARG;
     if (knobs[0] < 0 || knobs[0] > 1 || knobs[1] < 64 || knobs[1] > 65536 || knobs[2] < 1 || knobs[2] > 64 ||
-        knobs[3] < 1 || knobs[3] > 64 || knobs[4] < 0 || knobs[4] > 1)
+        knobs[3] < 1 || knobs[3] > 64 || knobs[4] < 0 || knobs[4] > 2)
         return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
     ctx->tune.refill = knobs[0];
     ctx->tune.chunk = knobs[1];

@@ -92,7 +92,15 @@ __global__ __launch_bounds__(WF_BLOCK) void k_generate(FrameConst fc, int32_t fr
     W.beta[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
     W.L[i] = make_float4(0.0f, 0.0f, 0.0f, bitsf(0u));
     W.prev_pos[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    W.queue[0][i] = (int32_t)i;
+    // queue order: 8x8 pixel tiles, one per wave, when the frame divides into
+    // them (a wave's camera rays then share more nodes than a 64-pixel row);
+    // paths stay indexed by pixel, only the processing order changes
+    int64_t slot = i;
+    if ((fc.width & 7) == 0 && (fc.height & 7) == 0) {
+        const int64_t tile = (int64_t)(py >> 3) * (fc.width >> 3) + (px >> 3);
+        slot = tile * 64 + (py & 7) * 8 + (px & 7);
+    }
+    W.queue[0][slot] = (int32_t)i;
     if (i == 0) W.counters[C_QCOUNT] = (int32_t)npx;
 }
 
@@ -663,7 +671,9 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
         }
         if (timed) (void)hipEventRecord(ins->end[ins->launches++], s);
         const int32_t* shade_q = W.queue[parity];
-        if (sort_by_material && nkeys > 1) {
+        // sort 2: the primary hits keep their tile order (their shadow rays and
+        // the bounce-1 rays then leave from neighbouring points)
+        if (sort_by_material && nkeys > 1 && !(sort_by_material == 2 && bounce == 0)) {
             hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(2 * MAX_KEYS), 0, s, W.hist, 0, 2 * MAX_KEYS);
             hipLaunchKernelGGL(k_sort_hist, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, W, parity, nkeys);
             hipLaunchKernelGGL(k_sort_offsets, dim3(1), dim3(64), 0, s, W, nkeys);

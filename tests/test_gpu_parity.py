@@ -170,12 +170,13 @@ def test_wavefront_equals_megakernel(plain):
 
 
 @pytest.mark.parametrize("knobs", [{"ray_sort": 1}, {"ray_sort": 1, "refill": 1}, {"sort": 0, "blocks_per_cu": 3},
-                                   {"tail": 1}, {"tail": 2}, {"tail": 4, "ray_sort": 1}])
+                                   {"tail": 1}, {"tail": 2}, {"tail": 4, "ray_sort": 1}, {"sort": 2, "tail": 2}])
 def test_launch_shapes_do_not_change_the_image(inst_small, knobs):
     """Every tuning knob only reorders work: the image and G-buffer are the
-    default launch shape's, bit for bit."""
+    default launch shape's, bit for bit (80x64: the camera rays run in 8x8
+    tile order)."""
     sc, ctx, orc = inst_small
-    W, H = 80, 60
+    W, H = 80, 64
     out = []
     for kn in ({}, knobs):
         ctx.set_tuning(refill=0, chunk=128, refill_min=16, blocks_per_cu=8, sort=1, graph=0, ray_sort=0, tail=0)
