@@ -125,13 +125,18 @@ struct LaneStack {
     }
 };
 
-// Closest-hit traversal state of one lane.
+// Closest-hit traversal state of one lane.  `cur` is the entry to process;
+// when it was reached from its parent (not popped), its header words (child
+// or first index a, count or instance b) are already known from the parent's
+// child test and are not loaded again (hk, ha, hb).
 struct ClosestTrav {
     Ray w, o;
     float tmin, tmax;
     Hit h;
     int32_t cur_inst;
     uint32_t cur;
+    uint32_t ha, hb;
+    bool hk;
     LaneStack st;
 
     // TLAS::intersect entry: empty TLAS or a miss of the root box ends the
@@ -147,13 +152,19 @@ struct ClosestTrav {
         float te;
         if (!slab(rlo, rhi, w, tmin, tmax, &te)) return false;
         cur = TLAS_BIT | 0u;
+        descend(rlo, rhi);
         return true;
+    }
+
+    KD void descend(const float4& lo, const float4& hi) {
+        ha = __builtin_bit_cast(uint32_t, lo.w); hb = __builtin_bit_cast(uint32_t, hi.w); hk = true;
     }
 
     // Continues with the stack top; false when the stack is empty.
     KD bool next() {
         if (st.sp == 0 || st.overflow) return false;
         cur = st.pop();
+        hk = false;
         return true;
     }
 
@@ -163,9 +174,12 @@ struct ClosestTrav {
         const uint32_t e = cur;
         if (e & TLAS_BIT) {
             const uint32_t ni = e & ~TLAS_BIT;
-            float4 lo, hi;
-            load_node(S.tlas_nodes, ni, &lo, &hi);
-            const uint32_t a = __builtin_bit_cast(uint32_t, lo.w), b = __builtin_bit_cast(uint32_t, hi.w);
+            if (!hk) {
+                float4 lo, hi;
+                load_node(S.tlas_nodes, ni, &lo, &hi);
+                descend(lo, hi);
+            }
+            const uint32_t a = ha, b = hb;
             if (COUNT) cnt->tlas++;
             if (a != 0u) {
                 const uint32_t i1 = a & 0xFFFFu, i2 = (a >> 16) & 0xFFFFu;
@@ -176,24 +190,27 @@ struct ClosestTrav {
                 const bool k1 = slab(l1, h1, w, tmin, tmax, &c1) && c1 < h.t;
                 const bool k2 = slab(l2, h2, w, tmin, tmax, &c2) && c2 < h.t;
                 if (k1 && k2) {
-                    if (c1 > c2) { st.push(TLAS_BIT | i1); cur = TLAS_BIT | i2; }
-                    else { st.push(TLAS_BIT | i2); cur = TLAS_BIT | i1; }
+                    if (c1 > c2) { st.push(TLAS_BIT | i1); cur = TLAS_BIT | i2; descend(l2, h2); }
+                    else { st.push(TLAS_BIT | i2); cur = TLAS_BIT | i1; descend(l1, h1); }
                     return !st.overflow;
                 }
-                if (k1) { cur = TLAS_BIT | i1; return true; }
-                if (k2) { cur = TLAS_BIT | i2; return true; }
+                if (k1) { cur = TLAS_BIT | i1; descend(l1, h1); return true; }
+                if (k2) { cur = TLAS_BIT | i2; descend(l2, h2); return true; }
                 return next();
             }
             if (COUNT) cnt->entries++;
             cur_inst = (int32_t)b;
             const InstTrav it = S.inst_trav[cur_inst];
             o = object_ray(it, w);
-            if (it.root >= 0) { cur = (uint32_t)it.root; return true; }
+            if (it.root >= 0) { cur = (uint32_t)it.root; hk = false; return true; }
             return next();
         }
-        float4 lo, hi;
-        load_node(S.blas_nodes, e, &lo, &hi);
-        const uint32_t a = __builtin_bit_cast(uint32_t, lo.w), b = __builtin_bit_cast(uint32_t, hi.w);
+        if (!hk) {
+            float4 lo, hi;
+            load_node(S.blas_nodes, e, &lo, &hi);
+            descend(lo, hi);
+        }
+        const uint32_t a = ha, b = hb;
         const float bt = fminf_(tmax, h.t);
         if (b == 0u) {
             if (COUNT) cnt->interior++;
@@ -204,12 +221,12 @@ struct ClosestTrav {
             const bool k1 = slab(l1, h1, o, tmin, bt, &c1) && c1 < h.t;
             const bool k2 = slab(l2, h2, o, tmin, bt, &c2) && c2 < h.t;
             if (k1 && k2) {
-                if (c1 > c2) { st.push(a); cur = a + 1; }
-                else { st.push(a + 1); cur = a; }
+                if (c1 > c2) { st.push(a); cur = a + 1; descend(l2, h2); }
+                else { st.push(a + 1); cur = a; descend(l1, h1); }
                 return !st.overflow;
             }
-            if (k1) { cur = a; return true; }
-            if (k2) { cur = a + 1; return true; }
+            if (k1) { cur = a; descend(l1, h1); return true; }
+            if (k2) { cur = a + 1; descend(l2, h2); return true; }
             return next();
         }
         if (COUNT) cnt->leaf++;
@@ -225,12 +242,14 @@ struct ClosestTrav {
     }
 };
 
-// Any-hit traversal state of one lane.
+// Any-hit traversal state of one lane (header words as in ClosestTrav).
 struct AnyTrav {
     Ray w, o;
     float tmin, tmax;
     bool hit;
     uint32_t cur;
+    uint32_t ha, hb;
+    bool hk;
     LaneStack st;
 
     KD bool init(const SceneView& S, const Ray& ray, float t0, float t1, uint32_t* lds, uint32_t* spill) {
@@ -242,11 +261,16 @@ struct AnyTrav {
         float te;
         if (!slab(rlo, rhi, w, tmin, tmax, &te)) return false;
         cur = TLAS_BIT | 0u;
+        descend(rlo, rhi);
         return true;
+    }
+    KD void descend(const float4& lo, const float4& hi) {
+        ha = __builtin_bit_cast(uint32_t, lo.w); hb = __builtin_bit_cast(uint32_t, hi.w); hk = true;
     }
     KD bool next() {
         if (st.sp == 0 || st.overflow) return false;
         cur = st.pop();
+        hk = false;
         return true;
     }
     template <bool COUNT>
@@ -254,9 +278,12 @@ struct AnyTrav {
         const uint32_t e = cur;
         if (e & TLAS_BIT) {
             const uint32_t ni = e & ~TLAS_BIT;
-            float4 lo, hi;
-            load_node(S.tlas_nodes, ni, &lo, &hi);
-            const uint32_t a = __builtin_bit_cast(uint32_t, lo.w), b = __builtin_bit_cast(uint32_t, hi.w);
+            if (!hk) {
+                float4 lo, hi;
+                load_node(S.tlas_nodes, ni, &lo, &hi);
+                descend(lo, hi);
+            }
+            const uint32_t a = ha, b = hb;
             if (COUNT) cnt->tlas++;
             if (a != 0u) {
                 const uint32_t i1 = a & 0xFFFFu, i2 = (a >> 16) & 0xFFFFu;
@@ -266,20 +293,23 @@ struct AnyTrav {
                 float c1, c2;
                 const bool k1 = slab(l1, h1, w, tmin, tmax, &c1);
                 const bool k2 = slab(l2, h2, w, tmin, tmax, &c2);
-                if (k1 && k2) { st.push(TLAS_BIT | i1); cur = TLAS_BIT | i2; return !st.overflow; }
-                if (k1) { cur = TLAS_BIT | i1; return true; }
-                if (k2) { cur = TLAS_BIT | i2; return true; }
+                if (k1 && k2) { st.push(TLAS_BIT | i1); cur = TLAS_BIT | i2; descend(l2, h2); return !st.overflow; }
+                if (k1) { cur = TLAS_BIT | i1; descend(l1, h1); return true; }
+                if (k2) { cur = TLAS_BIT | i2; descend(l2, h2); return true; }
                 return next();
             }
             if (COUNT) cnt->entries++;
             const InstTrav it = S.inst_trav[b];
             o = object_ray(it, w);
-            if (it.root >= 0) { cur = (uint32_t)it.root; return true; }
+            if (it.root >= 0) { cur = (uint32_t)it.root; hk = false; return true; }
             return next();
         }
-        float4 lo, hi;
-        load_node(S.blas_nodes, e, &lo, &hi);
-        const uint32_t a = __builtin_bit_cast(uint32_t, lo.w), b = __builtin_bit_cast(uint32_t, hi.w);
+        if (!hk) {
+            float4 lo, hi;
+            load_node(S.blas_nodes, e, &lo, &hi);
+            descend(lo, hi);
+        }
+        const uint32_t a = ha, b = hb;
         if (b == 0u) {
             if (COUNT) cnt->interior++;
             float4 l1, h1, l2, h2;
@@ -288,9 +318,9 @@ struct AnyTrav {
             float c1, c2;
             const bool k1 = slab(l1, h1, o, tmin, tmax, &c1);
             const bool k2 = slab(l2, h2, o, tmin, tmax, &c2);
-            if (k1 && k2) { st.push(a); cur = a + 1; return !st.overflow; }
-            if (k1) { cur = a; return true; }
-            if (k2) { cur = a + 1; return true; }
+            if (k1 && k2) { st.push(a); cur = a + 1; descend(l2, h2); return !st.overflow; }
+            if (k1) { cur = a; descend(l1, h1); return true; }
+            if (k2) { cur = a + 1; descend(l2, h2); return true; }
             return next();
         }
         if (COUNT) cnt->leaf++;
