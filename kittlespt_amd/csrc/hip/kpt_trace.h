@@ -108,8 +108,10 @@ KD bool tri_test(const TriPos& tp, const Ray& r, float tmin, float tmax, float* 
 // Per-lane stack: LDS part [LDS_STACK][TRACE_BLOCK] + a private spill array
 // owned by the caller (kept outside the traversal state so the state itself
 // stays in registers).
+typedef __attribute__((address_space(3))) uint32_t lds_u32;   // a pointer that is known to be LDS
+
 struct LaneStack {
-    uint32_t* lds;      // &s_stack[0][lane]
+    lds_u32* lds;       // &s_stack[0][lane]
     uint32_t* spill;    // private array of SPILL_STACK entries
     int sp;
     bool overflow;
@@ -145,7 +147,7 @@ struct ClosestTrav {
         w = ray; o = ray; tmin = t0; tmax = t1;
         h.t = __builtin_inff(); h.slot = -1; h.inst = -1; h.u = 0.0f; h.v = 0.0f;
         cur_inst = -1;
-        st.lds = lds; st.spill = spill; st.sp = 0; st.overflow = false;
+        st.lds = (lds_u32*)lds; st.spill = spill; st.sp = 0; st.overflow = false;
         if (S.tlas_root < 0) return false;
         float4 rlo, rhi;
         load_node(S.tlas_nodes, 0, &rlo, &rhi);
@@ -254,7 +256,7 @@ struct AnyTrav {
 
     KD bool init(const SceneView& S, const Ray& ray, float t0, float t1, uint32_t* lds, uint32_t* spill) {
         w = ray; o = ray; tmin = t0; tmax = t1; hit = false;
-        st.lds = lds; st.spill = spill; st.sp = 0; st.overflow = false;
+        st.lds = (lds_u32*)lds; st.spill = spill; st.sp = 0; st.overflow = false;
         if (S.tlas_root < 0) return false;
         float4 rlo, rhi;
         load_node(S.tlas_nodes, 0, &rlo, &rhi);
