@@ -232,12 +232,22 @@ struct ClosestTrav {
             return next();
         }
         if (COUNT) cnt->leaf++;
-        for (uint32_t k = 0; k < b; ++k) {
-            if (COUNT) cnt->tri++;
-            const TriPos tp = S.tris[a + k];
+        // two triangles' loads in flight at a time; tested in slot order
+        for (uint32_t k = 0; k < b; k += 2) {
+            const TriPos tp0 = S.tris[a + k];
+            const bool two = k + 1 < b;
+            TriPos tp1;
+            if (two) tp1 = S.tris[a + k + 1];
             float t, u, v;
-            if (tri_test(tp, o, tmin, fminf_(tmax, h.t), &t, &u, &v) && t < h.t) {
+            if (COUNT) cnt->tri++;
+            if (tri_test(tp0, o, tmin, fminf_(tmax, h.t), &t, &u, &v) && t < h.t) {
                 h.t = t; h.slot = (int32_t)(a + k); h.inst = cur_inst; h.u = u; h.v = v;
+            }
+            if (two) {
+                if (COUNT) cnt->tri++;
+                if (tri_test(tp1, o, tmin, fminf_(tmax, h.t), &t, &u, &v) && t < h.t) {
+                    h.t = t; h.slot = (int32_t)(a + k + 1); h.inst = cur_inst; h.u = u; h.v = v;
+                }
             }
         }
         return next();
@@ -326,11 +336,18 @@ struct AnyTrav {
             return next();
         }
         if (COUNT) cnt->leaf++;
-        for (uint32_t k = 0; k < b; ++k) {
-            if (COUNT) cnt->tri++;
-            const TriPos tp = S.tris[a + k];
+        for (uint32_t k = 0; k < b; k += 2) {
+            const TriPos tp0 = S.tris[a + k];
+            const bool two = k + 1 < b;
+            TriPos tp1;
+            if (two) tp1 = S.tris[a + k + 1];
             float t, u, v;
-            if (tri_test(tp, o, tmin, tmax, &t, &u, &v)) { hit = true; return false; }
+            if (COUNT) cnt->tri++;
+            if (tri_test(tp0, o, tmin, tmax, &t, &u, &v)) { hit = true; return false; }
+            if (two) {
+                if (COUNT) cnt->tri++;
+                if (tri_test(tp1, o, tmin, tmax, &t, &u, &v)) { hit = true; return false; }
+            }
         }
         return next();
     }
