@@ -35,6 +35,10 @@ constexpr int TRACE_BLOCK = 256;      // threads per workgroup of trace-bearing 
 constexpr int LDS_STACK = 16;         // stack entries per lane held in LDS
 constexpr int SPILL_STACK = 112;      // further entries in private memory
 constexpr uint32_t TLAS_BIT = 0x80000000u;
+#ifndef KPT_TRI_BATCH
+#define KPT_TRI_BATCH 4
+#endif
+constexpr int TRI_BATCH = KPT_TRI_BATCH;   // leaf triangles loaded together
 
 struct Ray {
     v3 o, d, inv;
@@ -232,21 +236,20 @@ struct ClosestTrav {
             return next();
         }
         if (COUNT) cnt->leaf++;
-        // two triangles' loads in flight at a time; tested in slot order
-        for (uint32_t k = 0; k < b; k += 2) {
-            const TriPos tp0 = S.tris[a + k];
-            const bool two = k + 1 < b;
-            TriPos tp1;
-            if (two) tp1 = S.tris[a + k + 1];
-            float t, u, v;
-            if (COUNT) cnt->tri++;
-            if (tri_test(tp0, o, tmin, fminf_(tmax, h.t), &t, &u, &v) && t < h.t) {
-                h.t = t; h.slot = (int32_t)(a + k); h.inst = cur_inst; h.u = u; h.v = v;
-            }
-            if (two) {
-                if (COUNT) cnt->tri++;
-                if (tri_test(tp1, o, tmin, fminf_(tmax, h.t), &t, &u, &v) && t < h.t) {
-                    h.t = t; h.slot = (int32_t)(a + k + 1); h.inst = cur_inst; h.u = u; h.v = v;
+        // TRI_BATCH triangles' loads in flight at a time; tested in slot order
+        for (uint32_t k = 0; k < b; k += TRI_BATCH) {
+            TriPos tp[TRI_BATCH];
+#pragma unroll
+            for (int j = 0; j < TRI_BATCH; ++j)
+                if (k + j < b) tp[j] = S.tris[a + k + j];
+#pragma unroll
+            for (int j = 0; j < TRI_BATCH; ++j) {
+                if (k + j < b) {
+                    if (COUNT) cnt->tri++;
+                    float t, u, v;
+                    if (tri_test(tp[j], o, tmin, fminf_(tmax, h.t), &t, &u, &v) && t < h.t) {
+                        h.t = t; h.slot = (int32_t)(a + k + j); h.inst = cur_inst; h.u = u; h.v = v;
+                    }
                 }
             }
         }
@@ -336,17 +339,18 @@ struct AnyTrav {
             return next();
         }
         if (COUNT) cnt->leaf++;
-        for (uint32_t k = 0; k < b; k += 2) {
-            const TriPos tp0 = S.tris[a + k];
-            const bool two = k + 1 < b;
-            TriPos tp1;
-            if (two) tp1 = S.tris[a + k + 1];
-            float t, u, v;
-            if (COUNT) cnt->tri++;
-            if (tri_test(tp0, o, tmin, tmax, &t, &u, &v)) { hit = true; return false; }
-            if (two) {
-                if (COUNT) cnt->tri++;
-                if (tri_test(tp1, o, tmin, tmax, &t, &u, &v)) { hit = true; return false; }
+        for (uint32_t k = 0; k < b; k += TRI_BATCH) {
+            TriPos tp[TRI_BATCH];
+#pragma unroll
+            for (int j = 0; j < TRI_BATCH; ++j)
+                if (k + j < b) tp[j] = S.tris[a + k + j];
+#pragma unroll
+            for (int j = 0; j < TRI_BATCH; ++j) {
+                if (k + j < b) {
+                    if (COUNT) cnt->tri++;
+                    float t, u, v;
+                    if (tri_test(tp[j], o, tmin, tmax, &t, &u, &v)) { hit = true; return false; }
+                }
             }
         }
         return next();
