@@ -22,7 +22,11 @@
 //     closest hit (including which of two equal-distance hits wins) is the
 //     same;
 //   * any-hit needs no ordering: its result is "some reachable triangle is
-//     hit", which does not depend on the visiting order.
+//     hit", which does not depend on the visiting order; it visits the
+//     nearer child first, which finds occluders close to the origin sooner;
+//   * a node reached from its parent reuses the header words read in the
+//     parent's child test, and a leaf's triangles are loaded TRI_BATCH at a
+//     time (tested in slot order, so the closest-hit tie rule is unchanged).
 // Traversal is a state machine (init / step) so persistent kernels can refill
 // a lane with a new ray as soon as its ray finishes.
 #pragma once
@@ -308,7 +312,11 @@ struct AnyTrav {
                 float c1, c2;
                 const bool k1 = slab(l1, h1, w, tmin, tmax, &c1);
                 const bool k2 = slab(l2, h2, w, tmin, tmax, &c2);
-                if (k1 && k2) { st.push(TLAS_BIT | i1); cur = TLAS_BIT | i2; descend(l2, h2); return !st.overflow; }
+                if (k1 && k2) {   // nearer first: an occluder near the origin ends the query sooner
+                    if (c1 > c2) { st.push(TLAS_BIT | i1); cur = TLAS_BIT | i2; descend(l2, h2); }
+                    else { st.push(TLAS_BIT | i2); cur = TLAS_BIT | i1; descend(l1, h1); }
+                    return !st.overflow;
+                }
                 if (k1) { cur = TLAS_BIT | i1; descend(l1, h1); return true; }
                 if (k2) { cur = TLAS_BIT | i2; descend(l2, h2); return true; }
                 return next();
@@ -333,7 +341,11 @@ struct AnyTrav {
             float c1, c2;
             const bool k1 = slab(l1, h1, o, tmin, tmax, &c1);
             const bool k2 = slab(l2, h2, o, tmin, tmax, &c2);
-            if (k1 && k2) { st.push(a); cur = a + 1; descend(l2, h2); return !st.overflow; }
+            if (k1 && k2) {
+                if (c1 > c2) { st.push(a); cur = a + 1; descend(l2, h2); }
+                else { st.push(a + 1); cur = a; descend(l1, h1); }
+                return !st.overflow;
+            }
             if (k1) { cur = a; descend(l1, h1); return true; }
             if (k2) { cur = a + 1; descend(l2, h2); return true; }
             return next();
