@@ -108,10 +108,11 @@ def fp64_reference(scene, st, cam, W, H, pixels, spp):
     return ref.astype(np.float64)
 
 
-def measured_traffic(W, H, instances, depth, spp):
+def measured_traffic(W, H, instances, depth, spp, tail):
     """HBM bytes per k_trace launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE passes of this workload (scaled by the copy-kernel
-    calibration), or (None, None) for another workload."""
+    calibration), or (None, None) for another workload or another set of
+    k_trace launches (the `tail` knob decides which bounces they cover)."""
     path = os.path.join(ROOT, "profiles", "k_trace_instanced1080p_traffic.json")
     try:
         with open(path) as f:
@@ -119,7 +120,8 @@ def measured_traffic(W, H, instances, depth, spp):
     except (OSError, ValueError):
         return None, None
     w = t.get("workload", {})
-    if (w.get("width"), w.get("height"), w.get("instances"), w.get("depth"), w.get("spp")) != (W, H, instances, depth, spp):
+    if ((w.get("width"), w.get("height"), w.get("instances"), w.get("depth"), w.get("spp"), w.get("tail")) !=
+            (W, H, instances, depth, spp, tail)):
         return None, None
     b = 1024 * (t["fetch_kib_per_dispatch"] * t["fetch_scale"] + t["write_kib_per_dispatch"] * t["write_scale"])
     return int(b), os.path.relpath(path, ROOT)
@@ -377,7 +379,8 @@ def main():
         bytes_per_launch = bytes_step / count_launches
         avg_launch_s = (trace_ms / launches) / 1e3
         achieved = bytes_per_launch / avg_launch_s / 1e9
-        traffic, traffic_src = measured_traffic(W, H, args.instances, args.depth, args.spp)
+        traffic, traffic_src = measured_traffic(W, H, args.instances, args.depth, args.spp,
+                                                (tuning or {}).get("tail", 0))
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": "k_trace (wavefront closest hit)",
