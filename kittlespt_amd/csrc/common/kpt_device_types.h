@@ -23,6 +23,20 @@ struct alignas(16) Node {
 };
 static_assert(sizeof(Node) == 32, "Node is 32 bytes");
 
+// Two BVH levels in one 192-byte record, for the traversal kernels: the
+// children pair of an interior node (c0, c1: copies of nodes 2q and 2q+1 of
+// the level's pair layout, where q is the record index) and, for each
+// interior child, its own children (g0, g1 for c0; g2, g3 for c1; zero
+// otherwise).  BLAS records follow the BLAS node array (sibling pairs at even
+// indices); TLAS records follow a renumbered copy of the TLAS in the same
+// pair layout (root at 0, pairs from 2 on; interior a = children pair index,
+// leaf a = 0, b = instance).
+struct alignas(64) NodeBlock {
+    Node c[2];
+    Node g[4];
+};
+static_assert(sizeof(NodeBlock) == 192, "NodeBlock is 192 bytes");
+
 // Triangle positions in BVH-leaf (slot) order: v0 and the two edges the
 // Moller-Trumbore test needs (triangle.cuh:66-102 computes v1-v0 and v2-v0 per
 // test; the host computes the same IEEE differences once).  v0.w holds the
@@ -58,7 +72,8 @@ struct alignas(16) InstTrav {
     float r2[4];
     int32_t root;
     int32_t geom;
-    int32_t pad[2];
+    uint32_t root_entry;   // traversal entry of the BLAS root (0xFFFFFFFF: empty)
+    int32_t pad;
 };
 static_assert(sizeof(InstTrav) == 64, "InstTrav is 64 bytes");
 
@@ -137,7 +152,9 @@ struct alignas(16) FrameConst {
 // Scene pointers passed by value to kernels.
 struct SceneView {
     const Node* blas_nodes;
-    const Node* tlas_nodes;
+    const Node* tlas_nodes;      // pair layout (see NodeBlock)
+    const NodeBlock* blas_blocks;
+    const NodeBlock* tlas_blocks;
     const TriPos* tris;
     const TriAttr* attrs;
     const InstTrav* inst_trav;

@@ -49,7 +49,9 @@ struct kpt_ctx {
     bool scene_ready = false;
     // device scene
     Node* d_blas = nullptr;
-    Node* d_tlas = nullptr;
+    Node* d_tlas = nullptr;          // TLAS in the pair layout of the traversal
+    NodeBlock* d_bblocks = nullptr;  // two-level records of the BLAS pairs
+    NodeBlock* d_tblocks = nullptr;  // two-level records of the TLAS pairs
     TriPos* d_tris = nullptr;
     TriAttr* d_attrs = nullptr;
     InstTrav* d_itrav = nullptr;
@@ -134,6 +136,8 @@ SceneView scene_view(const kpt_ctx* c) {
     SceneView S;
     S.blas_nodes = c->d_blas;
     S.tlas_nodes = c->d_tlas;
+    S.blas_blocks = c->d_bblocks;
+    S.tlas_blocks = c->d_tblocks;
     S.tris = c->d_tris;
     S.attrs = c->d_attrs;
     S.inst_trav = c->d_itrav;
@@ -151,6 +155,58 @@ SceneView scene_view(const kpt_ctx* c) {
 
 void store_mat(const hm::Mat4& m, float* dst) { std::memcpy(dst, m.m, 64); }
 
+// Two-level records of a pair-layout node array (see NodeBlock): record q
+// holds nodes 2q, 2q+1 and the children of each interior one.  `interior`
+// tells the level's interior test; `pair` gives an interior node's children
+// pair index.  Returns false when a children pair does not sit at an even
+// index (the layout the traversal relies on).
+template <class IsInterior>
+bool make_blocks(const std::vector<Node>& nodes, IsInterior interior, std::vector<NodeBlock>* out) {
+    out->assign(nodes.size() / 2 + 1, NodeBlock{});
+    for (size_t i = 0; i < nodes.size(); ++i) {
+        const Node& x = nodes[i];
+        if (!interior(x)) continue;
+        const uint32_t p = x.a;
+        if ((p & 1u) || (size_t)p + 1 >= nodes.size()) return false;
+        NodeBlock& blk = (*out)[p >> 1];
+        blk.c[0] = nodes[p];
+        blk.c[1] = nodes[p + 1];
+        for (int k = 0; k < 2; ++k) {
+            const Node& ch = nodes[p + k];
+            if (!interior(ch)) continue;
+            if ((ch.a & 1u) || (size_t)ch.a + 1 >= nodes.size()) return false;
+            blk.g[2 * k] = nodes[ch.a];
+            blk.g[2 * k + 1] = nodes[ch.a + 1];
+        }
+    }
+    return true;
+}
+
+// The TLAS renumbered into the pair layout: root at 0, index 1 unused, the
+// children of every interior node at a fresh even pair (first child even),
+// interior a = that pair's index, leaf a = 0 (b = instance); boxes unchanged.
+std::vector<Node> tlas_pair_layout(const std::vector<Node>& orig) {
+    std::vector<Node> out;
+    if (orig.empty()) return out;
+    out.push_back(orig[0]);
+    out.push_back(Node{});
+    std::vector<std::pair<uint32_t, uint32_t>> work{{0u, 0u}};   // (original index, new index)
+    for (size_t w = 0; w < work.size(); ++w) {
+        const Node src = orig[work[w].first];
+        if (src.a == 0u) { out[work[w].second] = src; continue; }
+        const uint32_t i1 = src.a & 0xFFFFu, i2 = (src.a >> 16) & 0xFFFFu;
+        const uint32_t p = (uint32_t)out.size();
+        out.push_back(orig[i1]);
+        out.push_back(orig[i2]);
+        Node n = src;
+        n.a = p;
+        out[work[w].second] = n;
+        work.push_back({i1, p});
+        work.push_back({i2, p + 1});
+    }
+    return out;
+}
+
 // Per-instance records from a model matrix (Renderer::setMeshTransform,
 // renderer.cu:250-269; BLAS2::setTransform, blas.cuh:190-200).
 void make_instance(kpt_ctx* c, int32_t i, const float* model, bool keep_prev) {
@@ -161,6 +217,11 @@ void make_instance(kpt_ctx* c, int32_t i, const float* model, bool keep_prev) {
     int32_t g = c->inst_mesh[i];
     it.root = c->blas[g].used > 0 ? c->geom_base[g] : -1;
     it.geom = g;
+    it.root_entry = 0xFFFFFFFFu;
+    if (it.root >= 0) {
+        const Node& r = c->nodes[(size_t)it.root];
+        it.root_entry = blas_entry((uint32_t)it.root, r.a, r.b);
+    }
     InstShade& sh = c->ishade[i];
     hm::Mat4 curr_inv = hm::glm_inverse(M);
     hm::Mat4 prev_inv = keep_prev ? hm::load(sh.curr_inv) : curr_inv;
@@ -175,9 +236,14 @@ int upload_tlas(kpt_ctx* ctx) {
     ctx->tlas_dirty = false;
     if (ctx->tlas.nodes.size() > 65535) return fail(ctx, KPT_ERR_UNSUPPORTED, "TLAS exceeds 65535 nodes (16-bit child fields)");
     if (ctx->host_only) return KPT_OK;
+    const std::vector<Node> tp = tlas_pair_layout(ctx->tlas.nodes);
+    std::vector<NodeBlock> tb;
+    if (!make_blocks(tp, [](const Node& n) { return n.a != 0u; }, &tb))
+        return fail(ctx, KPT_ERR_UNSUPPORTED, "TLAS pair layout failed");
     // the previous frame may still read the old arrays
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    HIPCHK(dev_upload(&ctx->d_tlas, ctx->tlas.nodes));
+    HIPCHK(dev_upload(&ctx->d_tlas, tp));
+    HIPCHK(dev_upload(&ctx->d_tblocks, tb));
     HIPCHK(dev_upload(&ctx->d_itrav, ctx->itrav));
     HIPCHK(dev_upload(&ctx->d_ishade, ctx->ishade));
     return KPT_OK;
@@ -494,6 +560,7 @@ int kpt_ctx_destroy(kpt_ctx* ctx) {
         (void)hipSetDevice(ctx->device);
         if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
         dev_free(&ctx->d_blas); dev_free(&ctx->d_tlas); dev_free(&ctx->d_tris); dev_free(&ctx->d_attrs);
+        dev_free(&ctx->d_bblocks); dev_free(&ctx->d_tblocks);
         dev_free(&ctx->d_itrav); dev_free(&ctx->d_ishade); dev_free(&ctx->d_mat); dev_free(&ctx->d_tex);
         dev_free(&ctx->d_pix); dev_free(&ctx->d_lights); dev_free(&ctx->d_lverts); dev_free(&ctx->d_overflow);
         free_frame(ctx);
@@ -684,6 +751,12 @@ int kpt_upload_scene(kpt_ctx* ctx, const kpt_scene_desc* d) {
     if (!ctx->host_only) {
         HIPCHK(hipSetDevice(ctx->device));
         HIPCHK(dev_upload(&ctx->d_blas, ctx->nodes));
+        std::vector<NodeBlock> bb;
+        if (!make_blocks(ctx->nodes, [](const Node& n) { return n.b == 0u; }, &bb))
+            return fail(ctx, KPT_ERR_UNSUPPORTED, "BLAS sibling pair not at an even node index");
+        if (ctx->nodes.size() >= (size_t)ENTRY_PAYLOAD)
+            return fail(ctx, KPT_ERR_UNSUPPORTED, "BLAS node count exceeds the traversal entry range");
+        HIPCHK(dev_upload(&ctx->d_bblocks, bb));
         HIPCHK(dev_upload(&ctx->d_tris, tris));
         HIPCHK(dev_upload(&ctx->d_attrs, attrs));
         HIPCHK(dev_upload(&ctx->d_mat, ctx->materials));

@@ -135,18 +135,55 @@ struct LaneStack {
     }
 };
 
-// Closest-hit traversal state of one lane.  `cur` is the entry to process;
-// when it was reached from its parent (not popped), its header words (child
-// or first index a, count or instance b) are already known from the parent's
-// child test and are not loaded again (hk, ha, hb).
+// ---- stack / current entries.  A BVH2 node is represented by what the
+// traversal needs to process it without loading the node itself:
+//   interior BLAS node:  pair index of its children (children at 2q, 2q+1)
+//   interior TLAS node:  TLAS_BIT | pair index of its children
+//   BLAS leaf:           LEAF_BIT | count << 23 | first slot   (count < 64, first < 2^23)
+//   larger BLAS leaf:    BIG_BIT | node index (its header is loaded)
+//   TLAS leaf:           TLAS_BIT | LEAF_BIT | instance
+// Processing an interior entry loads the 192-byte NodeBlock of its children
+// pair: both children and their children.  That replays two consecutive steps
+// of the BVH2 traversal (the node, then its nearer child when that child is
+// interior) from one block load, at the same closest distance h.t — no
+// triangle is tested between the two — so every culling decision, every push
+// and the visiting order are those of the node-at-a-time traversal.
+constexpr uint32_t LEAF_BIT = 0x40000000u;
+constexpr uint32_t BIG_BIT = 0x20000000u;
+constexpr uint32_t ENTRY_PAYLOAD = 0x1FFFFFFFu;
+
+KD uint32_t bits_u(float f) { return __builtin_bit_cast(uint32_t, f); }
+
+// Entry of a BLAS child given its node index and header words.
+KD uint32_t blas_entry(uint32_t idx, uint32_t a, uint32_t b) {
+    if (b == 0u) return a >> 1;
+    if (a < (1u << 23) && b < 64u) return LEAF_BIT | (b << 23) | a;
+    return BIG_BIT | idx;
+}
+// Entry of a TLAS child given its header words.
+KD uint32_t tlas_entry(uint32_t a, uint32_t b) {
+    return a != 0u ? (TLAS_BIT | (a >> 1)) : (TLAS_BIT | LEAF_BIT | b);
+}
+
+struct BlockView {
+    float4 c0l, c0h, c1l, c1h;   // children of the pair's parent
+    float4 g0l, g0h, g1l, g1h;   // children of child 0 (when interior)
+    float4 g2l, g2h, g3l, g3h;   // children of child 1 (when interior)
+};
+KD void load_block(const NodeBlock* blocks, uint32_t q, BlockView* v) {
+    const float4* p = reinterpret_cast<const float4*>(blocks + q);
+    v->c0l = p[0]; v->c0h = p[1]; v->c1l = p[2]; v->c1h = p[3];
+    v->g0l = p[4]; v->g0h = p[5]; v->g1l = p[6]; v->g1h = p[7];
+    v->g2l = p[8]; v->g2h = p[9]; v->g3l = p[10]; v->g3h = p[11];
+}
+
+// Closest-hit traversal state of one lane.
 struct ClosestTrav {
     Ray w, o;
     float tmin, tmax;
     Hit h;
     int32_t cur_inst;
     uint32_t cur;
-    uint32_t ha, hb;
-    bool hk;
     LaneStack st;
 
     // TLAS::intersect entry: empty TLAS or a miss of the root box ends the
@@ -161,83 +198,98 @@ struct ClosestTrav {
         load_node(S.tlas_nodes, 0, &rlo, &rhi);
         float te;
         if (!slab(rlo, rhi, w, tmin, tmax, &te)) return false;
-        cur = TLAS_BIT | 0u;
-        descend(rlo, rhi);
+        cur = tlas_entry(bits_u(rlo.w), bits_u(rhi.w));
         return true;
-    }
-
-    KD void descend(const float4& lo, const float4& hi) {
-        ha = __builtin_bit_cast(uint32_t, lo.w); hb = __builtin_bit_cast(uint32_t, hi.w); hk = true;
     }
 
     // Continues with the stack top; false when the stack is empty.
     KD bool next() {
         if (st.sp == 0 || st.overflow) return false;
         cur = st.pop();
-        hk = false;
         return true;
+    }
+
+    // One BVH2 interior step on the two children (l1,h1 = index i1; l2,h2 =
+    // index i2) of the node being processed: TLAS (level bit set) or BLAS.
+    // Returns 0 = nothing hit (pop), 1 = the nearer (or only) child is in
+    // *near_* (after the farther one was pushed).
+    template <bool TL>
+    KD bool children(const float4& l1, const float4& h1, const float4& l2, const float4& h2, uint32_t i1, uint32_t i2,
+                     int* near_k) {
+        const Ray& r = TL ? w : o;
+        const float bt = TL ? tmax : fminf_(tmax, h.t);
+        float c1, c2;
+        const bool k1 = slab(l1, h1, r, tmin, bt, &c1) && c1 < h.t;
+        const bool k2 = slab(l2, h2, r, tmin, bt, &c2) && c2 < h.t;
+        if (k1 && k2) {
+            if (c1 > c2) {
+                st.push(TL ? tlas_entry(bits_u(l1.w), bits_u(h1.w)) : blas_entry(i1, bits_u(l1.w), bits_u(h1.w)));
+                *near_k = 1;
+            } else {
+                st.push(TL ? tlas_entry(bits_u(l2.w), bits_u(h2.w)) : blas_entry(i2, bits_u(l2.w), bits_u(h2.w)));
+                *near_k = 0;
+            }
+            return true;
+        }
+        if (k1) { *near_k = 0; return true; }
+        if (k2) { *near_k = 1; return true; }
+        return false;
     }
 
     // Processes the current entry; returns false when the query is finished.
     template <bool COUNT>
     KD bool step(const SceneView& S, TraceCounts* cnt) {
         const uint32_t e = cur;
+        if (!(e & (LEAF_BIT | BIG_BIT))) {
+            // interior node (TLAS or BLAS): its children pair's block
+            const bool tl = (e & TLAS_BIT) != 0;
+            const uint32_t q = e & ENTRY_PAYLOAD;
+            BlockView v;
+            load_block(tl ? S.tlas_blocks : S.blas_blocks, q, &v);
+            int nk;
+            bool any;
+            if (COUNT) { if (tl) cnt->tlas++; else cnt->interior++; }
+            any = tl ? children<true>(v.c0l, v.c0h, v.c1l, v.c1h, 2 * q, 2 * q + 1, &nk)
+                     : children<false>(v.c0l, v.c0h, v.c1l, v.c1h, 2 * q, 2 * q + 1, &nk);
+            if (!any) return next();
+            if (st.overflow) return false;
+            const float4 nl = nk ? v.c1l : v.c0l, nh = nk ? v.c1h : v.c0h;
+            const uint32_t na = bits_u(nl.w), nb = bits_u(nh.w);
+            const bool n_interior = tl ? (na != 0u) : (nb == 0u);
+            if (!n_interior) {
+                cur = tl ? tlas_entry(na, nb) : blas_entry(2 * q + (uint32_t)nk, na, nb);
+                return true;
+            }
+            // the nearer child is interior: its step, from the same block
+            const float4 al = nk ? v.g2l : v.g0l, ah = nk ? v.g2h : v.g0h;
+            const float4 bl = nk ? v.g3l : v.g1l, bh = nk ? v.g3h : v.g1h;
+            const uint32_t p = na >> 1;   // the nearer child's children pair
+            if (COUNT) { if (tl) cnt->tlas++; else cnt->interior++; }
+            any = tl ? children<true>(al, ah, bl, bh, 2 * p, 2 * p + 1, &nk)
+                     : children<false>(al, ah, bl, bh, 2 * p, 2 * p + 1, &nk);
+            if (!any) return next();
+            if (st.overflow) return false;
+            const float4 ml = nk ? bl : al, mh = nk ? bh : ah;
+            cur = tl ? tlas_entry(bits_u(ml.w), bits_u(mh.w)) : blas_entry(2 * p + (uint32_t)nk, bits_u(ml.w), bits_u(mh.w));
+            return true;
+        }
         if (e & TLAS_BIT) {
-            const uint32_t ni = e & ~TLAS_BIT;
-            if (!hk) {
-                float4 lo, hi;
-                load_node(S.tlas_nodes, ni, &lo, &hi);
-                descend(lo, hi);
-            }
-            const uint32_t a = ha, b = hb;
-            if (COUNT) cnt->tlas++;
-            if (a != 0u) {
-                const uint32_t i1 = a & 0xFFFFu, i2 = (a >> 16) & 0xFFFFu;
-                float4 l1, h1, l2, h2;
-                load_node(S.tlas_nodes, i1, &l1, &h1);
-                load_node(S.tlas_nodes, i2, &l2, &h2);
-                float c1, c2;
-                const bool k1 = slab(l1, h1, w, tmin, tmax, &c1) && c1 < h.t;
-                const bool k2 = slab(l2, h2, w, tmin, tmax, &c2) && c2 < h.t;
-                if (k1 && k2) {
-                    if (c1 > c2) { st.push(TLAS_BIT | i1); cur = TLAS_BIT | i2; descend(l2, h2); }
-                    else { st.push(TLAS_BIT | i2); cur = TLAS_BIT | i1; descend(l1, h1); }
-                    return !st.overflow;
-                }
-                if (k1) { cur = TLAS_BIT | i1; descend(l1, h1); return true; }
-                if (k2) { cur = TLAS_BIT | i2; descend(l2, h2); return true; }
-                return next();
-            }
-            if (COUNT) cnt->entries++;
-            cur_inst = (int32_t)b;
+            // TLAS leaf: enter the instance's BLAS
+            if (COUNT) { cnt->tlas++; cnt->entries++; }
+            cur_inst = (int32_t)(e & ENTRY_PAYLOAD);
             const InstTrav it = S.inst_trav[cur_inst];
             o = object_ray(it, w);
-            if (it.root >= 0) { cur = (uint32_t)it.root; hk = false; return true; }
+            if (it.root_entry != 0xFFFFFFFFu) { cur = it.root_entry; return true; }
             return next();
         }
-        if (!hk) {
+        uint32_t a, b;
+        if (e & LEAF_BIT) {
+            a = e & 0x7FFFFFu;
+            b = (e >> 23) & 0x3Fu;
+        } else {
             float4 lo, hi;
-            load_node(S.blas_nodes, e, &lo, &hi);
-            descend(lo, hi);
-        }
-        const uint32_t a = ha, b = hb;
-        const float bt = fminf_(tmax, h.t);
-        if (b == 0u) {
-            if (COUNT) cnt->interior++;
-            float4 l1, h1, l2, h2;
-            load_node(S.blas_nodes, a, &l1, &h1);
-            load_node(S.blas_nodes, a + 1, &l2, &h2);
-            float c1, c2;
-            const bool k1 = slab(l1, h1, o, tmin, bt, &c1) && c1 < h.t;
-            const bool k2 = slab(l2, h2, o, tmin, bt, &c2) && c2 < h.t;
-            if (k1 && k2) {
-                if (c1 > c2) { st.push(a); cur = a + 1; descend(l2, h2); }
-                else { st.push(a + 1); cur = a; descend(l1, h1); }
-                return !st.overflow;
-            }
-            if (k1) { cur = a; descend(l1, h1); return true; }
-            if (k2) { cur = a + 1; descend(l2, h2); return true; }
-            return next();
+            load_node(S.blas_nodes, e & ENTRY_PAYLOAD, &lo, &hi);
+            a = bits_u(lo.w); b = bits_u(hi.w);
         }
         if (COUNT) cnt->leaf++;
         // TRI_BATCH triangles' loads in flight at a time; tested in slot order
@@ -261,14 +313,13 @@ struct ClosestTrav {
     }
 };
 
-// Any-hit traversal state of one lane (header words as in ClosestTrav).
+// Any-hit traversal state of one lane (entries and blocks as above; no
+// closest-distance bound, nearer child first).
 struct AnyTrav {
     Ray w, o;
     float tmin, tmax;
     bool hit;
     uint32_t cur;
-    uint32_t ha, hb;
-    bool hk;
     LaneStack st;
 
     KD bool init(const SceneView& S, const Ray& ray, float t0, float t1, uint32_t* lds, uint32_t* spill) {
@@ -279,76 +330,84 @@ struct AnyTrav {
         load_node(S.tlas_nodes, 0, &rlo, &rhi);
         float te;
         if (!slab(rlo, rhi, w, tmin, tmax, &te)) return false;
-        cur = TLAS_BIT | 0u;
-        descend(rlo, rhi);
+        cur = tlas_entry(bits_u(rlo.w), bits_u(rhi.w));
         return true;
-    }
-    KD void descend(const float4& lo, const float4& hi) {
-        ha = __builtin_bit_cast(uint32_t, lo.w); hb = __builtin_bit_cast(uint32_t, hi.w); hk = true;
     }
     KD bool next() {
         if (st.sp == 0 || st.overflow) return false;
         cur = st.pop();
-        hk = false;
         return true;
+    }
+    template <bool TL>
+    KD bool children(const float4& l1, const float4& h1, const float4& l2, const float4& h2, uint32_t i1, uint32_t i2,
+                     int* near_k) {
+        const Ray& r = TL ? w : o;
+        float c1, c2;
+        const bool k1 = slab(l1, h1, r, tmin, tmax, &c1);
+        const bool k2 = slab(l2, h2, r, tmin, tmax, &c2);
+        if (k1 && k2) {   // nearer first: an occluder near the origin ends the query sooner
+            if (c1 > c2) {
+                st.push(TL ? tlas_entry(bits_u(l1.w), bits_u(h1.w)) : blas_entry(i1, bits_u(l1.w), bits_u(h1.w)));
+                *near_k = 1;
+            } else {
+                st.push(TL ? tlas_entry(bits_u(l2.w), bits_u(h2.w)) : blas_entry(i2, bits_u(l2.w), bits_u(h2.w)));
+                *near_k = 0;
+            }
+            return true;
+        }
+        if (k1) { *near_k = 0; return true; }
+        if (k2) { *near_k = 1; return true; }
+        return false;
     }
     template <bool COUNT>
     KD bool step(const SceneView& S, TraceCounts* cnt) {
         const uint32_t e = cur;
+        if (!(e & (LEAF_BIT | BIG_BIT))) {
+            const bool tl = (e & TLAS_BIT) != 0;
+            const uint32_t q = e & ENTRY_PAYLOAD;
+            BlockView v;
+            load_block(tl ? S.tlas_blocks : S.blas_blocks, q, &v);
+            int nk;
+            bool any;
+            if (COUNT) { if (tl) cnt->tlas++; else cnt->interior++; }
+            any = tl ? children<true>(v.c0l, v.c0h, v.c1l, v.c1h, 2 * q, 2 * q + 1, &nk)
+                     : children<false>(v.c0l, v.c0h, v.c1l, v.c1h, 2 * q, 2 * q + 1, &nk);
+            if (!any) return next();
+            if (st.overflow) return false;
+            const float4 nl = nk ? v.c1l : v.c0l, nh = nk ? v.c1h : v.c0h;
+            const uint32_t na = bits_u(nl.w), nb = bits_u(nh.w);
+            const bool n_interior = tl ? (na != 0u) : (nb == 0u);
+            if (!n_interior) {
+                cur = tl ? tlas_entry(na, nb) : blas_entry(2 * q + (uint32_t)nk, na, nb);
+                return true;
+            }
+            const float4 al = nk ? v.g2l : v.g0l, ah = nk ? v.g2h : v.g0h;
+            const float4 bl = nk ? v.g3l : v.g1l, bh = nk ? v.g3h : v.g1h;
+            const uint32_t p = na >> 1;
+            if (COUNT) { if (tl) cnt->tlas++; else cnt->interior++; }
+            any = tl ? children<true>(al, ah, bl, bh, 2 * p, 2 * p + 1, &nk)
+                     : children<false>(al, ah, bl, bh, 2 * p, 2 * p + 1, &nk);
+            if (!any) return next();
+            if (st.overflow) return false;
+            const float4 ml = nk ? bl : al, mh = nk ? bh : ah;
+            cur = tl ? tlas_entry(bits_u(ml.w), bits_u(mh.w)) : blas_entry(2 * p + (uint32_t)nk, bits_u(ml.w), bits_u(mh.w));
+            return true;
+        }
         if (e & TLAS_BIT) {
-            const uint32_t ni = e & ~TLAS_BIT;
-            if (!hk) {
-                float4 lo, hi;
-                load_node(S.tlas_nodes, ni, &lo, &hi);
-                descend(lo, hi);
-            }
-            const uint32_t a = ha, b = hb;
-            if (COUNT) cnt->tlas++;
-            if (a != 0u) {
-                const uint32_t i1 = a & 0xFFFFu, i2 = (a >> 16) & 0xFFFFu;
-                float4 l1, h1, l2, h2;
-                load_node(S.tlas_nodes, i1, &l1, &h1);
-                load_node(S.tlas_nodes, i2, &l2, &h2);
-                float c1, c2;
-                const bool k1 = slab(l1, h1, w, tmin, tmax, &c1);
-                const bool k2 = slab(l2, h2, w, tmin, tmax, &c2);
-                if (k1 && k2) {   // nearer first: an occluder near the origin ends the query sooner
-                    if (c1 > c2) { st.push(TLAS_BIT | i1); cur = TLAS_BIT | i2; descend(l2, h2); }
-                    else { st.push(TLAS_BIT | i2); cur = TLAS_BIT | i1; descend(l1, h1); }
-                    return !st.overflow;
-                }
-                if (k1) { cur = TLAS_BIT | i1; descend(l1, h1); return true; }
-                if (k2) { cur = TLAS_BIT | i2; descend(l2, h2); return true; }
-                return next();
-            }
-            if (COUNT) cnt->entries++;
-            const InstTrav it = S.inst_trav[b];
+            if (COUNT) { cnt->tlas++; cnt->entries++; }
+            const InstTrav it = S.inst_trav[e & ENTRY_PAYLOAD];
             o = object_ray(it, w);
-            if (it.root >= 0) { cur = (uint32_t)it.root; hk = false; return true; }
+            if (it.root_entry != 0xFFFFFFFFu) { cur = it.root_entry; return true; }
             return next();
         }
-        if (!hk) {
+        uint32_t a, b;
+        if (e & LEAF_BIT) {
+            a = e & 0x7FFFFFu;
+            b = (e >> 23) & 0x3Fu;
+        } else {
             float4 lo, hi;
-            load_node(S.blas_nodes, e, &lo, &hi);
-            descend(lo, hi);
-        }
-        const uint32_t a = ha, b = hb;
-        if (b == 0u) {
-            if (COUNT) cnt->interior++;
-            float4 l1, h1, l2, h2;
-            load_node(S.blas_nodes, a, &l1, &h1);
-            load_node(S.blas_nodes, a + 1, &l2, &h2);
-            float c1, c2;
-            const bool k1 = slab(l1, h1, o, tmin, tmax, &c1);
-            const bool k2 = slab(l2, h2, o, tmin, tmax, &c2);
-            if (k1 && k2) {
-                if (c1 > c2) { st.push(a); cur = a + 1; descend(l2, h2); }
-                else { st.push(a + 1); cur = a; descend(l1, h1); }
-                return !st.overflow;
-            }
-            if (k1) { cur = a; descend(l1, h1); return true; }
-            if (k2) { cur = a + 1; descend(l2, h2); return true; }
-            return next();
+            load_node(S.blas_nodes, e & ENTRY_PAYLOAD, &lo, &hi);
+            a = bits_u(lo.w); b = bits_u(hi.w);
         }
         if (COUNT) cnt->leaf++;
         for (uint32_t k = 0; k < b; k += TRI_BATCH) {
