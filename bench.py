@@ -91,9 +91,24 @@ def cpu_baseline(scene, st, cam, W, H, budget_s):
         if spent >= budget_s * 0.5:
             break
         n = int(min(1 << 20, n * max(2.0, min(8.0, (budget_s * 0.5 - spent) / max(dt, 1e-3)))))
+    # the same on one core, over a smaller sample
+    n1 = max(2048, int(total_samples / spent / cores * budget_s * 0.1))
+    pix = rng.choice(W * H, size=min(n1, W * H), replace=False).astype(np.int32)
+    t0 = time.time()
+    orc.render_pixels(st, cam, W, H, pix, 0, 1, threads=1)
+    one = len(pix) / (time.time() - t0) / 1e6
+    log(f"[cpu] 1 core: {len(pix)} pixel-samples, {one:.3f} Msamples/s")
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": total_samples / spent / 1e6, "unit": "Msamples/s", "cores": cores, "kind": "port",
             "sample": f"{total_samples} random pixels of the 1920x1080 frame, 1 sample each, depth "
-                      f"{st.max_depth}, same scene/camera/seeds, oracle C++ (IEEE float) on {cores} threads"}
+                      f"{st.max_depth}, same scene/camera/seeds, oracle C++ (IEEE float) on {cores} threads",
+            "single_core_value": round(one, 4), "single_core_sample": f"{len(pix)} random pixels, 1 sample each",
+            "cpu_model": model}
 
 
 def fp64_reference(scene, st, cam, W, H, pixels, spp):
@@ -272,6 +287,7 @@ def main():
     ap.add_argument("--views", type=int, default=64, help="cameras of the batched multi-view leg (0: skip)")
     ap.add_argument("--full4k", type=int, default=12, help="frames of the animated 4K full-frame leg (0: skip)")
     ap.add_argument("--no-graph-leg", action="store_true", help="skip the hipGraph replay leg")
+    ap.add_argument("--no-megakernel-leg", action="store_true", help="skip the megakernel comparison leg")
     ap.add_argument("--ttt-spp", type=int, default=256,
                     help="time-to-target run: the plain tracer's RMSE at this many spp is the target (0: skip)")
     ap.add_argument("--ttt-pixels", type=int, default=2048)
@@ -410,6 +426,24 @@ def main():
                      "note": "same workload, each 1-spp frame captured as one hipGraph and replayed"}
         log(f"[bench] graph replay: {graph_rec}")
 
+    # the plain megakernel (one thread runs a whole path, the reference's
+    # structure) on the same workload, for comparison
+    mega = None
+    if integ == abi.INTEGRATOR_WAVEFRONT and not args.no_megakernel_leg:
+        ctx.set_settings(make_settings(args.depth, abi.INTEGRATOR_MEGAKERNEL))
+        ctx.render(args.spp)
+        ctx.reset()
+        msteps = max(1, min(args.steps, 2))
+        t0 = time.perf_counter()
+        for i in range(msteps):
+            ctx.render(args.spp)
+        mwall = time.perf_counter() - t0
+        ctx.set_settings(st)
+        mega = {"value": round(W * H * args.spp * msteps / mwall / 1e6, 3),
+                "ms_per_step": round(mwall * 1e3 / msteps, 3), "steps": msteps,
+                "note": "same workload, megakernel integrator (one lane per path)"}
+        log(f"[bench] megakernel: {mega}")
+
     views = None
     if integ == abi.INTEGRATOR_WAVEFRONT and args.views > 0:
         views = batched_views(ctx, scene, args.views, 4)
@@ -467,7 +501,7 @@ def main():
                    "max_depth": args.depth, "triangles": int(len(scene.positions)),
                    "instances": int(scene.num_instances), "parallelism": "single GPU", "tuning": tuning},
         "roofline": roof, "cpu_baseline": cpu, "time_to_target": ttt, "batched_views": views,
-        "graph_replay": graph_rec, "full_frame_4k": full4k,
+        "graph_replay": graph_rec, "megakernel": mega, "full_frame_4k": full4k,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -213,6 +213,30 @@ public:
 
     // extensions of this library
     kpt_ctx* context() const { return m_ctx; }
+
+    // Batched cameras (no reference counterpart): n cameras of any
+    // resolutions rendered as one set of paths, `spp` samples each; returns
+    // every camera's per-pixel estimate (RGBA32F), cameras back to back.  The
+    // renderer's own frame, accumulation and post chain are untouched.
+    struct CameraDesc {
+        mat4 projection;
+        mat4 view;
+        uint32_t width = 0, height = 0;
+    };
+    std::vector<float> executeBatch(const CameraDesc* cams, size_t n, int spp = 1) {
+        std::vector<kpt_view> v(n);
+        size_t total = 0;
+        for (size_t k = 0; k < n; ++k) {
+            std::memcpy(v[k].projection, cams[k].projection.data(), 16 * sizeof(float));
+            std::memcpy(v[k].view, cams[k].view.data(), 16 * sizeof(float));
+            v[k].width = (int32_t)cams[k].width;
+            v[k].height = (int32_t)cams[k].height;
+            total += (size_t)cams[k].width * cams[k].height;
+        }
+        std::vector<float> out(total * 4);
+        check(kpt_render_views_host(m_ctx, v.data(), (int32_t)n, spp, out.data()), "kpt_render_views_host");
+        return out;
+    }
     void setSeedMode(int mode) { m_settings.seed_mode = mode; push(); }
 
 private:

@@ -341,6 +341,9 @@ int kpt_set_frame_delta(kpt_ctx* ctx, float delta_time_ms);
  * static-accumulation estimate after spp samples.  No G-buffer, post chain
  * or cache; the context's frame buffers are untouched. */
 int kpt_render_views(kpt_ctx* ctx, const kpt_view* views, int32_t n, int32_t spp, float* out);
+/* The same into HOST memory (sum of width*height RGBA32F; a temporary
+ * device buffer is allocated and freed per call). */
+int kpt_render_views_host(kpt_ctx* ctx, const kpt_view* views, int32_t n, int32_t spp, float* host_out);
 
 /* ---- radiance cache (no reference counterpart) ---- */
 /* Applies the settings; the first call (and any call with a new seed)

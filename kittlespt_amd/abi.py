@@ -247,6 +247,7 @@ SIGNATURES = [
     ("kpt_get_film", C.c_int, [_P, C.POINTER(KptFilmState)]),
     ("kpt_set_frame_delta", C.c_int, [_P, C.c_float]),
     ("kpt_render_views", C.c_int, [_P, C.POINTER(KptView), C.c_int32, C.c_int32, _P]),
+    ("kpt_render_views_host", C.c_int, [_P, C.POINTER(KptView), C.c_int32, C.c_int32, C.POINTER(C.c_float)]),
     ("kpt_cache_configure", C.c_int, [_P, C.POINTER(KptCacheSettings)]),
     ("kpt_cache_get_settings", C.c_int, [_P, C.POINTER(KptCacheSettings)]),
     ("kpt_cache_get_weights", C.c_int, [_P, C.POINTER(C.c_float), C.c_int64]),

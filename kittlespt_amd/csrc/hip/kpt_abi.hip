@@ -993,6 +993,25 @@ int kpt_render_views(kpt_ctx* ctx, const kpt_view* views, int32_t n, int32_t spp
     return KPT_OK;
 }
 
+int kpt_render_views_host(kpt_ctx* ctx, const kpt_view* views, int32_t n, int32_t spp, float* host_out) {
+    if (!ctx || !views || n <= 0 || !host_out) return KPT_ERR_INVALID_ARG;
+    if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
+    int64_t total = 0;
+    for (int32_t v = 0; v < n; ++v) {
+        if (views[v].width <= 0 || views[v].height <= 0) return fail(ctx, KPT_ERR_INVALID_ARG, "view with an empty resolution");
+        total += (int64_t)views[v].width * views[v].height;
+    }
+    float* d = nullptr;
+    HIPCHK(hipMalloc((void**)&d, (size_t)total * 16));
+    int rc = kpt_render_views(ctx, views, n, spp, d);
+    if (rc == KPT_OK) {
+        hipError_t e = hipMemcpy(host_out, d, (size_t)total * 16, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(ctx, KPT_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(d);
+    return rc;
+}
+
 int kpt_set_camera(kpt_ctx* ctx, const kpt_camera* cam) {
     if (!ctx || !cam) return KPT_ERR_INVALID_ARG;
     ctx->cam = *cam;

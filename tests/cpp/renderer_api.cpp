@@ -14,7 +14,8 @@
 //       f32 materials[NM*18] (MaterialSceneEntity fields in declaration order,
 //       texture ids stored as floats), u32 tex[NT*3] (w,h,channels),
 //       u64 pixel_bytes, u8 pixels, f32 proj[16] view[16]
-// out: W*H*4 f32 render estimate, then W*H*4 f32 V-buffer.
+// out: W*H*4 f32 render estimate, then W*H*4 f32 V-buffer, then the
+//      executeBatch estimates of the same view at W x H and W/2 x H/2.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -139,6 +140,13 @@ int main(int argc, char** argv) {
         if (!o) return 2;
         fwrite(img.data(), sizeof(float), img.size(), o);
         fwrite(vb.data(), sizeof(float), vb.size(), o);
+        // the batched-camera extension: this view at W x H and at W/2 x H/2
+        Renderer::CameraDesc cams[2];
+        cams[0].projection = P; cams[0].view = V; cams[0].width = (uint32_t)W; cams[0].height = (uint32_t)H;
+        cams[1].projection = P; cams[1].view = V; cams[1].width = (uint32_t)(W / 2); cams[1].height = (uint32_t)(H / 2);
+        std::vector<float> batch = r.executeBatch(cams, 2, spp);
+        CHECK(batch.size() == (size_t)(W * H + (W / 2) * (H / 2)) * 4);
+        fwrite(batch.data(), sizeof(float), batch.size(), o);
         fclose(o);
         bool threw = false;
         try {

@@ -83,7 +83,9 @@ def test_cpp_renderer_matches_binding(tmp_path, which):
     print(f"[cpp api] {which} {W}x{H} spp {spp}", flush=True)
     r = subprocess.run([_exe(), dump, out], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    got = np.fromfile(out, np.float32).reshape(2, H, W, 4)
+    raw = np.fromfile(out, np.float32)
+    got = raw[:2 * H * W * 4].reshape(2, H, W, 4)
+    batch = raw[2 * H * W * 4:]
 
     ctx = Context(0)
     try:
@@ -95,7 +97,10 @@ def test_cpp_renderer_matches_binding(tmp_path, which):
         ctx.set_frame_delta(16.0)
         ctx.render(spp)
         img, vb = ctx.readback(4), ctx.readback(3)
+        views = ctx.render_views([(p, v, W, H), (p, v, W // 2, H // 2)], spp=spp)
     finally:
         ctx.close()
     assert np.array_equal(got[0].view(np.uint32), img.view(np.uint32))
     assert np.array_equal(got[1].view(np.uint32), vb.view(np.uint32))
+    want = np.concatenate([x.reshape(-1) for x in views])
+    assert batch.shape == want.shape and np.array_equal(batch.view(np.uint32), want.view(np.uint32))
