@@ -307,7 +307,7 @@ int kpt_set_trace_counting(kpt_ctx* ctx, int32_t enable);
  * rays by direction octant and Morton cell of the origin before tracing
  * (0/1; not applied inside captured graphs), [7] (when n >= 8) the bounce
  * from which the paths still alive finish in one persistent launch, a lane
- * running its whole remaining path (0: never; 0..64; frames in which the
+ * running its whole remaining path (0: never; 0..64; default 2; frames in which the
  * radiance cache renders or trains keep the stage kernels).  n >= 5. */
 int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n);
 int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n);
