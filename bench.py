@@ -113,7 +113,7 @@ def cpu_baseline(scene, st, cam, W, H, budget_s):
 
 def fp64_reference(scene, st, cam, W, H, pixels, spp):
     """Converged FP64 render of a pixel subset on the host cores (the oracle's
-    double instantiation): the judge of the RMSE, never the thing measured."""
+    double instantiation): the yardstick of the RMSE, never the thing measured."""
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     orc = oracle_scene(scene)
     t0 = time.time()
@@ -313,7 +313,7 @@ def main():
         return
 
     if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} is not supported: this build measures one GPU "
+        print(f"bench.py: --gpus {args.gpus} is not supported: this benchmark measures one GPU "
               f"(multi-GPU sharding is a design note, not built)", file=sys.stderr)
         sys.exit(2)
 

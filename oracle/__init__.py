@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/_build/libkpt_oracle.so).
 
 TEST INFRASTRUCTURE ONLY.  Imported by tests/, by __graft_entry__.smoke() and by
-bench.py's cpu_baseline leg, always as the checker; the product never imports
+bench.py's cpu_baseline leg, always as the reference results are compared with; the product never imports
 it.  Build with ``make -C oracle`` (or ``oracle.build()``).
 """
 from __future__ import annotations

@@ -1,6 +1,6 @@
 // kpt_oracle.cpp — CPU oracle: a restatement of the KittlesPT reference's hot
 // path in IEEE semantics.  TEST INFRASTRUCTURE ONLY: tests/, the smoke check
-// and bench.py's cpu_baseline leg call it as the checker; the product library
+// and bench.py's cpu_baseline leg call it as the reference to compare with; the product library
 // never links or loads it.
 //
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off -fno-fast-math).

@@ -3,7 +3,7 @@
  * The oracle restates the reference's host builders and device routines as
  * plain single-threaded-per-pixel C++ in IEEE semantics.  Only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, as the
- * checker; the product never links it.
+ * reference results are compared with; the product never links it.
  */
 #ifndef KPT_ORACLE_H
 #define KPT_ORACLE_H

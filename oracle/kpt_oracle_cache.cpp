@@ -1,6 +1,6 @@
 // kpt_oracle_cache.cpp — CPU oracle of the radiance-cache network: input
 // encoding, forward, relative-L2 loss, backward and the Adam step.
-// TEST INFRASTRUCTURE ONLY: tests/ call it as the checker of the GPU kernels
+// TEST INFRASTRUCTURE ONLY: tests/ call it as the reference the GPU kernels are compared with
 // (kittlespt_amd/csrc/hip/kpt_cache.hip); the product never links it.
 //
 // The reference renderer has no radiance cache; the definition followed here

@@ -1,7 +1,7 @@
 // kpt_oracle_post.cpp — CPU oracle of the frame's post chain: albedo
 // modulation, log-luminance histogram and auto exposure, bloom mip chain,
 // exposure in Yxy and the AgX film transform.  TEST INFRASTRUCTURE ONLY: tests/
-// call it as the checker; the product never links it.
+// call it as the reference to compare with; the product never links it.
 //
 // Scalar float code in IEEE semantics (built with -ffp-contract=off), one
 // expression per reference expression in the same order, transcendentals from
