@@ -482,6 +482,11 @@ def main():
                "plain_s": round(t_plain, 4), "cached_s": None if t_cached is None else round(t_cached, 4),
                "speedup": None if t_cached is None else round(t_plain / t_cached, 3),
                "cached_rmse_at_equal_spp": round(cached[-1][2], 5),
+               # throughput of each tracer over the accumulated frames (HIP-event frame times)
+               "plain_msamples_per_s": round(W * H * (plain[-1][1] - plain[0][1]) / max(plain[-1][0] - plain[0][0], 1e-9)
+                                             / 1e6, 3),
+               "cached_msamples_per_s": round(W * H * (cached[-1][1] - cached[0][1])
+                                              / max(cached[-1][0] - cached[0][0], 1e-9) / 1e6, 3),
                "cache_warmup_frames": args.cache_warmup, "cache_nee_at_vertex": args.cache_nee,
                "cache_wide_roughness": args.cache_wide,
                "reference": f"FP64 oracle render of {args.ttt_pixels} random pixels x {args.ref_spp} spp "
