@@ -213,22 +213,21 @@ struct ClosestTrav {
     // index i2) of the node being processed: TLAS (level bit set) or BLAS.
     // Returns 0 = nothing hit (pop), 1 = the nearer (or only) child is in
     // *near_* (after the farther one was pushed).
-    template <bool TL>
-    KD bool children(const float4& l1, const float4& h1, const float4& l2, const float4& h2, uint32_t i1, uint32_t i2,
-                     int* near_k) {
-        const Ray& r = TL ? w : o;
-        const float bt = TL ? tmax : fminf_(tmax, h.t);
+    // One code path for both levels (the level is a per-lane select), so a
+    // wave whose lanes sit at different levels does not run it twice.
+    KD bool children(bool tl, const float4& l1, const float4& h1, const float4& l2, const float4& h2, uint32_t i1,
+                     uint32_t i2, int* near_k) {
+        Ray r;
+        r.o = tl ? w.o : o.o; r.d = tl ? w.d : o.d; r.inv = tl ? w.inv : o.inv;
+        const float bt = tl ? tmax : fminf_(tmax, h.t);
         float c1, c2;
         const bool k1 = slab(l1, h1, r, tmin, bt, &c1) && c1 < h.t;
         const bool k2 = slab(l2, h2, r, tmin, bt, &c2) && c2 < h.t;
         if (k1 && k2) {
-            if (c1 > c2) {
-                st.push(TL ? tlas_entry(bits_u(l1.w), bits_u(h1.w)) : blas_entry(i1, bits_u(l1.w), bits_u(h1.w)));
-                *near_k = 1;
-            } else {
-                st.push(TL ? tlas_entry(bits_u(l2.w), bits_u(h2.w)) : blas_entry(i2, bits_u(l2.w), bits_u(h2.w)));
-                *near_k = 0;
-            }
+            const bool far1 = c1 > c2;
+            const uint32_t fa = bits_u(far1 ? l1.w : l2.w), fb = bits_u(far1 ? h1.w : h2.w);
+            st.push(tl ? tlas_entry(fa, fb) : blas_entry(far1 ? i1 : i2, fa, fb));
+            *near_k = far1 ? 1 : 0;
             return true;
         }
         if (k1) { *near_k = 0; return true; }
@@ -249,8 +248,7 @@ struct ClosestTrav {
             int nk;
             bool any;
             if (COUNT) { if (tl) cnt->tlas++; else cnt->interior++; }
-            any = tl ? children<true>(v.c0l, v.c0h, v.c1l, v.c1h, 2 * q, 2 * q + 1, &nk)
-                     : children<false>(v.c0l, v.c0h, v.c1l, v.c1h, 2 * q, 2 * q + 1, &nk);
+            any = children(tl, v.c0l, v.c0h, v.c1l, v.c1h, 2 * q, 2 * q + 1, &nk);
             if (!any) return next();
             if (st.overflow) return false;
             const float4 nl = nk ? v.c1l : v.c0l, nh = nk ? v.c1h : v.c0h;
@@ -265,8 +263,7 @@ struct ClosestTrav {
             const float4 bl = nk ? v.g3l : v.g1l, bh = nk ? v.g3h : v.g1h;
             const uint32_t p = na >> 1;   // the nearer child's children pair
             if (COUNT) { if (tl) cnt->tlas++; else cnt->interior++; }
-            any = tl ? children<true>(al, ah, bl, bh, 2 * p, 2 * p + 1, &nk)
-                     : children<false>(al, ah, bl, bh, 2 * p, 2 * p + 1, &nk);
+            any = children(tl, al, ah, bl, bh, 2 * p, 2 * p + 1, &nk);
             if (!any) return next();
             if (st.overflow) return false;
             const float4 ml = nk ? bl : al, mh = nk ? bh : ah;
@@ -338,21 +335,18 @@ struct AnyTrav {
         cur = st.pop();
         return true;
     }
-    template <bool TL>
-    KD bool children(const float4& l1, const float4& h1, const float4& l2, const float4& h2, uint32_t i1, uint32_t i2,
-                     int* near_k) {
-        const Ray& r = TL ? w : o;
+    KD bool children(bool tl, const float4& l1, const float4& h1, const float4& l2, const float4& h2, uint32_t i1,
+                     uint32_t i2, int* near_k) {
+        Ray r;
+        r.o = tl ? w.o : o.o; r.d = tl ? w.d : o.d; r.inv = tl ? w.inv : o.inv;
         float c1, c2;
         const bool k1 = slab(l1, h1, r, tmin, tmax, &c1);
         const bool k2 = slab(l2, h2, r, tmin, tmax, &c2);
         if (k1 && k2) {   // nearer first: an occluder near the origin ends the query sooner
-            if (c1 > c2) {
-                st.push(TL ? tlas_entry(bits_u(l1.w), bits_u(h1.w)) : blas_entry(i1, bits_u(l1.w), bits_u(h1.w)));
-                *near_k = 1;
-            } else {
-                st.push(TL ? tlas_entry(bits_u(l2.w), bits_u(h2.w)) : blas_entry(i2, bits_u(l2.w), bits_u(h2.w)));
-                *near_k = 0;
-            }
+            const bool far1 = c1 > c2;
+            const uint32_t fa = bits_u(far1 ? l1.w : l2.w), fb = bits_u(far1 ? h1.w : h2.w);
+            st.push(tl ? tlas_entry(fa, fb) : blas_entry(far1 ? i1 : i2, fa, fb));
+            *near_k = far1 ? 1 : 0;
             return true;
         }
         if (k1) { *near_k = 0; return true; }
@@ -370,8 +364,7 @@ struct AnyTrav {
             int nk;
             bool any;
             if (COUNT) { if (tl) cnt->tlas++; else cnt->interior++; }
-            any = tl ? children<true>(v.c0l, v.c0h, v.c1l, v.c1h, 2 * q, 2 * q + 1, &nk)
-                     : children<false>(v.c0l, v.c0h, v.c1l, v.c1h, 2 * q, 2 * q + 1, &nk);
+            any = children(tl, v.c0l, v.c0h, v.c1l, v.c1h, 2 * q, 2 * q + 1, &nk);
             if (!any) return next();
             if (st.overflow) return false;
             const float4 nl = nk ? v.c1l : v.c0l, nh = nk ? v.c1h : v.c0h;
@@ -385,8 +378,7 @@ struct AnyTrav {
             const float4 bl = nk ? v.g3l : v.g1l, bh = nk ? v.g3h : v.g1h;
             const uint32_t p = na >> 1;
             if (COUNT) { if (tl) cnt->tlas++; else cnt->interior++; }
-            any = tl ? children<true>(al, ah, bl, bh, 2 * p, 2 * p + 1, &nk)
-                     : children<false>(al, ah, bl, bh, 2 * p, 2 * p + 1, &nk);
+            any = children(tl, al, ah, bl, bh, 2 * p, 2 * p + 1, &nk);
             if (!any) return next();
             if (st.overflow) return false;
             const float4 ml = nk ? bl : al, mh = nk ? bh : ah;
