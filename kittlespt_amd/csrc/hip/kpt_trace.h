@@ -184,6 +184,7 @@ struct ClosestTrav {
     Hit h;
     int32_t cur_inst;
     uint32_t cur;
+    uint32_t la, lb;   // leaf in progress: next triangle slot, triangles left (0: none)
     LaneStack st;
 
     // TLAS::intersect entry: empty TLAS or a miss of the root box ends the
@@ -192,6 +193,7 @@ struct ClosestTrav {
         w = ray; o = ray; tmin = t0; tmax = t1;
         h.t = __builtin_inff(); h.slot = -1; h.inst = -1; h.u = 0.0f; h.v = 0.0f;
         cur_inst = -1;
+        la = 0u; lb = 0u;
         st.lds = (lds_u32*)lds; st.spill = spill; st.sp = 0; st.overflow = false;
         if (S.tlas_root < 0) return false;
         float4 rlo, rhi;
@@ -235,9 +237,13 @@ struct ClosestTrav {
         return false;
     }
 
-    // Processes the current entry; returns false when the query is finished.
+    // Processes the current entry — an interior step, an instance entry, or
+    // one batch of TRI_BATCH triangles of a leaf (a leaf spans several steps,
+    // so a wave's lanes at interior nodes are not held up by another lane's
+    // whole leaf); returns false when the query is finished.
     template <bool COUNT>
     KD bool step(const SceneView& S, TraceCounts* cnt) {
+      if (lb == 0u) {
         const uint32_t e = cur;
         if (!(e & (LEAF_BIT | BIG_BIT))) {
             // interior node (TLAS or BLAS): its children pair's block
@@ -279,33 +285,34 @@ struct ClosestTrav {
             if (it.root_entry != 0xFFFFFFFFu) { cur = it.root_entry; return true; }
             return next();
         }
-        uint32_t a, b;
         if (e & LEAF_BIT) {
-            a = e & 0x7FFFFFu;
-            b = (e >> 23) & 0x3Fu;
+            la = e & 0x7FFFFFu;
+            lb = (e >> 23) & 0x3Fu;
         } else {
             float4 lo, hi;
             load_node(S.blas_nodes, e & ENTRY_PAYLOAD, &lo, &hi);
-            a = bits_u(lo.w); b = bits_u(hi.w);
+            la = bits_u(lo.w); lb = bits_u(hi.w);
         }
         if (COUNT) cnt->leaf++;
-        // TRI_BATCH triangles' loads in flight at a time; tested in slot order
-        for (uint32_t k = 0; k < b; k += TRI_BATCH) {
-            TriPos tp[TRI_BATCH];
+      }
+        // the next TRI_BATCH triangles of the leaf, loads in flight together,
+        // tested in slot order
+        TriPos tp[TRI_BATCH];
 #pragma unroll
-            for (int j = 0; j < TRI_BATCH; ++j)
-                if (k + j < b) tp[j] = S.tris[a + k + j];
+        for (int j = 0; j < TRI_BATCH; ++j)
+            if ((uint32_t)j < lb) tp[j] = S.tris[la + j];
 #pragma unroll
-            for (int j = 0; j < TRI_BATCH; ++j) {
-                if (k + j < b) {
-                    if (COUNT) cnt->tri++;
-                    float t, u, v;
-                    if (tri_test(tp[j], o, tmin, fminf_(tmax, h.t), &t, &u, &v) && t < h.t) {
-                        h.t = t; h.slot = (int32_t)(a + k + j); h.inst = cur_inst; h.u = u; h.v = v;
-                    }
+        for (int j = 0; j < TRI_BATCH; ++j) {
+            if ((uint32_t)j < lb) {
+                if (COUNT) cnt->tri++;
+                float t, u, v;
+                if (tri_test(tp[j], o, tmin, fminf_(tmax, h.t), &t, &u, &v) && t < h.t) {
+                    h.t = t; h.slot = (int32_t)(la + j); h.inst = cur_inst; h.u = u; h.v = v;
                 }
             }
         }
+        if (lb > (uint32_t)TRI_BATCH) { la += TRI_BATCH; lb -= TRI_BATCH; return true; }
+        lb = 0u;
         return next();
     }
 };
@@ -317,10 +324,12 @@ struct AnyTrav {
     float tmin, tmax;
     bool hit;
     uint32_t cur;
+    uint32_t la, lb;
     LaneStack st;
 
     KD bool init(const SceneView& S, const Ray& ray, float t0, float t1, uint32_t* lds, uint32_t* spill) {
         w = ray; o = ray; tmin = t0; tmax = t1; hit = false;
+        la = 0u; lb = 0u;
         st.lds = (lds_u32*)lds; st.spill = spill; st.sp = 0; st.overflow = false;
         if (S.tlas_root < 0) return false;
         float4 rlo, rhi;
@@ -355,6 +364,7 @@ struct AnyTrav {
     }
     template <bool COUNT>
     KD bool step(const SceneView& S, TraceCounts* cnt) {
+      if (lb == 0u) {
         const uint32_t e = cur;
         if (!(e & (LEAF_BIT | BIG_BIT))) {
             const bool tl = (e & TLAS_BIT) != 0;
@@ -392,30 +402,30 @@ struct AnyTrav {
             if (it.root_entry != 0xFFFFFFFFu) { cur = it.root_entry; return true; }
             return next();
         }
-        uint32_t a, b;
         if (e & LEAF_BIT) {
-            a = e & 0x7FFFFFu;
-            b = (e >> 23) & 0x3Fu;
+            la = e & 0x7FFFFFu;
+            lb = (e >> 23) & 0x3Fu;
         } else {
             float4 lo, hi;
             load_node(S.blas_nodes, e & ENTRY_PAYLOAD, &lo, &hi);
-            a = bits_u(lo.w); b = bits_u(hi.w);
+            la = bits_u(lo.w); lb = bits_u(hi.w);
         }
         if (COUNT) cnt->leaf++;
-        for (uint32_t k = 0; k < b; k += TRI_BATCH) {
-            TriPos tp[TRI_BATCH];
+      }
+        TriPos tp[TRI_BATCH];
 #pragma unroll
-            for (int j = 0; j < TRI_BATCH; ++j)
-                if (k + j < b) tp[j] = S.tris[a + k + j];
+        for (int j = 0; j < TRI_BATCH; ++j)
+            if ((uint32_t)j < lb) tp[j] = S.tris[la + j];
 #pragma unroll
-            for (int j = 0; j < TRI_BATCH; ++j) {
-                if (k + j < b) {
-                    if (COUNT) cnt->tri++;
-                    float t, u, v;
-                    if (tri_test(tp[j], o, tmin, tmax, &t, &u, &v)) { hit = true; return false; }
-                }
+        for (int j = 0; j < TRI_BATCH; ++j) {
+            if ((uint32_t)j < lb) {
+                if (COUNT) cnt->tri++;
+                float t, u, v;
+                if (tri_test(tp[j], o, tmin, tmax, &t, &u, &v)) { hit = true; lb = 0u; return false; }
             }
         }
+        if (lb > (uint32_t)TRI_BATCH) { la += TRI_BATCH; lb -= TRI_BATCH; return true; }
+        lb = 0u;
         return next();
     }
 };
