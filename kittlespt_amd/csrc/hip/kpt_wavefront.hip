@@ -21,6 +21,20 @@ namespace kpt {
 namespace {
 
 constexpr int WF_BLOCK = 256;
+// Occupancy targets in waves per SIMD (512 VGPRs per lane per SIMD): the
+// traversal kernels at 5 (at most 96 VGPRs; they need 100 unconstrained and
+// fit without spilling), the path-per-lane tail at 4 (128 VGPRs, from 166).
+// Both measured faster than the unconstrained allocation on the instanced
+// 1080p workload.
+#ifndef KPT_WPE_TRACE
+#define KPT_WPE_TRACE __attribute__((amdgpu_waves_per_eu(5)))
+#endif
+#ifndef KPT_WPE_TAIL
+#define KPT_WPE_TAIL __attribute__((amdgpu_waves_per_eu(4)))
+#endif
+#ifndef KPT_WPE_SHADE
+#define KPT_WPE_SHADE
+#endif
 constexpr int MAX_KEYS = 64;
 // counters[] slots (zeroed once per frame by a fill kernel)
 constexpr int C_QCOUNT = 0;      // + bounce parity: [0],[1] live-queue sizes
@@ -201,7 +215,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shadow_keys(FrameConst fc, Wavefro
 
 // Closest hit of every live path.  COUNT accumulates traversal events.
 template <bool COUNT>
-__global__ __launch_bounds__(WF_BLOCK) void k_trace(SceneView S, WavefrontState W, int parity, int pull_slot,
+__global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TRACE void k_trace(SceneView S, WavefrontState W, int parity, int pull_slot,
                                                     int32_t* overflow, unsigned long long* counters,
                                                     TraceTuning tune, const int32_t* sorted) {
     __shared__ uint32_t s_stack[LDS_STACK][TRACE_BLOCK];
@@ -326,7 +340,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_sort_scatter(SceneView S, Wavefron
 }
 
 // One bounce of Li for every live path (integrator.cuh:266-398).
-__global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, WavefrontState W, int parity,
+__global__ __launch_bounds__(WF_BLOCK) KPT_WPE_SHADE void k_shade(SceneView S, FrameConst fc, WavefrontState W, int parity,
                                                     int pull_slot, const int32_t* q, CacheBuffers cb, CacheFrame cf) {
     const int count = W.counters[C_QCOUNT + parity];
     int32_t* nq = W.queue[parity ^ 1];
@@ -486,7 +500,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shade(SceneView S, FrameConst fc, 
 // several dependent trace/shade/shadow rounds, and per-bounce launches would
 // be bound by the latency of their few rays.  Same arithmetic and random
 // numbers as the stage kernels, so the radiance is unchanged.
-__global__ __launch_bounds__(WF_BLOCK) void k_tail(SceneView S, FrameConst fc, WavefrontState W, int parity,
+__global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TAIL void k_tail(SceneView S, FrameConst fc, WavefrontState W, int parity,
                                                    int pull_slot, int32_t* overflow) {
     __shared__ uint32_t s_stack[LDS_STACK][WF_BLOCK];
     const int count = W.counters[C_QCOUNT + parity];
@@ -531,7 +545,7 @@ __device__ __forceinline__ void shadow_resolve(WavefrontState& W, float4* t_lbef
     if (rec >= 0) t_lbefore[rec] = make_float4(nl.x, nl.y, nl.z, 0.0f);
 }
 
-__global__ __launch_bounds__(WF_BLOCK) void k_shadow(SceneView S, WavefrontState W, int pull_slot, int32_t* overflow,
+__global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TRACE void k_shadow(SceneView S, WavefrontState W, int pull_slot, int32_t* overflow,
                                                      TraceTuning tune, const int32_t* order, float4* t_lbefore) {
     __shared__ uint32_t s_stack[LDS_STACK][TRACE_BLOCK];
     uint32_t spill[SPILL_STACK];
