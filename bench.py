@@ -39,6 +39,13 @@ HBM_PEAK_GBS = 8000.0
 # (BVH2Node 32 B, Triangle positions 36 B + 4 B index, BLAS2 matrix + ids 72 B)
 BYTES_PER_EVENT = {"blas_interior_pops": 64, "blas_leaf_pops": 32, "tri_tests": 40, "blas_entries": 72,
                    "tlas_pops": 32}
+# path-per-lane tail, per path it finishes: its queue entry (4 B) and state read
+# (ray origin, direction, throughput, radiance, previous vertex: 5 x 16 B),
+# its radiance written (16 B); shading reads (attributes, textures) not counted
+TAIL_BYTES_PER_PATH = 100
+STAGE_KERNELS = {"trace": "k_trace (wavefront closest hit)",
+                 "tail": "k_tail (path-per-lane tail: closest and any-hit traversal, shading)"}
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "wavefront_instanced1080p_traffic.json")
 
 
 def log(msg):
@@ -123,23 +130,26 @@ def fp64_reference(scene, st, cam, W, H, pixels, spp):
     return ref.astype(np.float64)
 
 
-def measured_traffic(W, H, instances, depth, spp, tail):
-    """HBM bytes per k_trace launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE passes of this workload (scaled by the copy-kernel
-    calibration), or (None, None) for another workload or another set of
-    k_trace launches (the `tail` knob decides which bounces they cover)."""
-    path = os.path.join(ROOT, "profiles", "k_trace_instanced1080p_traffic.json")
+def measured_traffic(stage, W, H, instances, depth, spp, tail):
+    """HBM bytes per launch of a stage's kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes of this workload (scaled by the
+    copy-kernel calibration), or (None, None) for another workload or another
+    split of the bounces (the `tail` knob decides which bounces each kernel
+    covers)."""
     try:
-        with open(path) as f:
+        with open(TRAFFIC_JSON) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None, None
-    w = t.get("workload", {})
-    if ((w.get("width"), w.get("height"), w.get("instances"), w.get("depth"), w.get("spp"), w.get("tail")) !=
-            (W, H, instances, depth, spp, tail)):
-        return None, None
-    b = 1024 * (t["fetch_kib_per_dispatch"] * t["fetch_scale"] + t["write_kib_per_dispatch"] * t["write_scale"])
-    return int(b), os.path.relpath(path, ROOT)
+    for e in t.get("entries", []):
+        w = e.get("workload", {})
+        if e.get("stage") == stage and (
+                (w.get("width"), w.get("height"), w.get("instances"), w.get("depth"), w.get("spp"), w.get("tail")) ==
+                (W, H, instances, depth, spp, tail)):
+            b = 1024 * (e["fetch_kib_per_dispatch"] * t["fetch_scale"] +
+                        e["write_kib_per_dispatch"] * t["write_scale"])
+            return int(b), os.path.relpath(TRAFFIC_JSON, ROOT)
+    return None, None
 
 
 def rel_rmse(est, ref):
@@ -369,40 +379,60 @@ def main():
     if integ == abi.INTEGRATOR_WAVEFRONT:
         ctx.set_trace_counting(True)
         ctx.render(args.spp)
-        counted = ctx.render_counters()
-        count_launches = ctx.timing()[2]
+        counted = {sname: ctx.stage_stats(sname) for sname in STAGE_KERNELS}
         ctx.set_trace_counting(False)
-        log(f"[bench] counted step: {counted} over {count_launches:.0f} trace launches")
+        for sname, c in counted.items():
+            log(f"[bench] counted step, {sname}: {c}")
 
     ctx.reset()
-    render_ms, trace_ms, launches = 0.0, 0.0, 0.0
+    render_ms = 0.0
+    stage_ms = {sname: 0.0 for sname in STAGE_KERNELS}
+    stage_launches = {sname: 0 for sname in STAGE_KERNELS}
     t0 = time.perf_counter()
     for i in range(args.steps):
         ctx.render(args.spp)
-        tm = ctx.timing()
-        render_ms += tm[0]
-        trace_ms += tm[1]
-        launches += tm[2]
+        render_ms += ctx.timing()[0]
+        if integ == abi.INTEGRATOR_WAVEFRONT:
+            for sname in STAGE_KERNELS:
+                sst = ctx.stage_stats(sname)
+                stage_ms[sname] += sst["ms"]
+                stage_launches[sname] += sst["launches"]
     wall = time.perf_counter() - t0
     samples = W * H * args.spp * args.steps
     value = samples / wall / 1e6
-    log(f"[bench] {args.steps} steps in {wall:.3f}s (event time {render_ms:.1f} ms), trace {trace_ms:.1f} ms "
-        f"over {launches:.0f} launches")
+    log(f"[bench] {args.steps} steps in {wall:.3f}s (event time {render_ms:.1f} ms), " +
+        ", ".join(f"{sname} {stage_ms[sname]:.1f} ms over {stage_launches[sname]} launches"
+                  for sname in STAGE_KERNELS))
 
-    roof = None
-    if counted is not None and launches > 0:
-        bytes_step = sum(BYTES_PER_EVENT[k] * counted[k] for k in BYTES_PER_EVENT)
-        bytes_per_launch = bytes_step / count_launches
-        avg_launch_s = (trace_ms / launches) / 1e3
-        achieved = bytes_per_launch / avg_launch_s / 1e9
-        traffic, traffic_src = measured_traffic(W, H, args.instances, args.depth, args.spp,
-                                                (tuning or {}).get("tail", 0))
-        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "kernel": "k_trace (wavefront closest hit)",
-                "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(avg_launch_s * 1e3, 4)}
-        if traffic_src:
-            roof["traffic_source"] = traffic_src
+    # roofline of each instrumented stage kernel; the bench line's `roofline`
+    # is the one with the most time in the timed steps
+    roof, stage_roofs = None, {}
+    if counted is not None:
+        tail_knob = (tuning or {}).get("tail", 0)
+        for sname, c in counted.items():
+            if stage_launches[sname] <= 0 or c["launches"] <= 0:
+                continue
+            bytes_step = sum(BYTES_PER_EVENT[k] * c["events"][k] for k in BYTES_PER_EVENT)
+            if sname == "tail":
+                bytes_step += TAIL_BYTES_PER_PATH * c["paths"]
+            bytes_per_launch = bytes_step / c["launches"]
+            avg_launch_s = (stage_ms[sname] / stage_launches[sname]) / 1e3
+            achieved = bytes_per_launch / avg_launch_s / 1e9
+            traffic, traffic_src = measured_traffic(sname, W, H, args.instances, args.depth, args.spp, tail_knob)
+            r = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                 "kernel": STAGE_KERNELS[sname],
+                 "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+                 "ms_per_step": round(stage_ms[sname] / args.steps, 3)}
+            if traffic_src:
+                r["traffic_source"] = traffic_src
+            stage_roofs[sname] = r
+        if stage_roofs:
+            dom = max(stage_roofs, key=lambda k: stage_ms[k])
+            roof = dict(stage_roofs[dom])
+            roof["stages"] = {k: {x: v[x] for x in ("kernel", "achieved", "frac", "avg_launch_ms", "ms_per_step",
+                                                    "traffic")}
+                              for k, v in stage_roofs.items()}
 
     cpu = None
     if not args.no_cpu_baseline:

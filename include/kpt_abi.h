@@ -307,11 +307,29 @@ int kpt_set_trace_counting(kpt_ctx* ctx, int32_t enable);
  * rays by direction octant and Morton cell of the origin before tracing
  * (0/1; not applied inside captured graphs), [7] (when n >= 8) the bounce
  * from which the paths still alive finish in one persistent launch, a lane
- * running its whole remaining path (0: never; 0..64; default 2; frames in which the
+ * running one path at a time vertex by vertex and taking the next queued path
+ * when its own ends (0: never; 0..64; default 1; frames in which the
  * radiance cache renders or trains keep the stage kernels).  n >= 5. */
 int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n);
 int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n);
 int kpt_get_render_counters(const kpt_ctx* ctx, kpt_trace_counters* out);
+/* Per-stage measurements of the last wavefront kpt_render (no reference
+ * counterpart): stage 0 = the closest-hit trace launches, stage 1 = the
+ * path-per-lane tail launches.  ms and launches come from HIP events around
+ * each launch (zero when the frames were replayed as hipGraphs); events,
+ * paths and vertices are filled in counting mode (kpt_set_trace_counting):
+ * the traversal events of the stage (the tail's closest and any-hit
+ * traversals together) and, for the tail, the paths it finished and the
+ * path vertices it processed. */
+typedef struct kpt_stage_stats {
+    double ms;
+    int32_t launches;
+    int32_t reserved;
+    kpt_trace_counters events;
+    uint64_t paths;
+    uint64_t vertices;
+} kpt_stage_stats;
+int kpt_get_stage_stats(const kpt_ctx* ctx, int32_t stage, kpt_stage_stats* out);
 
 /* ---- post chain (executeRendering after the path trace, renderer.cu:117-186) ---- */
 /* Renderer::setExposure (renderer.cu:312-329): records the values for the

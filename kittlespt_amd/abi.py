@@ -102,6 +102,21 @@ class KptTraceCounters(C.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class KptStageStats(C.Structure):
+    _fields_ = [
+        ("ms", C.c_double),
+        ("launches", C.c_int32),
+        ("reserved", C.c_int32),
+        ("events", KptTraceCounters),
+        ("paths", C.c_uint64),
+        ("vertices", C.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {"ms": float(self.ms), "launches": int(self.launches), "events": self.events.as_dict(),
+                "paths": int(self.paths), "vertices": int(self.vertices)}
+
+
 class KptCamera(C.Structure):
     _fields_ = [
         ("inv_view", C.c_float * 16),
@@ -240,6 +255,7 @@ SIGNATURES = [
     ("kpt_last_timing", C.c_int, [_P, C.POINTER(C.c_double)]),
     ("kpt_set_trace_counting", C.c_int, [_P, C.c_int32]),
     ("kpt_get_render_counters", C.c_int, [_P, C.POINTER(KptTraceCounters)]),
+    ("kpt_get_stage_stats", C.c_int, [_P, C.c_int32, C.POINTER(KptStageStats)]),
     ("kpt_set_tuning", C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32]),
     ("kpt_get_tuning", C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32]),
     ("kpt_set_exposure", C.c_int, [_P, C.POINTER(KptExposure)]),

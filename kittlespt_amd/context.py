@@ -232,6 +232,17 @@ class Context:
         self._ck(self.L.kpt_get_render_counters(self.h, C.byref(c)))
         return c.as_dict()
 
+    STAGES = {"trace": 0, "tail": 1}
+
+    def stage_stats(self, stage):
+        """Per-stage measurements of the last wavefront render: 'trace' (the
+        closest-hit launches) or 'tail' (the path-per-lane tail): HIP-event
+        ms and launch count, and in counting mode the traversal events (and,
+        for the tail, the paths and vertices it processed)."""
+        st = abi.KptStageStats()
+        self._ck(self.L.kpt_get_stage_stats(self.h, self.STAGES[stage], C.byref(st)))
+        return st.as_dict()
+
     # ---- post chain
     def set_exposure(self, exposure):
         """Renderer::setExposure: a KptExposure (abi.KptExposure.camera(...))."""

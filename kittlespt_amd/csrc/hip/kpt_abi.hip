@@ -74,12 +74,13 @@ struct kpt_ctx {
     WfInstrument ins[INS_RING]{};
     int32_t use_graph = 0;               // capture each frame as a hipGraph and replay it
     hipGraphExec_t graph_exec = nullptr;
-    TraceTuning tune{0, 128, 16, 8, 0, 2};   // the autotuner's usual choice on the 1080p workload
+    TraceTuning tune{0, 128, 16, 8, 0, 1};   // the autotuner's usual choice on the 1080p workload
     int32_t sort_by_material = 1;
     bool count_trace = false;
     unsigned long long* d_counts = nullptr;
     kpt_trace_counters render_counts{};
     double timing[4] = {0, 0, 0, 0};
+    kpt_stage_stats stages[2]{};         // closest-hit trace stage, path-per-lane tail
     bool tlas_dirty = false;
     ViewConst* d_views = nullptr;    // multi-view batch cameras
     // post chain
@@ -527,7 +528,7 @@ int kpt_ctx_create(int device, kpt_ctx** out) {
         hipMemset(ctx->d_overflow, 0, sizeof(int32_t)) != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||
         hipEventCreate(&ctx->ev1) != hipSuccess || hipEventCreate(&ctx->tev0) != hipSuccess ||
         hipEventCreate(&ctx->tev1) != hipSuccess ||
-        hipMalloc((void**)&ctx->d_counts, 8 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_counts, 16 * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc((void**)&ctx->pb.expo, sizeof(ExposureState)) != hipSuccess ||
         hipMalloc((void**)&ctx->pb.hist, 256 * sizeof(unsigned long long)) != hipSuccess ||
         hipMemset(ctx->pb.hist, 0, 256 * sizeof(unsigned long long)) != hipSuccess) {
@@ -548,6 +549,11 @@ int kpt_ctx_create(int device, kpt_ctx** out) {
                 delete ctx;
                 return KPT_ERR_HIP;
             }
+        }
+        if (hipEventCreate(&ctx->ins[r].tail_begin) != hipSuccess ||
+            hipEventCreate(&ctx->ins[r].tail_end) != hipSuccess) {
+            delete ctx;
+            return KPT_ERR_HIP;
         }
     }
     *out = ctx;
@@ -574,6 +580,8 @@ int kpt_ctx_destroy(kpt_ctx* ctx) {
                 if (ctx->ins[r].begin[k]) (void)hipEventDestroy(ctx->ins[r].begin[k]);
                 if (ctx->ins[r].end[k]) (void)hipEventDestroy(ctx->ins[r].end[k]);
             }
+            if (ctx->ins[r].tail_begin) (void)hipEventDestroy(ctx->ins[r].tail_begin);
+            if (ctx->ins[r].tail_end) (void)hipEventDestroy(ctx->ins[r].tail_end);
         }
         if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
         dev_free(&ctx->d_counts);
@@ -1097,11 +1105,11 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
         hooks.num_cus = ctx->num_cus;
     }
     if (ctx->count_trace) {
-        HIPCHK(hipMemsetAsync(ctx->d_counts, 0, 8 * sizeof(unsigned long long), ctx->stream));
+        HIPCHK(hipMemsetAsync(ctx->d_counts, 0, 16 * sizeof(unsigned long long), ctx->stream));
     }
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    double trace_ms = 0.0;
-    int trace_launches = 0;
+    double trace_ms = 0.0, tail_ms = 0.0;
+    int trace_launches = 0, tail_launches = 0;
     const bool wavefront = ctx->settings.integrator == KPT_INTEGRATOR_WAVEFRONT;
     const bool graph = ctx->use_graph && !ctx->count_trace;
     // trace times of a frame whose ring slot is about to be reused (or at the end)
@@ -1115,9 +1123,20 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
         }
         trace_launches += ins.launches;
         ins.launches = 0;
+        if (e == hipSuccess && ins.tail_launches > 0) {
+            float t = 0.0f;
+            e = hipEventSynchronize(ins.tail_end);
+            if (e == hipSuccess) e = hipEventElapsedTime(&t, ins.tail_begin, ins.tail_end);
+            tail_ms += t;
+            tail_launches += ins.tail_launches;
+            ins.tail_launches = 0;
+        }
         return e;
     };
-    for (int r = 0; r < kpt_ctx::INS_RING; ++r) ctx->ins[r].launches = 0;
+    for (int r = 0; r < kpt_ctx::INS_RING; ++r) {
+        ctx->ins[r].launches = 0;
+        ctx->ins[r].tail_launches = 0;
+    }
     for (int32_t k = 0; k < spp; ++k) {
         FrameConst fc = make_frame(ctx, ctx->frame_index);
         WfInstrument* ins = nullptr;
@@ -1208,8 +1227,15 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
     ctx->timing[0] = ms;
     ctx->timing[1] = (wavefront && !graph) ? trace_ms : (wavefront ? 0.0 : ms);
     ctx->timing[2] = trace_launches;
+    const bool timed = wavefront && !graph;
+    ctx->stages[0] = kpt_stage_stats{};
+    ctx->stages[1] = kpt_stage_stats{};
+    ctx->stages[0].ms = timed ? trace_ms : 0.0;
+    ctx->stages[0].launches = timed ? trace_launches : 0;
+    ctx->stages[1].ms = timed ? tail_ms : 0.0;
+    ctx->stages[1].launches = timed ? tail_launches : 0;
     if (ctx->count_trace) {
-        unsigned long long h[8];
+        unsigned long long h[16];
         HIPCHK(hipMemcpy(h, ctx->d_counts, sizeof(h), hipMemcpyDeviceToHost));
         ctx->render_counts.blas_interior_pops = h[0];
         ctx->render_counts.blas_leaf_pops = h[1];
@@ -1217,6 +1243,16 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
         ctx->render_counts.blas_entries = h[3];
         ctx->render_counts.tlas_pops = h[4];
         ctx->render_counts.rays = 0;
+        ctx->stages[0].events = ctx->render_counts;
+        kpt_trace_counters& t = ctx->stages[1].events;
+        t.blas_interior_pops = h[8];
+        t.blas_leaf_pops = h[9];
+        t.tri_tests = h[10];
+        t.blas_entries = h[11];
+        t.tlas_pops = h[12];
+        t.rays = 0;
+        ctx->stages[1].paths = h[13];
+        ctx->stages[1].vertices = h[14];
     }
     int32_t ovf = 0;
     HIPCHK(hipMemcpy(&ovf, ctx->d_overflow, sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -1477,6 +1513,12 @@ int kpt_set_trace_counting(kpt_ctx* ctx, int32_t enable) {
 int kpt_get_render_counters(const kpt_ctx* ctx, kpt_trace_counters* out) {
     if (!ctx || !out) return KPT_ERR_INVALID_ARG;
     *out = ctx->render_counts;
+    return KPT_OK;
+}
+
+int kpt_get_stage_stats(const kpt_ctx* ctx, int32_t stage, kpt_stage_stats* out) {
+    if (!ctx || !out || stage < 0 || stage > 1) return KPT_ERR_INVALID_ARG;
+    *out = ctx->stages[stage];
     return KPT_OK;
 }
 

@@ -165,76 +165,80 @@ struct PathState {
     v3 prev_pos;
 };
 
-// Li from a path state to the end of the path, in one thread (the megakernel
-// from depth 0; the wavefront's path-per-lane tail from a later bounce).
-KD v3 li_loop(const SceneView& S, const FrameConst& fc, PathState& ps, uint32_t& seed, uint32_t* lds, bool* ovf,
-              VisOut* vs) {
-    Ray ray = ps.ray;
-    v3 L = ps.L, beta = ps.beta;
-    bool any_nonspec = ps.any_nonspec;
+// One vertex of Li's loop (integrator.cuh:266-399): trace from the path's
+// ray, add emission and the next-event estimate, sample the next direction.
+// Returns false where the reference's loop breaks (miss, last depth, zero
+// BSDF sample, Russian roulette); the caller also stops when beta is zero.
+template <bool COUNT = false>
+KD bool li_bounce(const SceneView& S, const FrameConst& fc, PathState& ps, uint32_t& seed, uint32_t* lds,
+                  bool* ovf, VisOut* vs, TraceCounts* cnt = nullptr) {
     const float eta_scale = 1.0f;
-    int32_t depth = ps.depth;
-    float p_b = ps.p_b;
-    v3 prev_pos = ps.prev_pos;
-    while (!is_zero(beta)) {
-        seed = seed + (uint32_t)depth;
-        const bool first = (depth == 0);
-        bool o = false;
-        Hit h = trace_closest<false>(S, ray, 0.0f, __builtin_inff(), lds, &o, nullptr);
-        *ovf = *ovf || o;
-        if (h.slot < 0) {
-            v3 sky = miss_radiance(fc, ray.d, first);
-            if (first) {
-                vs->albedo = sky; vs->depth = __builtin_inff(); vs->normal = mk3(0.0f, 0.0f, 0.0f);
-                vs->pos = mk3(0.0f, 0.0f, 0.0f); vs->inst = -1; vs->prim = -1;
-            }
-            L = L + sky * beta;
-            break;
-        }
-        Surf s = make_surf(S, h, ray);
-        v3 Le, add;
-        if (emitted(S, fc, s, first, p_b, prev_pos, beta, &Le, &add)) L = L + add;
-        BSDF b = get_bsdf(S, s);
+    seed = seed + (uint32_t)ps.depth;
+    const bool first = (ps.depth == 0);
+    bool o = false;
+    Hit h = trace_closest<COUNT>(S, ps.ray, 0.0f, __builtin_inff(), lds, &o, cnt);
+    *ovf = *ovf || o;
+    if (h.slot < 0) {
+        v3 sky = miss_radiance(fc, ps.ray.d, first);
         if (first) {
-            vs->albedo = !is_zero(Le) ? Le : b.albedo;
-            vs->depth = s.distance; vs->normal = s.n; vs->pos = s.pos; vs->inst = s.inst; vs->prim = s.prim;
-            b.albedo = mk3(1.0f, 1.0f, 1.0f);
+            vs->albedo = sky; vs->depth = __builtin_inff(); vs->normal = mk3(0.0f, 0.0f, 0.0f);
+            vs->pos = mk3(0.0f, 0.0f, 0.0f); vs->inst = -1; vs->prim = -1;
         }
-        if (any_nonspec) bsdf_regularize(b);
-        if (!(b.alpha < 0.001f)) {
-            NeeResult ne = next_event(S, fc, s, b, ray.d, beta, seed);
-            if (ne.trace) {
-                bool oo = false;
-                Ray sr = make_ray(ne.org, ne.dir);
-                bool occluded = trace_any<false>(S, sr, 0.0f, ne.tmax, lds, &oo, nullptr);
-                *ovf = *ovf || oo;
-                L = L + (occluded ? ne.occl : ne.unoccl);
-            } else {
-                L = L + ne.occl;
-            }
-        }
-        if (depth++ == fc.max_depth) break;
-        v3 wo = -ray.d;
-        v2 u2, X2;
-        u2.x = rng1(seed); u2.y = rng1(seed);
-        X2.x = rng1(seed); X2.y = rng1(seed);
-        BSample bs = bsdf_sample(b, wo, u2, X2);
-        if (is_zero(bs.f)) break;
-        beta = beta * (bs.f * absdot(s.n, bs.wi) / bs.pdf);
-        p_b = bsdf_pdf(b, wo, bs.wi);
-        any_nonspec = any_nonspec || !((bs.scatter & SC_SPECULAR) != 0);
-        prev_pos = s.pos;
-        v3 org = (bs.scatter & SC_TRANSMITTED) ? s.pos - (s.n * KPT_HIT_EPS) : s.pos + (s.n * KPT_HIT_EPS);
-        ray = make_ray(org, bs.wi);
-        v3 rr = beta * eta_scale;
-        if (max_comp(rr) < 1.0f && depth > 1) {
-            float q = fmaxf_(0.0f, 1.0f - max_comp(rr));
-            if (rng1(seed) < q) break;
-            beta = beta / (1.0f - q);
+        ps.L = ps.L + sky * ps.beta;
+        return false;
+    }
+    Surf s = make_surf(S, h, ps.ray);
+    v3 Le, add;
+    if (emitted(S, fc, s, first, ps.p_b, ps.prev_pos, ps.beta, &Le, &add)) ps.L = ps.L + add;
+    BSDF b = get_bsdf(S, s);
+    if (first) {
+        vs->albedo = !is_zero(Le) ? Le : b.albedo;
+        vs->depth = s.distance; vs->normal = s.n; vs->pos = s.pos; vs->inst = s.inst; vs->prim = s.prim;
+        b.albedo = mk3(1.0f, 1.0f, 1.0f);
+    }
+    if (ps.any_nonspec) bsdf_regularize(b);
+    if (!(b.alpha < 0.001f)) {
+        NeeResult ne = next_event(S, fc, s, b, ps.ray.d, ps.beta, seed);
+        if (ne.trace) {
+            bool oo = false;
+            Ray sr = make_ray(ne.org, ne.dir);
+            bool occluded = trace_any<COUNT>(S, sr, 0.0f, ne.tmax, lds, &oo, cnt);
+            *ovf = *ovf || oo;
+            ps.L = ps.L + (occluded ? ne.occl : ne.unoccl);
+        } else {
+            ps.L = ps.L + ne.occl;
         }
     }
-    ps.depth = depth;
-    return L;
+    if (ps.depth++ == fc.max_depth) return false;
+    v3 wo = -ps.ray.d;
+    v2 u2, X2;
+    u2.x = rng1(seed); u2.y = rng1(seed);
+    X2.x = rng1(seed); X2.y = rng1(seed);
+    BSample bs = bsdf_sample(b, wo, u2, X2);
+    if (is_zero(bs.f)) return false;
+    ps.beta = ps.beta * (bs.f * absdot(s.n, bs.wi) / bs.pdf);
+    ps.p_b = bsdf_pdf(b, wo, bs.wi);
+    ps.any_nonspec = ps.any_nonspec || !((bs.scatter & SC_SPECULAR) != 0);
+    ps.prev_pos = s.pos;
+    v3 org = (bs.scatter & SC_TRANSMITTED) ? s.pos - (s.n * KPT_HIT_EPS) : s.pos + (s.n * KPT_HIT_EPS);
+    ps.ray = make_ray(org, bs.wi);
+    v3 rr = ps.beta * eta_scale;
+    if (max_comp(rr) < 1.0f && ps.depth > 1) {
+        float q = fmaxf_(0.0f, 1.0f - max_comp(rr));
+        if (rng1(seed) < q) return false;
+        ps.beta = ps.beta / (1.0f - q);
+    }
+    return true;
+}
+
+// Li from a path state to the end of the path, in one thread (the megakernel
+// from depth 0).  ps.L holds the radiance on return.
+KD v3 li_loop(const SceneView& S, const FrameConst& fc, PathState& ps, uint32_t& seed, uint32_t* lds, bool* ovf,
+              VisOut* vs) {
+    while (!is_zero(ps.beta)) {
+        if (!li_bounce(S, fc, ps, seed, lds, ovf, vs)) break;
+    }
+    return ps.L;
 }
 
 // Megakernel form of Li: the whole path in one thread.

@@ -112,7 +112,12 @@ struct WfInstrument {
     hipEvent_t begin[WF_MAX_TRACE_LAUNCHES];
     hipEvent_t end[WF_MAX_TRACE_LAUNCHES];
     int launches;                          // filled by wavefront_frame
-    unsigned long long* counters;          // device [5] or nullptr (counted traversal)
+    hipEvent_t tail_begin, tail_end;       // around the path-per-lane tail launch
+    int tail_launches;                     // 0 or 1, filled by wavefront_frame
+    unsigned long long* counters;          // device [16] or nullptr (counted traversal):
+                                           // [0..4] closest-hit stage events, [8..12] tail
+                                           // events (closest and any hit), [13] tail paths,
+                                           // [14] tail vertices
 };
 
 // One camera of a multi-view batch (device array).

@@ -495,41 +495,74 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_SHADE void k_shade(SceneView S, F
     }
 }
 
-// The rest of every path still alive, one path per lane (li_loop from the
+// The rest of every path still alive, one path per lane (li_bounce from the
 // path's state in HBM): after the first bounces few paths remain, each needing
 // several dependent trace/shade/shadow rounds, and per-bounce launches would
 // be bound by the latency of their few rays.  Same arithmetic and random
-// numbers as the stage kernels, so the radiance is unchanged.
+// numbers as the stage kernels, so the radiance is unchanged.  Lanes refill
+// at vertex boundaries: a lane whose path ends takes the next queued path
+// before the wave's next vertex, so a wave is not held by its longest path.
+template <bool COUNT>
 __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TAIL void k_tail(SceneView S, FrameConst fc, WavefrontState W, int parity,
-                                                   int pull_slot, int32_t* overflow) {
+                                                   int pull_slot, int32_t* overflow,
+                                                   unsigned long long* counters) {
     __shared__ uint32_t s_stack[LDS_STACK][WF_BLOCK];
     const int count = W.counters[C_QCOUNT + parity];
     const int32_t* q = W.queue[parity];
+    int32_t* ctr = &W.counters[pull_slot];
     bool ovf = false;
+    bool active = false, drained = false;   // drained: wave-uniform
+    int p = -1;
+    uint32_t st = 0u, seed = 0u;
+    PathState ps;
+    VisOut vs;   // written only at depth 0, which never reaches this kernel
+    TraceCounts c = {0, 0, 0, 0, 0};
+    uint32_t n_paths = 0, n_vertices = 0;
     for (;;) {
-        const int base = wave_claim(&W.counters[pull_slot], WAVE);
-        if (base >= count) break;
-        const int k = base + lane_id();
-        if (k >= count) continue;
-        const int p = q[k];
-        const float4 o4 = W.ray_o[p], d4 = W.ray_d[p], b4 = W.beta[p], l4 = W.L[p], pp = W.prev_pos[p];
-        PathState ps;
-        ps.ray = make_ray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z));
-        ps.L = mk3(l4.x, l4.y, l4.z);
-        ps.beta = mk3(b4.x, b4.y, b4.z);
-        ps.p_b = b4.w;
-        const uint32_t st = fbits(l4.w);
-        ps.depth = (int32_t)(st & 0xFFFFu);
-        ps.any_nonspec = (st >> 16) & 1u;
-        ps.prev_pos = mk3(pp.x, pp.y, pp.z);
-        uint32_t seed = fbits(d4.w);
-        VisOut vs;   // written only at depth 0, which never reaches this kernel
-        bool o = false;
-        const v3 L = li_loop(S, fc, ps, seed, &s_stack[0][threadIdx.x], &o, &vs);
-        ovf = ovf || o;
-        W.L[p] = make_float4(L.x, L.y, L.z, bitsf((uint32_t)ps.depth | (st & 0x10000u)));
+        // wave-uniform: claim one queue entry per idle lane
+        const uint64_t idle = __ballot(!active);
+        if (idle != 0ull && !drained) {
+            const int base = wave_claim(ctr, __popcll(idle));
+            if (base + __popcll(idle) >= count) drained = true;
+            const int k = base + __popcll(idle & lanes_below());
+            if (!active && k < count) {
+                p = q[k];
+                const float4 o4 = W.ray_o[p], d4 = W.ray_d[p], b4 = W.beta[p], l4 = W.L[p], pp = W.prev_pos[p];
+                ps.ray = make_ray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z));
+                ps.L = mk3(l4.x, l4.y, l4.z);
+                ps.beta = mk3(b4.x, b4.y, b4.z);
+                ps.p_b = b4.w;
+                st = fbits(l4.w);
+                ps.depth = (int32_t)(st & 0xFFFFu);
+                ps.any_nonspec = (st >> 16) & 1u;
+                ps.prev_pos = mk3(pp.x, pp.y, pp.z);
+                seed = fbits(d4.w);
+                active = true;
+                if (COUNT) ++n_paths;
+            }
+        }
+        if (!__any(active)) break;   // wave-uniform: queue drained, every lane idle
+        if (active) {
+            bool o = false;
+            if (COUNT && !is_zero(ps.beta)) ++n_vertices;
+            const bool more = !is_zero(ps.beta) &&
+                              li_bounce<COUNT>(S, fc, ps, seed, &s_stack[0][threadIdx.x], &o, &vs, &c);
+            ovf = ovf || o;
+            if (!more || is_zero(ps.beta)) {
+                W.L[p] = make_float4(ps.L.x, ps.L.y, ps.L.z, bitsf((uint32_t)ps.depth | (st & 0x10000u)));
+                active = false;
+            }
+        }
     }
     if (ovf) atomicOr(overflow, 1);
+    if (COUNT) {
+        uint32_t v[7] = {c.interior, c.leaf, c.tri, c.entries, c.tlas, n_paths, n_vertices};
+        for (int k = 0; k < 7; ++k) {
+            unsigned long long x = v[k];
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+            if ((threadIdx.x & 63) == 0) atomicAdd(&counters[8 + k], x);
+        }
+    }
 }
 
 // Visibility of the deferred next-event estimates; adds their term to L.  An
@@ -623,7 +656,10 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
                            const FrameBuffers& fb, WavefrontState& W, int32_t* overflow, int32_t num_materials,
                            int32_t sort_by_material, const TraceTuning& tune, WfInstrument* ins,
                            const CacheHooks* cache, const ViewBatch* views) {
-    if (ins) ins->launches = 0;
+    if (ins) {
+        ins->launches = 0;
+        ins->tail_launches = 0;
+    }
     CacheBuffers cb{};
     CacheFrame cf{};
     if (cache && cache->cb && (cache->cf.enabled || cache->cf.train)) {
@@ -657,8 +693,17 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
     for (int bounce = 0; bounce < max_bounces; ++bounce) {
         const int parity = bounce & 1;
         if (tune.tail > 0 && bounce >= tune.tail && !(cf.enabled || cf.train)) {
-            hipLaunchKernelGGL(k_tail, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity, C_PULL + 3 * bounce,
-                               overflow);
+            if (ins) (void)hipEventRecord(ins->tail_begin, s);
+            if (ins && ins->counters)
+                hipLaunchKernelGGL(k_tail<true>, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity,
+                                   C_PULL + 3 * bounce, overflow, ins->counters);
+            else
+                hipLaunchKernelGGL(k_tail<false>, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity,
+                                   C_PULL + 3 * bounce, overflow, (unsigned long long*)nullptr);
+            if (ins) {
+                (void)hipEventRecord(ins->tail_end, s);
+                ins->tail_launches = 1;
+            }
             break;
         }
         // next queue and shadow count start at zero

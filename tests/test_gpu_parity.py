@@ -198,3 +198,37 @@ def test_reference_seed_mode(plain):
     img, gbuf, vbuf, orgb, og, ov = _render_both(sc, ctx, orc, 32, 32, 3, abi.INTEGRATOR_WAVEFRONT,
                                                  seed_mode=abi.SEED_REFERENCE)
     assert _rel_err(img, orgb).mean() <= 1e-4
+
+
+def test_stage_stats_of_the_tail(inst_small):
+    """Per-stage measurements: with the tail from bounce 1, each frame has one
+    closest-hit trace launch and one tail launch; counting mode fills the
+    traversal events and the tail's path and vertex counts, and does not
+    change the image."""
+    sc, ctx, orc = inst_small
+    W, H, spp = 80, 64, 2
+    ctx.set_tuning(refill=0, chunk=128, refill_min=16, blocks_per_cu=8, sort=1, graph=0, ray_sort=0, tail=1)
+    ctx.set_settings(settings(max_depth=6, integrator=abi.INTEGRATOR_WAVEFRONT, sun_theta=sc.sun_theta))
+    ctx.resize(W, H)
+    p, v = view_proj(sc, W, H)
+    ctx.set_view(p, v)
+    imgs, stats = [], []
+    for counting in (False, True):
+        ctx.reset()
+        ctx.set_trace_counting(counting)
+        print(f"[stages] counting={counting}", flush=True)
+        ctx.render(spp)
+        imgs.append(ctx.readback(1))
+        stats.append({s: ctx.stage_stats(s) for s in ("trace", "tail")})
+    ctx.set_trace_counting(False)
+    ctx.set_tuning(tail=0)
+    assert np.array_equal(imgs[0].view(np.uint32), imgs[1].view(np.uint32))
+    for st in stats:
+        assert st["trace"]["launches"] == spp and st["tail"]["launches"] == spp
+        assert st["trace"]["ms"] > 0 and st["tail"]["ms"] > 0
+    t, c = stats[1]["tail"], stats[1]["trace"]
+    assert 0 < t["paths"] <= W * H * spp
+    assert t["vertices"] >= t["paths"]
+    for k in ("blas_interior_pops", "blas_leaf_pops", "tri_tests", "tlas_pops"):
+        assert t["events"][k] > 0 and c["events"][k] > 0, k
+    assert stats[0]["tail"]["paths"] == 0 and stats[0]["tail"]["events"]["tri_tests"] == 0
