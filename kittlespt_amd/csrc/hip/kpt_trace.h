@@ -177,6 +177,53 @@ KD void load_block(const NodeBlock* blocks, uint32_t q, BlockView* v) {
     v->g2l = p[8]; v->g2h = p[9]; v->g3l = p[10]; v->g3h = p[11];
 }
 
+// What one traversal step of a lane reads, chosen before any of it is loaded
+// so that every lane of a wave issues its loads together and the wave waits
+// for one memory round trip per step whatever mix of work its lanes hold:
+//   STEP_INTERIOR  the 192-byte NodeBlock of the entry's children pair
+//   STEP_INSTANCE  the 64-byte InstTrav of a TLAS leaf
+//   STEP_TRIS      up to TRI_BATCH 48-byte TriPos of the current leaf
+//   STEP_HEADER    the 32-byte node of a leaf too large for an entry
+enum : int { STEP_INTERIOR = 0, STEP_INSTANCE = 1, STEP_TRIS = 2, STEP_HEADER = 3 };
+constexpr int STEP_LOADS = 12;   // float4 loads per step (192 bytes)
+static_assert(3 * TRI_BATCH <= STEP_LOADS, "a triangle batch fits one step's loads");
+
+// Loads float4 [0, n) of base into v (n >= 1); the slots past n repeat the
+// last one, so the loads are issued unconditionally and stay in bounds.
+KD void step_load(const float4* base, int n, float4 v[STEP_LOADS]) {
+#pragma unroll
+    for (int k = 0; k < STEP_LOADS; ++k) v[k] = base[k < n ? k : n - 1];
+}
+
+// Classifies the lane's next step and points base/n at what it reads.  A
+// LEAF entry is decoded here (first slot and count, no load), so its first
+// triangles are read in the same step.
+KD int step_kind(const SceneView& S, uint32_t cur, uint32_t& la, uint32_t& lb, const float4** base, int* n) {
+    if (lb == 0u) {
+        if (!(cur & (LEAF_BIT | BIG_BIT))) {
+            const NodeBlock* blocks = (cur & TLAS_BIT) ? S.tlas_blocks : S.blas_blocks;
+            *base = reinterpret_cast<const float4*>(blocks + (cur & ENTRY_PAYLOAD));
+            *n = 12;
+            return STEP_INTERIOR;
+        }
+        if (cur & TLAS_BIT) {
+            *base = reinterpret_cast<const float4*>(S.inst_trav + (cur & ENTRY_PAYLOAD));
+            *n = 4;
+            return STEP_INSTANCE;
+        }
+        if (!(cur & LEAF_BIT)) {
+            *base = reinterpret_cast<const float4*>(S.blas_nodes + (cur & ENTRY_PAYLOAD));
+            *n = 2;
+            return STEP_HEADER;
+        }
+        la = cur & 0x7FFFFFu;
+        lb = (cur >> 23) & 0x3Fu;
+    }
+    *base = reinterpret_cast<const float4*>(S.tris + la);
+    *n = 3 * (int)(lb < (uint32_t)TRI_BATCH ? lb : (uint32_t)TRI_BATCH);
+    return STEP_TRIS;
+}
+
 // Closest-hit traversal state of one lane.
 struct ClosestTrav {
     Ray w, o;
@@ -237,27 +284,32 @@ struct ClosestTrav {
         return false;
     }
 
-    // Processes the current entry — an interior step, an instance entry, or
-    // one batch of TRI_BATCH triangles of a leaf (a leaf spans several steps,
-    // so a wave's lanes at interior nodes are not held up by another lane's
-    // whole leaf); returns false when the query is finished.
+    // Processes the current entry — an interior step, an instance entry, a
+    // large leaf's header, or one batch of TRI_BATCH triangles of a leaf (a
+    // leaf spans several steps, so a wave's lanes at interior nodes are not
+    // held up by another lane's whole leaf); returns false when the query is
+    // finished.  Whatever the lane's kind of step, its reads are issued by
+    // the same instructions (step_load).
     template <bool COUNT>
     KD bool step(const SceneView& S, TraceCounts* cnt) {
-      if (lb == 0u) {
         const uint32_t e = cur;
-        if (!(e & (LEAF_BIT | BIG_BIT))) {
+        const bool new_leaf = (lb == 0u) && (e & LEAF_BIT) && !(e & TLAS_BIT);
+        const float4* base;
+        int n;
+        const int kind = step_kind(S, e, la, lb, &base, &n);
+        float4 v[STEP_LOADS];
+        step_load(base, n, v);
+        if (kind == STEP_INTERIOR) {
             // interior node (TLAS or BLAS): its children pair's block
             const bool tl = (e & TLAS_BIT) != 0;
             const uint32_t q = e & ENTRY_PAYLOAD;
-            BlockView v;
-            load_block(tl ? S.tlas_blocks : S.blas_blocks, q, &v);
             int nk;
             bool any;
             if (COUNT) { if (tl) cnt->tlas++; else cnt->interior++; }
-            any = children(tl, v.c0l, v.c0h, v.c1l, v.c1h, 2 * q, 2 * q + 1, &nk);
+            any = children(tl, v[0], v[1], v[2], v[3], 2 * q, 2 * q + 1, &nk);
             if (!any) return next();
             if (st.overflow) return false;
-            const float4 nl = nk ? v.c1l : v.c0l, nh = nk ? v.c1h : v.c0h;
+            const float4 nl = nk ? v[2] : v[0], nh = nk ? v[3] : v[1];
             const uint32_t na = bits_u(nl.w), nb = bits_u(nh.w);
             const bool n_interior = tl ? (na != 0u) : (nb == 0u);
             if (!n_interior) {
@@ -265,8 +317,8 @@ struct ClosestTrav {
                 return true;
             }
             // the nearer child is interior: its step, from the same block
-            const float4 al = nk ? v.g2l : v.g0l, ah = nk ? v.g2h : v.g0h;
-            const float4 bl = nk ? v.g3l : v.g1l, bh = nk ? v.g3h : v.g1h;
+            const float4 al = nk ? v[8] : v[4], ah = nk ? v[9] : v[5];
+            const float4 bl = nk ? v[10] : v[6], bh = nk ? v[11] : v[7];
             const uint32_t p = na >> 1;   // the nearer child's children pair
             if (COUNT) { if (tl) cnt->tlas++; else cnt->interior++; }
             any = children(tl, al, ah, bl, bh, 2 * p, 2 * p + 1, &nk);
@@ -276,38 +328,34 @@ struct ClosestTrav {
             cur = tl ? tlas_entry(bits_u(ml.w), bits_u(mh.w)) : blas_entry(2 * p + (uint32_t)nk, bits_u(ml.w), bits_u(mh.w));
             return true;
         }
-        if (e & TLAS_BIT) {
+        if (kind == STEP_INSTANCE) {
             // TLAS leaf: enter the instance's BLAS
             if (COUNT) { cnt->tlas++; cnt->entries++; }
             cur_inst = (int32_t)(e & ENTRY_PAYLOAD);
-            const InstTrav it = S.inst_trav[cur_inst];
+            InstTrav it;
+            __builtin_memcpy(&it, v, sizeof(InstTrav));
             o = object_ray(it, w);
             if (it.root_entry != 0xFFFFFFFFu) { cur = it.root_entry; return true; }
             return next();
         }
-        if (e & LEAF_BIT) {
-            la = e & 0x7FFFFFu;
-            lb = (e >> 23) & 0x3Fu;
-        } else {
-            float4 lo, hi;
-            load_node(S.blas_nodes, e & ENTRY_PAYLOAD, &lo, &hi);
-            la = bits_u(lo.w); lb = bits_u(hi.w);
+        if (kind == STEP_HEADER) {
+            // a leaf too large for an entry: its header, the triangles from the next step
+            la = bits_u(v[0].w);
+            lb = bits_u(v[1].w);
+            if (COUNT) cnt->leaf++;
+            return true;
         }
-        if (COUNT) cnt->leaf++;
-      }
-        // the next TRI_BATCH triangles of the leaf, loads in flight together,
-        // tested in slot order
-        TriPos tp[TRI_BATCH];
-#pragma unroll
-        for (int j = 0; j < TRI_BATCH; ++j)
-            if ((uint32_t)j < lb) tp[j] = S.tris[la + j];
+        if (COUNT && new_leaf) cnt->leaf++;
+        // the next TRI_BATCH triangles of the leaf, tested in slot order
 #pragma unroll
         for (int j = 0; j < TRI_BATCH; ++j) {
             if ((uint32_t)j < lb) {
                 if (COUNT) cnt->tri++;
-                float t, u, v;
-                if (tri_test(tp[j], o, tmin, fminf_(tmax, h.t), &t, &u, &v) && t < h.t) {
-                    h.t = t; h.slot = (int32_t)(la + j); h.inst = cur_inst; h.u = u; h.v = v;
+                TriPos tp;
+                __builtin_memcpy(&tp, &v[3 * j], sizeof(TriPos));
+                float t, u, vv;
+                if (tri_test(tp, o, tmin, fminf_(tmax, h.t), &t, &u, &vv) && t < h.t) {
+                    h.t = t; h.slot = (int32_t)(la + j); h.inst = cur_inst; h.u = u; h.v = vv;
                 }
             }
         }
@@ -364,28 +412,31 @@ struct AnyTrav {
     }
     template <bool COUNT>
     KD bool step(const SceneView& S, TraceCounts* cnt) {
-      if (lb == 0u) {
         const uint32_t e = cur;
-        if (!(e & (LEAF_BIT | BIG_BIT))) {
+        const bool new_leaf = (lb == 0u) && (e & LEAF_BIT) && !(e & TLAS_BIT);
+        const float4* base;
+        int n;
+        const int kind = step_kind(S, e, la, lb, &base, &n);
+        float4 v[STEP_LOADS];
+        step_load(base, n, v);
+        if (kind == STEP_INTERIOR) {
             const bool tl = (e & TLAS_BIT) != 0;
             const uint32_t q = e & ENTRY_PAYLOAD;
-            BlockView v;
-            load_block(tl ? S.tlas_blocks : S.blas_blocks, q, &v);
             int nk;
             bool any;
             if (COUNT) { if (tl) cnt->tlas++; else cnt->interior++; }
-            any = children(tl, v.c0l, v.c0h, v.c1l, v.c1h, 2 * q, 2 * q + 1, &nk);
+            any = children(tl, v[0], v[1], v[2], v[3], 2 * q, 2 * q + 1, &nk);
             if (!any) return next();
             if (st.overflow) return false;
-            const float4 nl = nk ? v.c1l : v.c0l, nh = nk ? v.c1h : v.c0h;
+            const float4 nl = nk ? v[2] : v[0], nh = nk ? v[3] : v[1];
             const uint32_t na = bits_u(nl.w), nb = bits_u(nh.w);
             const bool n_interior = tl ? (na != 0u) : (nb == 0u);
             if (!n_interior) {
                 cur = tl ? tlas_entry(na, nb) : blas_entry(2 * q + (uint32_t)nk, na, nb);
                 return true;
             }
-            const float4 al = nk ? v.g2l : v.g0l, ah = nk ? v.g2h : v.g0h;
-            const float4 bl = nk ? v.g3l : v.g1l, bh = nk ? v.g3h : v.g1h;
+            const float4 al = nk ? v[8] : v[4], ah = nk ? v[9] : v[5];
+            const float4 bl = nk ? v[10] : v[6], bh = nk ? v[11] : v[7];
             const uint32_t p = na >> 1;
             if (COUNT) { if (tl) cnt->tlas++; else cnt->interior++; }
             any = children(tl, al, ah, bl, bh, 2 * p, 2 * p + 1, &nk);
@@ -395,33 +446,29 @@ struct AnyTrav {
             cur = tl ? tlas_entry(bits_u(ml.w), bits_u(mh.w)) : blas_entry(2 * p + (uint32_t)nk, bits_u(ml.w), bits_u(mh.w));
             return true;
         }
-        if (e & TLAS_BIT) {
+        if (kind == STEP_INSTANCE) {
             if (COUNT) { cnt->tlas++; cnt->entries++; }
-            const InstTrav it = S.inst_trav[e & ENTRY_PAYLOAD];
+            InstTrav it;
+            __builtin_memcpy(&it, v, sizeof(InstTrav));
             o = object_ray(it, w);
             if (it.root_entry != 0xFFFFFFFFu) { cur = it.root_entry; return true; }
             return next();
         }
-        if (e & LEAF_BIT) {
-            la = e & 0x7FFFFFu;
-            lb = (e >> 23) & 0x3Fu;
-        } else {
-            float4 lo, hi;
-            load_node(S.blas_nodes, e & ENTRY_PAYLOAD, &lo, &hi);
-            la = bits_u(lo.w); lb = bits_u(hi.w);
+        if (kind == STEP_HEADER) {
+            la = bits_u(v[0].w);
+            lb = bits_u(v[1].w);
+            if (COUNT) cnt->leaf++;
+            return true;
         }
-        if (COUNT) cnt->leaf++;
-      }
-        TriPos tp[TRI_BATCH];
-#pragma unroll
-        for (int j = 0; j < TRI_BATCH; ++j)
-            if ((uint32_t)j < lb) tp[j] = S.tris[la + j];
+        if (COUNT && new_leaf) cnt->leaf++;
 #pragma unroll
         for (int j = 0; j < TRI_BATCH; ++j) {
             if ((uint32_t)j < lb) {
                 if (COUNT) cnt->tri++;
-                float t, u, v;
-                if (tri_test(tp[j], o, tmin, tmax, &t, &u, &v)) { hit = true; lb = 0u; return false; }
+                TriPos tp;
+                __builtin_memcpy(&tp, &v[3 * j], sizeof(TriPos));
+                float t, u, vv;
+                if (tri_test(tp, o, tmin, tmax, &t, &u, &vv)) { hit = true; lb = 0u; return false; }
             }
         }
         if (lb > (uint32_t)TRI_BATCH) { la += TRI_BATCH; lb -= TRI_BATCH; return true; }
