@@ -505,10 +505,10 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_SHADE void k_shade(SceneView S, F
 template <bool COUNT>
 __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TAIL void k_tail(SceneView S, FrameConst fc, WavefrontState W, int parity,
                                                    int pull_slot, int32_t* overflow,
-                                                   unsigned long long* counters) {
+                                                   unsigned long long* counters, const int32_t* order) {
     __shared__ uint32_t s_stack[LDS_STACK][WF_BLOCK];
     const int count = W.counters[C_QCOUNT + parity];
-    const int32_t* q = W.queue[parity];
+    const int32_t* q = order ? order : W.queue[parity];
     int32_t* ctr = &W.counters[pull_slot];
     bool ovf = false;
     bool active = false, drained = false;   // drained: wave-uniform
@@ -693,13 +693,23 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
     for (int bounce = 0; bounce < max_bounces; ++bounce) {
         const int parity = bounce & 1;
         if (tune.tail > 0 && bounce >= tune.tail && !(cf.enabled || cf.train)) {
+            // the tail's paths in spatial order of their rays (ray_sort knob)
+            const int32_t* tail_q = nullptr;
+            if (tune.ray_sort && W.rtemp) {
+                hipLaunchKernelGGL(k_ray_keys, dim3(gblocks), dim3(WF_BLOCK), 0, s, fc, W, parity, npx);
+                size_t bytes = W.rtemp_bytes;
+                hipError_t e = rocprim::radix_sort_pairs(W.rtemp, bytes, W.rkeys[0], W.rkeys[1], W.rvals[0],
+                                                         W.rvals[1], (size_t)npx, 0, RKEY_BITS, s);
+                if (e != hipSuccess) return e;
+                tail_q = W.rvals[1];
+            }
             if (ins) (void)hipEventRecord(ins->tail_begin, s);
             if (ins && ins->counters)
                 hipLaunchKernelGGL(k_tail<true>, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity,
-                                   C_PULL + 3 * bounce, overflow, ins->counters);
+                                   C_PULL + 3 * bounce, overflow, ins->counters, tail_q);
             else
                 hipLaunchKernelGGL(k_tail<false>, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity,
-                                   C_PULL + 3 * bounce, overflow, (unsigned long long*)nullptr);
+                                   C_PULL + 3 * bounce, overflow, (unsigned long long*)nullptr, tail_q);
             if (ins) {
                 (void)hipEventRecord(ins->tail_end, s);
                 ins->tail_launches = 1;
