@@ -169,15 +169,22 @@ struct PathState {
 // ray, add emission and the next-event estimate, sample the next direction.
 // Returns false where the reference's loop breaks (miss, last depth, zero
 // BSDF sample, Russian roulette); the caller also stops when beta is zero.
-template <bool COUNT = false>
+// DEFER_SKY (vertices past the first only): a miss sets *missed and leaves
+// L without the sky term — the last addition of the path, L + sky * beta,
+// which the caller then makes with ps.ray.d and ps.beta as they are.
+template <bool COUNT = false, bool DEFER_SKY = false>
 KD bool li_bounce(const SceneView& S, const FrameConst& fc, PathState& ps, uint32_t& seed, uint32_t* lds,
-                  bool* ovf, VisOut* vs, TraceCounts* cnt = nullptr) {
+                  bool* ovf, VisOut* vs, TraceCounts* cnt = nullptr, bool* missed = nullptr) {
     const float eta_scale = 1.0f;
     seed = seed + (uint32_t)ps.depth;
     const bool first = (ps.depth == 0);
     bool o = false;
     Hit h = trace_closest<COUNT>(S, ps.ray, 0.0f, __builtin_inff(), lds, &o, cnt);
     *ovf = *ovf || o;
+    if (DEFER_SKY && h.slot < 0 && !first) {
+        *missed = true;
+        return false;
+    }
     if (h.slot < 0) {
         v3 sky = miss_radiance(fc, ps.ray.d, first);
         if (first) {

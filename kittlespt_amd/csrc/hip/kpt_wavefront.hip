@@ -502,6 +502,10 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_SHADE void k_shade(SceneView S, F
 // numbers as the stage kernels, so the radiance is unchanged.  Lanes refill
 // at vertex boundaries: a lane whose path ends takes the next queued path
 // before the wave's next vertex, so a wave is not held by its longest path.
+// A path whose ray escapes leaves its sky term to k_sky: the lane appends
+// (path, direction, throughput) to the sky queue (the shadow-ray arrays,
+// unused while the tail runs; count in C_SHADOW, cleared before the launch)
+// instead of evaluating the sky with the few lanes that missed.
 template <bool COUNT>
 __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TAIL void k_tail(SceneView S, FrameConst fc, WavefrontState W, int parity,
                                                    int pull_slot, int32_t* overflow,
@@ -542,15 +546,28 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TAIL void k_tail(SceneView S, Fra
             }
         }
         if (!__any(active)) break;   // wave-uniform: queue drained, every lane idle
+        bool missed = false;
         if (active) {
             bool o = false;
             if (COUNT && !is_zero(ps.beta)) ++n_vertices;
             const bool more = !is_zero(ps.beta) &&
-                              li_bounce<COUNT>(S, fc, ps, seed, &s_stack[0][threadIdx.x], &o, &vs, &c);
+                              li_bounce<COUNT, true>(S, fc, ps, seed, &s_stack[0][threadIdx.x], &o, &vs, &c,
+                                                     &missed);
             ovf = ovf || o;
             if (!more || is_zero(ps.beta)) {
                 W.L[p] = make_float4(ps.L.x, ps.L.y, ps.L.z, bitsf((uint32_t)ps.depth | (st & 0x10000u)));
                 active = false;
+            }
+        }
+        // wave-uniform: one atomic per wave for the lanes whose ray escaped
+        const uint64_t mm = __ballot(missed);
+        if (mm != 0ull) {
+            const int base = wave_claim(&W.counters[C_SHADOW], __popcll(mm));
+            if (missed) {
+                const int k = base + __popcll(mm & lanes_below());
+                W.shadow_queue[k] = p;
+                W.shadow_d[k] = make_float4(ps.ray.d.x, ps.ray.d.y, ps.ray.d.z, 0.0f);
+                W.shadow_c[k] = make_float4(ps.beta.x, ps.beta.y, ps.beta.z, 0.0f);
             }
         }
     }
@@ -563,6 +580,18 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TAIL void k_tail(SceneView S, Fra
             if ((threadIdx.x & 63) == 0) atomicAdd(&counters[8 + k], x);
         }
     }
+}
+
+// Sky term of every path the tail ended by a miss: L + sky * beta, the same
+// operations li_bounce makes at the miss, with all lanes of a wave busy.
+__global__ __launch_bounds__(WF_BLOCK) void k_sky(FrameConst fc, WavefrontState W) {
+    const int i = (int)(blockIdx.x * WF_BLOCK + threadIdx.x);
+    if (i >= W.counters[C_SHADOW]) return;
+    const int p = W.shadow_queue[i];
+    const float4 d = W.shadow_d[i], b = W.shadow_c[i], l = W.L[p];
+    const v3 sky = miss_radiance(fc, mk3(d.x, d.y, d.z), false);
+    const v3 L = mk3(l.x, l.y, l.z) + sky * mk3(b.x, b.y, b.z);
+    W.L[p] = make_float4(L.x, L.y, L.z, l.w);
 }
 
 // Visibility of the deferred next-event estimates; adds their term to L.  An
@@ -703,6 +732,7 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
                 if (e != hipSuccess) return e;
                 tail_q = W.rvals[1];
             }
+            hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, W.counters + C_SHADOW, 0, 1);
             if (ins) (void)hipEventRecord(ins->tail_begin, s);
             if (ins && ins->counters)
                 hipLaunchKernelGGL(k_tail<true>, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity,
@@ -714,6 +744,7 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
                 (void)hipEventRecord(ins->tail_end, s);
                 ins->tail_launches = 1;
             }
+            hipLaunchKernelGGL(k_sky, dim3(gblocks), dim3(WF_BLOCK), 0, s, fc, W);
             break;
         }
         // next queue and shadow count start at zero
