@@ -277,6 +277,8 @@ private:
         m_settings.bloom_final_blend = m_rs.bloom_final_blend;
         m_settings.bloom_internal_blend = m_rs.bloom_internal_blend;
         m_settings.enable_auto_exposure = m_rs.tonemapper_enable_auto_exposure ? 1 : 0;
+        m_settings.postprocess_enable_effects = m_rs.postprocess_enable_effects ? 1 : 0;
+        m_settings.upscale_enable = m_rs.upscale_enable ? 1 : 0;
         m_settings.sun_angular_diameter_rad = m_env.sun_angular_diameter_rad;
         m_settings.sun_phi_rad = m_env.sun_phi_rad;
         m_settings.sun_theta_rad = m_env.sun_theta_rad;

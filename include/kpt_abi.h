@@ -151,7 +151,15 @@ typedef struct kpt_settings {
     float sun_emission_factor;
     int32_t seed_mode;        /* kpt_seed_mode */
     int32_t integrator;       /* kpt_integrator */
-    int32_t reserved[3];
+    /* RendererSettings::postprocess_enable_effects (pod_types.hpp:55): the
+     * fisheye + chromatic-aberration pass after the post-process
+     * (renderer.cu:172-178, kernels.cu:508-528); 0/1. */
+    int32_t postprocess_enable_effects;
+    /* RendererSettings::upscale_enable (pod_types.hpp:56): render at 1/2.5
+     * of the output size and upscale (renderer.cu:55-75, 138-160).  Not
+     * implemented: kpt_set_settings rejects 1 with KPT_ERR_UNSUPPORTED. */
+    int32_t upscale_enable;
+    int32_t reserved;
 } kpt_settings;
 
 /* Camera exposure as Renderer::setExposure takes it (ExposureValues,

@@ -141,7 +141,9 @@ class KptSettings(C.Structure):
         ("sun_emission_factor", C.c_float),
         ("seed_mode", C.c_int32),
         ("integrator", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("postprocess_enable_effects", C.c_int32),
+        ("upscale_enable", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

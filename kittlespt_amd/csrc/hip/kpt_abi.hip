@@ -312,6 +312,7 @@ void free_frame(kpt_ctx* c) {
     dev_free(&c->fb.accum); dev_free(&c->fb.render); dev_free(&c->fb.gbuffer);
     dev_free(&c->fb.prev_gbuffer); dev_free(&c->fb.vbuffer); dev_free(&c->fb.output);
     for (int k = 0; k < MAX_MIPS; ++k) dev_free(&c->pb.mip[k]);
+    dev_free(&c->pb.back);
     c->pb.levels = 0;
 }
 
@@ -920,8 +921,11 @@ int kpt_trace_closest_counted(kpt_ctx* ctx, const kpt_ray* rays, kpt_hit* hits, 
 
 int kpt_set_settings(kpt_ctx* ctx, const kpt_settings* s) {
     if (!ctx || !s) return KPT_ERR_INVALID_ARG;
-    if (s->max_depth < 0 || s->seed_mode < 0 || s->seed_mode > 1 || s->integrator < 0 || s->integrator > 1)
+    if (s->max_depth < 0 || s->seed_mode < 0 || s->seed_mode > 1 || s->integrator < 0 || s->integrator > 1 ||
+        s->postprocess_enable_effects < 0 || s->postprocess_enable_effects > 1 || s->upscale_enable < 0 ||
+        s->upscale_enable > 1)
         return fail(ctx, KPT_ERR_INVALID_ARG, "invalid settings");
+    if (s->upscale_enable) return fail(ctx, KPT_ERR_UNSUPPORTED, "upscale_enable: the upscaler is not implemented");
     ctx->settings = *s;
     return kpt_reset_accumulation(ctx);   // renderer.cu:301-305
 }
@@ -1045,6 +1049,7 @@ int kpt_resize(kpt_ctx* ctx, int32_t width, int32_t height) {
     HIPCHK(hipMalloc((void**)&ctx->fb.prev_gbuffer, n * sizeof(uint4)));
     HIPCHK(hipMalloc((void**)&ctx->fb.vbuffer, n * sizeof(float4)));
     HIPCHK(hipMalloc((void**)&ctx->fb.output, n * sizeof(float4)));
+    HIPCHK(hipMalloc((void**)&ctx->pb.back, n * sizeof(float4)));
     // bloom mip chain at output resolution (MipChain::resize, mipchain.cuh:17-35)
     ctx->pb.levels = bloom_levels(width, height);
     for (int k = 0; k <= ctx->pb.levels; ++k) {
@@ -1182,6 +1187,7 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
             pp.bloom_final_blend = ctx->settings.bloom_final_blend;
             pp.bloom_internal_blend = ctx->settings.bloom_internal_blend;
             pp.frame_delta_ms = ctx->frame_delta_ms;
+            pp.effects = ctx->settings.postprocess_enable_effects ? 1 : 0;
             e = launch_post_frame(ctx->stream, ctx->fb, ctx->pb, pp, ctx->num_cus);
             if (e != hipSuccess) return e;
             // end of executeRendering: gbuffer -> prev_gbuffer (renderer.cu:186)

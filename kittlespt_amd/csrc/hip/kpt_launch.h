@@ -71,6 +71,7 @@ struct PostBuffers {
     int32_t levels;                 // mips beyond mip 0
     unsigned long long* hist;       // 256 fixed-point bins (weight * 2^23)
     ExposureState* expo;
+    float4* back;                   // back buffer when the effects pass is on (output size)
 };
 struct PostParams {
     int32_t width, height;
@@ -79,6 +80,7 @@ struct PostParams {
     int32_t bloom, karis;
     float bloom_final_blend, bloom_internal_blend;
     float frame_delta_ms;
+    int32_t effects;                // fisheye + chromatic aberration from pb.back into fb.output
 };
 // executeRendering after the path trace (renderer.cu:117-170): modulate,
 // auto exposure, bloom, post.  Everything stays on the stream.

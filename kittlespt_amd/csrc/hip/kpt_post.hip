@@ -320,7 +320,7 @@ __global__ __launch_bounds__(POST_BLOCK) void k_bloom_up(Tex src, float4* dst, i
 
 // ------------------------------------------------------------ post
 // computePostProcess (kernels.cu:475-504): bloom blend, exposure in Yxy, AgX.
-// The effects pass is off, so the back buffer is the output.
+// Into the back buffer when the effects pass follows, else into the output.
 __global__ __launch_bounds__(POST_BLOCK) void k_post_final(FrameBuffers fb, PostBuffers pb, PostParams p) {
     const int64_t npx = (int64_t)p.width * p.height;
     const int64_t i = (int64_t)blockIdx.x * POST_BLOCK + threadIdx.x;
@@ -337,7 +337,69 @@ __global__ __launch_bounds__(POST_BLOCK) void k_post_final(FrameBuffers fb, Post
     yxy.x *= e->scalar;
     c = xyz_to_rgb(Yxy_to_xyz(yxy));
     const c3 o = film_agx(c, e->white_ev, e->black_ev);
-    fb.output[i] = make_float4(o.x, o.y, o.z, 1.0f);
+    (p.effects ? pb.back : fb.output)[i] = make_float4(o.x, o.y, o.z, 1.0f);
+}
+
+// ------------------------------------------------------------ effects
+// Float to int as the device's conversion does it (round toward zero,
+// saturating, NaN to 0), the make_int2(float2) of the texture accessors.
+__device__ __forceinline__ int f2i_sat(float x) {
+    if (!(x == x)) return 0;
+    if (x >= 2147483648.0f) return 2147483647;
+    if (x <= -2147483648.0f) return (-2147483647 - 1);
+    return (int)x;
+}
+
+// getFishEyeUV (kernels.cu:340-376) with power 1.65: the fisheye branch.
+// 0.5 / prop is a double division narrowed to float (make_float2 of a double
+// expression); normalize is v * (1 / sqrt(dot(v, v))), so the centre pixel,
+// where d = 0, gets NaN coordinates as the reference's does.
+__device__ __forceinline__ void fisheye_uv(float px, float py, float fw, float fh, float* u, float* v) {
+    const float p_x = px / fw, p_y = py / fw;
+    const float prop = fw / fh;
+    const float m_x = 0.5f, m_y = (float)(0.5 / (double)prop);
+    const float d_x = p_x - m_x, d_y = p_y - m_y;
+    const float r = __builtin_sqrtf((d_x * d_x) + (d_y * d_y));
+    const float power = 1.65f;
+    const float bind = __builtin_sqrtf((m_x * m_x) + (m_y * m_y));
+    const float inv = 1.0f / __builtin_sqrtf((d_x * d_x) + (d_y * d_y));
+    const float n_x = d_x * inv, n_y = d_y * inv;
+    const float t = dm::tanf(r * power);
+    const float tb = dm::tanf(bind * power);
+    const float a_x = n_x * t, a_y = n_y * t;
+    const float b_x = a_x * bind, b_y = a_y * bind;
+    const float c_x = b_x / tb, c_y = b_y / tb;
+    *u = m_x + c_x;
+    *v = (m_y + c_y) * prop;
+}
+
+// computeEffects (kernels.cu:508-528): fisheye coordinates, then R, G and B
+// point-sampled (textureReadNearestUV: truncate, clamp to the edge) at the
+// coordinates plus, at, and minus a radial offset.
+__global__ __launch_bounds__(POST_BLOCK) void k_fx(FrameBuffers fb, PostBuffers pb, PostParams p) {
+    const int64_t npx = (int64_t)p.width * p.height;
+    const int64_t i = (int64_t)blockIdx.x * POST_BLOCK + threadIdx.x;
+    if (i >= npx) return;
+    const int W = p.width, H = p.height;
+    const int x = (int)(i % W), y = (int)(i / W);
+    const float fw = (float)W, fh = (float)H;
+    const float uc = ((float)x + 0.5f) / fw, vc = ((float)y + 0.5f) / fh;   // getShadingJob uv
+    float u, v;
+    fisheye_uv((float)x, (float)y, fw, fh, &u, &v);
+    const float o_x = dm::powf(__builtin_fabsf(uc - 0.5f) * 2.0f, 2.0f) * 0.01f;
+    const float o_y = dm::powf(__builtin_fabsf(vc - 0.5f) * 2.0f, 2.0f) * 0.01f;
+    auto tap = [&](float tu, float tv) {
+        int ix = f2i_sat(tu * fw), iy = f2i_sat(tv * fh);
+        ix = ix < 0 ? 0 : (ix > W - 1 ? W - 1 : ix);
+        iy = iy < 0 ? 0 : (iy > H - 1 ? H - 1 : iy);
+        return pb.back[(size_t)iy * W + ix];
+    };
+    const float r = tap(u + o_x, v + o_y).x;
+    const float g = tap(u, v).y;
+    const float b = tap(u - o_x, v - o_y).z;
+    // textureWriteUV at the pixel's own uv (zero boundary: out of range dropped)
+    const int wx = f2i_sat(uc * fw), wy = f2i_sat(vc * fh);
+    if (wx >= 0 && wx < W && wy >= 0 && wy < H) fb.output[(size_t)wy * W + wx] = make_float4(r, g, b, 1.0f);
 }
 
 }  // namespace
@@ -371,6 +433,7 @@ hipError_t launch_post_frame(hipStream_t s, const FrameBuffers& fb, const PostBu
         }
     }
     hipLaunchKernelGGL(k_post_final, dim3((unsigned)blocks), dim3(POST_BLOCK), 0, s, fb, pb, p);
+    if (p.effects) hipLaunchKernelGGL(k_fx, dim3((unsigned)blocks), dim3(POST_BLOCK), 0, s, fb, pb, p);
     return hipGetLastError();
 }
 

@@ -331,6 +331,17 @@ def histogram_average(bins, white_ev=6.5, black_ev=-10.0, last=0.0, dt_ms=16.0):
         _ptr(b), white_ev, black_ev, last, dt_ms)
 
 
+def post_effects(back):
+    """computeEffects (kernels.cu:508-528): fisheye + chromatic aberration of
+    an [H, W, 4] back buffer (row 0 = the buffer's first row); returns the
+    [H, W, 4] output."""
+    b = np.ascontiguousarray(np.asarray(back, np.float32))
+    H, W = b.shape[:2]
+    out = np.zeros_like(b)
+    _sig("orc_post_effects", None, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p])(_ptr(b), W, H, _ptr(out))
+    return out
+
+
 def film(rgb, scalar=1.0, white_ev=6.5, black_ev=-10.0):
     x = _f32(rgb).reshape(-1, 3)
     out = np.zeros_like(x)

@@ -117,6 +117,7 @@ void orc_histogram(const float* rgba, int32_t width, int32_t height, float white
                    uint64_t* bins);
 float orc_histogram_average(const uint64_t* bins, float white_ev, float black_ev, float last, float dt_ms);
 void orc_film(const float* rgb, int64_t n, float scalar, float white_ev, float black_ev, float* out);
+void orc_post_effects(const float* back, int32_t width, int32_t height, float* out);
 void orc_agx_curve(const float* x, int64_t n, float* out);
 void orc_srgb_encode(const float* x, int64_t n, float* out);
 void orc_yxy_roundtrip(const float* rgb, int64_t n, float scalar, float* out_yxy, float* out_rgb);

@@ -428,6 +428,60 @@ float orc_histogram_average(const uint64_t* bins, float white_ev, float black_ev
     return HistogramAverage(bins, white_ev, black_ev, last, dt_ms);
 }
 
+// computeEffects (kernels.cu:508-528) over a back buffer of W x H RGBA32F
+// pixels (row 0 first), with getFishEyeUV (kernels.cu:340-376, power 1.65):
+// fisheye coordinates, then R, G and B point-sampled at the coordinates plus,
+// at, and minus the radial offset powf(|uv - 0.5| * 2, 2) * 0.01.  Reads
+// follow textureReadNearestUV (device_texture_buffer.cuh:35-40): the float
+// coordinate times the size, converted to int as the device converts (toward
+// zero, saturating, NaN to 0), clamped to the edge; the write goes to
+// make_int2(uv * size) of the pixel's own uv, dropped when out of range.
+// Operation order as written there: 0.5 / prop in double narrowed to float,
+// normalize as v * (1 / sqrt(dot)) (rsqrtf, IEEE), tan from kpt_dmath.h.
+void orc_post_effects(const float* back, int32_t W, int32_t H, float* out) {
+    auto f2i = [](float x) -> int {
+        if (std::isnan(x)) return 0;
+        if (x >= 2147483648.0f) return 2147483647;
+        if (x <= -2147483648.0f) return (-2147483647 - 1);
+        return (int)x;
+    };
+    const float fw = (float)W, fh = (float)H;
+    auto tap = [&](float tu, float tv, int ch) {
+        int ix = f2i(tu * fw), iy = f2i(tv * fh);
+        ix = ix < 0 ? 0 : (ix > W - 1 ? W - 1 : ix);
+        iy = iy < 0 ? 0 : (iy > H - 1 ? H - 1 : iy);
+        return back[4 * ((size_t)iy * W + ix) + ch];
+    };
+    for (int y = 0; y < H; ++y) {
+        for (int x = 0; x < W; ++x) {
+            // getFishEyeUV
+            const float p_x = (float)x / fw, p_y = (float)y / fw;
+            const float prop = fw / fh;
+            const float m_x = 0.5f, m_y = (float)(0.5 / (double)prop);
+            const float d_x = p_x - m_x, d_y = p_y - m_y;
+            const float r = std::sqrt((d_x * d_x) + (d_y * d_y));
+            const float power = 1.65f;
+            const float bind = std::sqrt((m_x * m_x) + (m_y * m_y));
+            const float inv = 1.0f / std::sqrt((d_x * d_x) + (d_y * d_y));
+            const float n_x = d_x * inv, n_y = d_y * inv;
+            const float t = kpt::dm::tanf(r * power), tb = kpt::dm::tanf(bind * power);
+            const float u = m_x + (((n_x * t) * bind) / tb);
+            const float v = (m_y + (((n_y * t) * bind) / tb)) * prop;
+            // chromatic aberration
+            const float uc = ((float)x + 0.5f) / fw, vc = ((float)y + 0.5f) / fh;
+            const float o_x = kpt::dm::powf(std::fabs(uc - 0.5f) * 2.0f, 2.0f) * 0.01f;
+            const float o_y = kpt::dm::powf(std::fabs(vc - 0.5f) * 2.0f, 2.0f) * 0.01f;
+            const float rr = tap(u + o_x, v + o_y, 0);
+            const float gg = tap(u, v, 1);
+            const float bb = tap(u - o_x, v - o_y, 2);
+            const int wx = f2i(uc * fw), wy = f2i(vc * fh);
+            if (wx < 0 || wx >= W || wy < 0 || wy >= H) continue;
+            float* o = out + 4 * ((size_t)wy * W + wx);
+            o[0] = rr; o[1] = gg; o[2] = bb; o[3] = 1.0f;
+        }
+    }
+}
+
 void orc_film(const float* rgb, int64_t n, float scalar, float white_ev, float black_ev, float* out) {
     for (int64_t i = 0; i < n; ++i) {
         F3 o = PostPixel(F3{rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]}, nullptr, 0.0f, scalar, white_ev, black_ev);

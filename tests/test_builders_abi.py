@@ -45,6 +45,25 @@ def test_host_only_context_errors():
     c.close()
 
 
+def test_settings_validation_effects_and_upscale():
+    from common import settings
+    c = Context(-2)
+    st = settings()
+    st.postprocess_enable_effects = 1
+    c.set_settings(st)
+    assert c.settings().postprocess_enable_effects == 1
+    st.postprocess_enable_effects = 2
+    with pytest.raises(abi.KptError) as e:
+        c.set_settings(st)
+    assert e.value.code == abi.KPT_ERR_INVALID_ARG
+    st.postprocess_enable_effects = 0
+    st.upscale_enable = 1            # no upscaler: refused, never silently ignored
+    with pytest.raises(abi.KptError) as e:
+        c.set_settings(st)
+    assert e.value.code == abi.KPT_ERR_UNSUPPORTED
+    c.close()
+
+
 def test_host_only_set_exposure_matches_oracle():
     """Renderer::setExposure's film scalar (host side of the C-ABI) equals the
     oracle's getStandardOutputBasedExposure, bit for bit; defaults match the

@@ -68,3 +68,30 @@ def test_post_chain_odd_size_bloom_and_adaptation(plain_ctx):
     # 97x61: five mip levels with odd extents; slow adaptation (dt 1 ms) over 4 frames
     _run(sc, ctx, 97, 61, 4, 1, 1, 0, dt=1.0,
          exposure=abi.KptExposure.camera(aperture=4.0, iso=400, shutter=1 / 125, ev_comp=0.5))
+
+
+@pytest.mark.parametrize("W,H", [(160, 96), (97, 61), (64, 32)])
+def test_effects_pass_parity(plain_ctx, W, H):
+    """postprocess_enable_effects: the output is computeEffects of the back
+    buffer (oracle.post_effects), bit for bit; the back buffer is the
+    effects-off output of the same frame (64x32 has the centre pixel whose
+    fisheye coordinates are NaN)."""
+    sc, ctx = plain_ctx
+    outs = []
+    for eff in (0, 1):
+        st = settings(max_depth=3, integrator=abi.INTEGRATOR_WAVEFRONT, sun_theta=sc.sun_theta)
+        st.bloom_generate_bloom = 1
+        st.postprocess_enable_effects = eff
+        ctx.set_settings(st)
+        ctx.resize(W, H)
+        p, v = view_proj(sc, W, H)
+        ctx.set_view(p, v)
+        ctx.set_exposure(abi.KptExposure.camera(aperture=16.0, iso=100, shutter=1 / 1000))
+        ctx.set_frame_delta(16.0)
+        print(f"[fx] {W}x{H} effects={eff}", flush=True)
+        ctx.render(1)
+        outs.append(ctx.readback(4))
+    want = oracle.post_effects(outs[0])
+    diff = (outs[1].view(np.uint32) != want.view(np.uint32)).any(-1)
+    assert not diff.any(), f"{int(diff.sum())} pixels differ"
+    assert not np.array_equal(outs[1], outs[0])          # the pass changed the image

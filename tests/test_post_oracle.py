@@ -267,3 +267,68 @@ def test_post_frame_composition():
     oracle.exposure_program(e_chk, avg)
     assert (e3.iso, e3.shutter_speed_secs) == (e_chk.iso, e_chk.shutter_speed_secs)
     assert film3.luminance_exposure_scalar == oracle.set_exposure(e_chk).luminance_exposure_scalar
+
+
+# ---------------------------------------------------------------- effects pass
+# computeEffects (kernels.cu:508-528) with getFishEyeUV (kernels.cu:340-376):
+# pinned by what the mapping fixes, not by retyping it.
+
+def _ramp(H, W, along="x"):
+    x = np.arange(W, dtype=np.float32)[None, :].repeat(H, 0)
+    y = np.arange(H, dtype=np.float32)[:, None].repeat(W, 1)
+    v = x if along == "x" else y
+    return np.stack([v, v, v, np.ones_like(v)], -1).astype(np.float32)
+
+
+def test_effects_constant_image_is_unchanged():
+    b = np.full((37, 53, 4), 0.25, np.float32)
+    o = oracle.post_effects(b)
+    assert np.array_equal(o[..., :3], b[..., :3]) and (o[..., 3] == 1.0).all()
+
+
+@pytest.mark.parametrize("W,H", [(16, 8), (64, 48), (97, 61)])
+def test_effects_corner_maps_to_itself(W, H):
+    # at the corner pixel d = -m, |d| = bind, so tan(r*k)*bind/tan(bind*k) = bind
+    # and uv = m + d = (0, 0): the green tap reads the source corner
+    b = np.random.default_rng(1).random((H, W, 4)).astype(np.float32)
+    o = oracle.post_effects(b)
+    assert o[0, 0, 1] == b[0, 0, 1]
+
+
+def test_effects_centre_pixel_reads_the_origin():
+    # W = 2H: the centre pixel has p == m exactly, d = 0, normalize(0) is NaN,
+    # and a NaN coordinate converts to pixel 0 as the device converts it
+    H, W = 8, 16
+    b = np.random.default_rng(2).random((H, W, 4)).astype(np.float32)
+    o = oracle.post_effects(b)
+    assert np.array_equal(o[H // 2, W // 2, :3], b[0, 0, :3])
+
+
+@pytest.mark.parametrize("along", ["x", "y"])
+def test_effects_chromatic_order_on_a_ramp(along):
+    # R samples at uv + offset, B at uv - offset, offset >= 0: on an image that
+    # grows along x (or y) the channels come out ordered R >= G >= B, and the
+    # spread is widest at the image border, zero at the centre column/row
+    H, W = 160, 200
+    b = _ramp(H, W, along)
+    o = oracle.post_effects(b)
+    assert (o[..., 0] >= o[..., 1]).all() and (o[..., 1] >= o[..., 2]).all()
+    spread = o[..., 0] - o[..., 2]
+    # at the border the offset is ~0.01 of the size (2 px each way here)
+    if along == "x":
+        assert spread[:, :3].mean() >= 3.0 and spread[:, W // 2 - 1:W // 2 + 2].mean() <= 1.0
+    else:
+        assert spread[:3, :].mean() >= 2.5 and spread[H // 2 - 1:H // 2 + 2, :].mean() <= 1.0
+
+
+def test_effects_fisheye_is_radially_monotone():
+    # along the centre row the green source column grows with the pixel column
+    # (the mapping tan(r k) / tan(bind k) is increasing on [0, bind])
+    H, W = 48, 64
+    b = _ramp(H, W, "x")
+    o = oracle.post_effects(b)
+    g = o[H // 2 - 1, :, 1]            # (the centre row's centre pixel is the NaN case)
+    assert (np.diff(g) >= 0).all()
+    # the corner distance is the radius kept fixed: along the centre row the
+    # edges pull inward (source columns inside the frame)
+    assert g[0] > 0 and g[-1] < W - 1
