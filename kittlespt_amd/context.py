@@ -209,7 +209,8 @@ class Context:
         self._ck(self.L.kpt_last_timing(self.h, out))
         return list(out)
 
-    TUNING_KEYS = ("refill", "chunk", "refill_min", "blocks_per_cu", "sort", "graph", "ray_sort", "tail")
+    TUNING_KEYS = ("refill", "chunk", "refill_min", "blocks_per_cu", "sort", "graph", "ray_sort", "tail",
+                   "tlas_device")
 
     def tuning(self):
         n = len(self.TUNING_KEYS)

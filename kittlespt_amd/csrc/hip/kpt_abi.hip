@@ -81,6 +81,13 @@ struct kpt_ctx {
     kpt_trace_counters render_counts{};
     double timing[4] = {0, 0, 0, 0};
     kpt_stage_stats stages[2]{};         // closest-hit trace stage, path-per-lane tail
+    // TLAS rebuilds on the device (kpt_tlas.hip); the host tree is then stale
+    int32_t tlas_device = 0;
+    bool tlas_on_device = false;
+    float* d_inst_bounds = nullptr;
+    Node* d_tlas_ref = nullptr;          // the device build in the reference numbering
+    int32_t* d_tlas_status = nullptr;
+    float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {1, 1, 1};   // box of all instances (ray-sort keys)
     bool tlas_dirty = false;
     ViewConst* d_views = nullptr;    // multi-view batch cameras
     // post chain
@@ -183,27 +190,30 @@ bool make_blocks(const std::vector<Node>& nodes, IsInterior interior, std::vecto
     return true;
 }
 
-// The TLAS renumbered into the pair layout: root at 0, index 1 unused, the
-// children of every interior node at a fresh even pair (first child even),
-// interior a = that pair's index, leaf a = 0 (b = instance); boxes unchanged.
+// The TLAS renumbered into the pair layout: root at 0, index 1 unused; the
+// two children of the merged node with reference index i (i > N, the
+// builder's numbering: leaves 1..N, merged nodes from N + 1, node 0 a copy
+// of the root, which is the last merged node 2N - 1) at nodes 2(i - N) and
+// 2(i - N) + 1, first child even; an interior node's a = its children's first
+// index, a leaf's a = 0 (b = instance); boxes unchanged.  Each node's place
+// follows from its parent alone, so the device rebuild (kpt_tlas.hip) lays
+// the tree out identically in parallel.
 std::vector<Node> tlas_pair_layout(const std::vector<Node>& orig) {
     std::vector<Node> out;
     if (orig.empty()) return out;
-    out.push_back(orig[0]);
-    out.push_back(Node{});
-    std::vector<std::pair<uint32_t, uint32_t>> work{{0u, 0u}};   // (original index, new index)
-    for (size_t w = 0; w < work.size(); ++w) {
-        const Node src = orig[work[w].first];
-        if (src.a == 0u) { out[work[w].second] = src; continue; }
-        const uint32_t i1 = src.a & 0xFFFFu, i2 = (src.a >> 16) & 0xFFFFu;
-        const uint32_t p = (uint32_t)out.size();
-        out.push_back(orig[i1]);
-        out.push_back(orig[i2]);
-        Node n = src;
-        n.a = p;
-        out[work[w].second] = n;
-        work.push_back({i1, p});
-        work.push_back({i2, p + 1});
+    const uint32_t N = (uint32_t)(orig.size() / 2);
+    out.assign(orig.size(), Node{});
+    auto relink = [&](Node n, uint32_t self) {
+        if (n.a != 0u) n.a = 2u * (self - N);
+        return n;
+    };
+    Node root = orig[0];
+    if (root.a != 0u) root.a = 2u * (N - 1u);
+    out[0] = root;
+    for (uint32_t i = N + 1; i < (uint32_t)orig.size(); ++i) {
+        const uint32_t q = i - N, na = orig[i].a & 0xFFFFu, nb = orig[i].a >> 16;
+        out[2 * q] = relink(orig[na], na);
+        out[2 * q + 1] = relink(orig[nb], nb);
     }
     return out;
 }
@@ -232,9 +242,22 @@ void make_instance(kpt_ctx* c, int32_t i, const float* model, bool keep_prev) {
     c->inst_bounds[i] = transform_bounds(c->blas[g].bounds, model);
 }
 
+void set_scene_box(kpt_ctx* ctx) {
+    for (int k = 0; k < 3; ++k) { ctx->scene_lo[k] = 0.0f; ctx->scene_hi[k] = 1.0f; }
+    if (ctx->inst_bounds.empty()) return;
+    for (int k = 0; k < 3; ++k) { ctx->scene_lo[k] = ctx->inst_bounds[0].lo[k]; ctx->scene_hi[k] = ctx->inst_bounds[0].hi[k]; }
+    for (const Aabb& b : ctx->inst_bounds)
+        for (int k = 0; k < 3; ++k) {
+            ctx->scene_lo[k] = std::min(ctx->scene_lo[k], b.lo[k]);
+            ctx->scene_hi[k] = std::max(ctx->scene_hi[k], b.hi[k]);
+        }
+}
+
 int upload_tlas(kpt_ctx* ctx) {
     ctx->tlas = build_tlas(ctx->inst_bounds);
     ctx->tlas_dirty = false;
+    ctx->tlas_on_device = false;
+    set_scene_box(ctx);
     if (ctx->tlas.nodes.size() > 65535) return fail(ctx, KPT_ERR_UNSUPPORTED, "TLAS exceeds 65535 nodes (16-bit child fields)");
     if (ctx->host_only) return KPT_OK;
     const std::vector<Node> tp = tlas_pair_layout(ctx->tlas.nodes);
@@ -243,6 +266,10 @@ int upload_tlas(kpt_ctx* ctx) {
         return fail(ctx, KPT_ERR_UNSUPPORTED, "TLAS pair layout failed");
     // the previous frame may still read the old arrays
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    // the device rebuild's buffers follow the instance count of this tree
+    dev_free(&ctx->d_inst_bounds);
+    dev_free(&ctx->d_tlas_ref);
+    dev_free(&ctx->d_tlas_status);
     HIPCHK(dev_upload(&ctx->d_tlas, tp));
     HIPCHK(dev_upload(&ctx->d_tblocks, tb));
     HIPCHK(dev_upload(&ctx->d_itrav, ctx->itrav));
@@ -250,10 +277,59 @@ int upload_tlas(kpt_ctx* ctx) {
     return KPT_OK;
 }
 
+// The TLAS rebuilt on the device, stream-ordered behind the frames that read
+// the old one (no host synchronisation): instance boxes and records copied
+// up, the tree built, laid out and blocked by kpt_tlas.hip.  The host
+// builder runs instead for an empty or too large instance set and for boxes
+// whose half areas could overflow (the device scan assumes finite ones).
+int rebuild_tlas_device(kpt_ctx* ctx) {
+    const int32_t N = (int32_t)ctx->inst_bounds.size();
+    bool ok = ctx->tlas_device && !ctx->host_only && N >= 1 && N <= TLAS_DEVICE_MAX &&
+              ctx->d_tlas && ctx->d_tblocks && (int64_t)ctx->tlas.nodes.size() == 2 * (int64_t)N;
+    for (int32_t i = 0; ok && i < N; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const float lo = ctx->inst_bounds[i].lo[k], hi = ctx->inst_bounds[i].hi[k];
+            if (!(std::fabs(lo) <= 1e18f) || !(std::fabs(hi) <= 1e18f)) ok = false;
+        }
+    if (!ok) return upload_tlas(ctx);
+    if (!ctx->d_inst_bounds) {
+        HIPCHK(hipMalloc((void**)&ctx->d_inst_bounds, (size_t)N * 6 * sizeof(float)));
+        HIPCHK(hipMalloc((void**)&ctx->d_tlas_ref, (size_t)2 * N * sizeof(Node)));
+        HIPCHK(hipMalloc((void**)&ctx->d_tlas_status, sizeof(int32_t)));
+    }
+    static_assert(sizeof(Aabb) == 6 * sizeof(float), "Aabb is lo xyz, hi xyz");
+    HIPCHK(hipMemcpyAsync(ctx->d_inst_bounds, ctx->inst_bounds.data(), (size_t)N * sizeof(Aabb),
+                          hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_itrav, ctx->itrav.data(), ctx->itrav.size() * sizeof(InstTrav),
+                          hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_ishade, ctx->ishade.data(), ctx->ishade.size() * sizeof(InstShade),
+                          hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(launch_tlas_build(ctx->stream, ctx->d_inst_bounds, N, ctx->d_tlas_ref, ctx->d_tlas, ctx->d_tblocks,
+                             ctx->d_tlas_status));
+    ctx->tlas_dirty = false;
+    ctx->tlas_on_device = true;
+    set_scene_box(ctx);
+    return KPT_OK;
+}
+
 // Rebuilds the TLAS after instance edits, before the next use.
 int ensure_tlas(kpt_ctx* ctx) {
     if (!ctx->tlas_dirty) return KPT_OK;
-    return upload_tlas(ctx);
+    return rebuild_tlas_device(ctx);
+}
+
+// The host copy of a device-built TLAS (export and statistics).
+int fetch_tlas(kpt_ctx* ctx) {
+    if (!ctx->tlas_on_device) return KPT_OK;
+    const size_t n = ctx->tlas.nodes.size();
+    int32_t built = 0;
+    HIPCHK(hipMemcpyAsync(&built, ctx->d_tlas_status, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->tlas.nodes.data(), ctx->d_tlas_ref, n * sizeof(Node), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if ((size_t)built != n) return fail(ctx, KPT_ERR_HIP, "device TLAS build failed");
+    ctx->tlas_on_device = false;
+    return KPT_OK;
 }
 #define ENSURE_TLAS(c)                      \
     do {                                    \
@@ -278,8 +354,8 @@ FrameConst make_frame(const kpt_ctx* c, int32_t /*frame*/) {
     fc.prev_cam_pos[2] = c->cam.prev_inv_view[14];
     fc.temporal = c->settings.use_temporal_accumulation ? 1 : 0;
     for (int k = 0; k < 3; ++k) {
-        const float lo = c->tlas.nodes.empty() ? 0.0f : c->tlas.nodes[0].lo[k];
-        const float hi = c->tlas.nodes.empty() ? 1.0f : c->tlas.nodes[0].hi[k];
+        const float lo = c->scene_lo[k];
+        const float hi = c->scene_hi[k];
         fc.scene_lo[k] = lo;
         fc.scene_inv[k] = (hi > lo) ? 1.0f / (hi - lo) : 1.0f;
     }
@@ -586,6 +662,9 @@ int kpt_ctx_destroy(kpt_ctx* ctx) {
         }
         if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
         dev_free(&ctx->d_counts);
+        dev_free(&ctx->d_inst_bounds);
+        dev_free(&ctx->d_tlas_ref);
+        dev_free(&ctx->d_tlas_status);
         dev_free(&ctx->pb.expo);
         dev_free(&ctx->pb.hist);
         dev_free(&ctx->d_views);
@@ -868,6 +947,10 @@ int kpt_get_tlas(const kpt_ctx* ctx, void* nodes32, int64_t cap_nodes, int64_t* 
     if (!ctx || !nodes32 || !out_count) return KPT_ERR_INVALID_ARG;
     if (ctx->tlas_dirty) {
         int rc = ensure_tlas(const_cast<kpt_ctx*>(ctx));
+        if (rc != KPT_OK) return rc;
+    }
+    {
+        int rc = fetch_tlas(const_cast<kpt_ctx*>(ctx));
         if (rc != KPT_OK) return rc;
     }
     if ((int64_t)ctx->tlas.nodes.size() > cap_nodes) return KPT_ERR_INVALID_ARG;
@@ -1494,6 +1577,10 @@ int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n) {
         if (knobs[7] < 0 || knobs[7] > 64) return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
         ctx->tune.tail = knobs[7];
     }
+    if (n >= 9) {
+        if (knobs[8] < 0 || knobs[8] > 1) return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
+        ctx->tlas_device = knobs[8];
+    }
     return KPT_OK;
 }
 
@@ -1507,6 +1594,7 @@ int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n) {
     if (n >= 6) knobs[5] = ctx->use_graph;
     if (n >= 7) knobs[6] = ctx->tune.ray_sort;
     if (n >= 8) knobs[7] = ctx->tune.tail;
+    if (n >= 9) knobs[8] = ctx->tlas_device;
     return KPT_OK;
 }
 

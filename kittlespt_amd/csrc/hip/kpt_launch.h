@@ -89,6 +89,14 @@ hipError_t launch_post_frame(hipStream_t s, const FrameBuffers& fb, const PostBu
 // Mip levels of the bloom chain (mipchain.cuh:17-25, 44-50).
 int bloom_levels(int width, int height);
 
+// TLAS rebuild on the device (kpt_tlas.hip): the host builder's tree for
+// n <= TLAS_DEVICE_MAX instance boxes (6 floats each, finite, |v| <= 1e18), in
+// the reference numbering (ref_nodes, 2n entries), the pair layout
+// (pair_nodes, 2n) and the node blocks (n + 1).  status[0] = nodes built.
+constexpr int TLAS_DEVICE_MAX = 2048;
+hipError_t launch_tlas_build(hipStream_t s, const float* bounds, int32_t n, Node* ref_nodes, Node* pair_nodes,
+                             NodeBlock* blocks, int32_t* status);
+
 hipError_t launch_trace_closest(hipStream_t s, const SceneView& S, const kpt_ray* rays, kpt_hit* hits, int64_t n,
                                 int32_t* overflow, kpt_trace_counters* counters);
 hipError_t launch_trace_any(hipStream_t s, const SceneView& S, const kpt_ray* rays, int32_t* occluded, int64_t n,
