@@ -62,8 +62,19 @@ public:
     std::vector<float> readOutput() const { return read(4); }
     // the modulated linear render buffer (render_texture)
     std::vector<float> readRender() const { return read(0); }
-    // replaces getDebugRenderTargetTexture (renderer.cu:202-205): the V-buffer
-    std::vector<float> readDebug() const { return read(3); }
+    // replaces getDebugRenderTargetTexture (renderer.cu:202-205): the debug
+    // texture the path-trace kernel writes, (velocity.x, velocity.y, 0, 1)
+    // per pixel (kernels.cu:447), taken from the V-buffer's velocity
+    std::vector<float> readDebug() const {
+        std::vector<float> v = read(3);
+        for (size_t i = 0; i + 3 < v.size(); i += 4) {
+            v[i + 2] = 0.0f;
+            v[i + 3] = 1.0f;
+        }
+        return v;
+    }
+    // the V-buffer itself (velocity xy, primitive and instance ids as floats)
+    std::vector<float> readVBuffer() const { return read(3); }
 
     size_t getMaterialsCount() const { return m_materials.size(); }
     size_t getMeshCount() const { return m_models.size(); }

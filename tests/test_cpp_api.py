@@ -101,6 +101,8 @@ def test_cpp_renderer_matches_binding(tmp_path, which):
     finally:
         ctx.close()
     assert np.array_equal(got[0].view(np.uint32), img.view(np.uint32))
-    assert np.array_equal(got[1].view(np.uint32), vb.view(np.uint32))
+    dbg = vb.copy()
+    dbg[..., 2], dbg[..., 3] = 0.0, 1.0          # the debug texture: (velocity.xy, 0, 1)
+    assert np.array_equal(got[1].view(np.uint32), dbg.view(np.uint32))
     want = np.concatenate([x.reshape(-1) for x in views])
     assert batch.shape == want.shape and np.array_equal(batch.view(np.uint32), want.view(np.uint32))
