@@ -42,8 +42,15 @@ public:
     ~Renderer() { shutdown(); }
 
     // renderer.cu:55-101 (no upscaler: render and output resolutions agree)
+    // renderer.cu:55-101: nothing happens when the output size and the render
+    // size (output / 2.5 with upscale_enable) are already these
     void resizeResolution(uint32_t width, uint32_t height) {
-        if (width == m_width && height == m_height) return;
+        const bool up = m_rs.upscale_enable;
+        const int32_t rw = up ? (int32_t)((float)width / 2.5f) : (int32_t)width;
+        const int32_t rh = up ? (int32_t)((float)height / 2.5f) : (int32_t)height;
+        int32_t res[4] = {0, 0, 0, 0};
+        check(kpt_get_resolution(m_ctx, res), "kpt_get_resolution");
+        if ((int32_t)width == res[2] && (int32_t)height == res[3] && rw == res[0] && rh == res[1]) return;
         check(kpt_resize(m_ctx, (int32_t)width, (int32_t)height), "kpt_resize");
         m_width = width;
         m_height = height;
@@ -251,8 +258,12 @@ public:
     void setSeedMode(int mode) { m_settings.seed_mode = mode; push(); }
 
 private:
+    // buffers 0-3 at the render size, 4 (output) at the output size
     std::vector<float> read(int which) const {
-        std::vector<float> out((size_t)m_width * m_height * 4);
+        int32_t res[4] = {0, 0, 0, 0};
+        check(kpt_get_resolution(m_ctx, res), "kpt_get_resolution");
+        const size_t n = which == 4 ? (size_t)res[2] * res[3] : (size_t)res[0] * res[1];
+        std::vector<float> out(n * 4);
         check(kpt_readback(m_ctx, which, out.data(), (int64_t)(out.size() * sizeof(float))), "kpt_readback");
         return out;
     }

@@ -156,8 +156,9 @@ typedef struct kpt_settings {
      * (renderer.cu:172-178, kernels.cu:508-528); 0/1. */
     int32_t postprocess_enable_effects;
     /* RendererSettings::upscale_enable (pod_types.hpp:56): render at 1/2.5
-     * of the output size and upscale (renderer.cu:55-75, 138-160).  Not
-     * implemented: kpt_set_settings rejects 1 with KPT_ERR_UNSUPPORTED. */
+     * of the output size and upscale bilinearly (renderer.cu:55-75, 138-160;
+     * upscale.cuh:260-298 without USE_FSR); the render size
+     * takes effect at the next kpt_resize (the reference's resizeResolution). */
     int32_t upscale_enable;
     int32_t reserved;
 } kpt_settings;
@@ -280,8 +281,16 @@ int kpt_set_view(kpt_ctx* ctx, const float projection[16], const float view[16])
 /* Direct access to the camera inverses (current and previous). */
 int kpt_set_camera(kpt_ctx* ctx, const kpt_camera* cam);
 int kpt_get_camera(const kpt_ctx* ctx, kpt_camera* out);
-/* Renderer::resizeResolution (renderer.cu:55-101): (re)allocates frame buffers. */
+/* Renderer::resizeResolution (renderer.cu:55-101): the output size is
+ * width x height, the render size the same or (int)(width / 2.5f) x
+ * (int)(height / 2.5f) when upscale_enable is set; (re)allocates the frame
+ * buffers and, as the reference, recomputes the projection for the render
+ * size keeping its field of view (near 0.1, far 100). */
 int kpt_resize(kpt_ctx* ctx, int32_t width, int32_t height);
+/* Sizes after the last kpt_resize: out[0..1] render width/height (buffers 0-3
+ * of kpt_readback), out[2..3] output width/height (buffer 4).  No reference
+ * counterpart (the reference keeps them in ShaderData). */
+int kpt_get_resolution(const kpt_ctx* ctx, int32_t out[4]);
 /* Renderer::resetAccumulation (renderer.cu:336-340). */
 int kpt_reset_accumulation(kpt_ctx* ctx);
 /* Renders `spp` reference frames of one sample each (static accumulation

@@ -184,8 +184,17 @@ class Context:
         return c
 
     def resize(self, w, h):
+        """Output size w x h; the render size is w x h, or w/2.5 x h/2.5 when
+        the settings ask for upscaling (see resolution())."""
         self._ck(self.L.kpt_resize(self.h, w, h))
         self.width, self.height = w, h
+        self.render_width, self.render_height = self.resolution()[:2]
+
+    def resolution(self):
+        """(render width, render height, output width, output height)."""
+        r = (C.c_int32 * 4)()
+        self._ck(self.L.kpt_get_resolution(self.h, r))
+        return tuple(int(x) for x in r)
 
     def reset(self):
         self._ck(self.L.kpt_reset_accumulation(self.h))
@@ -199,8 +208,12 @@ class Context:
         return v.value
 
     def readback(self, which=0):
+        """0 render, 1 accumulation, 2 G-buffer, 3 V-buffer (render size),
+        4 output (output size)."""
         dt = np.uint32 if which == 2 else np.float32
-        out = np.zeros((self.height, self.width, 4), dt)
+        rw, rh, ow, oh = self.resolution()
+        w, h = (ow, oh) if which == 4 else (rw, rh)
+        out = np.zeros((h, w, 4), dt)
         self._ck(self.L.kpt_readback(self.h, which, out.ctypes.data_as(C.c_void_p), out.nbytes))
         return out
 

@@ -66,7 +66,8 @@ struct kpt_ctx {
     kpt_settings settings{};
     kpt_camera cam{};
     int32_t frame_index = 0;
-    int32_t width = 0, height = 0;
+    int32_t width = 0, height = 0;           // render size (the output size / 2.5 when upscaling)
+    int32_t out_width = 0, out_height = 0;   // output size (resizeResolution's arguments)
     FrameBuffers fb{};
     WavefrontState ws{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, tev0 = nullptr, tev1 = nullptr;
@@ -389,6 +390,7 @@ void free_frame(kpt_ctx* c) {
     dev_free(&c->fb.prev_gbuffer); dev_free(&c->fb.vbuffer); dev_free(&c->fb.output);
     for (int k = 0; k < MAX_MIPS; ++k) dev_free(&c->pb.mip[k]);
     dev_free(&c->pb.back);
+    dev_free(&c->pb.up);
     c->pb.levels = 0;
 }
 
@@ -1008,7 +1010,6 @@ int kpt_set_settings(kpt_ctx* ctx, const kpt_settings* s) {
         s->postprocess_enable_effects < 0 || s->postprocess_enable_effects > 1 || s->upscale_enable < 0 ||
         s->upscale_enable > 1)
         return fail(ctx, KPT_ERR_INVALID_ARG, "invalid settings");
-    if (s->upscale_enable) return fail(ctx, KPT_ERR_UNSUPPORTED, "upscale_enable: the upscaler is not implemented");
     ctx->settings = *s;
     return kpt_reset_accumulation(ctx);   // renderer.cu:301-305
 }
@@ -1119,35 +1120,65 @@ int kpt_get_camera(const kpt_ctx* ctx, kpt_camera* out) {
     return KPT_OK;
 }
 
+// Renderer::resizeResolution (renderer.cu:55-101): the output size is
+// width x height; the render size is the same, or int(width / 2.5) x
+// int(height / 2.5) when upscale_enable is set.  Render-size buffers: render,
+// accumulation, G-buffers, V-buffer; output-size: output, back buffer, the
+// upscaled render, the bloom mip chain (MipChain::resize, mipchain.cuh:17-35).
+// The projection is recomputed for the render size with its field of view
+// kept (2 atan(1 / P[1][1])), near 0.1 and far 100, as the reference does on
+// every resize (renderer.cu:78-84); a following kpt_set_view replaces it.
 int kpt_resize(kpt_ctx* ctx, int32_t width, int32_t height) {
     if (!ctx || width <= 0 || height <= 0 || (int64_t)width * height > (int64_t)1 << 28)
         return KPT_ERR_INVALID_ARG;
     if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
+    const bool up = ctx->settings.upscale_enable != 0;
+    const int32_t rw = up ? (int32_t)((float)width / 2.5f) : width;
+    const int32_t rh = up ? (int32_t)((float)height / 2.5f) : height;
+    if (rw <= 0 || rh <= 0) return fail(ctx, KPT_ERR_INVALID_ARG, "render size below one pixel");
     HIPCHK(hipStreamSynchronize(ctx->stream));
     free_frame(ctx);
-    size_t n = (size_t)width * height;
+    const size_t n = (size_t)rw * rh, no = (size_t)width * height;
     HIPCHK(hipMalloc((void**)&ctx->fb.accum, n * sizeof(float4)));
     HIPCHK(hipMalloc((void**)&ctx->fb.render, n * sizeof(float4)));
     HIPCHK(hipMalloc((void**)&ctx->fb.gbuffer, n * sizeof(uint4)));
     HIPCHK(hipMalloc((void**)&ctx->fb.prev_gbuffer, n * sizeof(uint4)));
     HIPCHK(hipMalloc((void**)&ctx->fb.vbuffer, n * sizeof(float4)));
-    HIPCHK(hipMalloc((void**)&ctx->fb.output, n * sizeof(float4)));
-    HIPCHK(hipMalloc((void**)&ctx->pb.back, n * sizeof(float4)));
-    // bloom mip chain at output resolution (MipChain::resize, mipchain.cuh:17-35)
+    HIPCHK(hipMalloc((void**)&ctx->fb.output, no * sizeof(float4)));
+    HIPCHK(hipMalloc((void**)&ctx->pb.back, no * sizeof(float4)));
+    HIPCHK(hipMalloc((void**)&ctx->pb.up, no * sizeof(float4)));
     ctx->pb.levels = bloom_levels(width, height);
     for (int k = 0; k <= ctx->pb.levels; ++k) {
         const size_t mn = (size_t)(width >> k) * (size_t)(height >> k);
         HIPCHK(hipMalloc((void**)&ctx->pb.mip[k], mn * sizeof(float4)));
         HIPCHK(launch_clear(ctx->stream, ctx->pb.mip[k], (int64_t)mn));
     }
-    ctx->width = width;
-    ctx->height = height;
+    ctx->width = rw;
+    ctx->height = rh;
+    ctx->out_width = width;
+    ctx->out_height = height;
     HIPCHK(launch_clear(ctx->stream, (float4*)ctx->fb.prev_gbuffer, (int64_t)n));
     HIPCHK(launch_clear(ctx->stream, ctx->fb.vbuffer, (int64_t)n));
     HIPCHK(launch_clear(ctx->stream, (float4*)ctx->fb.gbuffer, (int64_t)n));
     HIPCHK(launch_clear(ctx->stream, ctx->fb.render, (int64_t)n));
-    HIPCHK(launch_clear(ctx->stream, ctx->fb.output, (int64_t)n));
+    HIPCHK(launch_clear(ctx->stream, ctx->fb.output, (int64_t)no));
+    HIPCHK(launch_clear(ctx->stream, ctx->pb.up, (int64_t)no));
+    const hm::Mat4 old_proj = hm::ref_inverse(hm::load(ctx->cam.inv_proj));
+    const float fov = 2.0f * std::atan(1.0f / old_proj.m[5]);
+    if (std::isfinite(fov) && fov > 0.0f) {
+        const hm::Mat4 P = hm::perspective_fov_lh(fov, (float)rw, (float)rh, 0.1f, 100.0f);
+        std::memcpy(ctx->cam.inv_proj, hm::ref_inverse(P).m, 64);
+    }
     return kpt_reset_accumulation(ctx);
+}
+
+int kpt_get_resolution(const kpt_ctx* ctx, int32_t out[4]) {
+    if (!ctx || !out) return KPT_ERR_INVALID_ARG;
+    out[0] = ctx->width;
+    out[1] = ctx->height;
+    out[2] = ctx->out_width;
+    out[3] = ctx->out_height;
+    return KPT_OK;
 }
 
 int kpt_reset_accumulation(kpt_ctx* ctx) {
@@ -1271,6 +1302,10 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
             pp.bloom_internal_blend = ctx->settings.bloom_internal_blend;
             pp.frame_delta_ms = ctx->frame_delta_ms;
             pp.effects = ctx->settings.postprocess_enable_effects ? 1 : 0;
+            pp.out_width = ctx->out_width;
+            pp.out_height = ctx->out_height;
+            pp.upscale = (ctx->settings.upscale_enable || ctx->out_width != ctx->width ||
+                          ctx->out_height != ctx->height) ? 1 : 0;
             e = launch_post_frame(ctx->stream, ctx->fb, ctx->pb, pp, ctx->num_cus);
             if (e != hipSuccess) return e;
             // end of executeRendering: gbuffer -> prev_gbuffer (renderer.cu:186)
@@ -1538,7 +1573,8 @@ int kpt_readback(kpt_ctx* ctx, int32_t which, void* host_dst, int64_t bytes) {
         default: return fail(ctx, KPT_ERR_INVALID_ARG, "unknown buffer");
     }
     if (!src) return fail(ctx, KPT_ERR_STATE, "no frame buffers");
-    int64_t need = (int64_t)ctx->width * ctx->height * 16;
+    const int64_t need = (which == 4 ? (int64_t)ctx->out_width * ctx->out_height
+                                     : (int64_t)ctx->width * ctx->height) * 16;
     if (bytes < need) return fail(ctx, KPT_ERR_INVALID_ARG, "destination too small");
     HIPCHK(hipMemcpyAsync(host_dst, src, (size_t)need, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -72,6 +72,7 @@ struct PostBuffers {
     unsigned long long* hist;       // 256 fixed-point bins (weight * 2^23)
     ExposureState* expo;
     float4* back;                   // back buffer when the effects pass is on (output size)
+    float4* up;                     // the render buffer upscaled to the output size
 };
 struct PostParams {
     int32_t width, height;
@@ -81,6 +82,8 @@ struct PostParams {
     float bloom_final_blend, bloom_internal_blend;
     float frame_delta_ms;
     int32_t effects;                // fisheye + chromatic aberration from pb.back into fb.output
+    int32_t out_width, out_height;  // output size (width x height is the render size)
+    int32_t upscale;                // bilinear upscale render -> pb.up, which bloom and post read
 };
 // executeRendering after the path trace (renderer.cu:117-170): modulate,
 // auto exposure, bloom, post.  Everything stays on the stream.

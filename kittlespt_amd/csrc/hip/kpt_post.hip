@@ -198,7 +198,7 @@ __global__ __launch_bounds__(POST_BLOCK) void k_post_prepare(FrameBuffers fb, Po
         const float m2 = r.z * ((float)((a >> 16) & 0xFFu) / 255.0f);
         const float4 m = make_float4(m0, m1, m2, 1.0f);
         fb.render[i] = m;
-        if (p.bloom) pb.mip[0][i] = m;
+        if (p.bloom && !p.upscale) pb.mip[0][i] = m;
         if (p.auto_exposure) {
             const int px = (int)(i % p.width), py = (int)(i / p.width);
             const uint32_t b = lum_to_bin(luminance(m0, m1, m2), black, inv_range);
@@ -318,15 +318,34 @@ __global__ __launch_bounds__(POST_BLOCK) void k_bloom_up(Tex src, float4* dst, i
     dst[di] = make_float4(oc.x, oc.y, oc.z, 1.0f);
 }
 
+// ------------------------------------------------------------ upscale
+// upscale (upscale.cuh:260-298, the build without USE_FSR): output pixel
+// (x, y) reads the render buffer bilinearly (textureReadBilinear with alpha
+// filtered) at (x / out_width * width, y / out_height * height); the result
+// is the output texture that bloom (as its mip 0) and the post pass read.
+__global__ __launch_bounds__(POST_BLOCK) void k_upscale(FrameBuffers fb, PostBuffers pb, PostParams p) {
+    const int64_t npx = (int64_t)p.out_width * p.out_height;
+    const int64_t i = (int64_t)blockIdx.x * POST_BLOCK + threadIdx.x;
+    if (i >= npx) return;
+    const int x = (int)(i % p.out_width), y = (int)(i / p.out_width);
+    const float u = (float)x / (float)p.out_width, v = (float)y / (float)p.out_height;
+    const Tex src{fb.render, p.width, p.height};
+    const float4 c = st(src.bilinear(u * (float)p.width, v * (float)p.height));
+    pb.up[i] = c;
+    if (p.bloom) pb.mip[0][i] = c;
+}
+
 // ------------------------------------------------------------ post
-// computePostProcess (kernels.cu:475-504): bloom blend, exposure in Yxy, AgX.
+// computePostProcess (kernels.cu:475-504): bloom blend, exposure in Yxy, AgX,
+// at the output size over the output texture (the render buffer, or its
+// upscale).
 // Into the back buffer when the effects pass follows, else into the output.
 __global__ __launch_bounds__(POST_BLOCK) void k_post_final(FrameBuffers fb, PostBuffers pb, PostParams p) {
-    const int64_t npx = (int64_t)p.width * p.height;
+    const int64_t npx = (int64_t)p.out_width * p.out_height;
     const int64_t i = (int64_t)blockIdx.x * POST_BLOCK + threadIdx.x;
     if (i >= npx) return;
     const ExposureState* e = pb.expo;
-    const float4 r = fb.render[i];
+    const float4 r = p.upscale ? pb.up[i] : fb.render[i];
     c3 c = C3(r.x, r.y, r.z);
     if (p.bloom) {
         const float4 b = pb.mip[0][i];
@@ -377,10 +396,10 @@ __device__ __forceinline__ void fisheye_uv(float px, float py, float fw, float f
 // point-sampled (textureReadNearestUV: truncate, clamp to the edge) at the
 // coordinates plus, at, and minus a radial offset.
 __global__ __launch_bounds__(POST_BLOCK) void k_fx(FrameBuffers fb, PostBuffers pb, PostParams p) {
-    const int64_t npx = (int64_t)p.width * p.height;
+    const int64_t npx = (int64_t)p.out_width * p.out_height;
     const int64_t i = (int64_t)blockIdx.x * POST_BLOCK + threadIdx.x;
     if (i >= npx) return;
-    const int W = p.width, H = p.height;
+    const int W = p.out_width, H = p.out_height;
     const int x = (int)(i % W), y = (int)(i / W);
     const float fw = (float)W, fh = (float)H;
     const float uc = ((float)x + 0.5f) / fw, vc = ((float)y + 0.5f) / fh;   // getShadingJob uv
@@ -418,22 +437,27 @@ hipError_t launch_post_frame(hipStream_t s, const FrameBuffers& fb, const PostBu
     const unsigned pgrid = (unsigned)std::min<int64_t>(blocks, (int64_t)std::max(1, num_cus) * 8);
     hipLaunchKernelGGL(k_post_prepare, dim3(pgrid), dim3(POST_BLOCK), 0, s, fb, pb, p);
     if (p.auto_exposure) hipLaunchKernelGGL(k_auto_exposure, dim3(1), dim3(HIST_BINS), 0, s, pb, p.frame_delta_ms);
+    // from here on the output size: bloom, post and effects
+    const int64_t onpx = (int64_t)p.out_width * p.out_height;
+    const unsigned oblocks = (unsigned)((onpx + POST_BLOCK - 1) / POST_BLOCK);
+    if (p.upscale) hipLaunchKernelGGL(k_upscale, dim3(oblocks), dim3(POST_BLOCK), 0, s, fb, pb, p);
+    const int OW = p.out_width, OH = p.out_height;
     if (p.bloom) {
         for (int k = 0; k < pb.levels; ++k) {
-            const int sw = p.width >> k, sh = p.height >> k, dw = p.width >> (k + 1), dh = p.height >> (k + 1);
+            const int sw = OW >> k, sh = OH >> k, dw = OW >> (k + 1), dh = OH >> (k + 1);
             Tex src{pb.mip[k], sw, sh};
             hipLaunchKernelGGL(k_bloom_down, dim3((dw + 15) / 16, (dh + 15) / 16), dim3(POST_BLOCK), 0, s, src,
                                pb.mip[k + 1], dw, dh, (p.karis && k == 0) ? 1 : 0);
         }
         for (int k = pb.levels; k > 0; --k) {
-            const int sw = p.width >> k, sh = p.height >> k, dw = p.width >> (k - 1), dh = p.height >> (k - 1);
+            const int sw = OW >> k, sh = OH >> k, dw = OW >> (k - 1), dh = OH >> (k - 1);
             Tex src{pb.mip[k], sw, sh};
             hipLaunchKernelGGL(k_bloom_up, dim3((dw + 15) / 16, (dh + 15) / 16), dim3(POST_BLOCK), 0, s, src,
                                pb.mip[k - 1], dw, dh, p.bloom_internal_blend);
         }
     }
-    hipLaunchKernelGGL(k_post_final, dim3((unsigned)blocks), dim3(POST_BLOCK), 0, s, fb, pb, p);
-    if (p.effects) hipLaunchKernelGGL(k_fx, dim3((unsigned)blocks), dim3(POST_BLOCK), 0, s, fb, pb, p);
+    hipLaunchKernelGGL(k_post_final, dim3(oblocks), dim3(POST_BLOCK), 0, s, fb, pb, p);
+    if (p.effects) hipLaunchKernelGGL(k_fx, dim3(oblocks), dim3(POST_BLOCK), 0, s, fb, pb, p);
     return hipGetLastError();
 }
 

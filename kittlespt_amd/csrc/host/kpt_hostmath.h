@@ -18,6 +18,21 @@ inline Mat4 identity() {
 }
 inline Mat4 load(const float* p) { Mat4 r; std::memcpy(r.m, p, sizeof(r.m)); return r; }
 
+// glm::perspectiveFovLH with the [-1, 1] depth range (ext/matrix_clip_space.inl):
+// h = cos(fov/2) / sin(fov/2), w = h * height / width.
+inline Mat4 perspective_fov_lh(float fov, float width, float height, float znear, float zfar) {
+    Mat4 r;
+    std::memset(r.m, 0, sizeof(r.m));
+    const float h = std::cos(0.5f * fov) / std::sin(0.5f * fov);
+    const float w = h * height / width;
+    r.m[0] = w;
+    r.m[5] = h;
+    r.m[4 * 2 + 3] = 1.0f;
+    r.m[4 * 2 + 2] = (zfar + znear) / (zfar - znear);
+    r.m[4 * 3 + 2] = -(2.0f * zfar * znear) / (zfar - znear);
+    return r;
+}
+
 // Matrix4x4::inverse (matrix_maths.cuh:304-354): 2x2 sub-factors of the last
 // two columns, cofactor expansion along the first column, identity when
 // |det| < 1e-6.  Used for the traversal transform (blas.cuh:191) and for every

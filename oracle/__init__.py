@@ -268,24 +268,38 @@ def _sig(name, res, args):
     return f
 
 
-def post_frame(estimate, gbuffer, settings, exposure, film, frame_delta_ms=16.0):
-    """One frame of the post chain (modulate, auto exposure, bloom, post) over
-    the demodulated estimate [H,W,4] and gbuffer [H,W,4] u32.  exposure and
-    film (KptExposure / KptFilmState) are updated in place.  Returns
-    (render, output, bloom_mip0, histogram)."""
+def post_frame(estimate, gbuffer, settings, exposure, film, frame_delta_ms=16.0, out_size=None):
+    """One frame of the post chain (modulate, auto exposure, upscale, bloom,
+    post) over the demodulated estimate [H,W,4] and gbuffer [H,W,4] u32 at the
+    render size; out_size = (output width, output height) (default: the
+    render size; the upscale runs when it differs or settings.upscale_enable
+    is set).  exposure and film (KptExposure / KptFilmState) are updated in
+    place.  Returns (render [H,W,4], output, bloom_mip0, histogram), the last
+    two at the output size."""
     H, W = estimate.shape[:2]
+    OW, OH = out_size if out_size is not None else (W, H)
     est = _f32(estimate)
     gb = np.ascontiguousarray(gbuffer, np.uint32)
     render = np.zeros((H, W, 4), np.float32)
-    out = np.zeros((H, W, 4), np.float32)
-    bloom = np.zeros((H, W, 4), np.float32)
+    out = np.zeros((OH, OW, 4), np.float32)
+    bloom = np.zeros((OH, OW, 4), np.float32)
     hist = np.zeros(256, np.uint64)
-    f = _sig("orc_post_frame", None, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(abi.KptSettings),
-                                      C.c_float, C.POINTER(abi.KptExposure), C.POINTER(abi.KptFilmState)] +
+    f = _sig("orc_post_frame_sized", None, [C.c_void_p, C.c_void_p] + [C.c_int32] * 5 +
+             [C.POINTER(abi.KptSettings), C.c_float, C.POINTER(abi.KptExposure), C.POINTER(abi.KptFilmState)] +
              [C.c_void_p] * 4)
-    f(_ptr(est), _ptr(gb), W, H, C.byref(settings), frame_delta_ms, C.byref(exposure), C.byref(film),
-      _ptr(render), _ptr(out), _ptr(bloom), _ptr(hist))
+    f(_ptr(est), _ptr(gb), W, H, OW, OH, int(settings.upscale_enable), C.byref(settings), frame_delta_ms,
+      C.byref(exposure), C.byref(film), _ptr(render), _ptr(out), _ptr(bloom), _ptr(hist))
     return render, out, bloom, hist
+
+
+def upscale_bilinear(src, out_w, out_h):
+    """upscale (upscale.cuh:287-292) of an [H, W, 4] image to [out_h, out_w, 4]."""
+    s = _f32(src)
+    H, W = s.shape[:2]
+    out = np.zeros((out_h, out_w, 4), np.float32)
+    _sig("orc_upscale_bilinear", None, [C.c_void_p] + [C.c_int32] * 4 + [C.c_void_p])(
+        _ptr(s), W, H, out_w, out_h, _ptr(out))
+    return out
 
 
 def set_exposure(exposure):

@@ -118,6 +118,11 @@ void orc_histogram(const float* rgba, int32_t width, int32_t height, float white
 float orc_histogram_average(const uint64_t* bins, float white_ev, float black_ev, float last, float dt_ms);
 void orc_film(const float* rgb, int64_t n, float scalar, float white_ev, float black_ev, float* out);
 void orc_post_effects(const float* back, int32_t width, int32_t height, float* out);
+void orc_post_frame_sized(const float* estimate, const uint32_t* gbuffer, int32_t W, int32_t H, int32_t OW,
+                          int32_t OH, int32_t upscale, const kpt_settings* s, float frame_delta_ms,
+                          kpt_exposure* exposure, kpt_film_state* film, float* out_render, float* out_output,
+                          float* out_bloom, uint64_t* out_hist);
+void orc_upscale_bilinear(const float* src, int32_t W, int32_t H, int32_t OW, int32_t OH, float* out);
 void orc_agx_curve(const float* x, int64_t n, float* out);
 void orc_srgb_encode(const float* x, int64_t n, float* out);
 void orc_yxy_roundtrip(const float* rgb, int64_t n, float scalar, float* out_yxy, float* out_rgb);

@@ -368,9 +368,16 @@ using namespace orp;
 
 extern "C" {
 
-void orc_post_frame(const float* estimate, const uint32_t* gbuffer, int32_t W, int32_t H, const kpt_settings* s,
-                    float frame_delta_ms, kpt_exposure* exposure, kpt_film_state* film, float* out_render,
-                    float* out_output, float* out_bloom, uint64_t* out_hist) {
+// executeRendering's post chain with the render size W x H and the output
+// size OW x OH: modulation and auto exposure at the render size; the output
+// texture is the render buffer (same size, no upscale) or its bilinear upscale
+// (upscale.cuh:287-292: output pixel (x, y) reads textureReadBilinear at
+// (x / OW * W, y / OH * H), alpha filtered); bloom and the post pass at the
+// output size.  out_output and out_bloom hold OW x OH pixels.
+void orc_post_frame_sized(const float* estimate, const uint32_t* gbuffer, int32_t W, int32_t H, int32_t OW,
+                          int32_t OH, int32_t upscale, const kpt_settings* s, float frame_delta_ms,
+                          kpt_exposure* exposure, kpt_film_state* film, float* out_render, float* out_output,
+                          float* out_bloom, uint64_t* out_hist) {
     const size_t n = (size_t)W * H;
     std::vector<F4> render(n);
     // modulateSamples (kernels.cu:453-472): render *= 8-bit albedo (byteToFloat, packing.cuh:19-22)
@@ -390,21 +397,54 @@ void orc_post_frame(const float* estimate, const uint32_t* gbuffer, int32_t W, i
         ExposureProgram(exposure, film->scene_average_luminance);
         SetExposure(exposure, film);
     }
-    // output <- render (renderer.cu:149-156), bloom (:159-162), post (:164)
+    if (out_render) std::memcpy(out_render, render.data(), n * 16);
+    // output <- render, or its upscale (renderer.cu:138-156), bloom (:159-162), post (:164)
+    const bool up = upscale || OW != W || OH != H;
+    const size_t no = (size_t)OW * OH;
+    std::vector<F4> outtex(no);
+    if (up) {
+        Img src{render.data(), W, H};
+        for (int y = 0; y < OH; ++y)
+            for (int x = 0; x < OW; ++x) {
+                const float u = (float)x / (float)OW, v = (float)y / (float)OH;
+                outtex[(size_t)y * OW + x] = src.bilinear(u * (float)W, v * (float)H, true);
+            }
+    } else {
+        outtex = render;
+    }
     std::vector<std::vector<F4>> mips;
     const bool bloom = s->bloom_generate_bloom != 0;
-    if (bloom) Bloom(render.data(), W, H, s->bloom_use_karis_average != 0, s->bloom_internal_blend, mips);
-    for (size_t i = 0; i < n; ++i) {
-        F3 c{render[i].x, render[i].y, render[i].z};
+    if (bloom) Bloom(outtex.data(), OW, OH, s->bloom_use_karis_average != 0, s->bloom_internal_blend, mips);
+    for (size_t i = 0; i < no; ++i) {
+        F3 c{outtex[i].x, outtex[i].y, outtex[i].z};
         F3 b{0, 0, 0};
         if (bloom) b = F3{mips[0][i].x, mips[0][i].y, mips[0][i].z};
         F3 o = PostPixel(c, bloom ? &b : nullptr, s->bloom_final_blend, film->luminance_exposure_scalar,
                          film->white_point_ev, film->black_point_ev);
-        if (out_render) std::memcpy(out_render + 4 * i, &render[i], 16);
         float ov[4] = {o.x, o.y, o.z, 1.0f};
         if (out_output) std::memcpy(out_output + 4 * i, ov, 16);
         if (out_bloom && bloom) std::memcpy(out_bloom + 4 * i, &mips[0][i], 16);
     }
+}
+
+void orc_post_frame(const float* estimate, const uint32_t* gbuffer, int32_t W, int32_t H, const kpt_settings* s,
+                    float frame_delta_ms, kpt_exposure* exposure, kpt_film_state* film, float* out_render,
+                    float* out_output, float* out_bloom, uint64_t* out_hist) {
+    orc_post_frame_sized(estimate, gbuffer, W, H, W, H, 0, s, frame_delta_ms, exposure, film, out_render, out_output,
+                         out_bloom, out_hist);
+}
+
+// The upscale alone (upscale.cuh:287-292) of a W x H RGBA image to OW x OH.
+void orc_upscale_bilinear(const float* src, int32_t W, int32_t H, int32_t OW, int32_t OH, float* out) {
+    std::vector<F4> s4((size_t)W * H);
+    std::memcpy(s4.data(), src, s4.size() * 16);
+    Img img{s4.data(), W, H};
+    for (int y = 0; y < OH; ++y)
+        for (int x = 0; x < OW; ++x) {
+            const float u = (float)x / (float)OW, v = (float)y / (float)OH;
+            const F4 c = img.bilinear(u * (float)W, v * (float)H, true);
+            std::memcpy(out + 4 * ((size_t)y * OW + x), &c, 16);
+        }
 }
 
 void orc_set_exposure(const kpt_exposure* e, kpt_film_state* f) { SetExposure(e, f); }

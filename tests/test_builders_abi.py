@@ -57,10 +57,13 @@ def test_settings_validation_effects_and_upscale():
         c.set_settings(st)
     assert e.value.code == abi.KPT_ERR_INVALID_ARG
     st.postprocess_enable_effects = 0
-    st.upscale_enable = 1            # no upscaler: refused, never silently ignored
+    st.upscale_enable = 1
+    c.set_settings(st)
+    assert c.settings().upscale_enable == 1
+    st.upscale_enable = 3
     with pytest.raises(abi.KptError) as e:
         c.set_settings(st)
-    assert e.value.code == abi.KPT_ERR_UNSUPPORTED
+    assert e.value.code == abi.KPT_ERR_INVALID_ARG
     c.close()
 
 

@@ -95,3 +95,57 @@ def test_effects_pass_parity(plain_ctx, W, H):
     diff = (outs[1].view(np.uint32) != want.view(np.uint32)).any(-1)
     assert not diff.any(), f"{int(diff.sum())} pixels differ"
     assert not np.array_equal(outs[1], outs[0])          # the pass changed the image
+
+
+@pytest.mark.parametrize("ae,bloom", [(0, 0), (0, 1), (1, 1)])
+def test_upscale_frames_parity(plain_ctx, ae, bloom):
+    """upscale_enable: the path trace, modulation and histogram at
+    int(W/2.5) x int(H/2.5), the bilinear upscale to W x H, then bloom and the
+    post pass at the output size — against the oracle's post chain at the two
+    sizes, frame after frame."""
+    sc, ctx = plain_ctx
+    W, H = 160, 96
+    st = settings(max_depth=3, integrator=abi.INTEGRATOR_WAVEFRONT, sun_theta=sc.sun_theta)
+    st.enable_auto_exposure, st.bloom_generate_bloom = ae, bloom
+    st.upscale_enable = 1
+    ctx.set_settings(st)
+    ctx.resize(W, H)
+    rw, rh, ow, oh = ctx.resolution()
+    assert (rw, rh, ow, oh) == (int(np.float32(W) / np.float32(2.5)), int(np.float32(H) / np.float32(2.5)), W, H)
+    p, v = view_proj(sc, rw, rh)
+    ctx.set_view(p, v)
+    ctx.set_exposure(abi.KptExposure.camera(aperture=16.0, iso=100, shutter=1 / 1000))
+    ctx.set_frame_delta(16.0)
+    e, film = ctx.exposure(), ctx.film()
+    for f in range(3):
+        print(f"[upscale] ae={ae} bloom={bloom} frame {f}", flush=True)
+        ctx.render(1)
+        est = np.zeros((rh, rw, 4), np.float32)
+        est[..., :3] = ctx.estimate()
+        gb = ctx.readback(2)
+        r_o, out_o, _, _ = oracle.post_frame(est, gb, st, e, film, 16.0, out_size=(W, H))
+        r_g, out_g = ctx.readback(0), ctx.readback(4)
+        assert r_g.shape == (rh, rw, 4) and out_g.shape == (H, W, 4)
+        assert np.array_equal(r_g.view(np.uint32), r_o.view(np.uint32)), "modulated render differs"
+        diff = (out_g.view(np.uint32) != out_o.view(np.uint32)).any(-1)
+        assert not diff.any(), f"{int(diff.sum())} output pixels differ"
+    st.upscale_enable = 0
+    ctx.set_settings(st)
+
+
+def test_resize_recomputes_the_projection(plain_ctx):
+    """Renderer::resizeResolution (renderer.cu:78-84): the projection becomes
+    perspectiveFovLH(fov, render width, render height, 0.1, 100) with the
+    field of view of the one set before."""
+    sc, ctx = plain_ctx
+    st = settings(max_depth=3, integrator=abi.INTEGRATOR_WAVEFRONT, sun_theta=sc.sun_theta)
+    ctx.set_settings(st)
+    ctx.resize(64, 48)
+    fov = 0.9
+    p = scenes.perspective_fov_lh(fov, 64, 48, 1.0, 100.0)
+    _, v = view_proj(sc, 64, 48)
+    ctx.set_view(p, v)
+    ctx.resize(100, 40)
+    want = np.linalg.inv(scenes.perspective_fov_lh(fov, 100, 40, 0.1, 100.0).reshape(4, 4).astype(np.float64))
+    got = np.array(ctx.camera().inv_proj, np.float64).reshape(4, 4)
+    assert np.allclose(got, want, rtol=1e-5, atol=1e-6)

@@ -332,3 +332,46 @@ def test_effects_fisheye_is_radially_monotone():
     # the corner distance is the radius kept fixed: along the centre row the
     # edges pull inward (source columns inside the frame)
     assert g[0] > 0 and g[-1] < W - 1
+
+
+# ---------------------------------------------------------------- upscale
+# upscale (upscale.cuh:287-292): output (x, y) reads the source bilinearly at
+# (x / OW * W, y / OH * H).
+
+def test_upscale_constant_and_corner():
+    src = np.full((5, 7, 4), 0.5, np.float32)
+    assert np.array_equal(oracle.upscale_bilinear(src, 17, 12), np.full((12, 17, 4), 0.5, np.float32))
+    src = np.random.default_rng(3).random((5, 7, 4)).astype(np.float32)
+    out = oracle.upscale_bilinear(src, 17, 12)
+    assert np.array_equal(out[0, 0], src[0, 0])        # sample point (0, 0): weights 0
+
+
+def test_upscale_reproduces_a_linear_ramp():
+    # bilinear interpolation of a linear function is the function (up to
+    # rounding) until the right-hand tap is clamped to the last column
+    W, H, OW, OH = 8, 3, 20, 7
+    x = np.arange(W, dtype=np.float32)
+    src = np.stack([np.tile(x, (H, 1))] * 3 + [np.ones((H, W), np.float32)], -1)
+    out = oracle.upscale_bilinear(src, OW, OH)
+    s = np.arange(OW, dtype=np.float64) / OW * W
+    want = np.where(s <= W - 1, s, W - 1)
+    assert np.allclose(out[:, :, 0], want[None, :], atol=1e-5)
+    assert np.allclose(out[..., 3], 1.0)
+
+
+def test_post_frame_at_the_render_size_is_unchanged_by_the_sized_form():
+    # out_size equal to the render size and no upscale: the same frame as before
+    H, W = 12, 20
+    rng = np.random.default_rng(4)
+    est = rng.random((H, W, 4)).astype(np.float32)
+    gb = np.zeros((H, W, 4), np.uint32)
+    gb[..., 2] = rng.integers(0, 1 << 24, (H, W), dtype=np.uint32)
+    from common import settings
+    st = settings()
+    st.bloom_generate_bloom = 1
+    outs = []
+    for size in (None, (W, H)):
+        e = abi.KptExposure.camera(aperture=8.0, iso=100, shutter=1 / 250)
+        film = oracle.set_exposure(e)
+        outs.append(oracle.post_frame(est, gb, st, e, film, 16.0, out_size=size))
+    assert all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(outs[0][:3], outs[1][:3]))
