@@ -329,8 +329,9 @@ int kpt_set_trace_counting(kpt_ctx* ctx, int32_t enable);
  * radiance cache renders or trains keep the stage kernels), [8] (when n >= 9)
  * rebuild the TLAS after instance edits on the device (1: the same tree,
  * built and laid out on the stream without a host synchronisation; slower
- * than the host builder for 1,000 instances) or on the host (0, default).
- * n >= 5. */
+ * than the host builder for 1,000 instances), refit it on the device (2: the
+ * topology of the last build, boxes recomputed; not the reference's tree once
+ * instances move) or rebuild it on the host (0, default).  n >= 5. */
 int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n);
 int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n);
 int kpt_get_render_counters(const kpt_ctx* ctx, kpt_trace_counters* out);

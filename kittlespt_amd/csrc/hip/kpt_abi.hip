@@ -88,6 +88,8 @@ struct kpt_ctx {
     float* d_inst_bounds = nullptr;
     Node* d_tlas_ref = nullptr;          // the device build in the reference numbering
     int32_t* d_tlas_status = nullptr;
+    int32_t* d_tlas_parent = nullptr;    // refit: parent of every node of the last build (0 for the root)
+    int32_t* d_tlas_arrivals = nullptr;  // refit: per-node arrival counters
     float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {1, 1, 1};   // box of all instances (ray-sort keys)
     bool tlas_dirty = false;
     ViewConst* d_views = nullptr;    // multi-view batch cameras
@@ -271,10 +273,28 @@ int upload_tlas(kpt_ctx* ctx) {
     dev_free(&ctx->d_inst_bounds);
     dev_free(&ctx->d_tlas_ref);
     dev_free(&ctx->d_tlas_status);
+    dev_free(&ctx->d_tlas_parent);
+    dev_free(&ctx->d_tlas_arrivals);
     HIPCHK(dev_upload(&ctx->d_tlas, tp));
     HIPCHK(dev_upload(&ctx->d_tblocks, tb));
     HIPCHK(dev_upload(&ctx->d_itrav, ctx->itrav));
     HIPCHK(dev_upload(&ctx->d_ishade, ctx->ishade));
+    if (ctx->tlas_device == 2 && !ctx->tlas.nodes.empty()) {
+        // refit mode: the tree just built, with its parents, kept on the device
+        const size_t n2 = ctx->tlas.nodes.size();
+        std::vector<int32_t> parent(n2, 0);
+        for (size_t i = 1; i < n2; ++i) {
+            const uint32_t a = ctx->tlas.nodes[i].a;
+            if (a == 0u) continue;
+            parent[a & 0xFFFFu] = (int32_t)i;
+            parent[a >> 16] = (int32_t)i;
+        }
+        HIPCHK(hipMalloc((void**)&ctx->d_inst_bounds, (n2 / 2) * 6 * sizeof(float)));
+        HIPCHK(hipMalloc((void**)&ctx->d_tlas_status, sizeof(int32_t)));
+        HIPCHK(hipMalloc((void**)&ctx->d_tlas_arrivals, n2 * sizeof(int32_t)));
+        HIPCHK(dev_upload(&ctx->d_tlas_ref, ctx->tlas.nodes));
+        HIPCHK(dev_upload(&ctx->d_tlas_parent, parent));
+    }
     return KPT_OK;
 }
 
@@ -285,14 +305,31 @@ int upload_tlas(kpt_ctx* ctx) {
 // whose half areas could overflow (the device scan assumes finite ones).
 int rebuild_tlas_device(kpt_ctx* ctx) {
     const int32_t N = (int32_t)ctx->inst_bounds.size();
-    bool ok = ctx->tlas_device && !ctx->host_only && N >= 1 && N <= TLAS_DEVICE_MAX &&
-              ctx->d_tlas && ctx->d_tblocks && (int64_t)ctx->tlas.nodes.size() == 2 * (int64_t)N;
+    const bool refit = ctx->tlas_device == 2;
+    bool ok = ctx->tlas_device && !ctx->host_only && N >= 1 && (refit || N <= TLAS_DEVICE_MAX) &&
+              ctx->d_tlas && ctx->d_tblocks && (int64_t)ctx->tlas.nodes.size() == 2 * (int64_t)N &&
+              (!refit || (ctx->d_tlas_parent && ctx->d_tlas_ref));
     for (int32_t i = 0; ok && i < N; ++i)
         for (int k = 0; k < 3; ++k) {
             const float lo = ctx->inst_bounds[i].lo[k], hi = ctx->inst_bounds[i].hi[k];
             if (!(std::fabs(lo) <= 1e18f) || !(std::fabs(hi) <= 1e18f)) ok = false;
         }
     if (!ok) return upload_tlas(ctx);
+    if (refit) {
+        // the topology of the last host build, boxes from the current instances
+        HIPCHK(hipMemcpyAsync(ctx->d_inst_bounds, ctx->inst_bounds.data(), (size_t)N * sizeof(Aabb),
+                              hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->d_itrav, ctx->itrav.data(), ctx->itrav.size() * sizeof(InstTrav),
+                              hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->d_ishade, ctx->ishade.data(), ctx->ishade.size() * sizeof(InstShade),
+                              hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(launch_tlas_refit(ctx->stream, ctx->d_inst_bounds, N, ctx->d_tlas_ref, ctx->d_tlas_parent,
+                                 ctx->d_tlas_arrivals, ctx->d_tlas, ctx->d_tblocks, ctx->d_tlas_status));
+        ctx->tlas_dirty = false;
+        ctx->tlas_on_device = true;
+        set_scene_box(ctx);
+        return KPT_OK;
+    }
     if (!ctx->d_inst_bounds) {
         HIPCHK(hipMalloc((void**)&ctx->d_inst_bounds, (size_t)N * 6 * sizeof(float)));
         HIPCHK(hipMalloc((void**)&ctx->d_tlas_ref, (size_t)2 * N * sizeof(Node)));
@@ -667,6 +704,8 @@ int kpt_ctx_destroy(kpt_ctx* ctx) {
         dev_free(&ctx->d_inst_bounds);
         dev_free(&ctx->d_tlas_ref);
         dev_free(&ctx->d_tlas_status);
+        dev_free(&ctx->d_tlas_parent);
+        dev_free(&ctx->d_tlas_arrivals);
         dev_free(&ctx->pb.expo);
         dev_free(&ctx->pb.hist);
         dev_free(&ctx->d_views);
@@ -1614,7 +1653,8 @@ int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n) {
         ctx->tune.tail = knobs[7];
     }
     if (n >= 9) {
-        if (knobs[8] < 0 || knobs[8] > 1) return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
+        if (knobs[8] < 0 || knobs[8] > 2) return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
+        if (knobs[8] != ctx->tlas_device && ctx->scene_ready) ctx->tlas_dirty = true;   // rebuilt for the new mode
         ctx->tlas_device = knobs[8];
     }
     return KPT_OK;

@@ -99,6 +99,11 @@ int bloom_levels(int width, int height);
 constexpr int TLAS_DEVICE_MAX = 2048;
 hipError_t launch_tlas_build(hipStream_t s, const float* bounds, int32_t n, Node* ref_nodes, Node* pair_nodes,
                              NodeBlock* blocks, int32_t* status);
+// TLAS refit on the device: the topology of ref_nodes (the last build, whose
+// parent[] maps every node to its parent, 0 for the root), boxes recomputed
+// from the instance boxes; then the pair layout and blocks as above.
+hipError_t launch_tlas_refit(hipStream_t s, const float* bounds, int32_t n, Node* ref_nodes, const int32_t* parent,
+                             int32_t* arrivals, Node* pair_nodes, NodeBlock* blocks, int32_t* status);
 
 hipError_t launch_trace_closest(hipStream_t s, const SceneView& S, const kpt_ray* rays, kpt_hit* hits, int64_t n,
                                 int32_t* overflow, kpt_trace_counters* counters);

@@ -148,6 +148,11 @@ __global__ __launch_bounds__(TB_THREADS) void k_tlas_build(const float* bounds, 
     }
 }
 
+__global__ void k_tlas_fill(int32_t* p, int n) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = 0;
+}
+
 // Pair layout: the children of the merged node with reference index i (i > N)
 // go to the pair q = i - N, nodes 2q and 2q + 1; an interior node's a becomes
 // its own children's first index; the root (node 0) keeps its place; node 1
@@ -205,6 +210,73 @@ hipError_t launch_tlas_build(hipStream_t s, const float* bounds, int32_t n, Node
                              NodeBlock* blocks, int32_t* status) {
     if (n <= 0 || n > TLAS_DEVICE_MAX) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_tlas_build, dim3(1), dim3(TB_THREADS), 0, s, bounds, (int)n, ref_nodes, status);
+    const unsigned g1 = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_tlas_layout, dim3(g1), dim3(256), 0, s, ref_nodes, (int)n, pair_nodes, status);
+    const unsigned g2 = (unsigned)((n + 1 + 255) / 256);
+    hipLaunchKernelGGL(k_tlas_blocks, dim3(g2), dim3(256), 0, s, pair_nodes, (int)n, blocks, status);
+    return hipGetLastError();
+}
+
+}  // namespace kpt
+
+namespace kpt {
+namespace {
+
+// Refit: the topology of the last build kept, every box recomputed from the
+// instances' current boxes (leaves) up to the root, in parallel: one thread
+// per leaf walks up through the parents, and at each parent the second of
+// its two children to arrive (per-node arrival counters, cleared by the
+// caller) computes its box, as the build merged it: fminf/fmaxf of the first
+// child's box with the second's.  Node 0 (the root's copy) takes the root's
+// box at the end.  status[0] = 2N afterwards, so the layout kernels run.
+__global__ void k_tlas_refit(const float* bounds, int N, Node* nodes, const int32_t* parent, int32_t* arrivals,
+                             int32_t* status) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= N) return;
+    Node& leaf = nodes[1 + i];
+    for (int k = 0; k < 3; ++k) {
+        leaf.lo[k] = bounds[6 * i + k];
+        leaf.hi[k] = bounds[6 * i + 3 + k];
+    }
+    int cur = 1 + i;
+    for (;;) {
+        const int p = parent[cur];
+        if (p <= 0) break;                                // the root (or a single leaf)
+        __threadfence();                                  // this node's box before the arrival
+        if (atomicAdd(&arrivals[p], 1) == 0) return;      // the sibling finishes the parent
+        __threadfence();
+        // the children's boxes were written by other threads, maybe on other
+        // compute units: read them past the (non-coherent) L1
+        const volatile Node* vn = nodes;
+        const uint32_t a = vn[p].a;
+        const uint32_t na = a & 0xFFFFu, nb = a >> 16;
+        Node& n = nodes[p];
+        for (int k = 0; k < 3; ++k) {
+            n.lo[k] = fmin_host(vn[na].lo[k], vn[nb].lo[k]);
+            n.hi[k] = fmax_host(vn[na].hi[k], vn[nb].hi[k]);
+        }
+        cur = p;
+    }
+    // only the thread that completed the root gets here (or the single leaf)
+    __threadfence();
+    const volatile Node* vn = nodes;
+    Node root;
+    for (int k = 0; k < 3; ++k) { root.lo[k] = vn[cur].lo[k]; root.hi[k] = vn[cur].hi[k]; }
+    root.a = vn[cur].a;
+    root.b = vn[cur].b;
+    nodes[0] = root;
+    status[0] = 2 * N;
+}
+
+}  // namespace
+
+hipError_t launch_tlas_refit(hipStream_t s, const float* bounds, int32_t n, Node* ref_nodes, const int32_t* parent,
+                             int32_t* arrivals, Node* pair_nodes, NodeBlock* blocks, int32_t* status) {
+    if (n <= 0) return hipErrorInvalidValue;
+    const unsigned g = (unsigned)((2 * n + 255) / 256);
+    hipLaunchKernelGGL(k_tlas_fill, dim3(g), dim3(256), 0, s, arrivals, 2 * n);
+    hipLaunchKernelGGL(k_tlas_refit, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, bounds, (int)n, ref_nodes,
+                       parent, arrivals, status);
     const unsigned g1 = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_tlas_layout, dim3(g1), dim3(256), 0, s, ref_nodes, (int)n, pair_nodes, status);
     const unsigned g2 = (unsigned)((n + 1 + 255) / 256);

@@ -98,3 +98,56 @@ def test_device_and_host_rebuilds_render_the_same():
     for f in range(3):
         assert np.array_equal(imgs[0][f][0].view(np.uint32), imgs[1][f][0].view(np.uint32)), f"frame {f}"
         assert np.array_equal(imgs[0][f][1], imgs[1][f][1])
+
+
+def _union_ok(nodes):
+    """Every merged node's box is fminf/fmaxf of its children's (host C
+    library semantics: the second operand on ties)."""
+    lo, hi, lr = nodes["pmin"], nodes["pmax"], nodes["lr"]
+    for i in range(1, len(nodes)):
+        if lr[i] == 0:
+            continue
+        a, b = lr[i] & 0xFFFF, lr[i] >> 16
+        want_lo = np.where(lo[a] < lo[b], lo[a], lo[b])
+        want_hi = np.where(hi[a] > hi[b], hi[a], hi[b])
+        if not (np.array_equal(lo[i].view(np.uint32), want_lo.view(np.uint32)) and
+                np.array_equal(hi[i].view(np.uint32), want_hi.view(np.uint32))):
+            return False
+    return True
+
+
+def test_device_refit_keeps_topology_and_refits_boxes():
+    """tlas_device = 2: the topology of the last build, boxes recomputed from
+    the moved instances on the device; closest hits agree with the oracle's
+    rebuilt tree (they may differ only where a different tree order decides an
+    exact tie)."""
+    from common import random_rays
+    sc = scenes.scene_instanced(num_instances=1000, tex_size=8, tris_range=(60, 120), num_blas=8, num_materials=4)
+    ctx = Context(0)
+    ctx.upload(sc)
+    ctx.set_tuning(tlas_device=2)
+    orc = oracle.OracleScene(sc)
+    before = ctx.tlas()
+    rng = np.random.default_rng(21)
+    for rnd in range(2):
+        models = _moved(sc, 0, 100, rng)
+        ctx.set_instance_transforms(0, models)
+        for i in range(100):
+            orc.set_instance_transform(i, models[i])
+        print(f"[refit] round {rnd}", flush=True)
+        pt, ot = ctx.tlas(), orc.tlas()
+        assert np.array_equal(pt["lr"], before["lr"]) and np.array_equal(pt["blas"][1:], before["blas"][1:])
+        # leaves: the instances' current world boxes, as the rebuilt tree holds them
+        n = len(sc.instance_model)
+        assert np.array_equal(pt[1:n + 1].view(np.uint32), ot[1:n + 1].view(np.uint32))
+        assert _union_ok(pt)
+        assert np.array_equal(pt[0]["pmin"].view(np.uint32), pt[2 * n - 1]["pmin"].view(np.uint32))
+        rays = random_rays(100_000, (-10, -10, -10), (110, 110, 110), (50, 50, 50), 30.0, seed=31 + rnd)
+        g = ctx.trace_closest(rays)
+        o, _, _ = orc.trace(rays, oracle.MODE_CLOSEST)
+        same = (g["prim_id"] == o["prim_id"]) & (g["inst_id"] == o["inst_id"])
+        print(f"[refit] closest-hit agreement {same.mean():.6f}", flush=True)
+        assert same.mean() >= 0.9999
+        assert (o["prim_id"] >= 0).mean() > 0.05
+    ctx.close()
+    orc.close()
