@@ -5,6 +5,8 @@ textures, renderer.cu:197-205; here frames are read back and written out).
   tone-mapped and sRGB-encoded, values in [0, 1]) as 8-bit RGBA PNG;
 * `write_pfm(path, rgb)`: linear float data (`readback(0)`/`estimate()`) as a
   little-endian colour PFM;
+* `write_exr(path, rgba)`: the same linear data as an uncompressed scanline
+  OpenEXR (FLOAT A/B/G/R channels);
 * `read_png_rgba8(path)`: the inverse of `write_png` for 8-bit RGBA files it
   wrote (non-interlaced, filter type 0), used by the tests.
 
@@ -71,6 +73,50 @@ def read_png_rgba8(path: str) -> np.ndarray:
     if np.any(raw[:, 0] != 0):
         raise ValueError("only filter type 0 is read")
     return raw[:, 1:].reshape(h, w, 4).copy()
+
+
+def write_exr(path: str, rgba, flip_y: bool = True) -> None:
+    """Linear float data ([H, W, 3|4]) as an OpenEXR file: single part,
+    scanline, no compression, FLOAT channels A, B, G, R (stored in
+    alphabetical order, as the format requires), increasing-Y lines.  Rows
+    are flipped to the format's top-down order by default (the buffers keep
+    the GL bottom-up convention)."""
+    a = np.asarray(rgba, np.float32)
+    if a.ndim != 3 or a.shape[2] not in (3, 4):
+        raise ValueError(f"expected [H, W, 3|4], got {a.shape}")
+    if a.shape[2] == 3:
+        a = np.concatenate([a, np.ones(a.shape[:2] + (1,), np.float32)], axis=2)
+    if flip_y:
+        a = a[::-1]
+    h, w = a.shape[:2]
+
+    def attr(name, typ, data):
+        return name.encode() + b"\0" + typ.encode() + b"\0" + struct.pack("<i", len(data)) + data
+
+    chans = b""
+    for c in ("A", "B", "G", "R"):
+        chans += c.encode() + b"\0" + struct.pack("<iB3xii", 2, 0, 1, 1)   # FLOAT, pLinear 0, x/y sampling 1
+    chans += b"\0"
+    box = struct.pack("<iiii", 0, 0, w - 1, h - 1)
+    header = (struct.pack("<ii", 20000630, 2) +
+              attr("channels", "chlist", chans) +
+              attr("compression", "compression", b"\0") +
+              attr("dataWindow", "box2i", box) +
+              attr("displayWindow", "box2i", box) +
+              attr("lineOrder", "lineOrder", b"\0") +
+              attr("pixelAspectRatio", "float", struct.pack("<f", 1.0)) +
+              attr("screenWindowCenter", "v2f", struct.pack("<ff", 0.0, 0.0)) +
+              attr("screenWindowWidth", "float", struct.pack("<f", 1.0)) + b"\0")
+    line_bytes = 4 * w * 4
+    first = len(header) + 8 * h
+    offsets = struct.pack(f"<{h}Q", *[first + y * (8 + line_bytes) for y in range(h)])
+    planes = np.ascontiguousarray(a[..., [3, 2, 1, 0]].transpose(0, 2, 1), "<f4")   # [H, channel, W]
+    with open(path, "wb") as f:
+        f.write(header)
+        f.write(offsets)
+        for y in range(h):
+            f.write(struct.pack("<ii", y, line_bytes))
+            f.write(planes[y].tobytes())
 
 
 def write_pfm(path: str, rgb) -> None:

@@ -48,3 +48,51 @@ def test_pfm_header_and_rows(tmp_path):
     assert data.startswith(b"PF\n3 2\n-1.0\n")
     body = np.frombuffer(data[len(b"PF\n3 2\n-1.0\n"):], "<f4").reshape(2, 3, 3)
     assert np.array_equal(body, img[:, :, :3])
+
+
+def _read_exr(path):
+    """Minimal OpenEXR reader for single-part, uncompressed scanline files,
+    written from the file layout (magic, version, attributes, offset table,
+    lines of [y, size, channel planes in chlist order])."""
+    import struct
+    data = open(path, "rb").read()
+    assert struct.unpack("<ii", data[:8]) == (20000630, 2)
+    pos, attrs = 8, {}
+    while data[pos] != 0:
+        e = data.index(b"\0", pos)
+        name = data[pos:e].decode()
+        e2 = data.index(b"\0", e + 1)
+        typ = data[e + 1:e2].decode()
+        (n,) = struct.unpack("<i", data[e2 + 1:e2 + 5])
+        attrs[name] = (typ, data[e2 + 5:e2 + 5 + n])
+        pos = e2 + 5 + n
+    pos += 1
+    chl, names = attrs["channels"][1], []
+    q = 0
+    while chl[q] != 0:
+        e = chl.index(b"\0", q)
+        names.append(chl[q:e].decode())
+        assert struct.unpack("<i", chl[e + 1:e + 5])[0] == 2     # FLOAT
+        q = e + 1 + 16
+    assert attrs["compression"][1] == b"\0"
+    x0, y0, x1, y1 = struct.unpack("<iiii", attrs["dataWindow"][1])
+    w, h = x1 - x0 + 1, y1 - y0 + 1
+    offs = struct.unpack(f"<{h}Q", data[pos:pos + 8 * h])
+    img = np.zeros((h, w, len(names)), np.float32)
+    for y in range(h):
+        ly, size = struct.unpack("<ii", data[offs[y]:offs[y] + 8])
+        assert ly == y and size == 4 * w * len(names)
+        planes = np.frombuffer(data, "<f4", w * len(names), offs[y] + 8).reshape(len(names), w)
+        img[y] = planes.T
+    return names, img
+
+
+def test_exr_round_trip(tmp_path):
+    rng = np.random.default_rng(5)
+    rgba = (rng.random((6, 9, 4)) * 100.0 - 5.0).astype(np.float32)
+    p = tmp_path / "a.exr"
+    imageio.write_exr(str(p), rgba)
+    names, img = _read_exr(str(p))
+    assert names == ["A", "B", "G", "R"]
+    got = img[..., [3, 2, 1, 0]][::-1]                  # back to RGBA, bottom-up rows
+    assert np.array_equal(got.view(np.uint32), rgba.view(np.uint32))
