@@ -21,6 +21,7 @@ SOURCES = [
     os.path.join(CSRC, "hip", "kpt_cache.hip"),
     os.path.join(CSRC, "hip", "kpt_abi.hip"),
     os.path.join(CSRC, "hip", "kpt_tlas.hip"),
+    os.path.join(CSRC, "hip", "kpt_blas.hip"),
     os.path.join(CSRC, "host", "kpt_build.cpp"),
 ]
 

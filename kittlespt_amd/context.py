@@ -223,7 +223,7 @@ class Context:
         return list(out)
 
     TUNING_KEYS = ("refill", "chunk", "refill_min", "blocks_per_cu", "sort", "graph", "ray_sort", "tail",
-                   "tlas_device")
+                   "tlas_device", "blas_device")
 
     def tuning(self):
         n = len(self.TUNING_KEYS)

@@ -4,6 +4,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <cmath>
 #include <string>
 #include <thread>
 #include <vector>
@@ -82,6 +83,7 @@ struct kpt_ctx {
     kpt_trace_counters render_counts{};
     double timing[4] = {0, 0, 0, 0};
     kpt_stage_stats stages[2]{};         // closest-hit trace stage, path-per-lane tail
+    int32_t blas_device = 0;             // build the BLASes of the next upload on the device (kpt_blas.hip)
     // TLAS rebuilds on the device (kpt_tlas.hip); the host tree is then stale
     int32_t tlas_device = 0;
     bool tlas_on_device = false;
@@ -793,9 +795,32 @@ int kpt_upload_scene(kpt_ctx* ctx, const kpt_scene_desc* d) {
         t.channels = (int32_t)d->textures[i].channels; t.bit_depth = 8; t.offset = (int64_t)d->textures[i].offset;
     }
     ctx->num_pixel_bytes = (int64_t)d->num_pixel_bytes;
-    // BLAS per mesh, built in parallel
+    // BLAS per mesh: host threads, or all meshes at once on the device
     ctx->blas.assign(d->num_meshes, BlasTree());
-    {
+    if (ctx->blas_device && T > 0) {
+        if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "device BLAS build on a host-only context");
+        for (size_t i = 0; i < ctx->pos.size(); ++i)
+            if (!std::isfinite(ctx->pos[i]))
+                return fail(ctx, KPT_ERR_INVALID_ARG, "device BLAS build needs finite vertex positions");
+        std::vector<Node> dn;
+        std::vector<int32_t> dorder, dused, ddepth;
+        hipError_t be = device_build_blas(ctx->stream, ctx->pos.data(), ctx->mesh_first.data(), ctx->mesh_count.data(),
+                                          d->num_meshes, T, &dn, &dorder, &dused, &ddepth);
+        if (be != hipSuccess) return fail(ctx, KPT_ERR_HIP, std::string("device BLAS build: ") + hipGetErrorString(be));
+        size_t base = 0;
+        for (int32_t m = 0; m < d->num_meshes; ++m) {
+            const int32_t cnt = ctx->mesh_count[m];
+            if (cnt <= 0) continue;
+            BlasTree& bt = ctx->blas[m];
+            const size_t cap = 2 * (size_t)cnt - 1;
+            bt.nodes.assign(dn.begin() + base, dn.begin() + base + cap);
+            bt.order.assign(dorder.begin() + ctx->mesh_first[m], dorder.begin() + ctx->mesh_first[m] + cnt);
+            bt.used = dused[m];
+            bt.max_depth = ddepth[m];
+            for (int k = 0; k < 3; ++k) { bt.bounds.lo[k] = bt.nodes[0].lo[k]; bt.bounds.hi[k] = bt.nodes[0].hi[k]; }
+            base += cap;
+        }
+    } else {
         std::vector<std::thread> pool;
         const float* P = ctx->pos.data();
         for (int32_t m = 0; m < d->num_meshes; ++m)
@@ -1657,6 +1682,10 @@ int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n) {
         if (knobs[8] != ctx->tlas_device && ctx->scene_ready) ctx->tlas_dirty = true;   // rebuilt for the new mode
         ctx->tlas_device = knobs[8];
     }
+    if (n >= 10) {
+        if (knobs[9] < 0 || knobs[9] > 1) return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
+        ctx->blas_device = knobs[9];
+    }
     return KPT_OK;
 }
 
@@ -1671,6 +1700,7 @@ int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n) {
     if (n >= 7) knobs[6] = ctx->tune.ray_sort;
     if (n >= 8) knobs[7] = ctx->tune.tail;
     if (n >= 9) knobs[8] = ctx->tlas_device;
+    if (n >= 10) knobs[9] = ctx->blas_device;
     return KPT_OK;
 }
 

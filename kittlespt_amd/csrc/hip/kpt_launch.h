@@ -1,6 +1,7 @@
 // kpt_launch.h — host-side launch wrappers of the gfx950 kernels (defined in
 // kpt_kernels.hip, called by the C-ABI implementation).
 #pragma once
+#include <vector>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -104,6 +105,15 @@ hipError_t launch_tlas_build(hipStream_t s, const float* bounds, int32_t n, Node
 // from the instance boxes; then the pair layout and blocks as above.
 hipError_t launch_tlas_refit(hipStream_t s, const float* bounds, int32_t n, Node* ref_nodes, const int32_t* parent,
                              int32_t* arrivals, Node* pair_nodes, NodeBlock* blocks, int32_t* status);
+
+// BLASes of all meshes built on the device, node for node the host
+// builder's trees (kpt_blas.hip).  nodes: sum over meshes of 2*count-1
+// entries (mesh m's slice first, unused tail zeroed, local indices as
+// BlasTree); order: slot -> global triangle for all T slots; used and
+// max_depth per mesh.  Blocks the host until the trees are read back.
+hipError_t device_build_blas(hipStream_t st, const float* pos_host, const int32_t* mesh_first, const int32_t* mesh_count,
+                             int32_t M, int32_t T, std::vector<Node>* nodes, std::vector<int32_t>* order,
+                             std::vector<int32_t>* used, std::vector<int32_t>* max_depth);
 
 hipError_t launch_trace_closest(hipStream_t s, const SceneView& S, const kpt_ray* rays, kpt_hit* hits, int64_t n,
                                 int32_t* overflow, kpt_trace_counters* counters);

@@ -45,6 +45,21 @@ def test_host_only_context_errors():
     c.close()
 
 
+def test_blas_device_knob():
+    c = Context(-2)
+    assert c.tuning()["blas_device"] == 0
+    with pytest.raises(abi.KptError) as e:
+        c.set_tuning(blas_device=2)
+    assert e.value.code == abi.KPT_ERR_INVALID_ARG
+    c.set_tuning(blas_device=1)
+    with pytest.raises(abi.KptError) as e:
+        c.upload(scenes.scene_plain(subdiv=1))
+    assert e.value.code == abi.KPT_ERR_NO_DEVICE
+    c.set_tuning(blas_device=0)
+    c.upload(scenes.scene_plain(subdiv=1))
+    c.close()
+
+
 def test_settings_validation_effects_and_upscale():
     from common import settings
     c = Context(-2)
