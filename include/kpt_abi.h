@@ -309,6 +309,11 @@ int kpt_sync(kpt_ctx* ctx);
  * of its dominant closest-hit trace kernel launches. out[0]=render ms,
  * out[1]=trace ms summed, out[2]=trace launches. */
 int kpt_last_timing(const kpt_ctx* ctx, double out[4]);
+/* Wall milliseconds of the last kpt_upload_scene's BLAS builds (out[0]) and
+ * whether they ran on the device (out[1] = 1) or on host threads (0).  The
+ * reference builds on the host inside loadScene (renderer.cu:356-436) and
+ * reports no timing. */
+int kpt_last_build_timing(const kpt_ctx* ctx, double out[2]);
 /* Measurement mode: when enabled, the wavefront's closest-hit launches count
  * traversal events; kpt_get_render_counters returns the totals of the last
  * kpt_render.  No reference counterpart (the reference keeps per-ray

@@ -255,6 +255,7 @@ SIGNATURES = [
     ("kpt_readback", C.c_int, [_P, C.c_int32, _P, C.c_int64]),
     ("kpt_sync", C.c_int, [_P]),
     ("kpt_last_timing", C.c_int, [_P, C.POINTER(C.c_double)]),
+    ("kpt_last_build_timing", C.c_int, [_P, C.POINTER(C.c_double)]),
     ("kpt_set_trace_counting", C.c_int, [_P, C.c_int32]),
     ("kpt_get_render_counters", C.c_int, [_P, C.POINTER(KptTraceCounters)]),
     ("kpt_get_stage_stats", C.c_int, [_P, C.c_int32, C.POINTER(KptStageStats)]),

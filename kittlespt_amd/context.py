@@ -222,6 +222,12 @@ class Context:
         self._ck(self.L.kpt_last_timing(self.h, out))
         return list(out)
 
+    def build_timing(self):
+        """(ms, on_device) of the last upload's BLAS builds."""
+        out = (C.c_double * 2)()
+        self._ck(self.L.kpt_last_build_timing(self.h, out))
+        return out[0], bool(out[1])
+
     TUNING_KEYS = ("refill", "chunk", "refill_min", "blocks_per_cu", "sort", "graph", "ray_sort", "tail",
                    "tlas_device", "blas_device")
 

@@ -4,6 +4,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <chrono>
 #include <cmath>
 #include <string>
 #include <thread>
@@ -84,6 +85,7 @@ struct kpt_ctx {
     double timing[4] = {0, 0, 0, 0};
     kpt_stage_stats stages[2]{};         // closest-hit trace stage, path-per-lane tail
     int32_t blas_device = 0;             // build the BLASes of the next upload on the device (kpt_blas.hip)
+    double blas_build_ms[2] = {0, 0};    // last upload: BLAS build wall ms, 1 if built on the device
     // TLAS rebuilds on the device (kpt_tlas.hip); the host tree is then stale
     int32_t tlas_device = 0;
     bool tlas_on_device = false;
@@ -797,6 +799,7 @@ int kpt_upload_scene(kpt_ctx* ctx, const kpt_scene_desc* d) {
     ctx->num_pixel_bytes = (int64_t)d->num_pixel_bytes;
     // BLAS per mesh: host threads, or all meshes at once on the device
     ctx->blas.assign(d->num_meshes, BlasTree());
+    const auto build_t0 = std::chrono::steady_clock::now();
     if (ctx->blas_device && T > 0) {
         if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "device BLAS build on a host-only context");
         for (size_t i = 0; i < ctx->pos.size(); ++i)
@@ -827,6 +830,8 @@ int kpt_upload_scene(kpt_ctx* ctx, const kpt_scene_desc* d) {
             pool.emplace_back([ctx, P, m]() { ctx->blas[m] = build_blas(P, ctx->mesh_first[m], ctx->mesh_count[m]); });
         for (auto& th : pool) th.join();
     }
+    ctx->blas_build_ms[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - build_t0).count();
+    ctx->blas_build_ms[1] = (ctx->blas_device && T > 0) ? 1.0 : 0.0;
     // global node array: each root at an odd index so every sibling pair
     // starts at an even index (one 64-byte line)
     ctx->nodes.clear();
@@ -1725,6 +1730,13 @@ int kpt_get_stage_stats(const kpt_ctx* ctx, int32_t stage, kpt_stage_stats* out)
 int kpt_last_timing(const kpt_ctx* ctx, double out[4]) {
     if (!ctx || !out) return KPT_ERR_INVALID_ARG;
     for (int i = 0; i < 4; ++i) out[i] = ctx->timing[i];
+    return KPT_OK;
+}
+
+int kpt_last_build_timing(const kpt_ctx* ctx, double out[2]) {
+    if (!ctx || !out) return KPT_ERR_INVALID_ARG;
+    out[0] = ctx->blas_build_ms[0];
+    out[1] = ctx->blas_build_ms[1];
     return KPT_OK;
 }
 

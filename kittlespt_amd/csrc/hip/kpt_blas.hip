@@ -76,8 +76,9 @@ struct BB {
     uint32_t* acb;           // 6 per entry: centroid min keys, max keys
     uint32_t* abin;          // BIN_WORDS per entry
     int32_t* aaxis; float* asplit; int32_t* aleft;
-    int32_t* counters;       // [0] nodes allocated, [1] next level's split-list length, [2] error
-    int32_t node_cap, act_cap;
+    int32_t* counters;       // [0] nodes allocated, [1] next level's split-list length, [2] error, [3] tasks
+    int32_t node_cap, act_cap, task_cap;
+    int2* task;              // this level's (entry, chunk) tasks
     int32_t* mesh_depth;
     const int32_t* mesh_first; int32_t M;
 };
@@ -96,6 +97,7 @@ __global__ __launch_bounds__(BT) void k_bb_init(BB b, const int32_t* root_act) {
 
 __global__ __launch_bounds__(BT) void k_bb_level_init(BB b, int32_t nact) {
     const int32_t i = blockIdx.x * BT + threadIdx.x;
+    if (i == 0) { b.counters[1] = 0; b.counters[3] = 0; }
     if (i < nact * 6) b.acb[i] = (i % 6) < 3 ? okey(INFINITY) : okey(-INFINITY);
     if (i < nact * BIN_WORDS) {
         const int w = i % BIN_WORDS;
@@ -103,32 +105,65 @@ __global__ __launch_bounds__(BT) void k_bb_level_init(BB b, int32_t nact) {
     }
 }
 
-// Node keys of the slots' current node target; mode 0: the roots (target =
-// the slot's mesh), mode 1: the children of this level's splits.
-__device__ __forceinline__ int32_t box_target(const BB& b, int32_t s, int mode) {
-    if (mode == 0) {
-        int lo = 0, hi = b.M;
-        while (hi - lo > 1) { int mid = (lo + hi) >> 1; if (b.mesh_first[mid] <= s) lo = mid; else hi = mid; }
-        return lo;
-    }
-    const int32_t a = b.slot_act[s];
-    if (a < 0) return -1;
-    const int32_t n = b.act[a], c = b.nchild[n];
-    if (c < 0) return -1;
-    return c + ((s - b.nfirst[n]) >= b.aleft[a] ? 1 : 0);
+// Work of the reductions comes in tasks of up to TASK consecutive slots of
+// one node, one wave each: per-lane partials, a wave reduction, then one
+// atomic per value and task (a few hundred per address at the top levels
+// instead of one per lane).
+constexpr int TW = 64;
+constexpr int TASK = 512;
+
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) { const uint32_t w = (uint32_t)__shfl_xor((int)v, o); v = w < v ? w : v; }
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) { const uint32_t w = (uint32_t)__shfl_xor((int)v, o); v = w > v ? w : v; }
+    return v;
+}
+__device__ __forceinline__ unsigned long long shfl_xor64(unsigned long long v, int o) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, o), hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), o);
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ unsigned long long wave_min64(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) { const unsigned long long w = shfl_xor64(v, o); v = w < v ? w : v; }
+    return v;
+}
+__device__ __forceinline__ unsigned long long wave_max64(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) { const unsigned long long w = shfl_xor64(v, o); v = w > v ? w : v; }
+    return v;
 }
 
-__global__ __launch_bounds__(BT) void k_bb_box(BB b, int mode) {
-    __shared__ unsigned long long sk[6];
-    __shared__ int32_t st0;
-    const int32_t s = blockIdx.x * BT + threadIdx.x;
-    int32_t t = s < b.T ? box_target(b, s, mode) : -1;
-    if (threadIdx.x == 0) st0 = t;
-    if (threadIdx.x < 6) sk[threadIdx.x] = threadIdx.x < 3 ? ~0ull : 0ull;
-    __syncthreads();
-    const bool uni = __syncthreads_and(t >= 0 && t == st0);
-    unsigned long long lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0ull, 0ull, 0ull};
-    if (t >= 0) {
+// Tasks of this level's split list: (entry, chunk) pairs.
+__global__ __launch_bounds__(BT) void k_bb_tasks(BB b, int32_t nact) {
+    const int32_t a = blockIdx.x * BT + threadIdx.x;
+    if (a >= nact) return;
+    const int32_t count = b.ncount[b.act[a]], nt = (count + TASK - 1) / TASK;
+    const int32_t off = atomicAdd(&b.counters[3], nt);
+    if (off + nt > b.task_cap) { b.counters[2] = 1; return; }
+    for (int32_t j = 0; j < nt; ++j) b.task[off + j] = make_int2(a, j);
+}
+
+// Node boxes.  mode 0: tasks (mesh, chunk) over the roots; mode 1: tasks
+// (entry, chunk) over this level's splits, boxes of their two children, and
+// each slot's entry in the next level's split list.
+__global__ __launch_bounds__(TW) void k_bb_box(BB b, int mode, const int2* tasks, int32_t ntasks) {
+    const int32_t t = blockIdx.x;
+    if (t >= (mode == 0 ? ntasks : b.counters[3])) return;
+    const int2 tk = tasks[t];
+    const int32_t n = mode == 0 ? tk.x : b.act[tk.x];
+    const int32_t first = b.nfirst[n], count = b.ncount[n];
+    const int32_t c = mode == 0 ? n : b.nchild[n];
+    const int32_t L = mode == 0 ? count : b.aleft[tk.x];
+    const int32_t beg = tk.y * TASK, end = min(count, beg + TASK);
+    unsigned long long lo[2][3], hi[2][3];
+    for (int h = 0; h < 2; ++h) for (int k = 0; k < 3; ++k) { lo[h][k] = ~0ull; hi[h][k] = 0ull; }
+    for (int32_t r = beg + (int32_t)threadIdx.x; r < end; r += TW) {
+        const int32_t s = first + r;
+        if (mode == 1) {
+            if (c < 0) { b.slot_act[s] = -1; continue; }
+            b.slot_act[s] = b.nnext[c + (r >= L ? 1 : 0)];
+        }
+        const int h = r >= L ? 1 : 0;
         const float* p = b.pos + 9 * (size_t)b.idx[s];
         for (int v = 0; v < 3; ++v) {
             const uint64_t q = (uint64_t)(3u * (uint32_t)s + (uint32_t)v + 1u);
@@ -137,24 +172,22 @@ __global__ __launch_bounds__(BT) void k_bb_box(BB b, int mode) {
                 if (x != x) continue;
                 const uint64_t kk = (uint64_t)okey(x) << 32;
                 const unsigned long long kl = kk | (LOW - q), kh = kk | q;
-                lo[k] = kl < lo[k] ? kl : lo[k];
-                hi[k] = kh > hi[k] ? kh : hi[k];
+                lo[h][k] = kl < lo[h][k] ? kl : lo[h][k];
+                hi[h][k] = kh > hi[h][k] ? kh : hi[h][k];
             }
         }
     }
-    if (uni) {
-        for (int k = 0; k < 3; ++k) { atomicMin(&sk[k], lo[k]); atomicMax(&sk[3 + k], hi[k]); }
-        __syncthreads();
-        if (threadIdx.x < 3) atomicMin(&b.nkey[6 * (size_t)t + threadIdx.x], sk[threadIdx.x]);
-        else if (threadIdx.x < 6) atomicMax(&b.nkey[6 * (size_t)t + threadIdx.x], sk[threadIdx.x]);
-    } else if (t >= 0) {
+    if (c < 0) return;
+    for (int h = 0; h < (mode == 0 ? 1 : 2); ++h) {
         for (int k = 0; k < 3; ++k) {
-            atomicMin(&b.nkey[6 * (size_t)t + k], lo[k]);
-            atomicMax(&b.nkey[6 * (size_t)t + 3 + k], hi[k]);
+            const unsigned long long l = wave_min64(lo[h][k]), u = wave_max64(hi[h][k]);
+            if (threadIdx.x == 0) {
+                const int32_t tn = c + h;
+                if (l != ~0ull) atomicMin(&b.nkey[6 * (size_t)tn + k], l);
+                if (u != 0ull) atomicMax(&b.nkey[6 * (size_t)tn + 3 + k], u);
+            }
         }
     }
-    if (mode == 1 && s < b.T && b.slot_act[s] >= 0)       // the slot's node at the next level
-        b.slot_act[s] = t >= 0 ? b.nnext[t] : -1;
 }
 
 // Box floats of nodes [lo, counters[0]) (or [lo, hi) when hi >= 0) from
@@ -180,44 +213,44 @@ __global__ __launch_bounds__(BT) void k_bb_decode(BB b, int32_t lo, int32_t hi) 
     }
 }
 
-__global__ __launch_bounds__(BT) void k_bb_bounds(BB b) {
-    __shared__ uint32_t sk[6];
-    __shared__ int32_t sa0;
-    const int32_t s = blockIdx.x * BT + threadIdx.x;
-    const int32_t a = s < b.T ? b.slot_act[s] : -1;
-    if (threadIdx.x == 0) sa0 = a;
-    if (threadIdx.x < 6) sk[threadIdx.x] = threadIdx.x < 3 ? ~0u : 0u;
-    __syncthreads();
-    const bool uni = __syncthreads_and(a >= 0 && a == sa0);
-    uint32_t c[3] = {0, 0, 0};
-    if (a >= 0) {
-        const float* cp = b.cen + 3 * (size_t)b.idx[s];
-        for (int k = 0; k < 3; ++k) c[k] = okey(cp[k]);
+__global__ __launch_bounds__(TW) void k_bb_bounds(BB b) {
+    const int32_t t = blockIdx.x;
+    if (t >= b.counters[3]) return;
+    const int2 tk = b.task[t];
+    const int32_t a = tk.x, n = b.act[a], first = b.nfirst[n], count = b.ncount[n];
+    const int32_t beg = tk.y * TASK, end = min(count, beg + TASK);
+    uint32_t mn[3] = {~0u, ~0u, ~0u}, mx[3] = {0u, 0u, 0u};
+    for (int32_t r = beg + (int32_t)threadIdx.x; r < end; r += TW) {
+        const float* cp = b.cen + 3 * (size_t)b.idx[first + r];
+        for (int k = 0; k < 3; ++k) { const uint32_t c = okey(cp[k]); mn[k] = c < mn[k] ? c : mn[k]; mx[k] = c > mx[k] ? c : mx[k]; }
     }
-    if (uni) {
-        for (int k = 0; k < 3; ++k) { atomicMin(&sk[k], c[k]); atomicMax(&sk[3 + k], c[k]); }
-        __syncthreads();
-        if (threadIdx.x < 3) atomicMin(&b.acb[6 * (size_t)a + threadIdx.x], sk[threadIdx.x]);
-        else if (threadIdx.x < 6) atomicMax(&b.acb[6 * (size_t)a + threadIdx.x], sk[threadIdx.x]);
-    } else if (a >= 0) {
-        for (int k = 0; k < 3; ++k) { atomicMin(&b.acb[6 * (size_t)a + k], c[k]); atomicMax(&b.acb[6 * (size_t)a + 3 + k], c[k]); }
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t l = wave_min(mn[k]), u = wave_max(mx[k]);
+        if (threadIdx.x == 0) { atomicMin(&b.acb[6 * (size_t)a + k], l); atomicMax(&b.acb[6 * (size_t)a + 3 + k], u); }
     }
 }
 
 // Bin boxes only feed surface areas, where the sign of a zero bound cannot
 // change a product's comparison, so value-only keys suffice here.
-__global__ __launch_bounds__(BT) void k_bb_bins(BB b) {
+__global__ __launch_bounds__(TW) void k_bb_bins(BB b) {
     __shared__ uint32_t sb[BIN_WORDS];
-    __shared__ int32_t sa0;
-    const int32_t s = blockIdx.x * BT + threadIdx.x;
-    const int32_t a = s < b.T ? b.slot_act[s] : -1;
-    if (threadIdx.x == 0) sa0 = a;
-    for (int w = threadIdx.x; w < BIN_WORDS; w += BT) sb[w] = w < 24 ? 0u : w < 96 ? okey(FLT_MAX) : okey(-FLT_MAX);
+    const int32_t t = blockIdx.x;
+    if (t >= b.counters[3]) return;
+    const int2 tk = b.task[t];
+    const int32_t a = tk.x, n = b.act[a], first = b.nfirst[n], count = b.ncount[n];
+    const int32_t beg = tk.y * TASK, end = min(count, beg + TASK);
+    for (int w = threadIdx.x; w < BIN_WORDS; w += TW) sb[w] = w < 24 ? 0u : w < 96 ? okey(FLT_MAX) : okey(-FLT_MAX);
+    float bmin[3], scale[3];
+    bool live[3];
+    for (int ax = 0; ax < 3; ++ax) {
+        bmin[ax] = okey_value(b.acb[6 * (size_t)a + ax]);
+        const float bmax = okey_value(b.acb[6 * (size_t)a + 3 + ax]);
+        live[ax] = bmin[ax] != bmax;
+        scale[ax] = live[ax] ? 8.0f / (bmax - bmin[ax]) : 0.0f;
+    }
     __syncthreads();
-    const bool uni = __syncthreads_and(a >= 0 && a == sa0);
-    if (a >= 0) {
-        uint32_t* dst = uni ? sb : b.abin + (size_t)a * BIN_WORDS;
-        const int32_t tri = b.idx[s];
+    for (int32_t r = beg + (int32_t)threadIdx.x; r < end; r += TW) {
+        const int32_t tri = b.idx[first + r];
         const float* p = b.pos + 9 * (size_t)tri;
         uint32_t tlo[3], thi[3];
         for (int k = 0; k < 3; ++k) {
@@ -227,24 +260,21 @@ __global__ __launch_bounds__(BT) void k_bb_bins(BB b) {
         }
         const float* cp = b.cen + 3 * (size_t)tri;
         for (int ax = 0; ax < 3; ++ax) {
-            const float bmin = okey_value(b.acb[6 * (size_t)a + ax]), bmax = okey_value(b.acb[6 * (size_t)a + 3 + ax]);
-            if (bmin == bmax) continue;
-            const float scale = 8.0f / (bmax - bmin);
-            const int32_t raw = (int32_t)((cp[ax] - bmin) * scale);
-            const int bi = raw < NBINS - 1 ? raw : NBINS - 1;
-            const int e = ax * NBINS + bi;
-            atomicAdd(&dst[e], 1u);
-            for (int k = 0; k < 3; ++k) { atomicMin(&dst[24 + 3 * e + k], tlo[k]); atomicMax(&dst[96 + 3 * e + k], thi[k]); }
+            if (!live[ax]) continue;
+            const int32_t raw = (int32_t)((cp[ax] - bmin[ax]) * scale[ax]);
+            const int e = ax * NBINS + (raw < NBINS - 1 ? raw : NBINS - 1);
+            atomicAdd(&sb[e], 1u);
+            for (int k = 0; k < 3; ++k) { atomicMin(&sb[24 + 3 * e + k], tlo[k]); atomicMax(&sb[96 + 3 * e + k], thi[k]); }
         }
     }
-    if (uni) {
-        __syncthreads();
-        uint32_t* g = b.abin + (size_t)a * BIN_WORDS;
-        for (int w = threadIdx.x; w < BIN_WORDS; w += BT) {
-            if (w < 24) { if (sb[w]) atomicAdd(&g[w], sb[w]); }
-            else if (w < 96) atomicMin(&g[w], sb[w]);
-            else atomicMax(&g[w], sb[w]);
-        }
+    __syncthreads();
+    uint32_t* g = b.abin + (size_t)a * BIN_WORDS;
+    for (int w = threadIdx.x; w < BIN_WORDS; w += TW) {
+        const int e = w < 24 ? w : w < 96 ? (w - 24) / 3 : (w - 96) / 3;
+        if (sb[e] == 0u) continue;                    // an empty bin keeps its initial box
+        if (w < 24) atomicAdd(&g[w], sb[w]);
+        else if (w < 96) atomicMin(&g[w], sb[w]);
+        else atomicMax(&g[w], sb[w]);
     }
 }
 
@@ -420,14 +450,16 @@ void launch_n(K kernel, int64_t n, hipStream_t st, Args... args) {
     if (n > 0) hipLaunchKernelGGL(kernel, dim3(blocks_for(n)), dim3(BT), 0, st, args...);
 }
 
-template <typename T>
-hipError_t dalloc(T** p, size_t n, std::vector<void*>* owned) {
-    *p = nullptr;
-    if (n == 0) n = 1;
-    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
-    if (e == hipSuccess) owned->push_back(*p);
-    return e;
-}
+// Sub-buffers of one device allocation, 256-byte aligned.
+struct Arena {
+    size_t off = 0;
+    char* base = nullptr;
+    template <typename T>
+    void take(T** p, size_t n) {
+        *p = (T*)(base + off);
+        off += (n * sizeof(T) + 255) & ~(size_t)255;
+    }
+};
 
 }  // namespace
 
@@ -436,41 +468,48 @@ hipError_t device_build_blas(hipStream_t st, const float* pos_host, const int32_
                              std::vector<int32_t>* used, std::vector<int32_t>* max_depth) {
     nodes->clear(); order->assign(T, 0); used->assign(M, 0); max_depth->assign(M, 0);
     if (M <= 0 || T <= 0) return hipSuccess;
-    std::vector<void*> owned;
+    char* arena = nullptr;
     hipError_t e = hipSuccess;
 #define BB_CK(x) do { e = (x); if (e != hipSuccess) goto done; } while (0)
     {
         const int64_t node_cap = 2 * (int64_t)T + M;
         const int32_t act_cap = T / (LEAF_MAX + 1) + 1;
+        const int32_t task_cap = T / TASK + act_cap + 1;
         std::vector<int64_t> base(M, 0);
         int64_t total_nodes = 0;
         for (int32_t m = 0; m < M; ++m) { base[m] = total_nodes; total_nodes += mesh_count[m] > 0 ? 2 * (int64_t)mesh_count[m] - 1 : 0; }
+        std::vector<int2> root_tasks;
+        for (int32_t m = 0; m < M; ++m)
+            for (int32_t j = 0; j * TASK < mesh_count[m]; ++j) root_tasks.push_back(make_int2(m, j));
+        size_t scan_bytes = 0;
+        BB_CK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (int32_t*)nullptr, (int32_t*)nullptr, T + 1, st));
         BB b{};
         b.T = T; b.M = M;
-        float* d_pos; BB_CK(dalloc(&d_pos, 9 * (size_t)T, &owned)); b.pos = d_pos;
-        BB_CK(dalloc(&b.cen, 3 * (size_t)T, &owned));
-        BB_CK(dalloc(&b.idx, T, &owned)); BB_CK(dalloc(&b.idx2, T, &owned));
-        BB_CK(dalloc(&b.slot_act, T, &owned));
-        BB_CK(dalloc(&b.flag, T + 1, &owned)); BB_CK(dalloc(&b.scan, T + 1, &owned));
-        BB_CK(dalloc(&b.fr, T, &owned)); BB_CK(dalloc(&b.bl, T, &owned));
-        BB_CK(dalloc(&b.nfirst, node_cap, &owned)); BB_CK(dalloc(&b.ncount, node_cap, &owned));
-        BB_CK(dalloc(&b.nchild, node_cap, &owned)); BB_CK(dalloc(&b.nmesh, node_cap, &owned));
-        BB_CK(dalloc(&b.ndepth, node_cap, &owned)); BB_CK(dalloc(&b.nnext, node_cap, &owned));
-        BB_CK(dalloc(&b.nkey, 6 * (size_t)node_cap, &owned)); BB_CK(dalloc(&b.nbox, 6 * (size_t)node_cap, &owned));
-        int32_t* d_act[2];
-        BB_CK(dalloc(&d_act[0], act_cap, &owned)); BB_CK(dalloc(&d_act[1], act_cap, &owned));
-        BB_CK(dalloc(&b.acb, 6 * (size_t)act_cap, &owned)); BB_CK(dalloc(&b.abin, (size_t)BIN_WORDS * act_cap, &owned));
-        BB_CK(dalloc(&b.aaxis, act_cap, &owned)); BB_CK(dalloc(&b.asplit, act_cap, &owned)); BB_CK(dalloc(&b.aleft, act_cap, &owned));
-        BB_CK(dalloc(&b.counters, 3, &owned));
-        b.node_cap = (int32_t)node_cap; b.act_cap = act_cap; BB_CK(dalloc(&b.mesh_depth, M, &owned));
-        int32_t* d_mfirst; BB_CK(dalloc(&d_mfirst, M, &owned)); b.mesh_first = d_mfirst;
-        int32_t* d_root_act; BB_CK(dalloc(&d_root_act, M, &owned));
-        int32_t *icnt, *rank, *id; int64_t* d_base; Node* d_out;
-        BB_CK(dalloc(&icnt, node_cap, &owned)); BB_CK(dalloc(&rank, node_cap, &owned)); BB_CK(dalloc(&id, node_cap, &owned));
-        BB_CK(dalloc(&d_base, M, &owned)); BB_CK(dalloc(&d_out, total_nodes, &owned));
-        size_t scan_bytes = 0;
-        BB_CK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, b.flag, b.scan, T + 1, st));
-        void* d_scan_tmp; BB_CK(dalloc((char**)&d_scan_tmp, scan_bytes, &owned));
+        b.node_cap = (int32_t)node_cap; b.act_cap = act_cap; b.task_cap = task_cap;
+        float* d_pos; int32_t* d_act[2]; int32_t *d_mfirst, *d_root_act, *icnt, *rank, *id; int64_t* d_base; Node* d_out;
+        int2* d_root_tasks; void* d_scan_tmp;
+        Arena A;
+        for (int pass = 0; pass < 2; ++pass) {       // sizes, then pointers
+            A.off = 0;
+            A.take(&d_pos, 9 * (size_t)T); A.take(&b.cen, 3 * (size_t)T);
+            A.take(&b.idx, T); A.take(&b.idx2, T); A.take(&b.slot_act, T);
+            A.take(&b.flag, T + 1); A.take(&b.scan, T + 1); A.take(&b.fr, T); A.take(&b.bl, T);
+            A.take(&b.nfirst, node_cap); A.take(&b.ncount, node_cap); A.take(&b.nchild, node_cap);
+            A.take(&b.nmesh, node_cap); A.take(&b.ndepth, node_cap); A.take(&b.nnext, node_cap);
+            A.take(&b.nkey, 6 * (size_t)node_cap); A.take(&b.nbox, 6 * (size_t)node_cap);
+            A.take(&d_act[0], act_cap); A.take(&d_act[1], act_cap);
+            A.take(&b.acb, 6 * (size_t)act_cap); A.take(&b.abin, (size_t)BIN_WORDS * act_cap);
+            A.take(&b.aaxis, act_cap); A.take(&b.asplit, act_cap); A.take(&b.aleft, act_cap);
+            A.take(&b.task, task_cap); A.take(&d_root_tasks, root_tasks.size());
+            A.take(&b.counters, 4); A.take(&b.mesh_depth, M);
+            A.take(&d_mfirst, M); A.take(&d_root_act, M);
+            A.take(&icnt, node_cap); A.take(&rank, node_cap); A.take(&id, node_cap);
+            A.take(&d_base, M); A.take(&d_out, total_nodes);
+            A.take((char**)&d_scan_tmp, scan_bytes);
+            if (pass == 0) { BB_CK(hipMalloc((void**)&arena, A.off)); A.base = arena; }
+        }
+        b.pos = d_pos;
+        b.mesh_first = d_mfirst;
 
         // roots: node m for mesh m
         std::vector<int32_t> rfirst(M), rcount(M), rminus(M, -1), rzero(M, 0), rmesh(M), ract(M, -1), ract_list;
@@ -495,16 +534,20 @@ hipError_t device_build_blas(hipStream_t st, const float* pos_host, const int32_
         BB_CK(hipMemcpyAsync(b.nkey, rkey.data(), 48 * (size_t)M, hipMemcpyHostToDevice, st));
         BB_CK(hipMemcpyAsync(b.mesh_depth, rzero.data(), 4 * (size_t)M, hipMemcpyHostToDevice, st));
         BB_CK(hipMemcpyAsync(d_base, base.data(), 8 * (size_t)M, hipMemcpyHostToDevice, st));
+        if (!root_tasks.empty())
+            BB_CK(hipMemcpyAsync(d_root_tasks, root_tasks.data(), sizeof(int2) * root_tasks.size(), hipMemcpyHostToDevice, st));
         if (!ract_list.empty())
             BB_CK(hipMemcpyAsync(d_act[0], ract_list.data(), 4 * ract_list.size(), hipMemcpyHostToDevice, st));
-        int32_t host_counters[3] = {M, 0, 0};
-        BB_CK(hipMemcpyAsync(b.counters, host_counters, 12, hipMemcpyHostToDevice, st));
+        int32_t host_counters[4] = {M, 0, 0, 0};
+        BB_CK(hipMemcpyAsync(b.counters, host_counters, 16, hipMemcpyHostToDevice, st));
         BB_CK(hipMemsetAsync(d_out, 0, sizeof(Node) * (size_t)(total_nodes > 0 ? total_nodes : 1), st));
         BB_CK(hipMemsetAsync(rank, 0, 4 * (size_t)M, st));
         BB_CK(hipMemsetAsync(id, 0, 4 * (size_t)M, st));
         b.act = d_act[0]; b.act_next = d_act[1];
         launch_n(k_bb_init, T + 1, st, b, d_root_act);
-        launch_n(k_bb_box, T, st, b, 0);
+        if (!root_tasks.empty())
+            hipLaunchKernelGGL(k_bb_box, dim3((unsigned)root_tasks.size()), dim3(TW), 0, st, b, 0, (const int2*)d_root_tasks,
+                               (int32_t)root_tasks.size());
         launch_n(k_bb_decode, M, st, b, 0, M);
         BB_CK(hipGetLastError());
 
@@ -514,22 +557,23 @@ hipError_t device_build_blas(hipStream_t st, const float* pos_host, const int32_
             if ((int64_t)lvl.size() > (int64_t)T + 2) { e = hipErrorUnknown; goto done; }   // depth is at most T
             b.act = d_act[cur]; b.act_next = d_act[cur ^ 1];
             const int32_t lo = lvl.back();
+            const unsigned max_tasks = (unsigned)(T / TASK + nact + 1);
             launch_n(k_bb_level_init, (int64_t)nact * BIN_WORDS, st, b, nact);
-            launch_n(k_bb_bounds, T, st, b);
-            launch_n(k_bb_bins, T, st, b);
+            launch_n(k_bb_tasks, nact, st, b, nact);
+            hipLaunchKernelGGL(k_bb_bounds, dim3(max_tasks), dim3(TW), 0, st, b);
+            hipLaunchKernelGGL(k_bb_bins, dim3(max_tasks), dim3(TW), 0, st, b);
             launch_n(k_bb_split, nact, st, b, nact);
             launch_n(k_bb_flag, T, st, b);
             BB_CK(hipGetLastError());
             BB_CK(hipcub::DeviceScan::ExclusiveSum(d_scan_tmp, scan_bytes, b.flag, b.scan, T + 1, st));
-            BB_CK(hipMemsetAsync(b.counters + 1, 0, 4, st));
             launch_n(k_bb_left, nact, st, b, nact);
             launch_n(k_bb_ranks, T, st, b);
             launch_n(k_bb_scatter, T, st, b);
             std::swap(b.idx, b.idx2);
-            launch_n(k_bb_box, T, st, b, 1);
+            hipLaunchKernelGGL(k_bb_box, dim3(max_tasks), dim3(TW), 0, st, b, 1, (const int2*)b.task, 0);
             launch_n(k_bb_decode, 2 * (int64_t)nact, st, b, lo, -1);
             BB_CK(hipGetLastError());
-            BB_CK(hipMemcpyAsync(host_counters, b.counters, 12, hipMemcpyDeviceToHost, st));
+            BB_CK(hipMemcpyAsync(host_counters, b.counters, 16, hipMemcpyDeviceToHost, st));
             BB_CK(hipStreamSynchronize(st));
             if (host_counters[2] || host_counters[0] > node_cap || host_counters[1] > act_cap) { e = hipErrorUnknown; goto done; }
             lvl.push_back(host_counters[0]);
@@ -546,20 +590,28 @@ hipError_t device_build_blas(hipStream_t st, const float* pos_host, const int32_
                            total_nodes, d_out);
         BB_CK(hipGetLastError());
         std::vector<int32_t> root_icnt(M);
-        nodes->resize((size_t)total_nodes);
-        BB_CK(hipMemcpyAsync(nodes->data(), d_out, sizeof(Node) * (size_t)total_nodes, hipMemcpyDeviceToHost, st));
-        BB_CK(hipMemcpyAsync(order->data(), b.idx, 4 * (size_t)T, hipMemcpyDeviceToHost, st));
         BB_CK(hipMemcpyAsync(root_icnt.data(), icnt, 4 * (size_t)M, hipMemcpyDeviceToHost, st));
         BB_CK(hipMemcpyAsync(max_depth->data(), b.mesh_depth, 4 * (size_t)M, hipMemcpyDeviceToHost, st));
-        BB_CK(hipMemcpyAsync(host_counters, b.counters, 12, hipMemcpyDeviceToHost, st));
+        BB_CK(hipMemcpyAsync(host_counters, b.counters, 16, hipMemcpyDeviceToHost, st));
         BB_CK(hipStreamSynchronize(st));
         if (host_counters[2]) { e = hipErrorUnknown; goto done; }
-        for (int32_t m = 0; m < M; ++m) (*used)[m] = mesh_count[m] > 0 ? 1 + 2 * root_icnt[m] : 0;
+        // only the allocated nodes travel back (about a fifth of the 2*count-1
+        // capacity with leaves of up to 8 triangles); the tails stay zero
+        nodes->resize((size_t)total_nodes);
+        for (int32_t m = 0; m < M; ++m) {
+            (*used)[m] = mesh_count[m] > 0 ? 1 + 2 * root_icnt[m] : 0;
+            if (mesh_count[m] > 0 && (*used)[m] > 2 * mesh_count[m] - 1) { e = hipErrorUnknown; goto done; }
+            if ((*used)[m] > 0)
+                BB_CK(hipMemcpyAsync(nodes->data() + base[m], d_out + base[m], sizeof(Node) * (size_t)(*used)[m],
+                                     hipMemcpyDeviceToHost, st));
+        }
+        BB_CK(hipMemcpyAsync(order->data(), b.idx, 4 * (size_t)T, hipMemcpyDeviceToHost, st));
+        BB_CK(hipStreamSynchronize(st));
     }
 done:
 #undef BB_CK
     if (e != hipSuccess) (void)hipStreamSynchronize(st);
-    for (void* p : owned) (void)hipFree(p);
+    if (arena) (void)hipFree(arena);
     return e;
 }
 
