@@ -279,6 +279,31 @@ def time_at(curve, target):
     return None
 
 
+def blas_build_leg(scene, host_ctx=None, host_ms=None):
+    """Wall time of the scene's BLAS builds on host threads and on the device
+    (tuning knob blas_device; the second device build, the first also loads
+    the kernels), and whether the trees are equal node for node."""
+    own = host_ctx is None
+    if own:
+        host_ctx = Context(0)
+        host_ctx.upload(scene)
+        host_ms = host_ctx.build_timing()[0]
+    bctx = Context(0)
+    bctx.set_tuning(blas_device=1)
+    dev_ms = []
+    for _ in range(2):
+        bctx.upload(scene)
+        dev_ms.append(bctx.build_timing()[0])
+    same = all(bctx.blas(m)[0].tobytes() == host_ctx.blas(m)[0].tobytes() and
+               np.array_equal(bctx.blas(m)[1], host_ctx.blas(m)[1]) for m in range(len(scene.mesh_count)))
+    bctx.close()
+    if own:
+        host_ctx.close()
+    return {"triangles": int(len(scene.positions)), "meshes": int(len(scene.mesh_count)),
+            "host_threads_ms": round(host_ms, 2), "device_ms": round(dev_ms[-1], 2),
+            "device_first_ms": round(dev_ms[0], 2), "trees_equal": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -310,6 +335,8 @@ def main():
                          "at least this rough (0: always at the bounce-1 vertex)")
     ap.add_argument("--profile-cached", type=int, default=0,
                     help="only render this many cached 1-spp frames of the workload (for rocprofv3)")
+    ap.add_argument("--blas-build-only", action="store_true",
+                    help="only build the instanced scene's BLASes on host threads and on the device, print one JSON line")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -327,6 +354,10 @@ def main():
               f"(multi-GPU sharding is a design note, not built)", file=sys.stderr)
         sys.exit(2)
 
+    if args.blas_build_only:
+        scene = scenes.scene_instanced(num_instances=args.instances, tex_size=8)
+        print(json.dumps(blas_build_leg(scene)), flush=True)
+        return
     t0 = time.time()
     scene = scenes.scene_instanced(num_instances=args.instances)
     log(f"[bench] scene generated in {time.time() - t0:.1f}s: {len(scene.positions)} triangles, "
@@ -336,6 +367,7 @@ def main():
     ctx = Context(0)
     t0 = time.time()
     ctx.upload(scene)
+    host_build_ms = ctx.build_timing()[0]
     log(f"[bench] scene built and uploaded in {time.time() - t0:.1f}s: info {ctx.info()}")
     st = make_settings(args.depth, integ)
     ctx.set_settings(st)
@@ -525,6 +557,9 @@ def main():
                "curve_cached": [[round(s, 4), n, round(e, 5)] for s, n, e in cached]}
         log(f"[ttt] {ttt}")
 
+    blas_build = blas_build_leg(scene, ctx, host_build_ms)
+    log(f"[bench] BLAS build {blas_build}")
+
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
@@ -536,7 +571,7 @@ def main():
                    "max_depth": args.depth, "triangles": int(len(scene.positions)),
                    "instances": int(scene.num_instances), "parallelism": "single GPU", "tuning": tuning},
         "roofline": roof, "cpu_baseline": cpu, "time_to_target": ttt, "batched_views": views,
-        "graph_replay": graph_rec, "megakernel": mega, "full_frame_4k": full4k,
+        "graph_replay": graph_rec, "megakernel": mega, "full_frame_4k": full4k, "blas_build": blas_build,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
