@@ -286,6 +286,7 @@ def blas_build_leg(scene, host_ctx=None, host_ms=None):
     own = host_ctx is None
     if own:
         host_ctx = Context(0)
+        host_ctx.set_tuning(blas_device=0)
         host_ctx.upload(scene)
         host_ms = host_ctx.build_timing()[0]
     bctx = Context(0)
@@ -365,6 +366,7 @@ def main():
     integ = abi.INTEGRATOR_WAVEFRONT if args.integrator == "wavefront" else abi.INTEGRATOR_MEGAKERNEL
     W, H = args.width, args.height
     ctx = Context(0)
+    ctx.set_tuning(blas_device=0)          # host threads here; the device build is timed beside it
     t0 = time.time()
     ctx.upload(scene)
     host_build_ms = ctx.build_timing()[0]

@@ -337,10 +337,11 @@ int kpt_set_trace_counting(kpt_ctx* ctx, int32_t enable);
  * than the host builder for 1,000 instances), refit it on the device (2: the
  * topology of the last build, boxes recomputed; not the reference's tree once
  * instances move) or rebuild it on the host (0, default), [9] (when n >= 10)
- * build the BLASes of the next kpt_upload_scene on the device (1: the host
- * builder's trees node for node, all meshes in one level-by-level build;
- * needs finite vertex positions and a device) or on host threads (0,
- * default).  n >= 5. */
+ * build the BLASes of the next kpt_upload_scene on the device (1, the
+ * default of a context with a device: the host builder's trees node for
+ * node, all meshes in one level-by-level build; needs finite vertex
+ * positions) or on host threads (0, the only choice of a host-only
+ * context).  n >= 5. */
 int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n);
 int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n);
 int kpt_get_render_counters(const kpt_ctx* ctx, kpt_trace_counters* out);

@@ -84,7 +84,7 @@ struct kpt_ctx {
     kpt_trace_counters render_counts{};
     double timing[4] = {0, 0, 0, 0};
     kpt_stage_stats stages[2]{};         // closest-hit trace stage, path-per-lane tail
-    int32_t blas_device = 0;             // build the BLASes of the next upload on the device (kpt_blas.hip)
+    int32_t blas_device = 0;             // build the BLASes of the next upload on the device (kpt_blas.hip); 1 on device contexts
     double blas_build_ms[2] = {0, 0};    // last upload: BLAS build wall ms, 1 if built on the device
     // TLAS rebuilds on the device (kpt_tlas.hip); the host tree is then stale
     int32_t tlas_device = 0;
@@ -643,6 +643,7 @@ int kpt_ctx_create(int device, kpt_ctx** out) {
     if (device < 0) { if (hipGetDevice(&device) != hipSuccess) device = 0; }
     if (device >= count) { delete ctx; return KPT_ERR_INVALID_ARG; }
     ctx->device = device;
+    ctx->blas_device = 1;                // same trees as the host builder, built faster on the device
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc((void**)&ctx->d_overflow, sizeof(int32_t)) != hipSuccess ||
         hipMemset(ctx->d_overflow, 0, sizeof(int32_t)) != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||

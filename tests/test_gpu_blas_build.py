@@ -83,6 +83,14 @@ def test_device_blas_edge_cases():
     dev[0].close()
 
 
+def test_device_build_is_the_default_on_a_device():
+    ctx = Context(0)
+    assert ctx.tuning()["blas_device"] == 1
+    ctx.upload(scenes.scene_instanced_small())
+    assert ctx.build_timing()[1] is True
+    ctx.close()
+
+
 def test_device_blas_renders_the_same():
     sc = scenes.scene_instanced_small()
     W, H = 64, 48
