@@ -803,6 +803,8 @@ int kpt_upload_scene(kpt_ctx* ctx, const kpt_scene_desc* d) {
     const auto build_t0 = std::chrono::steady_clock::now();
     if (ctx->blas_device && T > 0) {
         if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "device BLAS build on a host-only context");
+        if (T > BLAS_DEVICE_MAX_TRIS)
+            return fail(ctx, KPT_ERR_UNSUPPORTED, "device BLAS build: more than 2^29 triangles (use blas_device = 0)");
         for (size_t i = 0; i < ctx->pos.size(); ++i)
             if (!std::isfinite(ctx->pos[i]))
                 return fail(ctx, KPT_ERR_INVALID_ARG, "device BLAS build needs finite vertex positions");

@@ -468,6 +468,7 @@ hipError_t device_build_blas(hipStream_t st, const float* pos_host, const int32_
                              std::vector<int32_t>* used, std::vector<int32_t>* max_depth) {
     nodes->clear(); order->assign(T, 0); used->assign(M, 0); max_depth->assign(M, 0);
     if (M <= 0 || T <= 0) return hipSuccess;
+    if (T > BLAS_DEVICE_MAX_TRIS) return hipErrorInvalidValue;
     char* arena = nullptr;
     hipError_t e = hipSuccess;
 #define BB_CK(x) do { e = (x); if (e != hipSuccess) goto done; } while (0)

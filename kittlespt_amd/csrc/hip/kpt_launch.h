@@ -106,6 +106,10 @@ hipError_t launch_tlas_build(hipStream_t s, const float* bounds, int32_t n, Node
 hipError_t launch_tlas_refit(hipStream_t s, const float* bounds, int32_t n, Node* ref_nodes, const int32_t* parent,
                              int32_t* arrivals, Node* pair_nodes, NodeBlock* blocks, int32_t* status);
 
+// Box keys hold the slot position 3*slot+vertex+1 in 32 bits and node ids
+// are int32 (up to 2T+M): 2^29 triangles keeps both in range (and the
+// build's scratch, ~430 bytes per triangle, inside one device's HBM).
+constexpr int32_t BLAS_DEVICE_MAX_TRIS = 1 << 29;
 // BLASes of all meshes built on the device, node for node the host
 // builder's trees (kpt_blas.hip).  nodes: sum over meshes of 2*count-1
 // entries (mesh m's slice first, unused tail zeroed, local indices as
