@@ -23,7 +23,7 @@
 //             head and a tail), Taylor series of sin/cos on |r| <= pi/4.
 //   acosf   : acos x = pi/2 - asin x; asin by its power series on |x| <= 1/2
 //             and asin x = pi/2 - 2·asin(sqrt((1-x)/2)) above, in double.
-// Accuracy against glibc is checked in tests/test_dmath.py.
+// Accuracy against glibc is checked in tests/test_oracle.py (test_dmath_*).
 #pragma once
 #include <stdint.h>
 

@@ -463,3 +463,40 @@ def cache_adam(w, m, v, grad, step, lr):
     _sig("orc_cache_adam", None, [C.c_void_p] * 4 + [C.c_int64, C.c_float])(
         _ptr(w), _ptr(m), _ptr(v), _ptr(g), step, lr)
     return w, m, v
+
+
+# ---------------------------------------------------------------- shading probes
+def shade_probe(scene_handle, rays):
+    """getSurfaceInteraction + Material::getBSDF at each ray's closest hit:
+    [n, 24] floats (see orc_shade_probe)."""
+    rays = np.ascontiguousarray(rays, abi.RAY_DTYPE)
+    out = np.zeros((len(rays), 24), np.float32)
+    _sig("orc_shade_probe", None, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p])(
+        scene_handle.h, _ptr(rays), len(rays), _ptr(out))
+    return out
+
+
+def light_sample(scene_handle, light, from_pos, u):
+    """Light::sampleLi of light `light` from positions [n, 3] with numbers
+    [n, 2]: ([n, 12] point, normal, pdf, L, uv; [n] pdf_Li of that point)."""
+    f, uu = _f32(from_pos).reshape(-1, 3), _f32(u).reshape(-1, 2)
+    out = np.zeros((len(f), 12), np.float32)
+    pli = np.zeros(len(f), np.float32)
+    _sig("orc_light_sample", None, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                    C.c_void_p])(scene_handle.h, light, _ptr(f), _ptr(uu), len(f), _ptr(out),
+                                                 _ptr(pli))
+    return out, pli
+
+
+def encode_unit(v):
+    v = _f32(v).reshape(-1, 3)
+    out = np.zeros((len(v), 2), np.float32)
+    _sig("orc_encode_unit", None, [C.c_void_p, C.c_int64, C.c_void_p])(_ptr(v), len(v), _ptr(out))
+    return out
+
+
+def float_to_byte(v):
+    v = _f32(v)
+    out = np.zeros(v.size, np.uint32)
+    _sig("orc_float_to_byte", None, [C.c_void_p, C.c_int64, C.c_void_p])(_ptr(v), v.size, _ptr(out))
+    return out

@@ -6,7 +6,8 @@
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off -fno-fast-math).
 // The float instantiation uses the deterministic transcendentals of
 // kittlespt_amd/csrc/common/kpt_dmath.h (the GPU kernels use the same header;
-// that header is checked against glibc independently in tests/test_dmath.py).
+// that header is checked against glibc independently in tests/test_oracle.py,
+// test_dmath_*).
 // The double instantiation uses libm and serves as the converged FP64 render.
 //
 // Every routine cites the reference file:line it restates.  Operation order
@@ -1271,6 +1272,33 @@ template <class R> SurfInt<R> GetSurf(const orc_scene* S, const Isect<R>& it, co
     return si;
 }
 
+// Light::sampleLi with Triangle::sample (light.cu:33-45, triangle.cuh:104-119):
+// a point of the light's triangle from two uniform numbers (the square-to-
+// triangle map of triangle.cuh:104-119), the light's world normal, the pdf
+// of that point in solid angle seen from `from`, and the emitted radiance.
+template <class R> struct LightSample { V3<R> p, n; R pdf = 0; V3<R> L; V2<R> uv; };
+template <class R> LightSample<R> SampleLight(const orc_scene* S, const Light& l, V3<R> from, V2<R> u2) {
+    const Triangle& tri = S->tris[l.prim_id];
+    M4<R> model = CvtM4<R>(Inverse(S->meshes[l.inst_id].curr_inv_model));
+    V2<R> bb;
+    if (u2.x < u2.y) { bb.x = u2.x / R(2); bb.y = u2.y - bb.x; }
+    else { bb.y = u2.y / R(2); bb.x = u2.x - bb.y; }
+    V3<R> bary(bb.x, bb.y, R(1) - bb.x - bb.y);
+    LightSample<R> ls;
+    ls.uv.x = (bary.x * R(tri.t[0].x)) + (bary.y * R(tri.t[1].x)) + (bary.z * R(tri.t[2].x));
+    ls.uv.y = (bary.x * R(tri.t[0].y)) + (bary.y * R(tri.t[1].y)) + (bary.z * R(tri.t[2].y));
+    V3<R> p = (cvt<R>(tri.p[0]) * bary.x) + (cvt<R>(tri.p[1]) * bary.y) + (cvt<R>(tri.p[2]) * bary.z);
+    ls.p = xyz(Mul(model, V4<R>(p, R(1))));
+    ls.n = xyz(Mul(model, V4<R>(cvt<R>(tri.local_geometric_normal), R(0))));
+    R area = R(0.5f) * Length(Cross(cvt<R>(tri.p[1]) - cvt<R>(tri.p[0]), cvt<R>(tri.p[2]) - cvt<R>(tri.p[0])));
+    R spdf = R(1) / area;
+    spdf *= Sqr(Distance(ls.p, from));
+    spdf /= AbsDot(Normalize(ls.p - from), ls.n);
+    ls.pdf = spdf;
+    ls.L = LightL(S, l, ls.uv);
+    return ls;
+}
+
 template <class R> bool Unoccluded(const orc_scene* S, const SurfInt<R>& si, V3<R> target) {  // integrator.cuh:124-131
     V3<R> org = si.backface ? si.pos - (si.n * R(HIT_EPS_F)) : si.pos + (si.n * R(HIT_EPS_F));
     Ray<R> sr(org, Normalize(target - org));
@@ -1404,25 +1432,11 @@ template <class R> V3<R> Li(const FrameConst& fc, Ray<R> ray, Sampler<R>& sample
                     const Light& l = S->lights[li];
                     R pmf = R(1) / R((float)nlights);
                     V2<R> u2 = sampler.get2D();
-                    // Light::sampleLi + Triangle::sample, light.cu:33-45, triangle.cuh:104-119
-                    const Triangle& tri = S->tris[l.prim_id];
-                    M4<R> model = CvtM4<R>(Inverse(S->meshes[l.inst_id].curr_inv_model));
-                    V2<R> bb;
-                    if (u2.x < u2.y) { bb.x = u2.x / R(2); bb.y = u2.y - bb.x; }
-                    else { bb.y = u2.y / R(2); bb.x = u2.x - bb.y; }
-                    V3<R> bary(bb.x, bb.y, R(1) - bb.x - bb.y);
-                    V2<R> luv;
-                    luv.x = (bary.x * R(tri.t[0].x)) + (bary.y * R(tri.t[1].x)) + (bary.z * R(tri.t[2].x));
-                    luv.y = (bary.x * R(tri.t[0].y)) + (bary.y * R(tri.t[1].y)) + (bary.z * R(tri.t[2].y));
-                    V3<R> p = (cvt<R>(tri.p[0]) * bary.x) + (cvt<R>(tri.p[1]) * bary.y) + (cvt<R>(tri.p[2]) * bary.z);
-                    p = xyz(Mul(model, V4<R>(p, R(1))));
-                    V3<R> gn = xyz(Mul(model, V4<R>(cvt<R>(tri.local_geometric_normal), R(0))));
-                    R area = R(0.5f) * Length(Cross(cvt<R>(tri.p[1]) - cvt<R>(tri.p[0]), cvt<R>(tri.p[2]) - cvt<R>(tri.p[0])));
-                    R spdf = R(1) / area;
-                    spdf *= Sqr(Distance(p, si.pos));
-                    spdf /= AbsDot(Normalize(p - si.pos), gn);
+                    LightSample<R> ls = SampleLight(S, l, si.pos, u2);
+                    V3<R> p = ls.p;
+                    R spdf = ls.pdf;
                     V3<R> wi = Normalize(p - si.pos);
-                    V3<R> lL = LightL(S, l, luv);
+                    V3<R> lL = ls.L;
                     bool ok = !(spdf <= R(0) || IsZero(lL));
                     V3<R> wo = -ray.d;
                     V3<R> fcos(0, 0, 0);
@@ -2033,5 +2047,78 @@ void orc_camera_rays(const kpt_camera* cam, int32_t W, int32_t H, const int32_t*
 
 void orc_glm_inverse(const float m[16], float out[16]) { StoreM4(GlmInverse(LoadM4(m)), out); }
 void orc_mat4_inverse(const float m[16], float out[16]) { StoreM4(Inverse(LoadM4(m)), out); }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ shading probes
+// Per-routine views of the integrator's building blocks for the pinning
+// tests; each calls the same functions Li calls.
+extern "C" {
+
+/* getSurfaceInteraction (interaction.cu:11-38) and Material::getBSDF
+ * (material.cu:17-70, texture.cuh:34-66) at each ray's closest hit.
+ * out [n][24]: hit, t, position xyz, world geometric normal xyz (flipped to
+ * face -d), uv, backface, material, albedo rgb, metallic, roughness,
+ * transmission, ior, shading normal xyz (the BSDF basis' third column),
+ * light index (-1 none), 0. */
+void orc_shade_probe(const orc_scene* s, const kpt_ray* rays, int64_t n, float* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        float* o = out + 24 * i;
+        std::memset(o, 0, 24 * sizeof(float));
+        Ray<float> r(V3<float>(rays[i].org[0], rays[i].org[1], rays[i].org[2]),
+                     V3<float>(rays[i].dir[0], rays[i].dir[1], rays[i].dir[2]));
+        Counters cn;
+        Isect<float> it = TlasIntersect(s, r, rays[i].tmin, rays[i].tmax, &cn);
+        o[22] = -1.0f;
+        if (it.prim < 0) continue;
+        SurfInt<float> si = GetSurf(s, it, r);
+        BSDF<float> b = GetBSDF(s, si);
+        o[0] = 1.0f; o[1] = si.distance;
+        o[2] = si.pos.x; o[3] = si.pos.y; o[4] = si.pos.z;
+        o[5] = si.n.x; o[6] = si.n.y; o[7] = si.n.z;
+        o[8] = si.uv.x; o[9] = si.uv.y;
+        o[10] = si.backface ? 1.0f : 0.0f;
+        o[11] = (float)si.material;
+        o[12] = b.albedo.x; o[13] = b.albedo.y; o[14] = b.albedo.z;
+        o[15] = b.metallic; o[16] = b.roughness; o[17] = b.transmission; o[18] = b.ior;
+        V3<float> z = Mul3(b.basis, V3<float>(0.0f, 0.0f, 1.0f));
+        o[19] = z.x; o[20] = z.y; o[21] = z.z;
+        o[22] = si.light ? (float)(si.light - s->lights.data()) : -1.0f;
+    }
+}
+
+/* Light::sampleLi (light.cu:33-45) of light `light` seen from positions
+ * from[n][3] with numbers u[n][2]: out [n][12] = point xyz, world normal
+ * xyz (unnormalised, as the reference transforms it), pdf in solid angle,
+ * emitted radiance rgb, uv.  pdf_li [n] (optional): Light::pdf_Li
+ * (light.cu:11-18) of the same point, the pdf the emitter-hit MIS weight
+ * uses. */
+void orc_light_sample(const orc_scene* s, int32_t light, const float* from, const float* u, int64_t n, float* out,
+                      float* pdf_li) {
+    const Light& l = s->lights[light];
+    for (int64_t i = 0; i < n; ++i) {
+        V3<float> f(from[3 * i], from[3 * i + 1], from[3 * i + 2]);
+        V2<float> u2;
+        u2.x = u[2 * i]; u2.y = u[2 * i + 1];
+        LightSample<float> ls = SampleLight(s, l, f, u2);
+        float* o = out + 12 * i;
+        o[0] = ls.p.x; o[1] = ls.p.y; o[2] = ls.p.z;
+        o[3] = ls.n.x; o[4] = ls.n.y; o[5] = ls.n.z;
+        o[6] = ls.pdf;
+        o[7] = ls.L.x; o[8] = ls.L.y; o[9] = ls.L.z;
+        o[10] = ls.uv.x; o[11] = ls.uv.y;
+        if (pdf_li) pdf_li[i] = LightPdfLi(l, f, ls.p, Normalize(ls.n));
+    }
+}
+
+/* encodeUnitVector (packing.cuh:105-125): out [n][2]. */
+void orc_encode_unit(const float* v, int64_t n, float* out) {
+    for (int64_t i = 0; i < n; ++i) EncodeUnit(V3<float>(v[3 * i], v[3 * i + 1], v[3 * i + 2]), &out[2 * i], &out[2 * i + 1]);
+}
+
+/* floatToByte (packing.cuh:13-17). */
+void orc_float_to_byte(const float* v, int64_t n, uint32_t* out) {
+    for (int64_t i = 0; i < n; ++i) out[i] = FloatToByte(v[i]);
+}
 
 }  // extern "C"

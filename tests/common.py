@@ -72,3 +72,35 @@ def random_rays(n, lo, hi, target_center, target_spread, seed=0, tmax=np.inf):
     rays["tmin"] = 0.0
     rays["tmax"] = tmax
     return rays
+
+
+def furnace_cube(half=5.0, albedo=(0.4, 0.25, 0.6), emission=(1.0, 0.5, 2.0), roughness=1.0, ior=1.0,
+                 scale_nits=1.0):
+    """A closed cube of 12 equal-area triangles facing inward, every one an
+    emitter of uniform radiance `emission` * `scale_nits` and an opaque
+    dielectric of albedo `albedo`.  With IOR 1 the dielectric's Fresnel term
+    is 0 and its lobe is Lambertian (f = albedo / pi), so the radiance inside
+    has a closed form (the furnace test).  The camera sits at the centre."""
+    c = np.array([[x, y, z] for x in (-half, half) for y in (-half, half) for z in (-half, half)], np.float64)
+    faces = []
+    for axis in range(3):
+        for sgn in (-1.0, 1.0):
+            idx = [i for i in range(8) if c[i, axis] * sgn > 0]
+            # quad corners in cyclic order around the face
+            q = sorted(idx, key=lambda i: math.atan2(c[i, (axis + 2) % 3], c[i, (axis + 1) % 3]))
+            inward = np.zeros(3)
+            inward[axis] = -sgn
+            for tri in ((q[0], q[1], q[2]), (q[0], q[2], q[3])):
+                p = c[list(tri)]
+                if np.dot(np.cross(p[1] - p[0], p[2] - p[0]), inward) < 0:
+                    p = p[[0, 2, 1]]
+                faces.append((p, np.tile(inward, (3, 1))))
+    P = np.stack([f[0] for f in faces])
+    N = np.stack([f[1] for f in faces])
+    UV = np.zeros((len(P), 3, 2))
+    mats = [scenes.material(albedo=albedo, metallic=0.0, roughness=roughness, transmission=0.0, ior=ior,
+                            emission=emission, emission_scale_nits=scale_nits)]
+    sc = scenes._pack([(P, N, UV, np.zeros(len(P), np.int32))], mats, None,
+                      camera_eye=(0.0, 0.0, 0.0), camera_target=(0.0, 0.0, 1.0),
+                      camera_fov=math.radians(30.0), sun_theta=math.radians(30.0))
+    return sc

@@ -1,0 +1,599 @@
+"""Independent pins for the oracle's integrator composition and shading
+building blocks: closed forms, textbook formulas and invariants that a
+plausible mistake (a dropped term, a wrong sign or index, a transposed
+operand) fails.  CPU only.
+
+Integrator::Li (integrator.cuh:241-402) is pinned by an emissive furnace:
+inside a closed, uniformly emitting enclosure whose surfaces are Lambertian
+with albedo a, every point receives radiance Le from every direction, so
+the expected estimate has a closed form whatever the next-event /
+BSDF-sampling split, multiple importance weights and Russian roulette do,
+provided each is unbiased.  The reference's opaque dielectric is exactly
+Lambertian at IOR 1 (its Fresnel term vanishes), which makes the test
+possible with its own material model.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from kittlespt_amd import abi, scenes
+
+from common import furnace_cube, host_camera, settings
+
+A = np.array([0.4, 0.25, 0.6])          # furnace albedo per channel
+LE = np.array([1.0, 0.5, 2.0])          # furnace emission per channel
+
+
+@pytest.fixture(scope="module")
+def furnace():
+    sc = furnace_cube(albedo=tuple(A), emission=tuple(LE))
+    orc = oracle.OracleScene(sc)
+    yield sc, orc
+    orc.close()
+
+
+def _centre_pixel_batches(orc, sc, st, batches=8, spp=40_000, double=False):
+    """Per-batch means of the centre pixel's estimate (a 1x1 image: every
+    sample has its own seed).  The float oracle returns the reference's
+    sum / (last frame + 1) (integrator.cuh:406-416), so batch k, frames
+    [k*spp, (k+1)*spp), is rescaled by k + 1; the double one returns the
+    batch mean."""
+    cam = host_camera(sc, 1, 1)
+    out = []
+    for k in range(batches):
+        e = orc.render_pixels(st, cam, 1, 1, np.zeros(1, np.int32), k * spp, spp, double=double, threads=1)[0]
+        out.append(e.astype(np.float64) * (1 if double else k + 1))
+    out = np.array(out)
+    return out.mean(0), out.std(0, ddof=1) / math.sqrt(batches)
+
+
+def _furnace_settings(depth):
+    st = settings(max_depth=depth)
+    st.sun_emission_factor = 0.0        # no sky or sun: the enclosure is the only light
+    return st
+
+
+@pytest.mark.parametrize("double", [False, True])
+def test_furnace_closed_form(furnace, double):
+    """Depth 20: L = Le * (1 + sum_{v<D} a^v) + O(a^D) per channel.  The first
+    '1' is the primary hit's own emission; the primary vertex is demodulated
+    (albedo 1, integrator.cuh:351-356), so its reflection of the direct light
+    counts 1 and the light reflected at vertex v >= 1 counts a^v.  Paths lose
+    throughput 2a < 1 per diffuse bounce, so Russian roulette (sampling.cuh:
+    19-31) runs from depth 2; the 50 % sun / area-light split with its 1/0.5
+    (integrator.cuh:366-377) and the power-heuristic weights on both sides
+    (:188-220, :334-343) must sum to one for the total to come out."""
+    sc, orc = furnace
+    D = 20
+    m, se = _centre_pixel_batches(orc, sc, _furnace_settings(D), double=double)
+    want = LE * (1.0 + (1.0 - A ** D) / (1.0 - A))
+    tol = 6 * se + LE * A ** D + 1e-4 * want
+    print("furnace", m, "+-", se, "want", want)
+    assert np.all(np.abs(m - want) <= tol), (m, want, se)
+
+
+def _nee_weight_at_face_centre(roughness, half=5.0, n=1_000_000, seed=5):
+    """W_l = integral of (cos/pi) * w_l over the hemisphere at the centre of
+    the cube's +z face, seen at normal incidence: w_l = p_l^2/(p_l^2 + p_b^2)
+    with p_l the solid-angle pdf of uniform light picking (12 equal
+    triangles) times uniform area sampling, and p_b the opaque dielectric's
+    pdf 0.5*(GGX half-vector pdf + cos/pi).  Textbook formulas evaluated in
+    numpy (cosine-weighted Monte Carlo with analytic ray-box exits)."""
+    rng = np.random.default_rng(seed)
+    u1, u2 = rng.random(n), rng.random(n)
+    r, phi = np.sqrt(u1), 2 * np.pi * u2
+    cos_i = np.sqrt(1 - u1)
+    wi = np.stack([r * np.cos(phi), r * np.sin(phi), -cos_i], 1)     # about the inward normal -z
+    p = np.array([0.0, 0.0, half])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t = np.where(wi > 0, (half - p) / wi, np.where(wi < 0, (-half - p) / wi, np.inf))
+    t = np.where(t > 1e-9, t, np.inf)
+    ax = t.argmin(1)
+    d = t[np.arange(n), ax]
+    cos_l = np.abs(wi[np.arange(n), ax])
+    p_l = (1.0 / 12.0) * (1.0 / (2 * half * half)) * d * d / cos_l
+    wo = np.array([0.0, 0.0, -1.0])
+    h = wi + wo
+    h /= np.linalg.norm(h, axis=1, keepdims=True)
+    noh = np.maximum(-h[:, 2], 0.0)
+    voh = np.maximum(h @ wo, 0.0)
+    a2 = (roughness ** 2) ** 2
+    D = a2 / (np.pi * (noh * noh * (a2 - 1) + 1) ** 2)
+    p_b = 0.5 * (D * noh / (4 * voh) + cos_i / np.pi)
+    w = p_l ** 2 / (p_l ** 2 + p_b ** 2)
+    return w.mean(), w.std() / math.sqrt(n)
+
+
+def test_furnace_depth0_next_event_weight(furnace):
+    """Depth 0: L = Le * (1 + W_l).  Only the primary vertex's next-event
+    estimate reflects light, weighted by the power heuristic against the
+    BSDF pdf; W_l comes from an independent numpy integral.  A balance
+    heuristic, a dropped 1/0.5 or a missing light-pick probability fails."""
+    sc, orc = furnace
+    m, se = _centre_pixel_batches(orc, sc, _furnace_settings(0))
+    wl, wl_se = _nee_weight_at_face_centre(1.0)
+    want = LE * (1.0 + wl)
+    print("depth 0", m / LE, "want", 1 + wl, "+-", wl_se)
+    assert np.all(np.abs(m - want) <= 6 * se + LE * 6 * wl_se)
+
+
+@pytest.mark.parametrize("depth", [1, 2, 3])
+def test_furnace_depth_count(furnace, depth):
+    """Depth D: L/Le = 1 + sum_{v<D} a^v + a^D * W where W (the next-event
+    weight at the last vertex, whose BSDF continuation is never traced) is
+    the same for all three channels.  Solving each channel for W must give
+    one value in [0, 1]: an off-by-one in the depth test (integrator.cuh:
+    380, 'depth++ == max_ray_depth') or a per-channel throughput mix-up
+    does not."""
+    sc, orc = furnace
+    m, se = _centre_pixel_batches(orc, sc, _furnace_settings(depth), batches=8, spp=30_000)
+    base = 1.0 + (1.0 - A ** depth) / (1.0 - A)
+    W = (m / LE - base) / A ** depth
+    W_se = se / LE / A ** depth
+    print("depth", depth, "W per channel", W, "+-", W_se)
+    assert np.all(W > -6 * W_se) and np.all(W < 1 + 6 * W_se)
+    pair_tol = 6 * np.hypot(W_se[:, None], W_se[None, :])
+    assert np.all(np.abs(W[:, None] - W[None, :]) <= pair_tol + 1e-9), W
+
+
+def test_sun_next_event_estimate():
+    """sampleLdSun (integrator.cuh:147-186) on an unoccluded Lambertian ground
+    at depth 0: the sun is picked with probability 0.5 and divided by it, the
+    point on the sun is uniform in a cube of half-size r around its centre
+    (r = D tan(theta_d / 2)) and its pdf is (1 / (pi r^2)) / (cos_s / D^2)
+    with cos_s the cosine between the normalised offset and the direction.
+    So E[L] = (1/pi) cos(theta_sun) * sky(sun) * 6e5 * pi tan^2(theta_d/2) *
+    E|offset_hat . sun|, the last factor a numpy integral over the cube and
+    sky(sun) the pinned Nishita routine."""
+    half = 50.0
+    p = np.array([[[-half, 0, -half], [half, 0, half], [half, 0, -half]],
+                  [[-half, 0, -half], [-half, 0, half], [half, 0, half]]], np.float64)
+    n = np.tile(np.array([0, 1, 0], np.float64), (2, 3, 1))
+    mats = [scenes.material(albedo=(0.7, 0.7, 0.7), roughness=1.0, ior=1.0)]
+    sc = scenes._pack([(p, n, np.zeros((2, 3, 2)), np.zeros(2, np.int32))], mats, None,
+                      camera_eye=(0.0, 5.0, 0.0), camera_target=(0.0, 0.0, 0.001),
+                      camera_fov=math.radians(30.0))
+    orc = oracle.OracleScene(sc)
+    st = settings(max_depth=0)
+    theta = st.sun_theta_rad
+    m, se = _centre_pixel_batches(orc, sc, st, batches=8, spp=40_000)
+    orc.close()
+    sun = np.array([math.cos(st.sun_phi_rad) * math.cos(theta), math.sin(theta),
+                    math.sin(st.sun_phi_rad) * math.cos(theta)])
+    sun /= np.linalg.norm(sun)
+    sky = oracle.sky_li([[0.0, 6378138.0, 0.0]], [sun], sun, emission=st.sun_emission_factor, double=True)[0]
+    rng = np.random.default_rng(3)
+    off = rng.uniform(-1, 1, (2_000_000, 3))
+    off /= np.linalg.norm(off, axis=1, keepdims=True)
+    e_cos = np.abs(off @ sun).mean()
+    want = (1 / math.pi) * sun[1] * sky * 6.0e5 * math.pi * math.tan(st.sun_angular_diameter_rad / 2) ** 2 * e_cos
+    print("sun NEE", m, "+-", se, "want", want)
+    assert np.all(np.abs(m - want) <= 6 * se + 2e-3 * want)
+
+
+# ----------------------------------------------------------------- glass and conductor lobes
+def _glass(rough, backface=0, ior=1.5):
+    return [1.0, 1.0, 1.0, 0.0, rough, 1.0, ior, float(backface)]
+
+
+def _conductor(rough, albedo=1.0):
+    return [albedo, albedo, albedo, 1.0, rough, 0.0, 1.5, 0.0]
+
+
+def _fresnel_textbook(c, eta):
+    """Unpolarised Fresnel reflectance for light arriving at cos c from the
+    outside of a medium of relative index eta (no TIR for eta > 1)."""
+    st = np.sqrt(np.maximum(0.0, 1 - c * c)) / eta
+    ct = np.sqrt(np.maximum(0.0, 1 - st * st))
+    rs = ((c - eta * ct) / (c + eta * ct)) ** 2
+    rp = ((eta * c - ct) / (eta * c + ct)) ** 2
+    return 0.5 * (rs + rp)
+
+
+def _upper_dirs(n, rng):
+    d = rng.normal(size=(n, 3))
+    d[:, 2] = np.abs(d[:, 2]) + 0.05
+    return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+@pytest.mark.parametrize("rough", [0.15, 0.5, 0.9])
+def test_glass_reflection_is_fresnel_times_microfacet_brdf(rough):
+    """fTransparentDielectric and pdfTransparentDielectric on the reflection
+    side (bsdf.cuh:447-490, 492-524): the conductor lobe's microfacet BRDF
+    and half-vector pdf (F0 = 1, so Schlick's term is 1, bsdf.cuh:367-386)
+    times the dielectric Fresnel reflectance at wo.h (textbook)."""
+    rng = np.random.default_rng(21)
+    wo, wi = _upper_dirs(20000, rng), _upper_dirs(20000, rng)
+    fg, pg = oracle.bsdf_eval(_glass(rough), wo, wi, double=True)
+    fc, pc = oracle.bsdf_eval(_conductor(rough), wo, wi, double=True)
+    h = wo + wi
+    h /= np.linalg.norm(h, axis=1, keepdims=True)
+    F = _fresnel_textbook(np.abs((wo * h).sum(1)), 1.5)
+    ok = fc[:, 0] > 1e-6
+    assert ok.mean() > 0.5
+    assert np.allclose(fg[ok, 0], F[ok] * fc[ok, 0], rtol=2e-5)
+    assert np.allclose(pg[ok], F[ok] * pc[ok], rtol=2e-5)
+
+
+def _transmitted_energy(wo, rough, backface, n=400_000, seed=23, eta_m=1.5):
+    """Integral of f_T(wo, wi) |cos wi| over the transmitted directions by
+    textbook microfacet importance sampling (Walter et al. 2007): h drawn
+    from D(h) cos(theta_h) (GGX, alpha = roughness^2 as the reference's GGX
+    stores it), wi = Snell refraction of wo about h, pdf(wi) = D cos(theta_h)
+    |dh/dwi| with |dh/dwi| = eta_i^2 |wi.h| / (eta_o (wo.h) + eta_i (wi.h))^2
+    (eta_o of wo's side, eta_i of wi's side).  f_T comes from the oracle."""
+    rng = np.random.default_rng(seed)
+    alpha = rough ** 2
+    u1, u2 = rng.random(n), rng.random(n)
+    tan2 = alpha * alpha * u1 / (1 - u1)
+    ch = 1 / np.sqrt(1 + tan2)
+    sh = np.sqrt(1 - ch * ch)
+    ph = 2 * np.pi * u2
+    h = np.stack([sh * np.cos(ph), sh * np.sin(ph), ch], 1)
+    eta_o, eta_i = (eta_m, 1.0) if backface else (1.0, eta_m)
+    r = eta_o / eta_i
+    c = h @ wo
+    k = 1 - r * r * (1 - c * c)
+    ok = (c > 0) & (k > 0)
+    wi = -r * wo[None, :] + (r * c - np.sqrt(np.maximum(k, 0)))[:, None] * h
+    wi = wi[ok]
+    hh, cc = h[ok], c[ok]
+    D = alpha * alpha / (np.pi * (hh[:, 2] ** 2 * (alpha * alpha - 1) + 1) ** 2)
+    wih = (wi * hh).sum(1)
+    jac = eta_i * eta_i * np.abs(wih) / (eta_o * cc + eta_i * wih) ** 2
+    pdf = D * hh[:, 2] * jac
+    f, _ = oracle.bsdf_eval(_glass(rough, backface, eta_m), np.tile(wo, (len(wi), 1)), wi, double=True)
+    w = np.zeros(n)
+    w[ok] = f[:, 0].astype(np.float64) * np.abs(wi[:, 2]) / pdf
+    return w.mean(), w.std() / math.sqrt(n)
+
+
+@pytest.mark.parametrize("theta", [0.0, 30.0, 60.0])
+def test_glass_entry_transmission_energy(theta):
+    """The refraction branch of fTransparentDielectric (bsdf.cuh:469-489) is
+    Walter's BTDF without its eta_o^2 radiance factor: entering glass
+    (eta 1.5), its transmitted energy at low roughness tends to
+    (1 - F(theta)) / eta^2.  A Jacobian with wo.h and wi.h swapped, or the
+    index on the wrong term of (eta wi.h + wo.h), is off by tens of percent."""
+    wo = np.array([math.sin(math.radians(theta)), 0.0, math.cos(math.radians(theta))])
+    e, se = _transmitted_energy(wo, 0.2, 0)
+    want = (1 - _fresnel_textbook(np.array([wo[2]]), 1.5)[0]) / 1.5 ** 2
+    print("entry", theta, e, "+-", se, "want", want)
+    assert abs(e - want) < 0.02 * want + 6 * se
+
+
+@pytest.mark.parametrize("theta", [0.0, 15.0, 25.0])
+def test_glass_exit_transmission_energy(theta):
+    """Leaving glass (backface, bsdf.cuh:451-455) the BTDF uses the inverse
+    index, and its Fresnel term is still fresnelDielectric(|wo.h|, eta) as if
+    entering (bsdf.cuh:463): below the critical angle the transmitted energy
+    tends to eta^2 (1 - F_entry(theta)), so an enter/exit pair carries
+    (1 - F)^2 with no net index factor.  (Closer to the critical angle,
+    41.8 degrees, rough microfacets reflect totally and the limit no longer
+    holds.)"""
+    wo = np.array([math.sin(math.radians(theta)), 0.0, math.cos(math.radians(theta))])
+    e, se = _transmitted_energy(wo, 0.2, 1)
+    want = 1.5 ** 2 * (1 - _fresnel_textbook(np.array([wo[2]]), 1.5)[0])
+    print("exit", theta, e, "+-", se, "want", want)
+    assert abs(e - want) < 0.02 * want + 6 * se
+
+
+def test_glass_samples_refract_by_snell_and_match_eval():
+    """sampleTransparentDielectric (bsdf.cuh:394-445), entering: refracted
+    samples obey Snell's law about their generalised half vector
+    h = normalize(eta wi + wo) (|wo x h| = eta |wi x h|, opposite sides of
+    h); both branches return f equal to f(wo, wi) divided by the branch
+    probability 0.5, and reflected samples are mirror images about h."""
+    n = 50_000
+    rng = np.random.default_rng(25)
+    u = rng.random((n, 4)).astype(np.float32)
+    u[:, 2] = 0.0                                   # transparent lobe (metallic 0, transmission 1)
+    wo = _upper_dirs(n, rng)
+    wi, f, pdf, fl = oracle.bsdf_sample(_glass(0.4), wo, u, double=True)
+    fe, _ = oracle.bsdf_eval(_glass(0.4), wo, wi, double=True)
+    wi = wi.astype(np.float64)
+    tr = (fl & 4) != 0
+    rf = (fl & 2) != 0
+    assert tr.mean() > 0.3 and rf.mean() > 0.3
+    h = 1.5 * wi[tr] + wo[tr]
+    h /= np.linalg.norm(h, axis=1, keepdims=True)
+    s_o = np.linalg.norm(np.cross(wo[tr], h), axis=1)
+    s_i = np.linalg.norm(np.cross(wi[tr], h), axis=1)
+    assert np.allclose(s_o, 1.5 * s_i, atol=2e-5)
+    assert np.all((wo[tr] @ np.array([0, 0, 1.0])) * wi[tr, 2] < 0)
+    # sampleWm draws from D(h) cos(theta_h), not from the microfacets visible
+    # from wo: a sample on a microfacet facing away from wo (a few in 10^4
+    # here) refracts through it, while f(wo, wi) sees the half vector of the
+    # resulting pair, so those few differ
+    for side in (tr, rf):
+        good = side & (f[:, 0] > 1e-4)
+        same = np.isclose(f[good, 0], 2 * fe[good, 0], rtol=1e-4)
+        assert same.mean() > 0.995, same.mean()
+    hr = wo[rf] + wi[rf]
+    hr /= np.linalg.norm(hr, axis=1, keepdims=True)
+    mirror = 2 * (wo[rf] * hr).sum(1)[:, None] * hr - wo[rf]
+    assert np.allclose(mirror, wi[rf], atol=2e-5)
+
+
+@pytest.mark.parametrize("albedo", [0.05, 0.5, 0.95])
+def test_conductor_is_schlick_fresnel(albedo):
+    """fConductor (bsdf.cuh:367-386) divided by the same lobe with albedo 1
+    is Schlick's approximation F0 + (1 - F0)(1 - v.h)^5 at the albedo,
+    clamped to [0, 1] (bsdf.cuh:144-148; Schlick 1994)."""
+    rng = np.random.default_rng(27)
+    wo, wi = _upper_dirs(20000, rng), _upper_dirs(20000, rng)
+    fa, _ = oracle.bsdf_eval(_conductor(0.5, albedo), wo, wi, double=True)
+    f1, _ = oracle.bsdf_eval(_conductor(0.5, 1.0), wo, wi, double=True)
+    h = wo + wi
+    h /= np.linalg.norm(h, axis=1, keepdims=True)
+    voh = np.clip((wo * h).sum(1), 0, 1)
+    want = np.clip(albedo + (1 - albedo) * (1 - voh) ** 5, 0, 1)
+    ok = f1[:, 0] > 1e-6
+    assert np.allclose(fa[ok, 0] / f1[ok, 0], want[ok], rtol=2e-5)
+
+
+# ----------------------------------------------------------------- textures, materials, interaction
+def _srgb_decode_iec(c):
+    """IEC 61966-2-1 sRGB electro-optical transfer function."""
+    c = np.asarray(c, np.float64)
+    return np.where(c <= 0.04045, c / 12.92, ((c + 0.055) / 1.055) ** 2.4)
+
+
+def _quad_scene(material, textures, pixels, u_range=(-1.25, 2.75), v_range=(-0.5, 1.5), half=2.0, z=3.0):
+    """One quad at depth z facing the camera (normal -z), its uv an affine
+    map of (x, y): u spans u_range over x in [-half, half], v spans v_range
+    over y."""
+    x0, x1, y0, y1 = -half, half, -half, half
+    P = np.array([[[x0, y0, z], [x1, y0, z], [x1, y1, z]], [[x0, y0, z], [x1, y1, z], [x0, y1, z]]], np.float64)
+    N = np.tile(np.array([0.0, 0.0, -1.0]), (2, 3, 1))
+    uv_of = lambda x, y: ((x - x0) / (x1 - x0) * (u_range[1] - u_range[0]) + u_range[0],  # noqa: E731
+                          (y - y0) / (y1 - y0) * (v_range[1] - v_range[0]) + v_range[0])
+    UV = np.array([[uv_of(p[0], p[1]) for p in tri] for tri in P])
+    sc = scenes._pack([(P, N, UV, np.zeros(2, np.int32))], [material], None, textures=textures,
+                      pixels=pixels)
+    xy_of = lambda u, v: (x0 + (u - u_range[0]) / (u_range[1] - u_range[0]) * (x1 - x0),  # noqa: E731
+                          y0 + (v - v_range[0]) / (v_range[1] - v_range[0]) * (y1 - y0))
+    return sc, xy_of
+
+
+def _rays_at(points, z0=0.0):
+    rays = np.zeros(len(points), abi.RAY_DTYPE)
+    rays["org"] = np.column_stack([np.asarray(points, np.float64), np.full(len(points), z0)])
+    rays["dir"] = (0.0, 0.0, 1.0)
+    rays["tmin"] = 0.0
+    rays["tmax"] = np.inf
+    return rays
+
+
+def _texel_uvs(W, H, rng, n, u_range=(-1.25, 2.75), v_range=(-0.5, 1.5)):
+    """uv at texel centres inside the quad's uv ranges, including ones
+    outside [0, 1) (wrap)."""
+    tx, ty = rng.integers(0, W, 4 * n), rng.integers(0, H, 4 * n)
+    u = (tx + 0.5) / W + rng.integers(-2, 3, 4 * n)
+    v = (ty + 0.5) / H + rng.integers(-2, 3, 4 * n)
+    ok = (u > u_range[0]) & (u < u_range[1]) & (v > v_range[0]) & (v < v_range[1])
+    assert ok.sum() >= n
+    return u[ok][:n], v[ok][:n], tx[ok][:n], ty[ok][:n]
+
+
+def test_texture_reads_wrap_and_srgb_decode():
+    """Material::getBSDF with albedo, ORM and transmission textures
+    (material.cu:17-44, texture.cuh:34-66): the nearest texel of the wrapped
+    uv (x = int(frac(u) W)), bytes times 1/255, the albedo factor times the
+    IEC 61966-2-1 decode of the albedo texel, roughness and metallic times
+    the ORM texel's green and blue, transmission times the red."""
+    rng = np.random.default_rng(31)
+    W3, H3 = 5, 3                    # 3-channel albedo
+    W4, H4 = 4, 6                    # 4-channel ORM / transmission
+    alb = rng.integers(0, 256, (H3, W3, 3), dtype=np.uint8)
+    orm = rng.integers(0, 256, (H4, W4, 4), dtype=np.uint8)
+    pixels = np.concatenate([alb.reshape(-1), orm.reshape(-1), np.zeros(4, np.uint8)])
+    textures = [(W3, H3, 3, 0), (W4, H4, 4, alb.size)]
+    mat = scenes.material(albedo=(0.9, 0.6, 0.3), roughness=0.8, metallic=0.7, transmission=0.5,
+                          albedo_texture_id=0, orm_texture_id=1, transmission_texture_id=1, ior=1.4)
+    # the albedo and ORM textures share the uv: pick texel centres of the finer grid in each axis
+    sc, xy_of = _quad_scene(mat, textures, pixels)
+    orc = oracle.OracleScene(sc)
+    n = 400
+    u, v, _, _ = _texel_uvs(W3 * W4, H3 * H4, rng, n)
+    x, y = xy_of(u, v)
+    pr = oracle.shade_probe(orc, _rays_at(np.column_stack([x, y])))
+    orc.close()
+    assert np.all(pr[:, 0] == 1)
+    fu, fv = u - np.floor(u), v - np.floor(v)
+    ax, ay = (fu * W3).astype(int) % W3, (fv * H3).astype(int) % H3
+    ox, oy = (fu * W4).astype(int) % W4, (fv * H4).astype(int) % H4
+    want_alb = np.array([0.9, 0.6, 0.3]) * _srgb_decode_iec(alb[ay, ax] / 255.0)
+    assert np.allclose(pr[:, 12:15], want_alb, rtol=2e-5, atol=1e-7)
+    t = orm[oy, ox] / 255.0
+    assert np.allclose(pr[:, 16], 0.8 * t[:, 1], rtol=1e-5)          # roughness * G
+    assert np.allclose(pr[:, 15], 0.7 * t[:, 2], rtol=1e-5)          # metallic * B
+    assert np.allclose(pr[:, 17], 0.5 * t[:, 0], rtol=1e-5)          # transmission * R
+    assert np.allclose(pr[:, 18], 1.4)
+    assert np.allclose(pr[:, 8:10], np.column_stack([u, v]), atol=2e-5)
+
+
+def test_normal_map_rotates_the_normal_rigidly():
+    """Normal mapping (material.cu:45-57): the texel maps to m = 2 s - 1 with
+    x, y scaled, and the shading normal is an orthonormal frame around the
+    geometric normal applied to normalize(m), so the angle between shading
+    and geometric normal equals the angle of m from +z; a mapped normal
+    facing away from wo falls back to the geometric normal."""
+    texels = np.array([[128, 128, 255], [200, 100, 230], [40, 220, 180], [255, 128, 0]], np.uint8)
+    pixels = np.concatenate([texels.reshape(-1), np.zeros(4, np.uint8)])
+    scale = 0.8
+    mat = scenes.material(albedo=(0.5, 0.5, 0.5), normal_texture_id=0, normal_scale=scale)
+    sc, xy_of = _quad_scene(mat, [(4, 1, 3, 0)], pixels, u_range=(0.0, 1.0), v_range=(0.0, 1.0))
+    orc = oracle.OracleScene(sc)
+    u = (np.arange(4) + 0.5) / 4
+    x, y = xy_of(u, np.full(4, 0.5))
+    pr = oracle.shade_probe(orc, _rays_at(np.column_stack([x, y])))
+    orc.close()
+    geo = pr[:, 5:8].astype(np.float64)
+    sn = pr[:, 19:22].astype(np.float64)
+    assert np.allclose(geo, [0.0, 0.0, -1.0])
+    m = texels / 255.0 * 2 - 1
+    m[:, :2] *= scale
+    m /= np.linalg.norm(m, axis=1, keepdims=True)
+    ang_sn = np.arccos(np.clip((sn * geo).sum(1), -1, 1))
+    ang_m = np.arccos(np.clip(m[:, 2], -1, 1))
+    facing = m[:, 2] > 0
+    assert np.allclose(ang_sn[facing], ang_m[facing], atol=2e-4), (ang_sn, ang_m)
+    assert ang_m[1] > 0.2 and ang_m[2] > 0.2
+    assert not facing[3] and np.allclose(sn[3], geo[3], atol=1e-6)
+
+
+def test_surface_interaction_of_instanced_hits():
+    """getSurfaceInteraction (interaction.cu:11-38) on rotated, uniformly
+    scaled instances: position o + t d, world normal = the local geometric
+    normal (cross product oriented by the averaged vertex normals) mapped by
+    the inverse transpose of the model matrix, flipped to face the ray
+    (backface flag), uv interpolated with the hit's barycentrics, and the
+    triangle's material."""
+    sc = scenes.scene_instanced_small()
+    orc = oracle.OracleScene(sc)
+    rng = np.random.default_rng(33)
+    n = 20000
+    o = rng.uniform(-10, 110, (n, 3))
+    centres = sc.instance_model.reshape(-1, 4, 4)[:, 3, :3].astype(np.float64)
+    tgt = centres[rng.integers(0, len(centres), n)] + rng.normal(size=(n, 3))
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros(n, abi.RAY_DTYPE)
+    rays["org"], rays["dir"], rays["tmax"] = o, d, np.inf
+    hits, _, _ = orc.trace(rays, oracle.MODE_CLOSEST)
+    pr = oracle.shade_probe(orc, rays)
+    orc.close()
+    hit = hits["prim_id"] >= 0
+    assert hit.mean() > 0.2 and np.array_equal(pr[:, 0] == 1, hit)
+    prim, inst = hits["prim_id"][hit], hits["inst_id"][hit]
+    ro, rd = rays["org"][hit].astype(np.float64), rays["dir"][hit].astype(np.float64)
+    t = hits["t"][hit].astype(np.float64)
+    assert np.allclose(pr[hit, 1], t)
+    assert np.allclose(pr[hit, 2:5], ro + t[:, None] * rd, rtol=1e-5, atol=1e-4)
+    P = sc.positions[prim].astype(np.float64)
+    gn = np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0])
+    avg = sc.normals[prim].astype(np.float64).mean(1)
+    gn *= np.where((gn * avg).sum(1) < 0, -1.0, 1.0)[:, None]
+    M = sc.instance_model[inst].astype(np.float64).reshape(-1, 4, 4).transpose(0, 2, 1)[:, :3, :3]
+    nw = np.einsum("nij,nj->ni", np.linalg.inv(M).transpose(0, 2, 1), gn)
+    nw /= np.linalg.norm(nw, axis=1, keepdims=True)
+    back = (nw * -rd).sum(1) <= 0
+    nw[back] *= -1
+    assert np.array_equal(pr[hit, 10] == 1, back)
+    assert np.all(np.abs(pr[hit, 5:8] - nw).max(1) < 2e-5)
+    b = hits["bary"][hit].astype(np.float64)
+    uv = (b[:, :, None] * sc.uvs[prim].astype(np.float64)).sum(1)
+    assert np.allclose(pr[hit, 8:10], uv, atol=2e-6)
+    assert np.array_equal(pr[hit, 11].astype(int), sc.material_ids[prim])
+
+
+# ----------------------------------------------------------------- packing
+def _oct_decode(e):
+    """Octahedral unit-vector decode (Cigolle et al. 2014)."""
+    x, y = e[:, 0].astype(np.float64), e[:, 1].astype(np.float64)
+    z = 1 - np.abs(x) - np.abs(y)
+    t = np.maximum(-z, 0)
+    x = x - np.where(x >= 0, t, -t)
+    y = y - np.where(y >= 0, t, -t)
+    v = np.stack([x, y, z], 1)
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def test_octahedral_encoding():
+    """encodeUnitVector (packing.cuh:105-125): the octahedral map; decoded by
+    the textbook inverse it returns the vector within float rounding; the
+    axes and poles have known codes; the upper hemisphere lands inside the
+    L1 unit ball, the lower one outside it."""
+    rng = np.random.default_rng(35)
+    v = rng.normal(size=(100_000, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    e = oracle.encode_unit(v)
+    back = _oct_decode(e)
+    ang = np.arccos(np.clip((back * v).sum(1), -1, 1))
+    assert ang.max() < 2e-6
+    l1 = np.abs(e).sum(1)
+    assert np.all(l1[v[:, 2] > 1e-3] <= 1 + 1e-6) and np.all(l1[v[:, 2] < -1e-3] >= 1 - 1e-6)
+    axes = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [-1, 0, 0], [0, -1, 0], [0, 0, -1]], np.float32)
+    assert np.array_equal(oracle.encode_unit(axes), [[1, 0], [0, 1], [0, 0], [-1, 0], [0, -1], [1, 1]])
+
+
+def test_float_to_byte():
+    """floatToByte (packing.cuh:13-17): uint(saturate(v) * 255) (truncation),
+    saturate = fmax(0, fmin(v, 1)) so NaN -> 1 -> 255."""
+    v = np.concatenate([np.linspace(-0.5, 1.5, 20001), [np.nan, np.inf, -np.inf, 1 / 255, 254.999 / 255]])
+    v = v.astype(np.float32)
+    got = oracle.float_to_byte(v)
+    sat = np.where(np.isnan(v), np.float32(1), np.clip(v, np.float32(0), np.float32(1)))
+    want = (sat * np.float32(255)).astype(np.float32).astype(np.uint32)
+    assert np.array_equal(got, want)
+    assert got[-5] == 255 and got[-4] == 255 and got[-3] == 0
+
+
+# ----------------------------------------------------------------- area lights
+def _emitter_scene(scale=1.0):
+    P = np.array([[[0.0, 0.0, 0.0], [2.0, 0.0, 0.5], [0.5, 0.0, 3.0]]], np.float64)
+    N = np.tile(np.array([0.0, 1.0, 0.0]), (1, 3, 1))
+    UV = np.array([[[0, 0], [1, 0], [0, 1]]], np.float64)
+    mats = [scenes.material(emission=(0.5, 1.0, 2.0), emission_scale_nits=3.0)]
+    rot = scenes.random_rotation(np.random.default_rng(1))
+    return scenes._pack([(P, N, UV, np.zeros(1, np.int32))], mats,
+                        np.stack([scenes.trs(translate=(1.0, 2.0, -1.0), rotate=rot, scale=scale)])), rot
+
+
+@pytest.mark.parametrize("scale", [1.0, 2.0])
+def test_area_light_sampling(scale):
+    """Light::sampleLi with Triangle::sample (light.cu:33-45, triangle.cuh:
+    104-119): the square-to-triangle map is uniform (chi-square over 16
+    congruent sub-triangles), its pdf is converted to solid angle by
+    d^2 / cos, and Light::pdf_Li (light.cu:11-18), the pdf the emitter-hit
+    MIS weight uses, gives the same value for the same point.  The reference
+    takes the area from the object-space vertices and the cosine against the
+    unnormalised world normal: on an instance scaled by k the sampling pdf is
+    k times d^2/(A_world cos) and pdf_Li k^2 times (kept as the reference
+    has it; every scene here places emitters unscaled)."""
+    sc, rot = _emitter_scene(scale)
+    orc = oracle.OracleScene(sc)
+    n = 64_000
+    rng = np.random.default_rng(37)
+    frm = rng.normal(size=(n, 3)) * 3 + np.array([1.0, 6.0, 0.0])
+    u = rng.random((n, 2))
+    out, pli = oracle.light_sample(orc, 0, frm, u)
+    orc.close()
+    M = np.eye(4)
+    M[:3, :3] = rot * scale
+    M[:3, 3] = (1.0, 2.0, -1.0)
+    P = sc.positions[0].astype(np.float64)
+    Pw = (np.column_stack([P, np.ones(3)]) @ M.T)[:, :3]
+    p = out[:, 0:3].astype(np.float64)
+    # barycentrics of the sampled points in the world triangle
+    e1, e2 = Pw[1] - Pw[0], Pw[2] - Pw[0]
+    G = np.array([[e1 @ e1, e1 @ e2], [e1 @ e2, e2 @ e2]])
+    rhs = np.column_stack([(p - Pw[0]) @ e1, (p - Pw[0]) @ e2])
+    b12 = np.linalg.solve(G, rhs.T).T
+    assert np.all(b12 > -1e-5) and np.all(b12.sum(1) < 1 + 1e-5)
+    cell = np.minimum((b12 * 4).astype(int), 3)
+    up = ((b12 * 4) - cell).sum(1) > 1                        # upper sub-triangle of the grid cell
+    idx = (cell[:, 0] * 4 + cell[:, 1]) * 2 + up
+    valid = cell.sum(1) + up <= 3
+    counts = np.bincount(idx[valid], minlength=32)
+    counts = counts[counts > 0]
+    assert len(counts) == 16 and valid.mean() > 0.999
+    expect = n / 16
+    assert ((counts - expect) ** 2 / expect).sum() < 45
+    area_w = 0.5 * np.linalg.norm(np.cross(e1, e2))
+    nw = np.cross(e1, e2)
+    nw /= np.linalg.norm(nw)
+    dvec = p - frm
+    dist = np.linalg.norm(dvec, axis=1)
+    cos = np.abs((dvec / dist[:, None]) @ nw)
+    pdf_true = dist ** 2 / (area_w * cos)
+    ok = cos > 0.02                        # float32 pdfs at grazing cosines are ill-conditioned
+    assert ok.mean() > 0.95
+    assert np.allclose(out[ok, 6], scale * pdf_true[ok], rtol=1e-4)
+    assert np.allclose(pli[ok], scale * scale * pdf_true[ok], rtol=1e-4)
+    assert np.allclose(out[:, 7:10], np.array([0.5, 1.0, 2.0]) * 3.0)
+    uv = np.column_stack([b12[:, 0], b12[:, 1]])              # vertex uvs (0,0), (1,0), (0,1)
+    assert np.allclose(out[:, 10:12], uv, atol=1e-4)
