@@ -188,14 +188,7 @@ def batched_views(ctx, scene, n_views, spp, seed=64):
     """The multi-view workload: n_views cameras around the instance box with
     square resolutions log-uniform in [128, 1024], all traced as one set of
     paths per sample (kpt_render_views)."""
-    rng = np.random.default_rng(seed)
-    views = []
-    for _ in range(n_views):
-        side = int(round(math.exp(rng.uniform(math.log(128), math.log(1024)))))
-        az, el = rng.uniform(0, 2 * math.pi), rng.uniform(0.1, 0.9)
-        eye = (50 + 110 * math.cos(el) * math.cos(az), 45 + 110 * math.sin(el), 50 + 110 * math.cos(el) * math.sin(az))
-        proj = scenes.perspective_fov_lh(scene.camera_fov, side, side, 1.0, 100.0)
-        views.append((proj, scenes.look_at_view(eye, (50.0, 45.0, 50.0)), side, side))
+    views = scenes.multi_view_cameras(scene, n_views, seed)
     pixels = sum(w * h for _, _, w, h in views)
     ctx.render_views(views, spp=1)                      # allocation and warm-up
     ctx.render_views(views, spp=spp)

@@ -261,7 +261,11 @@ int kpt_get_blas(const kpt_ctx* ctx, int32_t mesh, void* nodes32, int64_t cap_no
 int kpt_get_tlas(const kpt_ctx* ctx, void* nodes32, int64_t cap_nodes, int64_t* out_count);
 
 /* ---- traversal (TLAS::intersect / intersectP, tlas.cu:8-125) ---- */
-/* rays/hits are DEVICE pointers; runs on the context stream. */
+/* rays/hits are DEVICE pointers; runs on the context stream.  A ray whose
+ * traversal stack would pass 128 entries (the reference's 16-entry stacks,
+ * tlas.cuh:10, blas.cuh:37, overflow there) ends early; the first
+ * synchronising call after it (kpt_sync, kpt_render, kpt_render_views)
+ * returns KPT_ERR_OVERFLOW once. */
 int kpt_trace_closest(kpt_ctx* ctx, const kpt_ray* rays, kpt_hit* hits, int64_t n);
 /* occluded[i] = 1 when any triangle is hit in (tmin, tmax). */
 int kpt_trace_any(kpt_ctx* ctx, const kpt_ray* rays, int32_t* occluded, int64_t n);
@@ -303,7 +307,8 @@ int kpt_get_frame_index(const kpt_ctx* ctx, int32_t* out);
  * 1 = accumulation (running sum, RGBA32F), 2 = gbuffer (packed, 4xu32),
  * 3 = vbuffer (RGBA32F), 4 = output (post-processed sRGB, RGBA32F). */
 int kpt_readback(kpt_ctx* ctx, int32_t which, void* host_dst, int64_t bytes);
-/* Synchronises the context stream. */
+/* Synchronises the context stream; KPT_ERR_OVERFLOW when a traversal since
+ * the last such report overflowed its stack (reported once). */
 int kpt_sync(kpt_ctx* ctx);
 /* Milliseconds of the last kpt_render (HIP events on the context stream) and
  * of its dominant closest-hit trace kernel launches. out[0]=render ms,
@@ -389,7 +394,9 @@ int kpt_set_frame_delta(kpt_ctx* ctx, float delta_time_ms);
  * alone with kpt_render from frame 0.  out: DEVICE RGBA32F, views back to
  * back (view v starts at the sum of the earlier views' width*height), the
  * static-accumulation estimate after spp samples.  No G-buffer, post chain
- * or cache; the context's frame buffers are untouched. */
+ * or cache; the context's frame buffers are untouched.  With the graph
+ * tuning knob each sample's launches are captured and replayed as one
+ * hipGraph (knobs as for kpt_render). */
 int kpt_render_views(kpt_ctx* ctx, const kpt_view* views, int32_t n, int32_t spp, float* out);
 /* The same into HOST memory (sum of width*height RGBA32F; a temporary
  * device buffer is allocated and freed per call). */

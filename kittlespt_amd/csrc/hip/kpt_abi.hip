@@ -76,7 +76,8 @@ struct kpt_ctx {
     static constexpr int INS_RING = 4;   // frames whose trace events may be in flight
     WfInstrument ins[INS_RING]{};
     int32_t use_graph = 0;               // capture each frame as a hipGraph and replay it
-    hipGraphExec_t graph_exec = nullptr;
+    hipGraphExec_t graph_exec = nullptr;        // kpt_render's per-frame graph
+    hipGraphExec_t views_graph_exec = nullptr;  // kpt_render_views' per-sample graph
     TraceTuning tune{0, 128, 16, 8, 0, 1};   // the autotuner's usual choice on the 1080p workload
     int32_t sort_by_material = 2;
     bool count_trace = false;
@@ -616,6 +617,34 @@ int alloc_wavefront(kpt_ctx* ctx, int64_t n) {
 
 }  // namespace
 
+// Captures what enqueue() launches on the context stream (relaxed capture
+// mode) and replays it as one graph: the first capture is instantiated into
+// *exec, later ones update it in place (hipGraphExecUpdate; re-instantiated
+// when the topology changed).  Nothing enqueued here may be a memset node:
+// counters and queues are cleared by fill kernels.
+template <class Fn>
+static hipError_t replay_captured(kpt_ctx* ctx, hipGraphExec_t* exec, Fn&& enqueue) {
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) return e;
+    e = enqueue();
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess && *exec) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res;
+        if (hipGraphExecUpdate(*exec, g, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(*exec);
+            *exec = nullptr;
+        }
+    }
+    if (e == hipSuccess && !*exec) e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipGraphLaunch(*exec, ctx->stream);
+    if (g) (void)hipGraphDestroy(g);
+    return e;
+}
+
 extern "C" {
 
 int kpt_get_abi_version(void) { return KPT_ABI_VERSION; }
@@ -705,6 +734,7 @@ int kpt_ctx_destroy(kpt_ctx* ctx) {
             if (ctx->ins[r].tail_end) (void)hipEventDestroy(ctx->ins[r].tail_end);
         }
         if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+        if (ctx->views_graph_exec) (void)hipGraphExecDestroy(ctx->views_graph_exec);
         dev_free(&ctx->d_counts);
         dev_free(&ctx->d_inst_bounds);
         dev_free(&ctx->d_tlas_ref);
@@ -1040,6 +1070,19 @@ int kpt_get_tlas(const kpt_ctx* ctx, void* nodes32, int64_t cap_nodes, int64_t* 
     return KPT_OK;
 }
 
+// The traversal kernels raise the context's overflow flag when a lane's
+// stack would pass LDS_STACK + SPILL_STACK entries (the lane's query then
+// ends early).  Read after the stream has drained: a raised flag is cleared
+// and reported once, by the first synchronising call after the launch
+// (kpt_render, kpt_render_views, kpt_sync).
+static int report_overflow(kpt_ctx* ctx) {
+    int32_t ovf = 0;
+    HIPCHK(hipMemcpy(&ovf, ctx->d_overflow, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (!ovf) return KPT_OK;
+    HIPCHK(hipMemset(ctx->d_overflow, 0, sizeof(int32_t)));
+    return fail(ctx, KPT_ERR_OVERFLOW, "traversal stack overflow (depth > 128)");
+}
+
 int kpt_trace_closest(kpt_ctx* ctx, const kpt_ray* rays, kpt_hit* hits, int64_t n) {
     if (!ctx || (n > 0 && (!rays || !hits)) || n < 0) return KPT_ERR_INVALID_ARG;
     if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
@@ -1141,11 +1184,20 @@ int kpt_render_views(kpt_ctx* ctx, const kpt_view* views, int32_t n, int32_t spp
     HIPCHK(launch_clear(ctx->stream, vb.out, total));
     SceneView S = scene_view(ctx);
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    // with the graph knob, each sample's launches are captured and replayed
+    // as one graph (the library radix sort stays out of captured graphs)
+    const bool graph = ctx->use_graph && !ctx->count_trace;
+    TraceTuning tune = ctx->tune;
+    if (graph) tune.ray_sort = 0;
     for (int32_t k = 0; k < spp; ++k) {
         FrameConst fc = make_frame(ctx, k);
-        HIPCHK(wavefront_frame(ctx->stream, S, fc, k, ctx->fb, ctx->ws, ctx->d_overflow,
-                               (int32_t)ctx->materials.size(), ctx->sort_by_material, ctx->tune, nullptr, nullptr,
-                               &vb));
+        auto enqueue = [&]() -> hipError_t {
+            return wavefront_frame(ctx->stream, S, fc, k, ctx->fb, ctx->ws, ctx->d_overflow,
+                                   (int32_t)ctx->materials.size(), ctx->sort_by_material, tune, nullptr, nullptr,
+                                   &vb);
+        };
+        if (graph) HIPCHK(replay_captured(ctx, &ctx->views_graph_exec, enqueue));
+        else HIPCHK(enqueue());
     }
     HIPCHK(launch_views_mean(ctx->stream, vb.out, total, spp));
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
@@ -1155,10 +1207,7 @@ int kpt_render_views(kpt_ctx* ctx, const kpt_view* views, int32_t n, int32_t spp
     ctx->timing[0] = ms;
     ctx->timing[1] = 0.0;
     ctx->timing[2] = 0.0;
-    int32_t ovf = 0;
-    HIPCHK(hipMemcpy(&ovf, ctx->d_overflow, sizeof(int32_t), hipMemcpyDeviceToHost));
-    if (ovf) return fail(ctx, KPT_ERR_OVERFLOW, "traversal stack overflow (depth > 128)");
-    return KPT_OK;
+    return report_overflow(ctx);
 }
 
 int kpt_render_views_host(kpt_ctx* ctx, const kpt_view* views, int32_t n, int32_t spp, float* host_out) {
@@ -1387,25 +1436,7 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
         if (graph) {
             // the frame's ~80 launches captured once and replayed as one graph;
             // later frames update the executable graph in place
-            HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
-            hipError_t e = enqueue();
-            hipGraph_t g = nullptr;
-            hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
-            if (e == hipSuccess) e = e2;
-            if (e == hipSuccess && ctx->graph_exec) {
-                hipGraphNode_t bad = nullptr;
-                hipGraphExecUpdateResult res;
-                if (hipGraphExecUpdate(ctx->graph_exec, g, &bad, &res) != hipSuccess ||
-                    res != hipGraphExecUpdateSuccess) {
-                    (void)hipGetLastError();
-                    (void)hipGraphExecDestroy(ctx->graph_exec);
-                    ctx->graph_exec = nullptr;
-                }
-            }
-            if (e == hipSuccess && !ctx->graph_exec) e = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
-            if (e == hipSuccess) e = hipGraphLaunch(ctx->graph_exec, ctx->stream);
-            if (g) (void)hipGraphDestroy(g);
-            HIPCHK(e);
+            HIPCHK(replay_captured(ctx, &ctx->graph_exec, enqueue));
         } else {
             HIPCHK(enqueue());
         }
@@ -1450,10 +1481,7 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
         ctx->stages[1].paths = h[13];
         ctx->stages[1].vertices = h[14];
     }
-    int32_t ovf = 0;
-    HIPCHK(hipMemcpy(&ovf, ctx->d_overflow, sizeof(int32_t), hipMemcpyDeviceToHost));
-    if (ovf) return fail(ctx, KPT_ERR_OVERFLOW, "traversal stack overflow (depth > 128)");
-    return KPT_OK;
+    return report_overflow(ctx);
 }
 
 int kpt_set_exposure(kpt_ctx* ctx, const kpt_exposure* e) {
@@ -1657,10 +1685,7 @@ int kpt_sync(kpt_ctx* ctx) {
     if (!ctx) return KPT_ERR_INVALID_ARG;
     if (ctx->host_only) return KPT_OK;
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    int32_t ovf = 0;
-    HIPCHK(hipMemcpy(&ovf, ctx->d_overflow, sizeof(int32_t), hipMemcpyDeviceToHost));
-    if (ovf) return fail(ctx, KPT_ERR_OVERFLOW, "traversal stack overflow (depth > 128)");
-    return KPT_OK;
+    return report_overflow(ctx);
 }
 
 int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n) {

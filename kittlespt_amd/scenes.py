@@ -399,3 +399,19 @@ def scene_instanced_small(seed=0):
     4 small BLAS and 64^2 textures."""
     return scene_instanced(num_instances=40, tex_size=64, tris_range=(600, 1500), num_blas=4, num_materials=8,
                            seed=seed)
+
+
+def multi_view_cameras(scene, n_views=64, seed=64, lo=128, hi=1024):
+    """The multi-view workload's cameras: n_views views around the instance
+    box (azimuth uniform, elevation U[0.1, 0.9] rad, 110 units from its
+    centre), square resolutions with side round(exp(U[ln lo, ln hi]))."""
+    rng = np.random.default_rng(seed)
+    views = []
+    for _ in range(n_views):
+        side = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        az, el = rng.uniform(0, 2 * math.pi), rng.uniform(0.1, 0.9)
+        eye = (50 + 110 * math.cos(el) * math.cos(az), 45 + 110 * math.sin(el), 50 + 110 * math.cos(el) * math.sin(az))
+        proj = perspective_fov_lh(scene.camera_fov, side, side, 1.0, 100.0)
+        views.append((proj, look_at_view(eye, (50.0, 45.0, 50.0)), side, side))
+    return views
+

@@ -160,6 +160,25 @@ class OracleScene:
                               _ptr(r), _ptr(g), _ptr(v), threads or threads_default())
         return r, g, v
 
+    def render_temporal_pixels(self, settings, camera, width, height, frame, prev_accum, prev_gbuffer, pixels,
+                               threads=None):
+        """One temporal frame for the listed pixels only; returns (render
+        [n,4], gbuffer [n,4], vbuffer [n,4])."""
+        pa = np.ascontiguousarray(prev_accum, np.float32)
+        pg = np.ascontiguousarray(prev_gbuffer, np.uint32)
+        px = np.ascontiguousarray(pixels, np.int32)
+        n = len(px)
+        r = np.zeros((n, 4), np.float32)
+        g = np.zeros((n, 4), np.uint32)
+        v = np.zeros((n, 4), np.float32)
+        f = _sig("orc_render_temporal_pixels", None,
+                 [C.c_void_p, C.POINTER(abi.KptSettings), C.POINTER(abi.KptCamera), C.c_int32, C.c_int32,
+                  C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                  C.c_int32])
+        f(self.h, C.byref(settings), C.byref(camera), width, height, frame, _ptr(pa), _ptr(pg), _ptr(px), n,
+          _ptr(r), _ptr(g), _ptr(v), threads or threads_default())
+        return r, g, v
+
     def sample_pixels(self, settings, camera, width, height, pixels, frame, threads=None):
         pixels = np.ascontiguousarray(pixels, np.int32)
         rgb = np.zeros((len(pixels), 3), np.float32)

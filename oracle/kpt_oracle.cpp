@@ -610,7 +610,13 @@ template <class R> void TriIntersect(const Triangle& tri, const Ray<R>& ray, R t
     }
 }
 
-struct Counters { uint64_t interior = 0, leaf = 0, tri = 0, entries = 0, tlas = 0; int max_stack = 0; };
+// max_stack: the deepest the TLAS and BLAS stacks got together (tlas_depth
+// holds the TLAS stack's size while a BLAS is traversed).
+struct Counters {
+    uint64_t interior = 0, leaf = 0, tri = 0, entries = 0, tlas = 0;
+    int max_stack = 0, tlas_depth = 0;
+    void depth(int d) { if (d > max_stack) max_stack = d; }
+};
 
 // BLAS2::intersect, blas.cuh:113-188 (unbounded stack: the reference's 16
 // entries overflow on deep trees, which is undefined behaviour there).
@@ -623,7 +629,7 @@ template <class R> Isect<R> BlasIntersect(const orc_scene* S, const BLAS2& b, co
     stack.push_back((uint32_t)b.root);
     Isect<R> it;
     while (!stack.empty()) {
-        if ((int)stack.size() > cn->max_stack) cn->max_stack = (int)stack.size();
+        cn->depth(cn->tlas_depth + (int)stack.size());
         const BVH2Node& node = S->nodes[stack.back()];
         stack.pop_back();
         if (!node.isLeaf()) {
@@ -664,6 +670,7 @@ template <class R> bool BlasIntersectP(const orc_scene* S, const BLAS2& b, const
     stack.push_back((uint32_t)b.root);
     Isect<R> it;
     while (!stack.empty()) {
+        cn->depth(cn->tlas_depth + (int)stack.size());
         const BVH2Node& node = S->nodes[stack.back()];
         stack.pop_back();
         if (!node.isLeaf()) {
@@ -696,8 +703,10 @@ template <class R> Isect<R> TlasIntersect(const orc_scene* S, const Ray<R>& ray,
     stack.push_back(S->tlas_root);
     Isect<R> closest;
     while (!stack.empty()) {
+        cn->depth((int)stack.size());
         const TLASNode& node = S->tlas_nodes[stack.back()];
         stack.pop_back();
+        cn->tlas_depth = (int)stack.size();
         cn->tlas++;
         if (node.left_right_index != 0) {
             uint32_t i1 = node.left_right_index & 0xFFFF, i2 = (node.left_right_index >> 16) & 0xFFFF;
@@ -728,8 +737,10 @@ template <class R> bool TlasIntersectP(const orc_scene* S, const Ray<R>& ray, R 
     stack.reserve(32);
     stack.push_back(S->tlas_root);
     while (!stack.empty()) {
+        cn->depth((int)stack.size());
         const TLASNode& node = S->tlas_nodes[stack.back()];
         stack.pop_back();
+        cn->tlas_depth = (int)stack.size();
         cn->tlas++;
         if (node.left_right_index != 0) {
             uint32_t i1 = node.left_right_index & 0xFFFF, i2 = (node.left_right_index >> 16) & 0xFFFF;
@@ -1901,25 +1912,51 @@ void orc_set_instance_transform(orc_scene* s, int32_t i, const float model[16]) 
 // One frame with integrator_use_temporal_accumulation (kernels.cu:385-451):
 // every pixel's sample, G/V-buffers and the reprojected render value from the
 // previous frame's accumulation and G-buffer.
+}  // extern "C"
+
+namespace orc {
+// One pixel of a temporal-accumulation frame: the sample, its G/V-buffer
+// words and reprojectAccumulate against the previous frame's buffers.
+inline void TemporalPixel(const FrameConst& fc, int W, int H, int pidx, int frame, const float* prev_accum,
+                          const uint32_t* prev_gbuffer, V3<float> prev_cam, float* out_render, uint32_t* out_g,
+                          float* out_v) {
+    V3<float> smp;
+    VisSurf<float> vs;
+    RenderPixel<float>(fc, W, H, pidx, frame, &smp, &vs);
+    float ex, ey;
+    EncodeUnit(vs.normal, &ex, &ey);
+    uint32_t alb = FloatToByte(vs.albedo.x) | (FloatToByte(vs.albedo.y) << 8) | (FloatToByte(vs.albedo.z) << 16);
+    out_g[0] = kpt::dm::f2u(ex); out_g[1] = kpt::dm::f2u(ey);
+    out_g[2] = alb; out_g[3] = kpt::dm::f2u(vs.depth);
+    V2<float> vel = Velocity(fc, vs);
+    out_v[0] = vel.x; out_v[1] = vel.y;
+    out_v[2] = (float)vs.prim; out_v[3] = (float)vs.inst;
+    ReprojectAccumulate(fc, W, H, pidx % W, pidx / W, vs, vel, smp, prev_accum, prev_gbuffer, prev_cam, out_render);
+}
+}  // namespace orc
+
+extern "C" {
+
 void orc_render_temporal(const orc_scene* s, const kpt_settings* settings, const kpt_camera* cam, int32_t W,
                          int32_t H, int32_t frame, const float* prev_accum, const uint32_t* prev_gbuffer,
                          float* out_render, uint32_t* out_g, float* out_v, int32_t threads) {
     FrameConst fc = MakeFrame(s, settings, cam);
     V3<float> prev_cam = xyz(LoadM4(cam->prev_inv_view).c[3]);
     ParallelFor((int64_t)W * H, threads, [&](int64_t i) {
-        V3<float> smp;
-        VisSurf<float> vs;
-        RenderPixel<float>(fc, W, H, (int)i, frame, &smp, &vs);
-        float ex, ey;
-        EncodeUnit(vs.normal, &ex, &ey);
-        uint32_t alb = FloatToByte(vs.albedo.x) | (FloatToByte(vs.albedo.y) << 8) | (FloatToByte(vs.albedo.z) << 16);
-        out_g[4 * i] = kpt::dm::f2u(ex); out_g[4 * i + 1] = kpt::dm::f2u(ey);
-        out_g[4 * i + 2] = alb; out_g[4 * i + 3] = kpt::dm::f2u(vs.depth);
-        V2<float> vel = Velocity(fc, vs);
-        out_v[4 * i] = vel.x; out_v[4 * i + 1] = vel.y;
-        out_v[4 * i + 2] = (float)vs.prim; out_v[4 * i + 3] = (float)vs.inst;
-        ReprojectAccumulate(fc, W, H, (int)(i % W), (int)(i / W), vs, vel, smp, prev_accum, prev_gbuffer, prev_cam,
-                            out_render + 4 * i);
+        TemporalPixel(fc, W, H, (int)i, frame, prev_accum, prev_gbuffer, prev_cam, out_render + 4 * i, out_g + 4 * i,
+                      out_v + 4 * i);
+    });
+}
+
+void orc_render_temporal_pixels(const orc_scene* s, const kpt_settings* settings, const kpt_camera* cam, int32_t W,
+                                int32_t H, int32_t frame, const float* prev_accum, const uint32_t* prev_gbuffer,
+                                const int32_t* pixels, int64_t n, float* out_render, uint32_t* out_g, float* out_v,
+                                int32_t threads) {
+    FrameConst fc = MakeFrame(s, settings, cam);
+    V3<float> prev_cam = xyz(LoadM4(cam->prev_inv_view).c[3]);
+    ParallelFor(n, threads, [&](int64_t i) {
+        TemporalPixel(fc, W, H, pixels[i], frame, prev_accum, prev_gbuffer, prev_cam, out_render + 4 * i,
+                      out_g + 4 * i, out_v + 4 * i);
     });
 }
 

@@ -63,6 +63,12 @@ void orc_render_temporal(const orc_scene* s, const kpt_settings* settings, const
                          const uint32_t* prev_gbuffer, float* out_render, uint32_t* out_gbuffer,
                          float* out_vbuffer, int32_t threads);
 
+/* The same for the listed pixels only (out_* are [n][4]). */
+void orc_render_temporal_pixels(const orc_scene* s, const kpt_settings* settings, const kpt_camera* cam,
+                                int32_t width, int32_t height, int32_t frame, const float* prev_accum,
+                                const uint32_t* prev_gbuffer, const int32_t* pixels, int64_t n,
+                                float* out_render, uint32_t* out_gbuffer, float* out_vbuffer, int32_t threads);
+
 /* One sample (radiance before clamp/accumulation) per pixel for frame `frame`.
  * Used to check the accumulation kernels independently. */
 void orc_sample_pixels(const orc_scene* s, const kpt_settings* settings, const kpt_camera* cam,

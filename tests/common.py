@@ -104,3 +104,31 @@ def furnace_cube(half=5.0, albedo=(0.4, 0.25, 0.6), emission=(1.0, 0.5, 2.0), ro
                       camera_eye=(0.0, 0.0, 0.0), camera_target=(0.0, 0.0, 1.0),
                       camera_fov=math.radians(30.0), sun_theta=math.radians(30.0))
     return sc
+
+
+def instance_chain(n, ratio=1.5):
+    """A TLAS that the agglomerative builder makes a chain n - 1 levels deep:
+    instance k's box spans x in [r^k, r^(k+1)] and y, z in [-r^k, r^k]
+    (r = ratio), so the cluster of instances 0..k-1 is always the best
+    match of instance k.  Each instance holds two quads in the planes
+    z = -1 and z = +1 of its unit box, so a ray along the x axis (y = z = 0)
+    enters every box and hits nothing: closest-hit traversal from the origin
+    pushes the farther child (the next instance) at every level, n - 1
+    entries deep, before it reaches instance 0."""
+    quad = lambda z: np.array([[[0, -1, z], [1, -1, z], [1, 1, z]], [[0, -1, z], [1, 1, z], [0, 1, z]]], np.float64)  # noqa: E731
+    P = np.concatenate([quad(-1.0), quad(1.0)])
+    N = np.zeros_like(P)
+    N[:2, :, 2] = 1.0
+    N[2:, :, 2] = -1.0
+    UV = np.zeros((4, 3, 2))
+    mats = [scenes.material(albedo=(0.7, 0.7, 0.7), roughness=0.6)]
+    models = []
+    for k in range(n):
+        x0, x1, s = ratio ** k, ratio ** (k + 1), ratio ** k
+        m = np.diag([x1 - x0, s, s, 1.0])
+        m[0, 3] = x0
+        models.append(scenes.mat_colmajor(m))
+    sc = scenes._pack([(P, N, UV, np.zeros(4, np.int32))], mats, None,
+                      instance_mesh=np.zeros(n, np.int32), instance_model=np.stack(models),
+                      camera_eye=(-2.0, 0.0, 0.0), camera_target=(1.0, 0.0, 0.0), camera_fov=math.radians(20.0))
+    return sc
