@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libkpt_oracle.so")
 _lib = None
 
-MODE_CLOSEST, MODE_ANY, MODE_BRUTE_CLOSEST, MODE_BRUTE_ANY = 0, 1, 2, 3
+MODE_CLOSEST, MODE_ANY, MODE_BRUTE_CLOSEST, MODE_BRUTE_ANY, MODE_CLOSEST_ENTRY_BOUND = 0, 1, 2, 3, 4
 FN = {"exp": 0, "exp2": 1, "log2": 2, "pow": 3, "sin": 4, "cos": 5, "tan": 6, "acos": 7, "log": 8}
 
 
@@ -111,7 +111,8 @@ class OracleScene:
         ms = C.c_int32(0)
         lib().orc_trace(self.h, _ptr(rays), n, mode, _ptr(hits), _ptr(occ), C.byref(cnt), C.byref(ms),
                         threads or threads_default())
-        return (hits if mode in (MODE_CLOSEST, MODE_BRUTE_CLOSEST) else occ), cnt.as_dict(), ms.value
+        return (hits if mode in (MODE_CLOSEST, MODE_BRUTE_CLOSEST, MODE_CLOSEST_ENTRY_BOUND) else occ), cnt.as_dict(), \
+            ms.value
 
     def render_pixels(self, settings, camera, width, height, pixels, frame0, spp, double=False, threads=None,
                       want_buffers=False):

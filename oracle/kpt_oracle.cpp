@@ -620,8 +620,13 @@ struct Counters {
 
 // BLAS2::intersect, blas.cuh:113-188 (unbounded stack: the reference's 16
 // entries overflow on deep trees, which is undefined behaviour there).
-template <class R> Isect<R> BlasIntersect(const orc_scene* S, const BLAS2& b, const Ray<R>& wray, R tmin, R tmax, Counters* cn) {
+// init: the distance the BLAS's own closest hit starts from (+inf in the
+// reference; the closest hit so far in the GPU kernels' bound, see
+// TlasIntersect).
+template <class R> Isect<R> BlasIntersect(const orc_scene* S, const BLAS2& b, const Ray<R>& wray, R tmin, R tmax, Counters* cn,
+                                          R init = Inf<R>()) {
     Isect<R> closest;
+    closest.distance = init;
     if (b.root < 0) return closest;
     Ray<R> ray = Transform(wray, CvtM4<R>(b.inv_model));
     std::vector<uint32_t> stack;
@@ -692,8 +697,12 @@ template <class R> bool BlasIntersectP(const orc_scene* S, const BLAS2& b, const
     }
     return false;
 }
-// TLAS::intersect, tlas.cu:60-125
-template <class R> Isect<R> TlasIntersect(const orc_scene* S, const Ray<R>& ray, R tmin, R tmax, Counters* cn) {
+// TLAS::intersect, tlas.cu:60-125.  entry_bound: enter each BLAS with
+// tmax = min(tmax, closest so far) and cull its nodes against that closest
+// distance, as the GPU kernels do (not the reference; a test compares the
+// two).
+template <class R> Isect<R> TlasIntersect(const orc_scene* S, const Ray<R>& ray, R tmin, R tmax, Counters* cn,
+                                          bool entry_bound = false) {
     if (S->tlas_root < 0) return Isect<R>();
     R c1 = Inf<R>(), c2 = Inf<R>();
     if (!SlabHit(S->tlas_bounds, ray, tmin, tmax, &c1, &c2)) return Isect<R>();
@@ -722,7 +731,10 @@ template <class R> Isect<R> TlasIntersect(const orc_scene* S, const Ray<R>& ray,
             }
         } else {
             cn->entries++;
-            Isect<R> it = BlasIntersect(S, S->blas[node.blas_index], ray, tmin, tmax, cn);
+            Isect<R> it = entry_bound
+                              ? BlasIntersect(S, S->blas[node.blas_index], ray, tmin, Fmin(tmax, closest.distance), cn,
+                                              closest.distance)
+                              : BlasIntersect(S, S->blas[node.blas_index], ray, tmin, tmax, cn);
             if (it.distance < closest.distance) closest = it;
         }
     }
@@ -1796,8 +1808,9 @@ void orc_trace(const orc_scene* s, const kpt_ray* rays, int64_t n, int32_t mode,
         const kpt_ray& r = rays[i];
         Ray<float> ray(V3<float>(r.org[0], r.org[1], r.org[2]), V3<float>(r.dir[0], r.dir[1], r.dir[2]));
         Counters cn;
-        if (mode == 0 || mode == 2) {
-            Isect<float> it = mode == 0 ? TlasIntersect(s, ray, r.tmin, r.tmax, &cn) : BruteClosest(s, ray, r.tmin, r.tmax);
+        if (mode == 0 || mode == 2 || mode == 4) {
+            Isect<float> it = mode == 2 ? BruteClosest(s, ray, r.tmin, r.tmax)
+                                        : TlasIntersect(s, ray, r.tmin, r.tmax, &cn, mode == 4);
             hits[i].t = it.distance; hits[i].prim_id = it.prim; hits[i].inst_id = it.inst;
             hits[i].bary[0] = it.bary.x; hits[i].bary[1] = it.bary.y; hits[i].bary[2] = it.bary.z;
         } else {

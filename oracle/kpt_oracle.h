@@ -28,7 +28,9 @@ int64_t orc_get_tlas(const orc_scene* s, void* nodes32, int64_t cap);
 
 /* Traversal of host rays.  mode 0 = BVH closest hit (TLAS::intersect),
  * 1 = BVH any hit (TLAS::intersectP), 2 = brute-force closest
- * (INTERSECT_DEBUG, integrator.cuh:64-95), 3 = brute-force any hit. */
+ * (INTERSECT_DEBUG, integrator.cuh:64-95), 3 = brute-force any hit,
+ * 4 = BVH closest hit entering each BLAS with the closest distance so far as
+ * its tmax (the GPU kernels' bound; compared with mode 0 by a test). */
 void orc_trace(const orc_scene* s, const kpt_ray* rays, int64_t n, int32_t mode,
                kpt_hit* hits, int32_t* occluded, kpt_trace_counters* counters,
                int32_t* max_stack, int32_t threads);

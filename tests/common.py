@@ -132,3 +132,41 @@ def instance_chain(n, ratio=1.5):
                       instance_mesh=np.zeros(n, np.int32), instance_model=np.stack(models),
                       camera_eye=(-2.0, 0.0, 0.0), camera_target=(1.0, 0.0, 0.0), camera_fov=math.radians(20.0))
     return sc
+
+
+def coincident_quads(n=24, seed=0, plane_z=7.3):
+    """n instances of one axis-aligned quad (a flat BLAS box, z = 0 in object
+    space) laid over each other in the world plane z = plane_z: rotations by
+    multiples of 90 degrees about z (exact), or by a few 1e-6 rad about x and
+    y (boxes no longer exactly flat), translations within the plane and along
+    z by a few ulps of plane_z.  Rays from above hit several instances at
+    distances equal to within an ulp or two: the case in which entering a
+    BLAS with the closest distance so far as tmax could disagree with the
+    reference's per-BLAS traversal from the original tmax."""
+    rng = np.random.default_rng(seed)
+    P = np.array([[[-1, -1, 0], [1, -1, 0], [1, 1, 0]], [[-1, -1, 0], [1, 1, 0], [-1, 1, 0]]], np.float64)
+    N = np.tile(np.array([0.0, 0.0, 1.0]), (2, 3, 1))
+    UV = np.zeros((2, 3, 2))
+    ulp = np.spacing(np.float32(plane_z))
+    models = []
+    for k in range(n):
+        if k % 3 == 2:
+            ax, ay = rng.uniform(-3e-6, 3e-6, 2)
+            r = _rot_x(ax) @ _rot_y(ay)
+        else:
+            c, s = [(1, 0), (0, 1), (-1, 0), (0, -1)][k % 4]
+            r = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]], np.float64)
+        t = (rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3), plane_z + float(ulp) * int(rng.integers(-2, 3)))
+        models.append(scenes.trs(translate=t, rotate=r, scale=float(rng.choice([1.0, 1.5, 2.0]))))
+    mats = [scenes.material(albedo=(0.6, 0.6, 0.6), roughness=0.5)]
+    return scenes._pack([(P, N, UV, np.zeros(2, np.int32))], mats, None, instance_mesh=np.zeros(n, np.int32),
+                        instance_model=np.stack(models), camera_eye=(0.0, 0.0, 0.0),
+                        camera_target=(0.0, 0.0, 1.0), camera_fov=math.radians(40.0))
+
+
+def _rot_x(a):
+    return np.array([[1, 0, 0], [0, math.cos(a), -math.sin(a)], [0, math.sin(a), math.cos(a)]])
+
+
+def _rot_y(a):
+    return np.array([[math.cos(a), 0, math.sin(a)], [0, 1, 0], [-math.sin(a), 0, math.cos(a)]])

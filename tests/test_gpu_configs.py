@@ -167,9 +167,18 @@ def test_c3_cached_frame_matches_oracle(c1):
     want = np.where(np.isfinite(want).all(1, keepdims=True), np.clip(want, 0, 1e8), 0).astype(np.float32)
     miss = np.setdiff1d(np.arange(W1 * H1), hit)
     assert not _bits_differ(est[miss], want[miss]).any()
-    rel = np.abs(est[hit] - want[hit]) / (np.abs(want[hit]) + 1e-3 * np.abs(want).mean())
-    print(f"[c3] cached pixels: max rel err {rel.max():.2e}, mean {rel.mean():.2e}")
-    assert rel.max() < 5e-3 and rel.mean() < 1e-4
+    # the network's forward pass equals the oracle's within 2e-3 of its
+    # output range on almost every query (the matrix cores sum in FP32 in
+    # their own order, 10x the error the forward test observes); a sum that
+    # lands on the other side of a BF16 rounding point of a hidden
+    # activation moves the output further, rarely (a few in 10^6 queries)
+    err = np.abs(est[hit] - want[hit])
+    bound = gb[:, :3] * np.float32(1e4) * np.float32(2e-3 * np.abs(o).max()) + 1e-6 * np.abs(want[hit])
+    ratio = (err / bound).max(axis=1)
+    rel = err / (np.abs(want[hit]) + 1e-3 * np.abs(want).mean())
+    print(f"[c3] cached pixels: max rel err {rel.max():.2e}, mean {rel.mean():.2e}, "
+          f"beyond the bound {int((ratio > 1).sum())}, max err / bound {ratio.max():.2e}")
+    assert (ratio > 1).sum() <= len(hit) // 100_000 and ratio.max() < 10 and rel.mean() < 2e-6
 
 
 def _axis_angle(axis, ang):
@@ -283,7 +292,7 @@ def test_deep_tlas_spills_the_stack_and_matches_oracle():
         g = ctx.trace_closest(rays)
         assert np.array_equal(g["prim_id"], o["prim_id"]) and np.array_equal(g["inst_id"], o["inst_id"])
         hit = o["prim_id"] >= 0
-        assert 0.2 < hit.mean() < 0.95
+        assert 0.1 < hit.mean() < 0.95
         assert np.array_equal(g["t"][hit].view(np.uint32), o["t"][hit].view(np.uint32))
         assert np.array_equal(g["bary"][hit].view(np.uint32), o["bary"][hit].view(np.uint32))
         rays["tmax"] = np.random.default_rng(48).uniform(1.0, 1e4, len(rays)).astype(np.float32)
