@@ -272,6 +272,19 @@ int kpt_trace_any(kpt_ctx* ctx, const kpt_ray* rays, int32_t* occluded, int64_t 
 /* Same traversal with event counting (slower; measurement only). */
 int kpt_trace_closest_counted(kpt_ctx* ctx, const kpt_ray* rays, kpt_hit* hits, int64_t n,
                               kpt_trace_counters* host_counters);
+/* kpt_trace_closest_counted without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (serving loops that must not call hipMalloc, harnesses that forbid it).
+ * The counters live in a fixed 256-byte device array of the library (one
+ * per GPU, held until the process ends) instead of a buffer allocated per
+ * call; its size does not depend on n, so no ray count is refused.  Every
+ * call on a GPU shares that array and clears it first: calls that overlap on
+ * one GPU (contexts of the same device used from several threads) race.
+ * Refused with KPT_ERR_STATE while instance edits wait for their TLAS
+ * rebuild, which allocates (kpt_trace_closest or kpt_render performs it).
+ * Synchronises the context stream; same results as kpt_trace_closest_counted. */
+int kpt_trace_closest_counted_noalloc(kpt_ctx* ctx, const kpt_ray* rays, kpt_hit* hits, int64_t n,
+                                      kpt_trace_counters* host_counters);
 
 /* ---- rendering (Renderer::executeRendering, renderer.cu:103-195) ---- */
 /* Renderer::setRendererSettings + setProceduralEnvironmentData

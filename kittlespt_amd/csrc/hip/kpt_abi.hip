@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <chrono>
+#include <mutex>
 #include <cmath>
 #include <string>
 #include <thread>
@@ -25,6 +26,34 @@ template <class T> struct DevBuf {
     size_t n = 0;
 };
 
+}  // namespace
+
+// Device scratch of kpt_trace_closest_counted_noalloc: the traversal event
+// counters of one call (five 64-bit words used), a fixed 256 bytes whatever
+// the number of rays.  Every GPU holds its own copy until the process ends;
+// every call on a GPU shares it.  In host code `scratch` is only a stand-in
+// for the symbol: kernels and copies take the address kept in
+// g_scratch_addr, looked up once per device with that device current.
+constexpr int NOALLOC_SCRATCH_WORDS = 32;
+static __device__ __align__(256) unsigned long long scratch[NOALLOC_SCRATCH_WORDS];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<unsigned long long*> g_scratch_addr;   // by device; nullptr until looked up
+
+// The device address of `scratch` on `device` (current at the call).
+hipError_t scratch_address(int device, unsigned long long** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((size_t)device >= g_scratch_addr.size()) g_scratch_addr.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_addr[(size_t)device]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_addr[(size_t)device] = static_cast<unsigned long long*>(ptr);
+    }
+    *out = g_scratch_addr[(size_t)device];
+    return hipSuccess;
+}
 }  // namespace
 
 struct kpt_ctx {
@@ -64,6 +93,7 @@ struct kpt_ctx {
     kpt::Light* d_lights = nullptr;
     float* d_lverts = nullptr;
     int32_t* d_overflow = nullptr;
+    unsigned long long* scratch_addr = nullptr;   // this device's copy of `scratch`
     // rendering
     kpt_settings settings{};
     kpt_camera cam{};
@@ -685,6 +715,10 @@ int kpt_ctx_create(int device, kpt_ctx** out) {
         delete ctx;
         return KPT_ERR_HIP;
     }
+    if (scratch_address(device, &ctx->scratch_addr) != hipSuccess) {
+        delete ctx;
+        return KPT_ERR_HIP;
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->num_cus = prop.multiProcessorCount;
@@ -1115,6 +1149,29 @@ int kpt_trace_closest_counted(kpt_ctx* ctx, const kpt_ray* rays, kpt_hit* hits, 
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(d);
     if (e != hipSuccess) return fail(ctx, KPT_ERR_HIP, std::string("counted trace: ") + hipGetErrorString(e));
+    host_counters->rays = (uint64_t)n;
+    return KPT_OK;
+}
+
+int kpt_trace_closest_counted_noalloc(kpt_ctx* ctx, const kpt_ray* rays, kpt_hit* hits, int64_t n,
+                                      kpt_trace_counters* host_counters) {
+    if (!ctx || !host_counters || n < 0 || (n > 0 && (!rays || !hits))) return KPT_ERR_INVALID_ARG;
+    if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
+    if (!ctx->scene_ready) return fail(ctx, KPT_ERR_STATE, "no scene");
+    // the TLAS of pending instance edits is rebuilt into freshly allocated
+    // device buffers: that is left to an allocating call
+    if (ctx->tlas_dirty)
+        return fail(ctx, KPT_ERR_STATE,
+                    "instance transforms changed: the TLAS rebuild allocates; call kpt_trace_closest or kpt_render first");
+    if (!ctx->scratch_addr) return fail(ctx, KPT_ERR_STATE, "no device scratch on this context");
+    // the counters start at zero: a fill kernel, not a memset node (callers may capture)
+    HIPCHK(launch_clear(ctx->stream, reinterpret_cast<float4*>(ctx->scratch_addr),
+                        (int64_t)(NOALLOC_SCRATCH_WORDS * sizeof(unsigned long long) / sizeof(float4))));
+    HIPCHK(launch_trace_closest(ctx->stream, scene_view(ctx), rays, hits, n, ctx->d_overflow,
+                                reinterpret_cast<kpt_trace_counters*>(ctx->scratch_addr)));
+    HIPCHK(hipMemcpyAsync(host_counters, ctx->scratch_addr, sizeof(kpt_trace_counters), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     host_counters->rays = (uint64_t)n;
     return KPT_OK;
 }
