@@ -127,6 +127,7 @@ struct kpt_ctx {
     int32_t* d_tlas_arrivals = nullptr;  // refit: per-node arrival counters
     float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {1, 1, 1};   // box of all instances (ray-sort keys)
     bool tlas_dirty = false;
+    bool cache_adapt = false;   // the last frame trained the cache at this resolution: its counts set the next threshold
     ViewConst* d_views = nullptr;    // multi-view batch cameras
     // post chain
     PostBuffers pb{};
@@ -473,7 +474,7 @@ void free_frame(kpt_ctx* c) {
 void free_cache(kpt_ctx* c) {
     CacheBuffers& b = c->cb;
     dev_free(&b.w); dev_free(&b.adam_m); dev_free(&b.adam_v); dev_free(&b.grad); dev_free(&b.wb); dev_free(&b.wbt);
-    dev_free(&b.partial); dev_free(&b.loss_partial); dev_free(&b.stats); dev_free(&b.q_pos); dev_free(&b.q_dir);
+    dev_free(&b.partial); dev_free(&b.partial_sum); dev_free(&b.loss_partial); dev_free(&b.stats); dev_free(&b.q_pos); dev_free(&b.q_dir);
     dev_free(&b.q_beta); dev_free(&b.t_pos); dev_free(&b.t_dir); dev_free(&b.t_beta); dev_free(&b.t_lbefore);
     dev_free(&b.t_target); dev_free(&b.counters);
     b = CacheBuffers{};
@@ -503,6 +504,7 @@ int cache_alloc_params(kpt_ctx* ctx, int32_t batch) {
         HIPCHK(hipMalloc((void**)&b.wbt, RC_PADDED * sizeof(uint16_t)));
         HIPCHK(hipMalloc((void**)&b.stats, 4 * sizeof(float)));
         HIPCHK(hipMalloc((void**)&b.counters, 4 * sizeof(int32_t)));
+        HIPCHK(hipMalloc((void**)&b.partial_sum, (size_t)RC_CHUNKS * RC_PADDED * sizeof(float)));
         HIPCHK(hipMemsetAsync(b.grad, 0, RC_PARAMS * sizeof(float), ctx->stream));
         HIPCHK(hipMemsetAsync(b.stats, 0, 4 * sizeof(float), ctx->stream));
         HIPCHK(hipMemsetAsync(b.counters, 0, 4 * sizeof(int32_t), ctx->stream));
@@ -1309,6 +1311,7 @@ int kpt_get_camera(const kpt_ctx* ctx, kpt_camera* out) {
 int kpt_resize(kpt_ctx* ctx, int32_t width, int32_t height) {
     if (!ctx || width <= 0 || height <= 0 || (int64_t)width * height > (int64_t)1 << 28)
         return KPT_ERR_INVALID_ARG;
+    ctx->cache_adapt = false;
     if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
     const bool up = ctx->settings.upscale_enable != 0;
     const int32_t rw = up ? (int32_t)((float)width / 2.5f) : width;
@@ -1389,7 +1392,8 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
     if (use_cache) {
         if (!ctx->cache_weights) return fail(ctx, KPT_ERR_STATE, "radiance cache not configured");
         int rc = cache_alloc_params(ctx, batch);
-        if (rc == KPT_OK) rc = cache_alloc_records(ctx, npx, std::max(1, cs.train_samples));
+        // room for the record count's frame-to-frame spread around train_samples
+        if (rc == KPT_OK) rc = cache_alloc_records(ctx, npx, std::max(1, 2 * cs.train_samples));
         if (rc != KPT_OK) return rc;
         hooks.cb = &ctx->cb;
         hooks.cf.enabled = cs.enabled;
@@ -1398,6 +1402,7 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
         hooks.cf.wide_roughness = cs.wide_roughness;
         const double prob = std::min(1.0, (double)cs.train_samples / (double)std::max<int64_t>(1, npx));
         hooks.cf.train_threshold = (uint32_t)std::min(16777216.0, std::ceil(prob * 16777216.0));
+        hooks.cf.train_samples = cs.train_samples;
         hooks.enc = cache_encode(ctx);
         hooks.num_cus = ctx->num_cus;
     }
@@ -1443,6 +1448,7 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
             ins->counters = ctx->count_trace ? ctx->d_counts : nullptr;
         }
         hooks.cf.frame = (uint32_t)ctx->frame_index;
+        hooks.cf.adapt = ctx->cache_adapt ? 1 : 0;
         // one executeRendering frame, enqueued without a host synchronisation
         auto enqueue = [&]() -> hipError_t {
             hipError_t e;
@@ -1498,6 +1504,7 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
             HIPCHK(enqueue());
         }
         if (!wavefront) trace_launches += 1;
+        if (use_cache) ctx->cache_adapt = cs.train != 0;
         // camera prev <- curr, frame_index++ (renderer.cu:188-194)
         std::memcpy(ctx->cam.prev_inv_view, ctx->cam.inv_view, 64);
         std::memcpy(ctx->cam.prev_inv_proj, ctx->cam.inv_proj, 64);
@@ -1593,6 +1600,7 @@ int kpt_set_frame_delta(kpt_ctx* ctx, float delta_time_ms) {
 // ---------------------------------------------------------------- radiance cache
 int kpt_cache_configure(kpt_ctx* ctx, const kpt_cache_settings* s) {
     if (!ctx || !s) return KPT_ERR_INVALID_ARG;
+    ctx->cache_adapt = false;
     if (s->train_samples < 0 || s->train_samples > (1 << 22) || s->train_steps < 0 || s->train_steps > 64 ||
         !(s->learning_rate >= 0.0f) || !(s->radiance_scale > 0.0f) || (s->nee_at_vertex & ~1) ||
         !(s->wide_roughness >= 0.0f))

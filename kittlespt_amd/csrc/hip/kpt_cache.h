@@ -20,6 +20,7 @@ constexpr int RC_OUT = 3;
 constexpr int RC_LAYERS = 6;
 constexpr int RC_PARAMS = 18624;
 constexpr int RC_PADDED = 20480;
+constexpr int RC_CHUNKS = 32;    // gradient partials are summed in at most this many chunks first
 
 __host__ __device__ constexpr int rc_in(int l) { return l == 0 ? RC_IN : RC_HID; }
 __host__ __device__ constexpr int rc_out(int l) { return l == RC_LAYERS - 1 ? RC_OUT : RC_HID; }
@@ -44,6 +45,7 @@ struct CacheBuffers {
     uint16_t* wb;             // BF16 [RC_PADDED], row-major [out_pad][in] per layer
     uint16_t* wbt;            // BF16 [RC_PADDED], row-major [in][out_pad] per layer
     float* partial;           // per-workgroup gradient partials [max_wg][RC_PADDED]
+    float* partial_sum;       // sums of consecutive chunks of them [RC_CHUNKS][RC_PADDED]
     float* loss_partial;      // per-workgroup loss sums [max_wg]
     float* stats;             // [0] loss of the last step, [1] samples of the last step
     int32_t max_wg;
@@ -56,7 +58,8 @@ struct CacheBuffers {
     float4* t_beta;           // throughput x reflectance proxy at the vertex
     float4* t_lbefore;        // radiance of the path up to the vertex's next-event estimate
     float4* t_target;         // suffix radiance / throughput / scale
-    int32_t* counters;        // [0] queries, [1] training records
+    int32_t* counters;        // [0] queries, [1] training records, [2] this frame's training
+                              // threshold (of 2^24), [3] unused
     int64_t query_cap, train_cap;
 };
 
@@ -64,7 +67,9 @@ struct CacheBuffers {
 struct CacheFrame {
     int32_t enabled;          // terminate non-training paths into the cache after bounce 1
     int32_t train;            // record training suffixes
-    uint32_t train_threshold; // path is a training path when hash < threshold (of 2^24)
+    uint32_t train_threshold; // first-frame threshold: a path trains when its hash < it (of 2^24)
+    int32_t adapt;            // set the threshold from the previous frame's counts
+    int32_t train_samples;    // training records wanted per frame
     uint32_t frame;
     int32_t nee;              // trace the vertex's next-event estimate; the network learns the rest
     float wide_roughness;     // lobes at least this rough (or diffuse samples) lead to a cache vertex

@@ -23,8 +23,9 @@
 //                     LDS row buffer.
 //   k_rc_train        forward + relative-L2 loss + backward for 32 samples per
 //                     workgroup (2 waves); writes its weight-gradient partial.
-//   k_rc_adam         sums the partials in workgroup order (deterministic),
-//                     Adam step on the FP32 master weights, BF16 republish.
+//   k_rc_reduce       sums consecutive chunks of the partials (workgroup order);
+//   k_rc_adam         adds the chunk sums in chunk order (deterministic), Adam
+//                     step on the FP32 master weights, BF16 republish.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -311,7 +312,23 @@ struct AdamParams {
     float bc1, bc2;       // 1 - beta^t
 };
 
-__global__ __launch_bounds__(256) void k_rc_adam(CacheBuffers cb, int nwg, const int32_t* count_dev,
+// First level of the gradient sum: chunk c (blockIdx.y) of `per` consecutive
+// workgroup partials, summed in workgroup order, for 256 parameters per
+// workgroup.  Many chunks run at once; k_rc_adam then adds the chunk sums in
+// chunk order, so the total is the same for every launch.
+__global__ __launch_bounds__(256) void k_rc_reduce(CacheBuffers cb, int nwg, int per, const int32_t* count_dev,
+                                                   int64_t count_host, int64_t cap, int64_t first) {
+    const int64_t n = count_dev ? std::min<int64_t>((int64_t)*count_dev, cap) : count_host;
+    if (n - first <= 0) return;
+    const int i = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (i >= RC_PARAMS) return;
+    const int k0 = (int)blockIdx.y * per, k1 = std::min(nwg, k0 + per);
+    float g = 0.0f;
+    for (int k = k0; k < k1; ++k) g += cb.partial[(size_t)k * RC_PADDED + i];
+    cb.partial_sum[(size_t)blockIdx.y * RC_PADDED + i] = g;
+}
+
+__global__ __launch_bounds__(256) void k_rc_adam(CacheBuffers cb, int nwg, int nchunks, const int32_t* count_dev,
                                                  int64_t count_host, int64_t cap, int64_t first, int32_t batch,
                                                  AdamParams ap) {
     const int64_t n = count_dev ? std::min<int64_t>((int64_t)*count_dev, cap) : count_host;
@@ -326,7 +343,7 @@ __global__ __launch_bounds__(256) void k_rc_adam(CacheBuffers cb, int nwg, const
     }
     if (i >= RC_PARAMS) return;
     float gsum = 0.0f;
-    for (int k = 0; k < nwg; ++k) gsum += cb.partial[(size_t)k * RC_PADDED + i];
+    for (int c = 0; c < nchunks; ++c) gsum += cb.partial_sum[(size_t)c * RC_PADDED + i];
     cb.grad[i] = gsum;
     const float m = ap.beta1 * cb.adam_m[i] + (1.0f - ap.beta1) * gsum;
     const float v = ap.beta2 * cb.adam_v[i] + (1.0f - ap.beta2) * (gsum * gsum);
@@ -360,9 +377,36 @@ __global__ __launch_bounds__(256) void k_rc_targets(CacheBuffers cb, const float
     cb.t_target[i] = make_float4(suffix(l.x, b.x, be.x), suffix(l.y, b.y, be.y), suffix(l.z, b.z, be.z), 0.0f);
 }
 
+// Start of a frame's cache records: this frame's training threshold into
+// counters[2], then the query and record counts cleared.  With cf.adapt the
+// previous frame's counts (same context and resolution) tell how many paths
+// reached their cache vertex: when the cache rendered, each of them wrote a
+// query or a record; when it only trained, the records were the threshold's
+// share of them.  The threshold is then set so that about cf.train_samples
+// paths train; otherwise cf.train_threshold (train_samples over the frame's
+// paths) stands.
+__global__ void k_rc_frame_start(CacheBuffers cb, CacheFrame cf) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t thr = cf.train_threshold;
+    if (cf.adapt) {
+        const double q = (double)cb.counters[0], r = (double)cb.counters[1];
+        const double prev = (double)(uint32_t)cb.counters[2];
+        const double reach = cf.enabled ? q + r : (prev > 0.0 ? r * 16777216.0 / prev : 0.0);
+        if (reach >= 1.0) thr = (uint32_t)fmin(16777216.0, ceil(16777216.0 * (double)cf.train_samples / reach));
+    }
+    cb.counters[2] = (int32_t)thr;
+    cb.counters[0] = 0;
+    cb.counters[1] = 0;
+}
+
 }  // namespace
 
 int cache_train_workgroups(int32_t batch) { return (batch + TRAIN_Q - 1) / TRAIN_Q; }
+
+hipError_t launch_cache_frame_start(hipStream_t s, const CacheBuffers& cb, const CacheFrame& cf) {
+    hipLaunchKernelGGL(k_rc_frame_start, dim3(1), dim3(64), 0, s, cb, cf);
+    return hipGetLastError();
+}
 
 hipError_t launch_cache_publish(hipStream_t s, const CacheBuffers& cb) {
     hipLaunchKernelGGL(k_rc_publish, dim3((RC_PADDED + 255) / 256), dim3(256), 0, s, cb);
@@ -409,8 +453,12 @@ hipError_t launch_cache_train_step(hipStream_t s, const CacheBuffers& cb, const 
     ap.eps = 1e-15f;
     ap.bc1 = (float)(1.0 - std::pow(0.9, (double)step));
     ap.bc2 = (float)(1.0 - std::pow(0.99, (double)step));
-    hipLaunchKernelGGL(k_rc_adam, dim3((RC_PARAMS + 255) / 256), dim3(256), 0, s, cb, nwg, count_dev, count_host, cap,
-                       first, batch, ap);
+    const int nchunks = std::min(RC_CHUNKS, nwg);
+    const int per = (nwg + nchunks - 1) / nchunks;
+    hipLaunchKernelGGL(k_rc_reduce, dim3((RC_PARAMS + 255) / 256, (unsigned)((nwg + per - 1) / per)), dim3(256), 0, s,
+                       cb, nwg, per, count_dev, count_host, cap, first);
+    hipLaunchKernelGGL(k_rc_adam, dim3((RC_PARAMS + 255) / 256), dim3(256), 0, s, cb, nwg, (nwg + per - 1) / per,
+                       count_dev, count_host, cap, first, batch, ap);
     return hipGetLastError();
 }
 

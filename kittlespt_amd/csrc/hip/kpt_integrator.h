@@ -2,6 +2,7 @@
 // integrator.cuh:241-402) as device code, split into the pieces the
 // megakernel and the wavefront stages share.
 #pragma once
+#include "kpt_cache.h"
 #include "kpt_shade.h"
 
 namespace kpt {
@@ -163,7 +164,63 @@ struct PathState {
     int32_t depth;
     float p_b;
     v3 prev_pos;
+    bool wide_in = false;   // radiance cache: this vertex was reached through a wide lobe
+    bool cached = false;    // radiance cache: the path has made its training record
 };
+
+// The radiance cache at a path vertex (no reference counterpart).  A path's
+// cache vertex is the first vertex after the primary hit reached through a
+// wide lobe (a diffuse sample, or a lobe of roughness >= cf.wide_roughness),
+// where the lobe blurs the network's errors.  There a training path (chosen
+// by a hash of path and frame against the frame's threshold, which
+// k_rc_frame_start leaves in cb.counters[2]) records the
+// vertex and continues: its suffix becomes the network's target.  Any other
+// path, when the cache renders, writes a query and ends there: the network
+// completes it after the frame.  A training path ends at its next cache
+// vertex in the network (self-training).  The network learns radiance
+// divided by the throughput and a reflectance proxy of the vertex
+// (0.04 + 0.96 albedo).  With cf.nee the vertex's next-event estimate is
+// traced exactly first and the network stands for the continuation only; the
+// estimate then samples only the sun when the scene has no area lights (the
+// reference's 50 % area-light draw would be a wasted, zero sample).
+struct CacheVertex {
+    bool to_cache;    // the path ends here in the network (its query is written)
+    int32_t trec;     // training record made here (-1: none)
+    float sun_p;      // sun probability of this vertex's next-event estimate
+};
+KD CacheVertex cache_vertex(const FrameConst& fc, const CacheBuffers& cb, const CacheFrame& cf, int32_t p,
+                            int32_t depth, bool wide_in, bool& cached, const Surf& s, const BSDF& b, v3 ray_d,
+                            v3 beta) {
+    CacheVertex cv;
+    cv.to_cache = false;
+    cv.trec = -1;
+    if (depth >= 1 && wide_in && depth < fc.max_depth && (cf.enabled || cf.train) && !(cached && !cf.enabled)) {
+        const uint32_t hs = pcg_hash((uint32_t)p ^ (cf.frame * 0x9E3779B9u)) & 0xFFFFFFu;
+        const float4 qpos = make_float4(s.pos.x, s.pos.y, s.pos.z, b.roughness);
+        const float4 qdir = make_float4(-ray_d.x, -ray_d.y, -ray_d.z, __builtin_bit_cast(float, (uint32_t)p));
+        const v3 bq = beta * (mk3(0.04f, 0.04f, 0.04f) + b.albedo * 0.96f);
+        if (!cached && cf.train && hs < (uint32_t)cb.counters[2]) {
+            cached = true;
+            const int r = atomicAdd(&cb.counters[1], 1);
+            if (r < cb.train_cap) {
+                cb.t_pos[r] = qpos;
+                cb.t_dir[r] = qdir;
+                cb.t_beta[r] = make_float4(bq.x, bq.y, bq.z, 0.0f);
+                cv.trec = r;
+            }
+        } else if (cf.enabled) {
+            const int r = atomicAdd(&cb.counters[0], 1);
+            if (r < cb.query_cap) {
+                cb.q_pos[r] = qpos;
+                cb.q_dir[r] = qdir;
+                cb.q_beta[r] = make_float4(bq.x, bq.y, bq.z, 0.0f);
+                cv.to_cache = true;
+            }
+        }
+    }
+    cv.sun_p = (cf.nee && (cv.to_cache || cv.trec >= 0) && fc.num_lights == 0) ? 1.0f : 0.5f;
+    return cv;
+}
 
 // One vertex of Li's loop (integrator.cuh:266-399): trace from the path's
 // ray, add emission and the next-event estimate, sample the next direction.
@@ -172,9 +229,13 @@ struct PathState {
 // DEFER_SKY (vertices past the first only): a miss sets *missed and leaves
 // L without the sky term — the last addition of the path, L + sky * beta,
 // which the caller then makes with ps.ray.d and ps.beta as they are.
-template <bool COUNT = false, bool DEFER_SKY = false>
+// CACHE: the radiance cache's hooks at the path's cache vertex (cache_vertex
+// with path index `path`), as the wavefront's shade stage applies them; a
+// path that ends in the network returns false after its next-event estimate.
+template <bool COUNT = false, bool DEFER_SKY = false, bool CACHE = false>
 KD bool li_bounce(const SceneView& S, const FrameConst& fc, PathState& ps, uint32_t& seed, uint32_t* lds,
-                  bool* ovf, VisOut* vs, TraceCounts* cnt = nullptr, bool* missed = nullptr) {
+                  bool* ovf, VisOut* vs, TraceCounts* cnt = nullptr, bool* missed = nullptr,
+                  const CacheBuffers* cb = nullptr, const CacheFrame* cf = nullptr, int32_t path = -1) {
     const float eta_scale = 1.0f;
     seed = seed + (uint32_t)ps.depth;
     const bool first = (ps.depth == 0);
@@ -204,8 +265,19 @@ KD bool li_bounce(const SceneView& S, const FrameConst& fc, PathState& ps, uint3
         b.albedo = mk3(1.0f, 1.0f, 1.0f);
     }
     if (ps.any_nonspec) bsdf_regularize(b);
-    if (!(b.alpha < 0.001f)) {
-        NeeResult ne = next_event(S, fc, s, b, ps.ray.d, ps.beta, seed);
+    CacheVertex cv;
+    cv.to_cache = false;
+    cv.trec = -1;
+    cv.sun_p = 0.5f;
+    if (CACHE) {
+        cv = cache_vertex(fc, *cb, *cf, path, ps.depth, ps.wide_in, ps.cached, s, b, ps.ray.d, ps.beta);
+        if (cv.trec >= 0 && !cf->nee) {
+            cb->t_lbefore[cv.trec] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
+            cv.trec = -1;
+        }
+    }
+    if (!(CACHE && cv.to_cache && !cf->nee) && !(b.alpha < 0.001f)) {
+        NeeResult ne = next_event(S, fc, s, b, ps.ray.d, ps.beta, seed, cv.sun_p);
         if (ne.trace) {
             bool oo = false;
             Ray sr = make_ray(ne.org, ne.dir);
@@ -216,7 +288,10 @@ KD bool li_bounce(const SceneView& S, const FrameConst& fc, PathState& ps, uint3
             ps.L = ps.L + ne.occl;
         }
     }
+    // a training record's radiance so far includes the vertex's estimate
+    if (CACHE && cv.trec >= 0) cb->t_lbefore[cv.trec] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
     if (ps.depth++ == fc.max_depth) return false;
+    if (CACHE && cv.to_cache) return false;
     v3 wo = -ps.ray.d;
     v2 u2, X2;
     u2.x = rng1(seed); u2.y = rng1(seed);
@@ -226,6 +301,7 @@ KD bool li_bounce(const SceneView& S, const FrameConst& fc, PathState& ps, uint3
     ps.beta = ps.beta * (bs.f * absdot(s.n, bs.wi) / bs.pdf);
     ps.p_b = bsdf_pdf(b, wo, bs.wi);
     ps.any_nonspec = ps.any_nonspec || !((bs.scatter & SC_SPECULAR) != 0);
+    if (CACHE) ps.wide_in = ((bs.scatter & SC_DIFFUSE) != 0) || b.roughness >= cf->wide_roughness;
     ps.prev_pos = s.pos;
     v3 org = (bs.scatter & SC_TRANSMITTED) ? s.pos - (s.n * KPT_HIT_EPS) : s.pos + (s.n * KPT_HIT_EPS);
     ps.ray = make_ray(org, bs.wi);

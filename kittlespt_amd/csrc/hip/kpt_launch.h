@@ -192,6 +192,7 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
 // Radiance-cache kernels (kpt_cache.hip).
 int cache_train_workgroups(int32_t batch);
 hipError_t launch_cache_publish(hipStream_t s, const CacheBuffers& cb);
+hipError_t launch_cache_frame_start(hipStream_t s, const CacheBuffers& cb, const CacheFrame& cf);
 hipError_t launch_cache_infer_paths(hipStream_t s, const CacheBuffers& cb, const CacheEncode& enc, float4* L,
                                     int64_t max_queries, int num_cus);
 hipError_t launch_cache_infer_raw(hipStream_t s, const CacheBuffers& cb, const CacheEncode& enc, const float4* pos,

@@ -389,54 +389,11 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_SHADE void k_shade(SceneView S, F
                 b.albedo = mk3(1.0f, 1.0f, 1.0f);
             }
             if (any_nonspec) bsdf_regularize(b);
-            // radiance cache at the path's cache vertex: the first vertex after
-            // the primary hit that was reached through a wide lobe (a diffuse
-            // sample, or a lobe of roughness >= cf.wide_roughness), where the
-            // network's errors are blurred by that lobe; narrower lobes let
-            // the path continue.  Training paths record the radiance so far
-            // there (their suffix becomes the target), the others end there
-            // and the network completes them.  With cf.nee the vertex's
-            // next-event estimate is traced exactly first and the network
-            // stands for the continuation only; without it the network stands
-            // for all radiance scattered at the vertex and no shadow ray is
-            // traced for an ending path.  The network learns radiance divided
-            // by the path throughput and a reflectance proxy of the vertex
-            // (0.04 + 0.96 albedo), so the albedo texture's detail stays out
-            // of what it has to fit.  While the cache renders, a training
-            // path ends at its next cache vertex after the recorded one, in
-            // the network (self-training: short suffixes, and the target's
-            // tail is the network's own estimate instead of a long noisy one).
-            bool to_cache = false;
-            int32_t trec = -1;
-            if (depth >= 1 && wide_in && depth < fc.max_depth && (cf.enabled || cf.train) &&
-                !(cached && !cf.enabled)) {
-                const uint32_t hs = pcg_hash((uint32_t)p ^ (cf.frame * 0x9E3779B9u)) & 0xFFFFFFu;
-                const float4 qpos = make_float4(s.pos.x, s.pos.y, s.pos.z, b.roughness);
-                const float4 qdir = make_float4(-ray.d.x, -ray.d.y, -ray.d.z, bitsf((uint32_t)p));
-                const v3 bq = beta * (mk3(0.04f, 0.04f, 0.04f) + b.albedo * 0.96f);
-                if (!cached && cf.train && hs < cf.train_threshold) {
-                    cached = true;
-                    const int r = atomicAdd(&cb.counters[1], 1);
-                    if (r < cb.train_cap) {
-                        cb.t_pos[r] = qpos;
-                        cb.t_dir[r] = qdir;
-                        cb.t_beta[r] = make_float4(bq.x, bq.y, bq.z, 0.0f);
-                        trec = r;
-                    }
-                } else if (cf.enabled) {
-                    const int r = atomicAdd(&cb.counters[0], 1);
-                    if (r < cb.query_cap) {
-                        cb.q_pos[r] = qpos;
-                        cb.q_dir[r] = qdir;
-                        cb.q_beta[r] = make_float4(bq.x, bq.y, bq.z, 0.0f);
-                        to_cache = true;
-                    }
-                }
-            }
-            // the traced estimate at a cache vertex samples only the sun when
-            // the scene has no area lights (the reference's 50 % area-light
-            // draw would be a wasted, zero sample there)
-            const float sun_p = (cf.nee && (to_cache || trec >= 0) && fc.num_lights == 0) ? 1.0f : 0.5f;
+            // radiance cache at the path's cache vertex (cache_vertex)
+            const CacheVertex cv = cache_vertex(fc, cb, cf, p, depth, wide_in, cached, s, b, ray.d, beta);
+            const bool to_cache = cv.to_cache;
+            int32_t trec = cv.trec;
+            const float sun_p = cv.sun_p;
             if (trec >= 0 && !cf.nee) {
                 cb.t_lbefore[trec] = make_float4(L.x, L.y, L.z, 0.0f);
                 trec = -1;
@@ -506,10 +463,12 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_SHADE void k_shade(SceneView S, F
 // (path, direction, throughput) to the sky queue (the shadow-ray arrays,
 // unused while the tail runs; count in C_SHADOW, cleared before the launch)
 // instead of evaluating the sky with the few lanes that missed.
-template <bool COUNT>
+// CACHE: the radiance cache's hooks at each path's cache vertex (li_bounce).
+template <bool COUNT, bool CACHE>
 __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TAIL void k_tail(SceneView S, FrameConst fc, WavefrontState W, int parity,
                                                    int pull_slot, int32_t* overflow,
-                                                   unsigned long long* counters, const int32_t* order) {
+                                                   unsigned long long* counters, const int32_t* order,
+                                                   CacheBuffers cb, CacheFrame cf) {
     __shared__ uint32_t s_stack[LDS_STACK][WF_BLOCK];
     const int count = W.counters[C_QCOUNT + parity];
     const int32_t* q = order ? order : W.queue[parity];
@@ -539,6 +498,8 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TAIL void k_tail(SceneView S, Fra
                 st = fbits(l4.w);
                 ps.depth = (int32_t)(st & 0xFFFFu);
                 ps.any_nonspec = (st >> 16) & 1u;
+                ps.wide_in = (st >> 17) & 1u;
+                ps.cached = (st >> 18) & 1u;
                 ps.prev_pos = mk3(pp.x, pp.y, pp.z);
                 seed = fbits(d4.w);
                 active = true;
@@ -551,8 +512,8 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TAIL void k_tail(SceneView S, Fra
             bool o = false;
             if (COUNT && !is_zero(ps.beta)) ++n_vertices;
             const bool more = !is_zero(ps.beta) &&
-                              li_bounce<COUNT, true>(S, fc, ps, seed, &s_stack[0][threadIdx.x], &o, &vs, &c,
-                                                     &missed);
+                              li_bounce<COUNT, true, CACHE>(S, fc, ps, seed, &s_stack[0][threadIdx.x], &o, &vs, &c,
+                                                            &missed, &cb, &cf, p);
             ovf = ovf || o;
             if (!more || is_zero(ps.beta)) {
                 W.L[p] = make_float4(ps.L.x, ps.L.y, ps.L.z, bitsf((uint32_t)ps.depth | (st & 0x10000u)));
@@ -694,7 +655,8 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
     if (cache && cache->cb && (cache->cf.enabled || cache->cf.train)) {
         cb = *cache->cb;
         cf = cache->cf;
-        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, cb.counters, 0, 2);
+        hipError_t e = launch_cache_frame_start(s, cb, cf);
+        if (e != hipSuccess) return e;
     }
     if (g_num_cus == 0) {
         int dev = 0;
@@ -721,7 +683,7 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
     if (cf.enabled && !(cf.wide_roughness > 0.0f)) max_bounces = std::min(max_bounces, 3);
     for (int bounce = 0; bounce < max_bounces; ++bounce) {
         const int parity = bounce & 1;
-        if (tune.tail > 0 && bounce >= tune.tail && !(cf.enabled || cf.train)) {
+        if (tune.tail > 0 && bounce >= tune.tail) {
             // the tail's paths in spatial order of their rays (ray_sort knob)
             const int32_t* tail_q = nullptr;
             if (tune.ray_sort && W.rtemp) {
@@ -734,12 +696,20 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
             }
             hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, W.counters + C_SHADOW, 0, 1);
             if (ins) (void)hipEventRecord(ins->tail_begin, s);
-            if (ins && ins->counters)
-                hipLaunchKernelGGL(k_tail<true>, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity,
-                                   C_PULL + 3 * bounce, overflow, ins->counters, tail_q);
+            const bool with_cache = cf.enabled || cf.train;
+            unsigned long long* tc = (ins && ins->counters) ? ins->counters : nullptr;
+            if (tc && with_cache)
+                hipLaunchKernelGGL((k_tail<true, true>), dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity,
+                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf);
+            else if (tc)
+                hipLaunchKernelGGL((k_tail<true, false>), dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity,
+                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf);
+            else if (with_cache)
+                hipLaunchKernelGGL((k_tail<false, true>), dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity,
+                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf);
             else
-                hipLaunchKernelGGL(k_tail<false>, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity,
-                                   C_PULL + 3 * bounce, overflow, (unsigned long long*)nullptr, tail_q);
+                hipLaunchKernelGGL((k_tail<false, false>), dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity,
+                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf);
             if (ins) {
                 (void)hipEventRecord(ins->tail_end, s);
                 ins->tail_launches = 1;

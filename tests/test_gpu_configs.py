@@ -124,14 +124,16 @@ def test_c2_views_match_oracle(c1, mode):
     assert bad == 0
 
 
-def test_c3_cached_frame_matches_oracle(c1):
+@pytest.mark.parametrize("mode", ["tail1", "stages_sorted"])
+def test_c3_cached_frame_matches_oracle(c1, mode):
     """A 1080p frame of the C1 scene in which every path ends in the cache at
     its bounce-1 vertex after that vertex's next-event estimate (the bench's
     mode), the network at random weights (the query, not the training, is
-    what the oracle pins)."""
+    what the oracle pins); bounce 1 in the path-per-lane tail or as sorted
+    stage kernels."""
     from test_cache_oracle import he_weights
     sc, ctx, orc = c1
-    ctx.set_tuning(**TUNINGS["tail1"])
+    ctx.set_tuning(**TUNINGS[mode])
     st = settings(max_depth=8, integrator=abi.INTEGRATOR_WAVEFRONT, sun_theta=sc.sun_theta)
     ctx.set_settings(st)
     ctx.resize(W1, H1)
@@ -142,7 +144,7 @@ def test_c3_cached_frame_matches_oracle(c1):
     w = he_weights(51) * np.float32(0.5)
     ctx.cache_configure(enabled=True, train=False, seed=7, bounds=box, radiance_scale=1e4, nee_at_vertex=1)
     ctx.cache_set_weights(w)
-    print("[c3] one cached 1080p frame", flush=True)
+    print(f"[c3] one cached 1080p frame, {mode}", flush=True)
     try:
         ctx.render(1)
         gp, gd, gb = ctx.cache_queries()
