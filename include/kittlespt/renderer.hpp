@@ -41,7 +41,6 @@ public:
     }
     ~Renderer() { shutdown(); }
 
-    // renderer.cu:55-101 (no upscaler: render and output resolutions agree)
     // renderer.cu:55-101: nothing happens when the output size and the render
     // size (output / 2.5 with upscale_enable) are already these
     void resizeResolution(uint32_t width, uint32_t height) {

@@ -357,9 +357,11 @@ int kpt_set_trace_counting(kpt_ctx* ctx, int32_t enable);
  * instances move) or rebuild it on the host (0, default), [9] (when n >= 10)
  * build the BLASes of the next kpt_upload_scene on the device (1, the
  * default of a context with a device: the host builder's trees node for
- * node, all meshes in one level-by-level build; needs finite vertex
- * positions) or on host threads (0, the only choice of a host-only
- * context).  n >= 5. */
+ * node, all meshes in one level-by-level build) or on host threads (0, the
+ * only choice of a host-only context).  A scene the device builder cannot
+ * take (non-finite vertex positions, more than 2^29 triangles) or a device
+ * build that fails is built on host threads; kpt_last_build_timing says
+ * which ran.  n >= 5. */
 int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n);
 int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n);
 int kpt_get_render_counters(const kpt_ctx* ctx, kpt_trace_counters* out);
@@ -444,6 +446,14 @@ int kpt_cache_stats(kpt_ctx* ctx, double out[5]);
  * index), throughputs n x 4.  *count receives the number of queries; at most
  * cap are copied. */
 int kpt_cache_get_queries(kpt_ctx* ctx, float* pos, float* dir, float* beta, int64_t cap, int64_t* count);
+
+/* ---- image data (no reference counterpart: the reference's importer
+ * decodes images with stb_image, SampleApp model_importer.hpp) ---- */
+/* PNG scanline unfiltering (PNG specification, section 9): raw holds height
+ * rows of 1 filter byte + stride bytes (the inflated IDAT stream of an
+ * 8-bit, non-interlaced image with bpp bytes per pixel); out receives
+ * height x stride bytes.  KPT_ERR_INVALID_ARG on an unknown filter type. */
+int kpt_png_unfilter(const uint8_t* raw, int64_t height, int64_t stride, int32_t bpp, uint8_t* out);
 
 #ifdef __cplusplus
 }

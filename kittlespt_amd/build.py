@@ -23,6 +23,7 @@ SOURCES = [
     os.path.join(CSRC, "hip", "kpt_tlas.hip"),
     os.path.join(CSRC, "hip", "kpt_blas.hip"),
     os.path.join(CSRC, "host", "kpt_build.cpp"),
+    os.path.join(CSRC, "host", "kpt_image.cpp"),
 ]
 
 HIPCC_FLAGS = [

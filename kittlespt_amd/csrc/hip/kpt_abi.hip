@@ -867,32 +867,39 @@ int kpt_upload_scene(kpt_ctx* ctx, const kpt_scene_desc* d) {
     // BLAS per mesh: host threads, or all meshes at once on the device
     ctx->blas.assign(d->num_meshes, BlasTree());
     const auto build_t0 = std::chrono::steady_clock::now();
-    if (ctx->blas_device && T > 0) {
-        if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "device BLAS build on a host-only context");
-        if (T > BLAS_DEVICE_MAX_TRIS)
-            return fail(ctx, KPT_ERR_UNSUPPORTED, "device BLAS build: more than 2^29 triangles (use blas_device = 0)");
-        for (size_t i = 0; i < ctx->pos.size(); ++i)
-            if (!std::isfinite(ctx->pos[i]))
-                return fail(ctx, KPT_ERR_INVALID_ARG, "device BLAS build needs finite vertex positions");
+    // The device builder needs a device, at most BLAS_DEVICE_MAX_TRIS
+    // triangles and finite positions (its bin keys order finite floats); a
+    // scene outside that, or a device build that fails, is built by the host
+    // threads instead (the same trees; the reference builds any scene on the
+    // host, as_builder.cuh:26-35).
+    bool on_device = ctx->blas_device && T > 0 && !ctx->host_only && T <= BLAS_DEVICE_MAX_TRIS;
+    for (size_t i = 0; on_device && i < ctx->pos.size(); ++i)
+        if (!std::isfinite(ctx->pos[i])) on_device = false;
+    if (on_device) {
         std::vector<Node> dn;
         std::vector<int32_t> dorder, dused, ddepth;
         hipError_t be = device_build_blas(ctx->stream, ctx->pos.data(), ctx->mesh_first.data(), ctx->mesh_count.data(),
                                           d->num_meshes, T, &dn, &dorder, &dused, &ddepth);
-        if (be != hipSuccess) return fail(ctx, KPT_ERR_HIP, std::string("device BLAS build: ") + hipGetErrorString(be));
-        size_t base = 0;
-        for (int32_t m = 0; m < d->num_meshes; ++m) {
-            const int32_t cnt = ctx->mesh_count[m];
-            if (cnt <= 0) continue;
-            BlasTree& bt = ctx->blas[m];
-            const size_t cap = 2 * (size_t)cnt - 1;
-            bt.nodes.assign(dn.begin() + base, dn.begin() + base + cap);
-            bt.order.assign(dorder.begin() + ctx->mesh_first[m], dorder.begin() + ctx->mesh_first[m] + cnt);
-            bt.used = dused[m];
-            bt.max_depth = ddepth[m];
-            for (int k = 0; k < 3; ++k) { bt.bounds.lo[k] = bt.nodes[0].lo[k]; bt.bounds.hi[k] = bt.nodes[0].hi[k]; }
-            base += cap;
+        if (be != hipSuccess) {
+            (void)hipGetLastError();
+            on_device = false;
+        } else {
+            size_t base = 0;
+            for (int32_t m = 0; m < d->num_meshes; ++m) {
+                const int32_t cnt = ctx->mesh_count[m];
+                if (cnt <= 0) continue;
+                BlasTree& bt = ctx->blas[m];
+                const size_t cap = 2 * (size_t)cnt - 1;
+                bt.nodes.assign(dn.begin() + base, dn.begin() + base + cap);
+                bt.order.assign(dorder.begin() + ctx->mesh_first[m], dorder.begin() + ctx->mesh_first[m] + cnt);
+                bt.used = dused[m];
+                bt.max_depth = ddepth[m];
+                for (int k = 0; k < 3; ++k) { bt.bounds.lo[k] = bt.nodes[0].lo[k]; bt.bounds.hi[k] = bt.nodes[0].hi[k]; }
+                base += cap;
+            }
         }
-    } else {
+    }
+    if (!on_device) {
         std::vector<std::thread> pool;
         const float* P = ctx->pos.data();
         for (int32_t m = 0; m < d->num_meshes; ++m)
@@ -900,7 +907,7 @@ int kpt_upload_scene(kpt_ctx* ctx, const kpt_scene_desc* d) {
         for (auto& th : pool) th.join();
     }
     ctx->blas_build_ms[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - build_t0).count();
-    ctx->blas_build_ms[1] = (ctx->blas_device && T > 0) ? 1.0 : 0.0;
+    ctx->blas_build_ms[1] = on_device ? 1.0 : 0.0;
     // global node array: each root at an odd index so every sibling pair
     // starts at an even index (one 64-byte line)
     ctx->nodes.clear();

@@ -95,12 +95,15 @@ __global__ __launch_bounds__(BT) void k_bb_init(BB b, const int32_t* root_act) {
     b.slot_act[s] = root_act[lo];
 }
 
+// One thread per bin word of the level's nodes: nact * BIN_WORDS passes 2^31
+// once a level splits more than ~12.8 M nodes (meshes of ~115 M triangles),
+// so the index is 64-bit.
 __global__ __launch_bounds__(BT) void k_bb_level_init(BB b, int32_t nact) {
-    const int32_t i = blockIdx.x * BT + threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
     if (i == 0) { b.counters[1] = 0; b.counters[3] = 0; }
-    if (i < nact * 6) b.acb[i] = (i % 6) < 3 ? okey(INFINITY) : okey(-INFINITY);
-    if (i < nact * BIN_WORDS) {
-        const int w = i % BIN_WORDS;
+    if (i < (int64_t)nact * 6) b.acb[i] = (i % 6) < 3 ? okey(INFINITY) : okey(-INFINITY);
+    if (i < (int64_t)nact * BIN_WORDS) {
+        const int w = (int)(i % BIN_WORDS);
         b.abin[i] = w < 24 ? 0u : w < 96 ? okey(FLT_MAX) : okey(-FLT_MAX);
     }
 }
@@ -442,6 +445,14 @@ __global__ __launch_bounds__(BT) void k_bb_emit(BB b, int32_t nn, const int32_t*
 }
 
 inline int32_t blocks_for(int64_t n) { return (int32_t)((n + BT - 1) / BT); }
+
+// The widest launch of a build is one thread per bin word of a level's
+// nodes: at most (T / (LEAF_MAX + 1) + 1) * BIN_WORDS threads for T up to
+// BLAS_DEVICE_MAX_TRIS; its block count must fit the int32 grid size above,
+// and every per-slot and per-node index (< 2T) an int32.
+constexpr int64_t MAX_ACT = (int64_t)BLAS_DEVICE_MAX_TRIS / (LEAF_MAX + 1) + 1;
+static_assert((MAX_ACT * BIN_WORDS + BT - 1) / BT < ((int64_t)1 << 31), "bin-word grid fits an int32 block count");
+static_assert(2 * (int64_t)BLAS_DEVICE_MAX_TRIS < ((int64_t)1 << 31), "slot and node indices fit int32");
 
 // Launch over n threads; nothing to do for n == 0 (a level whose splits all
 // aborted adds no nodes).

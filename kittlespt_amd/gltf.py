@@ -37,12 +37,13 @@ import json
 import math
 import os
 import struct
+import ctypes
 import zlib
 from dataclasses import dataclass
 
 import numpy as np
 
-from . import scenes
+from . import abi, scenes
 
 _CTYPES = {5120: np.int8, 5121: np.uint8, 5122: np.int16, 5123: np.uint16, 5125: np.uint32, 5126: np.float32}
 _NCOMP = {"SCALAR": 1, "VEC2": 2, "VEC3": 3, "VEC4": 4, "MAT4": 16}
@@ -83,37 +84,14 @@ def decode_png(data: bytes) -> np.ndarray:
     raw = np.frombuffer(zlib.decompress(b"".join(idat)), np.uint8)
     if raw.size != h * (stride + 1):
         raise ValueError("PNG data size does not match its header")
-    raw = raw.reshape(h, stride + 1)
-    out = np.zeros((h, stride), np.int32)
-    prev = np.zeros(stride, np.int32)
-    bpp = chans
-    for y in range(h):
-        f, line = int(raw[y, 0]), raw[y, 1:].astype(np.int32)
-        cur = np.zeros(stride, np.int32)
-        if f == 0:
-            cur = line.copy()
-        elif f == 2:
-            cur = (line + prev) & 255
-        elif f in (1, 3, 4):
-            # left-dependent filters: byte by byte over the row
-            for x in range(stride):
-                a = cur[x - bpp] if x >= bpp else 0
-                b = prev[x]
-                c = prev[x - bpp] if x >= bpp else 0
-                if f == 1:
-                    pred = a
-                elif f == 3:
-                    pred = (a + b) >> 1
-                else:
-                    p = a + b - c
-                    pa, pb, pc = abs(p - a), abs(p - b), abs(p - c)
-                    pred = a if (pa <= pb and pa <= pc) else (b if pb <= pc else c)
-                cur[x] = (line[x] + pred) & 255
-        else:
-            raise ValueError(f"PNG row filter {f} unknown")
-        out[y] = cur
-        prev = cur
-    px = out.astype(np.uint8).reshape(h, w, chans)
+    out = np.empty((h, stride), np.uint8)
+    L = abi.lib()
+    f = L.kpt_png_unfilter
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p]
+    if f(raw.ctypes.data_as(ctypes.c_void_p), h, stride, chans, out.ctypes.data_as(ctypes.c_void_p)) != abi.KPT_OK:
+        raise ValueError("PNG row filter unknown")
+    px = out.reshape(h, w, chans)
     if ctype == 0:
         rgb = np.repeat(px, 3, axis=2)
     elif ctype == 2:

@@ -255,3 +255,21 @@ def test_tlas_small_cases():
     assert t["lr"][0] == t["lr"][5]
     leaves = t["blas"][t["lr"] == 0]
     assert sorted(leaves[1:].tolist()) == [0, 1, 2] if len(leaves) == 4 else sorted(leaves.tolist()) == [0, 1, 2]
+
+
+def test_host_only_context_builds_on_host_threads_whatever_the_knob():
+    """blas_device = 1 on a context without a device: the upload builds on
+    host threads (kpt_last_build_timing reports it) instead of failing."""
+    sc = scenes.scene_plain(subdiv=2)
+    a, b = Context(-2), Context(-2)
+    try:
+        a.set_tuning(blas_device=1)
+        a.upload(sc)
+        assert a.build_timing()[1] is False
+        b.upload(sc)
+        for m in range(len(sc.mesh_count)):
+            assert a.blas(m)[0].tobytes() == b.blas(m)[0].tobytes()
+    finally:
+        a.close()
+        b.close()
+

@@ -91,6 +91,33 @@ def test_device_build_is_the_default_on_a_device():
     ctx.close()
 
 
+def test_scene_the_device_builder_cannot_take_is_built_on_the_host():
+    """A vertex position that is not finite (the device builder's bin keys
+    order finite floats only): the upload still succeeds, on host threads,
+    with the host builder's trees."""
+    rng = np.random.default_rng(71)
+    p = rng.uniform(-1, 1, (40, 3, 3))
+    p[7, 1, 2] = np.nan
+    p[19, 0, 0] = np.inf
+    sc = scenes._pack([_mesh(p), _mesh(rng.uniform(-1, 1, (30, 3, 3)))],
+                      [scenes.material()], None)
+    ctx = Context(0)
+    host = Context(-2)
+    try:
+        assert ctx.tuning()["blas_device"] == 1
+        ctx.upload(sc)
+        assert ctx.build_timing()[1] is False
+        host.upload(sc)
+        for m in range(2):
+            assert ctx.blas(m)[0].tobytes() == host.blas(m)[0].tobytes()
+            assert np.array_equal(ctx.blas(m)[1], host.blas(m)[1])
+        ctx.upload(scenes.scene_instanced_small())          # finite again: on the device
+        assert ctx.build_timing()[1] is True
+    finally:
+        ctx.close()
+        host.close()
+
+
 def test_device_blas_renders_the_same():
     sc = scenes.scene_instanced_small()
     W, H = 64, 48
