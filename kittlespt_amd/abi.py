@@ -243,6 +243,8 @@ SIGNATURES = [
     ("kpt_trace_closest", C.c_int, [_P, _P, _P, C.c_int64]),
     ("kpt_trace_any", C.c_int, [_P, _P, _P, C.c_int64]),
     ("kpt_trace_closest_counted", C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(KptTraceCounters)]),
+    ("kpt_trace_closest_counted_noalloc", C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(KptTraceCounters)]),
+    ("kpt_png_unfilter", C.c_int, [_P, C.c_int64, C.c_int64, C.c_int32, _P]),
     ("kpt_set_settings", C.c_int, [_P, C.POINTER(KptSettings)]),
     ("kpt_get_settings", C.c_int, [_P, C.POINTER(KptSettings)]),
     ("kpt_set_view", C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
