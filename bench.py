@@ -46,6 +46,17 @@ TAIL_BYTES_PER_PATH = 100
 STAGE_KERNELS = {"trace": "k_trace (wavefront closest hit)",
                  "tail": "k_tail (path-per-lane tail: closest and any-hit traversal, shading)"}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "wavefront_instanced1080p_traffic.json")
+# VALU-bound stages: FP32 vector peak of the MI355X
+VALU_PEAK_TFLOPS = 157.3
+VALU_STAGES = {"shade": "k_shade (primary vertex: surface, BSDF, next-event estimate, primary misses' sky)",
+               "sky": "k_sky (sky of the tail's escaped paths)"}
+# radiance-cache MLP on the matrix cores: BF16 dense peak
+MFMA_PEAK_TFLOPS = 2500.0
+# 32 -> 64 -> 64 -> 64 -> 64 -> 64 -> 3, no biases
+MLP_MACS = 32 * 64 + 4 * 64 * 64 + 64 * 3                 # 18,624 per query
+MLP_FWD_FLOP = 2 * MLP_MACS                               # 37,248
+# training sample: forward, input gradients of layers 1..5 (64x64 x 4, 3x64), weight gradients
+MLP_TRAIN_FLOP = MLP_FWD_FLOP + 2 * (4 * 64 * 64 + 3 * 64) + 2 * MLP_MACS   # 107,648
 
 
 def log(msg):
@@ -130,6 +141,19 @@ def fp64_reference(scene, st, cam, W, H, pixels, spp):
     return ref.astype(np.float64)
 
 
+def op_counts(scene, st, cam, W, H, n):
+    """Algorithmic FP32 ops per sample of the frame, by phase, over a random
+    pixel sample (the oracle instantiated with a counting float; the
+    yardstick of the VALU rooflines, never the thing timed)."""
+    import oracle
+    orc = oracle_scene(scene)
+    pix = np.random.default_rng(4321).choice(W * H, size=n, replace=False).astype(np.int32)
+    t0 = time.time()
+    ops = oracle.count_ops(orc, st, cam, W, H, pix, frame=0)
+    log(f"[ops] counted {n} samples in {time.time() - t0:.1f}s")
+    return ops
+
+
 def measured_traffic(stage, W, H, instances, depth, spp, tail):
     """HBM bytes per launch of a stage's kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE passes of this workload (scaled by the
@@ -159,10 +183,13 @@ def rel_rmse(est, ref):
     return float(np.sqrt(np.mean((est - ref) ** 2 / (ref ** 2 + floor ** 2))))
 
 
-def run_curve(ctx, pixels, ref, frames, checkpoints, warmup_frames=0, label=""):
+def run_curve(ctx, pixels, ref, frames, checkpoints, warmup_frames=0, label="", cache_acc=None):
     """Renders `frames` frames of 1 spp (after `warmup_frames` frames whose
     time counts but whose samples are discarded); returns [(seconds, spp,
-    rmse)] at the checkpoints.  Seconds are HIP-event frame times."""
+    rmse)] at the checkpoints.  Seconds are HIP-event frame times.
+    cache_acc (a dict, or None): per accumulated frame, the cache stages'
+    HIP-event ms and launches and the frame's queries and training records
+    are added to it."""
     seconds = 0.0
     for _ in range(warmup_frames):
         ctx.render(1)
@@ -173,6 +200,18 @@ def run_curve(ctx, pixels, ref, frames, checkpoints, warmup_frames=0, label=""):
     for f in range(1, frames + 1):
         ctx.render(1)
         seconds += ctx.timing()[0] / 1e3
+        if cache_acc is not None:
+            for stage in ("cache_infer", "cache_train"):
+                sst = ctx.stage_stats(stage)
+                cache_acc[stage + "_ms"] = cache_acc.get(stage + "_ms", 0.0) + sst["ms"]
+                cache_acc[stage + "_launches"] = cache_acc.get(stage + "_launches", 0) + sst["launches"]
+            cs = ctx.cache_stats()
+            cache_acc["queries"] = cache_acc.get("queries", 0) + int(cs["queries"])
+            cache_acc["train_records"] = cache_acc.get("train_records", 0) + int(cs["train_records"])
+            # the frame's Adam steps consume at most train_samples of its records
+            cache_acc["trained"] = cache_acc.get("trained", 0) + min(int(cs["train_records"]),
+                                                                   cache_acc.get("train_samples", 1 << 62))
+            cache_acc["frames"] = cache_acc.get("frames", 0) + 1
         if f in checkpoints:
             est = ctx.estimate().reshape(-1, 3)[pixels].astype(np.float64)
             out.append((seconds, f, rel_rmse(est, ref)))
@@ -187,14 +226,23 @@ def run_curve(ctx, pixels, ref, frames, checkpoints, warmup_frames=0, label=""):
 def batched_views(ctx, scene, n_views, spp, seed=64):
     """The multi-view workload: n_views cameras around the instance box with
     square resolutions log-uniform in [128, 1024], all traced as one set of
-    paths per sample (kpt_render_views)."""
+    paths per sample (kpt_render_views), with the configuration's material
+    sort and a hipGraph per sample (the eager, unsorted figure beside it)."""
     views = scenes.multi_view_cameras(scene, n_views, seed)
     pixels = sum(w * h for _, _, w, h in views)
-    ctx.render_views(views, spp=1)                      # allocation and warm-up
-    ctx.render_views(views, spp=spp)
-    ms = ctx.timing()[0]
+    keep = ctx.tuning()
+    res = {}
+    for name, kn in (("graph_sorted", {"graph": 1, "sort": 1}), ("eager", {"graph": 0})):
+        ctx.set_tuning(**kn)
+        ctx.render_views(views, spp=1)                  # allocation, capture and warm-up
+        ctx.render_views(views, spp=spp)
+        res[name] = ctx.timing()[0]
+        ctx.set_tuning(**keep)
+    ms = res["graph_sorted"]
     out = {"views": n_views, "pixels": int(pixels), "spp": spp, "ms": round(ms, 3),
            "Msamples_per_s": round(pixels * spp / (ms / 1e3) / 1e6, 3),
+           "tuning": "hipGraph per sample, material sort at the bounces the stage kernels run",
+           "eager_unsorted_ms": round(res["eager"], 3),
            "resolutions": "square, side log-uniform in [128, 1024]"}
     log(f"[views] {out}")
     return out
@@ -209,11 +257,14 @@ def axis_angle(axis, ang):
                      [0, 0, 0, 1]], np.float64)
 
 
-def full_frame_4k(ctx, scene, frames, seed=100):
+def full_frame_4k(ctx, scene, frames, seed=100, tlas_device=2):
     """The animated full frame: 3840x2160, the first 100 instances moving with
-    seeded linear + angular velocities, TLAS rebuilt every frame, motion
-    vectors, temporal accumulation, 256-bin auto exposure (ISO 100, f/2.8,
-    1/60 s), 6-level bloom, AgX, radiance cache on (training online)."""
+    seeded linear + angular velocities, the TLAS refitted on the device every
+    frame (tlas_device 2; 0 rebuilds it on the host as the reference does),
+    motion vectors, temporal accumulation, 256-bin auto exposure (ISO 100,
+    f/2.8, 1/60 s), 6-level bloom, AgX, radiance cache on (training online)."""
+    keep = ctx.tuning()["tlas_device"]
+    ctx.set_tuning(tlas_device=tlas_device)
     W, H = 3840, 2160
     rng = np.random.default_rng(seed)
     base = np.array(scene.instance_model[:100], np.float64).reshape(100, 4, 4).transpose(0, 2, 1)  # row-major
@@ -246,13 +297,15 @@ def full_frame_4k(ctx, scene, frames, seed=100):
     out = ctx.readback(4)
     ctx.cache_configure(enabled=False, train=False, seed=13)
     ctx.set_instance_transforms(0, scene.instance_model[:100])   # the later legs render the scene at rest
+    ctx.set_tuning(tlas_device=keep)
     steady = times[2:] if len(times) > 4 else times
     rec = {"width": W, "height": H, "frames": frames, "animated_instances": 100,
            "gpu_ms_per_frame": round(float(np.mean(steady)), 3),
            "wall_ms_per_frame": round(1e3 * float(np.mean(host[2:] if len(host) > 4 else host)), 3),
            "Msamples_per_s": round(W * H / (float(np.mean(steady)) / 1e3) / 1e6, 3),
            "output_finite": bool(np.isfinite(out).all()),
-           "stages": "TLAS rebuild per frame, motion vectors, temporal, AE, bloom, AgX, cache"}
+           "tlas": "device refit per frame" if tlas_device == 2 else "host rebuild per frame",
+           "stages": "TLAS per frame, motion vectors, temporal, AE, bloom, AgX, cache"}
     log(f"[4k] {rec}")
     return rec
 
@@ -310,6 +363,8 @@ def main():
     ap.add_argument("--integrator", choices=["wavefront", "megakernel"], default="wavefront")
     ap.add_argument("--instances", type=int, default=1000)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--op-count-pixels", type=int, default=2048,
+                    help="pixel sample of the FP32 op count behind the shading stages' VALU rooflines (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", default="", help="fixed knobs, e.g. refill=0,blocks_per_cu=8,sort=1")
@@ -413,14 +468,15 @@ def main():
 
     ctx.reset()
     render_ms = 0.0
-    stage_ms = {sname: 0.0 for sname in STAGE_KERNELS}
-    stage_launches = {sname: 0 for sname in STAGE_KERNELS}
+    timed_stages = list(STAGE_KERNELS) + list(VALU_STAGES)
+    stage_ms = {sname: 0.0 for sname in timed_stages}
+    stage_launches = {sname: 0 for sname in timed_stages}
     t0 = time.perf_counter()
     for i in range(args.steps):
         ctx.render(args.spp)
         render_ms += ctx.timing()[0]
         if integ == abi.INTEGRATOR_WAVEFRONT:
-            for sname in STAGE_KERNELS:
+            for sname in timed_stages:
                 sst = ctx.stage_stats(sname)
                 stage_ms[sname] += sst["ms"]
                 stage_launches[sname] += sst["launches"]
@@ -468,6 +524,35 @@ def main():
         except Exception as e:  # the baseline is reported, never fatal
             log(f"[cpu] baseline failed: {e}")
 
+    # FP32 vector rooflines of the shading stages: the algorithmic op count
+    # per sample (the oracle instantiated with a counting float over a pixel
+    # sample: +, -, *, /, sqrt and transcendentals, one op each) times the
+    # frame's samples, over the launch's HIP-event time
+    if roof is not None and args.op_count_pixels > 0:
+        try:
+            ops = op_counts(scene, st, cam, W, H, args.op_count_pixels)
+            flop = {k: v["arith"] + v["sqrt"] + v["transcendental"] for k, v in ops.items()}
+            per_launch = {"shade": (flop["shade_primary"] + flop["sky_primary"]) * W * H,
+                          "sky": flop["sky_later"] * W * H}
+            for sname, kname in VALU_STAGES.items():
+                if stage_launches[sname] <= 0:
+                    continue
+                avg_s = stage_ms[sname] / stage_launches[sname] / 1e3
+                ach = per_launch[sname] / avg_s / 1e12
+                roof["stages"][sname] = {"kernel": kname, "bound": "valu", "unit": "TFLOP/s",
+                                         "achieved": round(ach, 3), "peak": VALU_PEAK_TFLOPS,
+                                         "frac": round(ach / VALU_PEAK_TFLOPS, 5),
+                                         "flop_per_launch": int(per_launch[sname]),
+                                         "avg_launch_ms": round(avg_s * 1e3, 4),
+                                         "ms_per_step": round(stage_ms[sname] / args.steps, 3)}
+            roof["op_counts_per_sample"] = {k: {x: round(y, 2) for x, y in v.items()} for k, v in ops.items()}
+            roof["op_counts_sample"] = (f"{args.op_count_pixels} random pixels, 1 sample each, oracle with a "
+                                        f"counting float; the sky toward the sun is a per-frame constant the "
+                                        f"kernels evaluate once (not counted per sample)")
+            log(f"[ops] per sample: {roof['op_counts_per_sample']}")
+        except Exception as e:  # measurement aid, never fatal
+            log(f"[ops] op count failed: {e}")
+
     # the same steps with every frame replayed as a captured hipGraph
     graph_rec = None
     if integ == abi.INTEGRATOR_WAVEFRONT and not args.no_graph_leg:
@@ -507,7 +592,9 @@ def main():
 
     full4k = None
     if integ == abi.INTEGRATOR_WAVEFRONT and args.full4k > 0:
-        full4k = full_frame_4k(ctx, scene, args.full4k)
+        full4k = full_frame_4k(ctx, scene, args.full4k, tlas_device=2)
+        rebuild = full_frame_4k(ctx, scene, args.full4k, tlas_device=0)
+        full4k["host_rebuild"] = {k: rebuild[k] for k in ("gpu_ms_per_frame", "wall_ms_per_frame", "Msamples_per_s")}
         # back to the headline frame for the time-to-target leg
         ctx.set_settings(st)
         ctx.resize(W, H)
@@ -531,8 +618,9 @@ def main():
         ctx.cache_configure(enabled=True, train=True, train_samples=65536, train_steps=4, learning_rate=1e-3,
                             radiance_scale=1e4, seed=11, nee_at_vertex=args.cache_nee,
                             wide_roughness=args.cache_wide)
+        cache_acc = {"train_samples": 65536}
         cached = run_curve(ctx, pix, ref, args.ttt_spp, set(plain_cp), warmup_frames=args.cache_warmup,
-                           label="cached")
+                           label="cached", cache_acc=cache_acc)
         t_cached = time_at(cached, target)
         ctx.cache_configure(enabled=False, train=False, seed=11)
         ttt = {"target_rel_rmse": round(target, 5), "target": f"plain wavefront at {args.ttt_spp} spp",
@@ -550,6 +638,29 @@ def main():
                             f"(host cores)",
                "curve_plain": [[round(s, 4), n, round(e, 5)] for s, n, e in plain],
                "curve_cached": [[round(s, 4), n, round(e, 5)] for s, n, e in cached]}
+        # matrix-core rooflines of the cache's MLP kernels over the accumulated frames
+        mlp = {}
+        nf = max(1, cache_acc.get("frames", 0))
+        if cache_acc.get("cache_infer_launches", 0) > 0:
+            avg_s = cache_acc["cache_infer_ms"] / cache_acc["cache_infer_launches"] / 1e3
+            fl = cache_acc["queries"] / nf * MLP_FWD_FLOP
+            mlp["infer"] = {"kernel": "k_rc_infer (BF16 MFMA forward of every cache query)", "bound": "mfma",
+                            "unit": "TFLOP/s", "achieved": round(fl / avg_s / 1e12, 3), "peak": MFMA_PEAK_TFLOPS,
+                            "frac": round(fl / avg_s / 1e12 / MFMA_PEAK_TFLOPS, 5), "queries_per_frame":
+                            int(cache_acc["queries"] / nf), "flop_per_launch": int(fl),
+                            "avg_launch_ms": round(avg_s * 1e3, 4)}
+        if cache_acc.get("cache_train_launches", 0) > 0:
+            avg_s = cache_acc["cache_train_ms"] / cache_acc["cache_train_launches"] / 1e3
+            used = cache_acc["trained"] / nf
+            fl = used * MLP_TRAIN_FLOP
+            mlp["train"] = {"kernel": "k_rc_train + k_rc_reduce + k_rc_adam (4 Adam steps per frame)",
+                            "bound": "mfma", "unit": "TFLOP/s", "achieved": round(fl / avg_s / 1e12, 3),
+                            "peak": MFMA_PEAK_TFLOPS, "frac": round(fl / avg_s / 1e12 / MFMA_PEAK_TFLOPS, 5),
+                            "samples_per_frame": int(used), "flop_per_frame": int(fl),
+                            "avg_ms_per_frame": round(avg_s * 1e3, 4)}
+        ttt["mlp_roofline"] = mlp
+        if roof is not None:
+            roof.setdefault("stages", {}).update({"cache_" + k: v for k, v in mlp.items()})
         log(f"[ttt] {ttt}")
 
     blas_build = blas_build_leg(scene, ctx, host_build_ms)

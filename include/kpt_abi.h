@@ -367,7 +367,9 @@ int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n);
 int kpt_get_render_counters(const kpt_ctx* ctx, kpt_trace_counters* out);
 /* Per-stage measurements of the last wavefront kpt_render (no reference
  * counterpart): stage 0 = the closest-hit trace launches, stage 1 = the
- * path-per-lane tail launches.  ms and launches come from HIP events around
+ * path-per-lane tail launches, 2 = the primary vertex's shade launch, 3 = the
+ * sky pass of the tail's escaped paths, 4 = the radiance cache's inference,
+ * 5 = its training steps (MLP forward/backward, partial sums, Adam).  ms and launches come from HIP events around
  * each launch (zero when the frames were replayed as hipGraphs); events,
  * paths and vertices are filled in counting mode (kpt_set_trace_counting):
  * the traversal events of the stage (the tail's closest and any-hit

@@ -1,7 +1,7 @@
 """Autotuner of the wavefront launch shapes.
 
 Knobs (kpt_set_tuning): lane refill on/off, queue entries claimed per wave
-atomic, refill threshold, persistent workgroups per compute unit, material
+atomic (with or without refill), refill threshold, persistent workgroups per compute unit, material
 sort on/off, per-frame hipGraph replay on/off, spatial ray sort on/off, and
 the bounce from which the remaining paths finish in one path-per-lane launch.
 The tuner renders real frames of the caller's scene and camera
@@ -45,7 +45,7 @@ def tune(ctx, spp=1, repeats=2, passes=2, log=None, skip=()):
     for _ in range(passes):
         improved = False
         for knob in ORDER:
-            if knob in skip or (knob in ("chunk", "refill_min") and not cur["refill"]):
+            if knob in skip or (knob == "refill_min" and not cur["refill"]):
                 continue
             for v in SPACE[knob]:
                 if v == cur[knob]:

@@ -252,7 +252,7 @@ class Context:
         self._ck(self.L.kpt_get_render_counters(self.h, C.byref(c)))
         return c.as_dict()
 
-    STAGES = {"trace": 0, "tail": 1}
+    STAGES = {"trace": 0, "tail": 1, "shade": 2, "sky": 3, "cache_infer": 4, "cache_train": 5}
 
     def stage_stats(self, stage):
         """Per-stage measurements of the last wavefront render: 'trace' (the

@@ -114,7 +114,8 @@ struct kpt_ctx {
     unsigned long long* d_counts = nullptr;
     kpt_trace_counters render_counts{};
     double timing[4] = {0, 0, 0, 0};
-    kpt_stage_stats stages[2]{};         // closest-hit trace stage, path-per-lane tail
+    kpt_stage_stats stages[6]{};         // closest-hit trace, path-per-lane tail, primary shade, sky pass,
+                                         // cache inference, cache training
     int32_t blas_device = 0;             // build the BLASes of the next upload on the device (kpt_blas.hip); 1 on device contexts
     double blas_build_ms[2] = {0, 0};    // last upload: BLAS build wall ms, 1 if built on the device
     // TLAS rebuilds on the device (kpt_tlas.hip); the host tree is then stale
@@ -736,7 +737,11 @@ int kpt_ctx_create(int device, kpt_ctx** out) {
                 return KPT_ERR_HIP;
             }
         }
-        if (hipEventCreate(&ctx->ins[r].tail_begin) != hipSuccess ||
+        bool xok = true;
+        for (int x = 0; x < WF_XSTAGES; ++x)
+            xok = xok && hipEventCreate(&ctx->ins[r].xbegin[x]) == hipSuccess &&
+                  hipEventCreate(&ctx->ins[r].xend[x]) == hipSuccess;
+        if (!xok || hipEventCreate(&ctx->ins[r].tail_begin) != hipSuccess ||
             hipEventCreate(&ctx->ins[r].tail_end) != hipSuccess) {
             delete ctx;
             return KPT_ERR_HIP;
@@ -768,6 +773,10 @@ int kpt_ctx_destroy(kpt_ctx* ctx) {
             }
             if (ctx->ins[r].tail_begin) (void)hipEventDestroy(ctx->ins[r].tail_begin);
             if (ctx->ins[r].tail_end) (void)hipEventDestroy(ctx->ins[r].tail_end);
+            for (int x = 0; x < WF_XSTAGES; ++x) {
+                if (ctx->ins[r].xbegin[x]) (void)hipEventDestroy(ctx->ins[r].xbegin[x]);
+                if (ctx->ins[r].xend[x]) (void)hipEventDestroy(ctx->ins[r].xend[x]);
+            }
         }
         if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
         if (ctx->views_graph_exec) (void)hipGraphExecDestroy(ctx->views_graph_exec);
@@ -1419,12 +1428,24 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     double trace_ms = 0.0, tail_ms = 0.0;
     int trace_launches = 0, tail_launches = 0;
+    double x_ms[WF_XSTAGES] = {0.0, 0.0, 0.0, 0.0};
+    int x_launches[WF_XSTAGES] = {0, 0, 0, 0};
     const bool wavefront = ctx->settings.integrator == KPT_INTEGRATOR_WAVEFRONT;
     const bool graph = ctx->use_graph && !ctx->count_trace;
     // trace times of a frame whose ring slot is about to be reused (or at the end)
     auto harvest = [&](WfInstrument& ins) -> hipError_t {
-        if (ins.launches <= 0) return hipSuccess;
-        hipError_t e = hipEventSynchronize(ins.end[ins.launches - 1]);
+        hipError_t e = hipSuccess;
+        for (int x = 0; e == hipSuccess && x < WF_XSTAGES; ++x) {
+            if (ins.xlaunches[x] <= 0) continue;
+            float t = 0.0f;
+            e = hipEventSynchronize(ins.xend[x]);
+            if (e == hipSuccess) e = hipEventElapsedTime(&t, ins.xbegin[x], ins.xend[x]);
+            x_ms[x] += t;
+            x_launches[x] += ins.xlaunches[x];
+            ins.xlaunches[x] = 0;
+        }
+        if (e != hipSuccess || ins.launches <= 0) return e;
+        e = hipEventSynchronize(ins.end[ins.launches - 1]);
         for (int j = 0; e == hipSuccess && j < ins.launches; ++j) {
             float t = 0.0f;
             e = hipEventElapsedTime(&t, ins.begin[j], ins.end[j]);
@@ -1445,6 +1466,7 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
     for (int r = 0; r < kpt_ctx::INS_RING; ++r) {
         ctx->ins[r].launches = 0;
         ctx->ins[r].tail_launches = 0;
+        for (int x = 0; x < WF_XSTAGES; ++x) ctx->ins[r].xlaunches[x] = 0;
     }
     for (int32_t k = 0; k < spp; ++k) {
         FrameConst fc = make_frame(ctx, ctx->frame_index);
@@ -1469,12 +1491,17 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
                 if (e != hipSuccess) return e;
                 if (use_cache && cs.train) {
                     // Adam steps over this frame's suffixes (they shape the next frames)
+                    if (ins) (void)hipEventRecord(ins->xbegin[WF_X_TRAIN], ctx->stream);
                     for (int32_t st = 0; st < std::max(1, cs.train_steps); ++st) {
                         e = launch_cache_train_step(ctx->stream, ctx->cb, hooks.enc, ctx->cb.t_pos, ctx->cb.t_dir,
                                                     ctx->cb.t_target, ctx->cb.counters + 1, 0, ctx->cb.train_cap,
                                                     (int64_t)st * batch, batch, cs.learning_rate,
                                                     ++ctx->cache_steps);
                         if (e != hipSuccess) return e;
+                    }
+                    if (ins) {
+                        (void)hipEventRecord(ins->xend[WF_X_TRAIN], ctx->stream);
+                        ins->xlaunches[WF_X_TRAIN] = 1;
                     }
                 }
             } else {
@@ -1526,12 +1553,15 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
     ctx->timing[1] = (wavefront && !graph) ? trace_ms : (wavefront ? 0.0 : ms);
     ctx->timing[2] = trace_launches;
     const bool timed = wavefront && !graph;
-    ctx->stages[0] = kpt_stage_stats{};
-    ctx->stages[1] = kpt_stage_stats{};
+    for (int k = 0; k < 6; ++k) ctx->stages[k] = kpt_stage_stats{};
     ctx->stages[0].ms = timed ? trace_ms : 0.0;
     ctx->stages[0].launches = timed ? trace_launches : 0;
     ctx->stages[1].ms = timed ? tail_ms : 0.0;
     ctx->stages[1].launches = timed ? tail_launches : 0;
+    for (int x = 0; x < WF_XSTAGES; ++x) {
+        ctx->stages[2 + x].ms = timed ? x_ms[x] : 0.0;
+        ctx->stages[2 + x].launches = timed ? x_launches[x] : 0;
+    }
     if (ctx->count_trace) {
         unsigned long long h[16];
         HIPCHK(hipMemcpy(h, ctx->d_counts, sizeof(h), hipMemcpyDeviceToHost));
@@ -1822,7 +1852,7 @@ int kpt_get_render_counters(const kpt_ctx* ctx, kpt_trace_counters* out) {
 }
 
 int kpt_get_stage_stats(const kpt_ctx* ctx, int32_t stage, kpt_stage_stats* out) {
-    if (!ctx || !out || stage < 0 || stage > 1) return KPT_ERR_INVALID_ARG;
+    if (!ctx || !out || stage < 0 || stage > 5) return KPT_ERR_INVALID_ARG;
     *out = ctx->stages[stage];
     return KPT_OK;
 }

@@ -140,12 +140,18 @@ struct TraceTuning {
 // Events bracketing every closest-hit trace launch of a frame (recorded on the
 // context stream), and optional traversal event counters.
 constexpr int WF_MAX_TRACE_LAUNCHES = 40;
+// Further timed stages of a frame: the primary vertex's shade launch, the
+// sky pass of the tail's escaped paths, the radiance cache's inference and
+// its training steps (train, partial sums, Adam).
+enum { WF_X_SHADE0 = 0, WF_X_SKY = 1, WF_X_INFER = 2, WF_X_TRAIN = 3, WF_XSTAGES = 4 };
 struct WfInstrument {
     hipEvent_t begin[WF_MAX_TRACE_LAUNCHES];
     hipEvent_t end[WF_MAX_TRACE_LAUNCHES];
     int launches;                          // filled by wavefront_frame
     hipEvent_t tail_begin, tail_end;       // around the path-per-lane tail launch
     int tail_launches;                     // 0 or 1, filled by wavefront_frame
+    hipEvent_t xbegin[WF_XSTAGES], xend[WF_XSTAGES];
+    int xlaunches[WF_XSTAGES];             // 0 or 1 per frame
     unsigned long long* counters;          // device [16] or nullptr (counted traversal):
                                            // [0..4] closest-hit stage events, [8..12] tail
                                            // events (closest and any hit), [13] tail paths,

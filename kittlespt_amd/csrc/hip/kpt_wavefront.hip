@@ -227,21 +227,26 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TRACE void k_trace(SceneView S, W
     bool ovf = false;
     TraceCounts c = {0, 0, 0, 0, 0};
     if (!tune.refill) {
-        // one ray per lane to completion; a wave claims 64 entries at a time
+        // one ray per lane to completion; a wave claims tune.chunk entries
+        // (a multiple of 64) per atomic and runs them 64 at a time
+        const int chunk = (tune.chunk / WAVE) * WAVE;
         for (;;) {
-            const int base = wave_claim(ctr, WAVE);
+            const int base = wave_claim(ctr, chunk);
             if (base >= count) break;
-            const int k = base + lane_id();
-            if (k >= count) continue;
-            const int p = q[k];
-            const float4 o = W.ray_o[p], d = W.ray_d[p];
-            ClosestTrav tr;
-            if (tr.init(S, make_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z)), 0.0f, __builtin_inff(), lds, spill)) {
-                while (tr.template step<COUNT>(S, &c)) {}
+            for (int off = 0; off < chunk && base + off < count; off += WAVE) {
+                const int k = base + off + lane_id();
+                if (k >= count) continue;
+                const int p = q[k];
+                const float4 o = W.ray_o[p], d = W.ray_d[p];
+                ClosestTrav tr;
+                if (tr.init(S, make_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z)), 0.0f, __builtin_inff(), lds,
+                            spill)) {
+                    while (tr.template step<COUNT>(S, &c)) {}
+                }
+                ovf = ovf || tr.st.overflow;
+                W.hit[p] = make_float4(tr.h.t, bitsf((uint32_t)tr.h.slot), bitsf((uint32_t)tr.h.inst), tr.h.u);
+                W.hit_v[p] = tr.h.v;
             }
-            ovf = ovf || tr.st.overflow;
-            W.hit[p] = make_float4(tr.h.t, bitsf((uint32_t)tr.h.slot), bitsf((uint32_t)tr.h.inst), tr.h.u);
-            W.hit_v[p] = tr.h.v;
         }
     } else {
         WaveFeeder feeder;
@@ -577,19 +582,22 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TRACE void k_shadow(SceneView S, 
     uint32_t* lds = &s_stack[0][threadIdx.x];
     bool ovf = false;
     if (!tune.refill) {
+        const int chunk = (tune.chunk / WAVE) * WAVE;
         for (;;) {
-            const int base = wave_claim(ctr, WAVE);
+            const int base = wave_claim(ctr, chunk);
             if (base >= count) break;
-            const int kq = base + lane_id();
-            if (kq >= count) continue;
-            const int k = order ? order[kq] : kq;
-            const float4 o = W.shadow_o[k], d = W.shadow_d[k];
-            AnyTrav tr;
-            if (tr.init(S, make_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z)), 0.0f, o.w, lds, spill)) {
-                while (tr.template step<false>(S, nullptr)) {}
+            for (int off = 0; off < chunk && base + off < count; off += WAVE) {
+                const int kq = base + off + lane_id();
+                if (kq >= count) continue;
+                const int k = order ? order[kq] : kq;
+                const float4 o = W.shadow_o[k], d = W.shadow_d[k];
+                AnyTrav tr;
+                if (tr.init(S, make_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z)), 0.0f, o.w, lds, spill)) {
+                    while (tr.template step<false>(S, nullptr)) {}
+                }
+                ovf = ovf || tr.st.overflow;
+                shadow_resolve(W, t_lbefore, k, tr.hit);
             }
-            ovf = ovf || tr.st.overflow;
-            shadow_resolve(W, t_lbefore, k, tr.hit);
         }
     } else {
         WaveFeeder feeder;
@@ -649,6 +657,7 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
     if (ins) {
         ins->launches = 0;
         ins->tail_launches = 0;
+        for (int x = 0; x < WF_XSTAGES; ++x) ins->xlaunches[x] = 0;
     }
     CacheBuffers cb{};
     CacheFrame cf{};
@@ -714,7 +723,12 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
                 (void)hipEventRecord(ins->tail_end, s);
                 ins->tail_launches = 1;
             }
+            if (ins) (void)hipEventRecord(ins->xbegin[WF_X_SKY], s);
             hipLaunchKernelGGL(k_sky, dim3(gblocks), dim3(WF_BLOCK), 0, s, fc, W);
+            if (ins) {
+                (void)hipEventRecord(ins->xend[WF_X_SKY], s);
+                ins->xlaunches[WF_X_SKY] = 1;
+            }
             break;
         }
         // next queue and shadow count start at zero
@@ -750,8 +764,14 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
             hipLaunchKernelGGL(k_sort_scatter, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, W, parity, nkeys);
             shade_q = W.sort_tmp;
         }
+        const bool time_shade = ins && bounce == 0;
+        if (time_shade) (void)hipEventRecord(ins->xbegin[WF_X_SHADE0], s);
         hipLaunchKernelGGL(k_shade, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity, C_PULL + 3 * bounce + 1,
                            shade_q, cb, cf);
+        if (time_shade) {
+            (void)hipEventRecord(ins->xend[WF_X_SHADE0], s);
+            ins->xlaunches[WF_X_SHADE0] = 1;
+        }
         const int32_t* shadow_order = nullptr;
         if (tune.ray_sort && W.rtemp) {
             hipLaunchKernelGGL(k_shadow_keys, dim3(gblocks), dim3(WF_BLOCK), 0, s, fc, W, npx);
@@ -765,8 +785,13 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
                            tune, shadow_order, cb.t_lbefore);
     }
     if (cf.enabled) {
+        if (ins) (void)hipEventRecord(ins->xbegin[WF_X_INFER], s);
         hipError_t e = launch_cache_infer_paths(s, cb, cache->enc, W.L, npx, cache->num_cus);
         if (e != hipSuccess) return e;
+        if (ins) {
+            (void)hipEventRecord(ins->xend[WF_X_INFER], s);
+            ins->xlaunches[WF_X_INFER] = 1;
+        }
     }
     if (cf.train) {
         hipError_t e = launch_cache_targets(s, cb, W.L, cache->enc.scale);

@@ -520,3 +520,20 @@ def float_to_byte(v):
     out = np.zeros(v.size, np.uint32)
     _sig("orc_float_to_byte", None, [C.c_void_p, C.c_int64, C.c_void_p])(_ptr(v), v.size, _ptr(out))
     return out
+
+
+OP_PHASES = ("traversal", "sky_primary", "sky_later", "sky_sun", "shade_primary", "shade_later", "camera_output")
+OP_KINDS = ("arith", "sqrt", "transcendental", "minmax")
+
+
+def count_ops(scene_handle, settings, camera, width, height, pixels, frame=0):
+    """Algorithmic FP32 op counts per sample (the oracle instantiated with a
+    counting float): {phase: {kind: ops per sample}} over the listed pixels'
+    samples of frame `frame`."""
+    px = np.ascontiguousarray(pixels, np.int32)
+    out = np.zeros(len(OP_PHASES) * len(OP_KINDS), np.uint64)
+    _sig("orc_count_ops", None, [C.c_void_p, C.POINTER(abi.KptSettings), C.POINTER(abi.KptCamera), C.c_int32,
+                                 C.c_int32, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p])(
+        scene_handle.h, C.byref(settings), C.byref(camera), width, height, _ptr(px), len(px), frame, _ptr(out))
+    o = out.reshape(len(OP_PHASES), len(OP_KINDS)).astype(np.float64) / max(1, len(px))
+    return {ph: dict(zip(OP_KINDS, map(float, o[i]))) for i, ph in enumerate(OP_PHASES)}

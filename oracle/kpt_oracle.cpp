@@ -52,14 +52,82 @@ template <> struct Mth<double> {
     static double tan(double x) { return std::tan(x); }
     static double acos(double x) { return std::acos(x); }
 };
+// ---- op counting.  CF is a float that counts the arithmetic done on it,
+// the algorithmic FP32 op count of a routine: +, -, *, / are arithmetic ops,
+// square roots and transcendentals ops of their own kinds, min / max / abs /
+// floor a fourth kind (comparisons, conversions and the integer RNG are not
+// counted).  Counts go to the bucket of the current phase, so the oracle
+// instantiated with CF tells how much of a path's work is traversal, sky and
+// shading; the float and double instantiations ignore the phases.
+enum OpKind { OP_ARITH, OP_SQRT, OP_TRANS, OP_MINMAX, OP_KINDS };
+enum OpPhase { PH_TRAV, PH_SKY0, PH_SKY, PH_SKYSUN, PH_SHADE0, PH_SHADE, PH_OTHER, PH_COUNT };
+struct OpCounts { uint64_t n[PH_COUNT][OP_KINDS]; };
+thread_local OpCounts* g_ops = nullptr;
+thread_local int g_phase = PH_OTHER;
+inline void count_op(int k) { if (g_ops) g_ops->n[g_phase][k]++; }
+struct CF {
+    float v = 0.0f;
+    CF() = default;
+    CF(float x) : v(x) {}
+    CF(double x) : v((float)x) {}
+    CF(int x) : v((float)x) {}
+    CF(uint32_t x) : v((float)x) {}
+    CF(int64_t x) : v((float)x) {}
+    explicit operator float() const { return v; }
+    explicit operator double() const { return (double)v; }
+    explicit operator int() const { return (int)v; }
+    explicit operator uint32_t() const { return (uint32_t)v; }
+    CF& operator+=(CF b) { count_op(OP_ARITH); v += b.v; return *this; }
+    CF& operator-=(CF b) { count_op(OP_ARITH); v -= b.v; return *this; }
+    CF& operator*=(CF b) { count_op(OP_ARITH); v *= b.v; return *this; }
+    CF& operator/=(CF b) { count_op(OP_ARITH); v /= b.v; return *this; }
+};
+inline CF operator+(CF a, CF b) { count_op(OP_ARITH); return CF(a.v + b.v); }
+inline CF operator-(CF a, CF b) { count_op(OP_ARITH); return CF(a.v - b.v); }
+inline CF operator*(CF a, CF b) { count_op(OP_ARITH); return CF(a.v * b.v); }
+inline CF operator/(CF a, CF b) { count_op(OP_ARITH); return CF(a.v / b.v); }
+inline CF operator-(CF a) { return CF(-a.v); }
+inline bool operator<(CF a, CF b) { return a.v < b.v; }
+inline bool operator>(CF a, CF b) { return a.v > b.v; }
+inline bool operator<=(CF a, CF b) { return a.v <= b.v; }
+inline bool operator>=(CF a, CF b) { return a.v >= b.v; }
+inline bool operator==(CF a, CF b) { return a.v == b.v; }
+inline bool operator!=(CF a, CF b) { return a.v != b.v; }
+template <> struct Mth<CF> {
+    static CF exp(CF x) { count_op(OP_TRANS); return CF(kpt::dm::expf(x.v)); }
+    static CF exp2(CF x) { count_op(OP_TRANS); return CF(kpt::dm::exp2f(x.v)); }
+    static CF log2(CF x) { count_op(OP_TRANS); return CF(kpt::dm::log2f(x.v)); }
+    static CF pow(CF x, CF y) { count_op(OP_TRANS); return CF(kpt::dm::powf(x.v, y.v)); }
+    static CF sin(CF x) { count_op(OP_TRANS); return CF(kpt::dm::sinf(x.v)); }
+    static CF cos(CF x) { count_op(OP_TRANS); return CF(kpt::dm::cosf(x.v)); }
+    static CF tan(CF x) { count_op(OP_TRANS); return CF(kpt::dm::tanf(x.v)); }
+    static CF acos(CF x) { count_op(OP_TRANS); return CF(kpt::dm::acosf(x.v)); }
+};
+// The phase of the work that follows (CF only); PhaseScope restores the
+// previous one when it ends.
+template <class R> inline void SetPhase(int) {}
+template <> inline void SetPhase<CF>(int p) { g_phase = p; }
+template <class R> struct PhaseScope { explicit PhaseScope(int) {} };
+template <> struct PhaseScope<CF> {
+    int old;
+    explicit PhaseScope(int p) : old(g_phase) { g_phase = p; }
+    ~PhaseScope() { g_phase = old; }
+};
+
 template <class R> R Sqrt(R x) { return std::sqrt(x); }
 template <class R> R Fabs(R x) { return std::fabs(x); }
 template <class R> R Fmin(R a, R b) { return std::fmin(a, b); }
 template <class R> R Fmax(R a, R b) { return std::fmax(a, b); }
 template <class R> R Floor(R x) { return std::floor(x); }
 template <class R> R Inf() { return std::numeric_limits<R>::infinity(); }
+template <> inline CF Sqrt<CF>(CF x) { count_op(OP_SQRT); return CF(std::sqrt(x.v)); }
+template <> inline CF Fabs<CF>(CF x) { count_op(OP_MINMAX); return CF(std::fabs(x.v)); }
+template <> inline CF Fmin<CF>(CF a, CF b) { count_op(OP_MINMAX); return CF(std::fmin(a.v, b.v)); }
+template <> inline CF Fmax<CF>(CF a, CF b) { count_op(OP_MINMAX); return CF(std::fmax(a.v, b.v)); }
+template <> inline CF Floor<CF>(CF x) { count_op(OP_MINMAX); return CF(std::floor(x.v)); }
+template <> inline CF Inf<CF>() { return CF(std::numeric_limits<float>::infinity()); }
 template <class R> bool IsNan(R x) { return x != x; }
-template <class R> bool IsInf(R x) { return std::fabs(x) == Inf<R>(); }
+template <class R> bool IsInf(R x) { return std::fabs((double)x) == (double)Inf<R>(); }
 // vector_types_extension.cuh:1168-1171
 template <class R> R Clamp(R f, R a, R b) { return Fmax(a, Fmin(f, b)); }
 template <class R> R Saturate(R v) { return Clamp<R>(v, R(0), R(1)); }   // linear_algebra.cuh:65-67
@@ -703,6 +771,7 @@ template <class R> bool BlasIntersectP(const orc_scene* S, const BLAS2& b, const
 // two).
 template <class R> Isect<R> TlasIntersect(const orc_scene* S, const Ray<R>& ray, R tmin, R tmax, Counters* cn,
                                           bool entry_bound = false) {
+    PhaseScope<R> phase_(PH_TRAV);
     if (S->tlas_root < 0) return Isect<R>();
     R c1 = Inf<R>(), c2 = Inf<R>();
     if (!SlabHit(S->tlas_bounds, ray, tmin, tmax, &c1, &c2)) return Isect<R>();
@@ -742,6 +811,7 @@ template <class R> Isect<R> TlasIntersect(const orc_scene* S, const Ray<R>& ray,
 }
 // TLAS::intersectP, tlas.cu:8-58
 template <class R> bool TlasIntersectP(const orc_scene* S, const Ray<R>& ray, R tmin, R tmax, Counters* cn) {
+    PhaseScope<R> phase_(PH_TRAV);
     if (S->tlas_root < 0) return false;
     R c1 = Inf<R>(), c2 = Inf<R>();
     if (!SlabHit(S->tlas_bounds, ray, tmin, tmax, &c1, &c2)) return false;
@@ -1365,11 +1435,13 @@ template <class R> V3<R> Li(const FrameConst& fc, Ray<R> ray, Sampler<R>& sample
     const V3<R> observer(0, R(EARTH_R_F + 1), 0);
     const int nlights = (int)S->lights.size();
     while (!IsZero(beta)) {
+        SetPhase<R>(depth == 0 ? PH_SHADE0 : PH_SHADE);
         sampler.seed = sampler.seed + (uint32_t)depth;
         bool first = (depth == 0);
         Counters cn;
         Isect<R> it = TlasIntersect(S, ray, R(0), Inf<R>(), &cn);
         if (it.prim < 0) {
+            SetPhase<R>(first ? PH_SKY0 : PH_SKY);
             V3<R> sky = SkyLi(Ray<R>(observer, ray.d), sun, emis);
             if (first) {
                 // LeSun, integrator.cuh:135-143
@@ -1436,7 +1508,12 @@ template <class R> V3<R> Li(const FrameConst& fc, Ray<R> ray, Sampler<R>& sample
                 V3<R> fcos = bsdf.f(wo, sun) * Fmax(R(0), Dot(sun, (si.backface ? R(-1) : R(1)) * si.n));
                 bool ok = !IsZero(fcos);
                 V3<R> srad(0, 0, 0);
-                if (ok) { srad = SkyLi(Ray<R>(observer, sun), sun, emis); ok = !IsZero(srad); }
+                if (ok) {
+                    // the sky toward the sun: the same for every call of a frame
+                    PhaseScope<R> phase_(PH_SKYSUN);
+                    srad = SkyLi(Ray<R>(observer, sun), sun, emis);
+                    ok = !IsZero(srad);
+                }
                 if (ok) ok = Unoccluded(S, si, target);
                 if (ok) {
                     R sun_area = R(PI_F) * Sqr(sun_radius);
@@ -1591,9 +1668,11 @@ void RenderPixel(const FrameConst& fc, int W, int H, int pidx, int frame, V3<R>*
     sm.initPixelSeed(px, py, W, frame, fc.st.seed_mode);
     // BoxFilter::sample draws two values whose jitter is then discarded (kernels.cu:397-402)
     (void)sm.get2D();
+    SetPhase<R>(PH_OTHER);
     Ray<R> ray = CameraRay<R>(fc, px, py, W, H);
     *vs = VisSurf<R>();
     V3<R> L = Li<R>(fc, ray, sm, vs);
+    SetPhase<R>(PH_OTHER);
     // RGBSpectrum(fs.weight) * camera_weight * Li: 1 * 1 * L (kernels.cu:411-412)
     V3<R> w(R(1.0f), R(1.0f), R(1.0f));
     L = (w * R(1.0f)) * L;
@@ -2172,3 +2251,31 @@ void orc_float_to_byte(const float* v, int64_t n, uint32_t* out) {
 }
 
 }  // extern "C"
+
+extern "C" {
+
+/* Algorithmic FP32 op counts of one sample of each listed pixel (frame
+ * `frame`): the oracle instantiated with the counting float CF.
+ * out[phase * 4 + kind], phases: 0 traversal, 1 sky of primary misses,
+ * 2 sky of later misses, 3 sky toward the sun (a per-frame constant the
+ * renderer evaluates once), 4 shading at the primary vertex, 5 shading at
+ * later vertices, 6 camera rays and output; kinds: 0 + - * /, 1 sqrt,
+ * 2 transcendentals, 3 min / max / abs / floor.  Single-threaded. */
+void orc_count_ops(const orc_scene* s, const kpt_settings* settings, const kpt_camera* cam, int32_t W, int32_t H,
+                   const int32_t* pixels, int64_t n, int32_t frame, uint64_t* out) {
+    FrameConst fc = MakeFrame(s, settings, cam);
+    OpCounts c;
+    std::memset(&c, 0, sizeof(c));
+    g_ops = &c;
+    for (int64_t i = 0; i < n; ++i) {
+        V3<CF> smp;
+        VisSurf<CF> vs;
+        RenderPixel<CF>(fc, W, H, pixels[i], frame, &smp, &vs);
+    }
+    g_ops = nullptr;
+    g_phase = PH_OTHER;
+    std::memcpy(out, &c, sizeof(c));
+}
+
+}  // extern "C"
+

@@ -636,3 +636,37 @@ def test_blas_entry_bound_keeps_the_reference_hit(seed):
         assert np.array_equal(got[k], ref[k])
     assert np.array_equal(got["t"].view(np.uint32), ref["t"].view(np.uint32))
     assert np.array_equal(got["bary"].view(np.uint32), ref["bary"].view(np.uint32))
+
+
+def test_op_counts_follow_the_path_structure(furnace):
+    """The counting-float instantiation of the oracle (the FP32 op counts the
+    shading and sky rooflines divide by) against what the path structure
+    fixes: a closed enclosure never reaches the sky model through a miss;
+    depth 0 shades no later vertex; every phase grows or stays with the
+    depth; the per-sample figures of two pixels average to those of both
+    together; camera set-up and output cost the same per sample whatever
+    the depth."""
+    sc, orc = furnace
+    cam = host_camera(sc, 8, 8)
+
+    def counts(depth, px):
+        st = _furnace_settings(depth)
+        return oracle.count_ops(orc, st, cam, 8, 8, np.array(px, np.int32))
+
+    allpx = list(range(64))
+    by_depth = [counts(d, allpx) for d in (0, 1, 3)]
+    for c in by_depth:
+        for ph in ("sky_primary", "sky_later"):
+            assert all(v == 0 for v in c[ph].values()), (ph, c[ph])
+        assert c["traversal"]["arith"] > 0 and c["shade_primary"]["arith"] > 0
+    assert all(v == 0 for v in by_depth[0]["shade_later"].values())
+    assert by_depth[1]["shade_later"]["arith"] > 0
+    for lo, hi in zip(by_depth, by_depth[1:]):
+        for ph in oracle.OP_PHASES:
+            for k in oracle.OP_KINDS:
+                assert hi[ph][k] >= lo[ph][k], (ph, k, lo[ph][k], hi[ph][k])
+    assert by_depth[0]["camera_output"] == by_depth[2]["camera_output"]
+    a, b, ab = counts(3, [3]), counts(3, [40]), counts(3, [3, 40])
+    for ph in oracle.OP_PHASES:
+        for k in oracle.OP_KINDS:
+            assert (a[ph][k] + b[ph][k]) / 2 == ab[ph][k], (ph, k)
