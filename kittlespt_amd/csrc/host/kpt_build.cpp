@@ -64,7 +64,12 @@ struct BlasWork {
             float scale = 8.0f / (bmax - bmin);
             for (uint32_t i = 0; i < count; ++i) {
                 int32_t t = idx[start + i];
-                int b = std::min(7, (int32_t)((cen[3 * (size_t)t + a] - bmin) * scale));
+                // min(BINS - 1, (int)((c - min) * scale)) for every finite
+                // centroid; a non-finite one (NaN, or inf * 0 when another
+                // centroid is infinite) goes to bin 0 instead of an undefined
+                // float-to-int conversion
+                const float f = (cen[3 * (size_t)t + a] - bmin) * scale;
+                int b = f >= 0.0f ? (int)std::fmin(7.0f, f) : 0;
                 bc[b]++;
                 bins[b].grow_pt(vtx(t, 0)); bins[b].grow_pt(vtx(t, 1)); bins[b].grow_pt(vtx(t, 2));
             }
@@ -211,12 +216,20 @@ TlasTree build_tlas(const std::vector<Aabb>& inst) {
             if (B == A) continue;
             if (sa[B] < smallest) { smallest = sa[B]; best = B; }
         }
+        // No merged area below infinity (non-finite or overflowing boxes):
+        // the reference returns -1 and then indexes with it; here the first
+        // other entry is taken, as if every such area were one equal value.
+        if (best < 0 && n > 1) best = A == 0 ? 1 : 0;
         return best;
     };
     int32_t left = N;
     int32_t A = 0, B = best_match(left, A);
-    while (left > 1) {
-        int32_t C = best_match(left, B);
+    // The walk merges N - 1 times and moves on a few times per merge; past
+    // the cap (never reached for finite boxes) A and B are merged as they
+    // stand, so a degenerate input still ends with a tree.
+    const int64_t max_steps = 64 * (int64_t)N + 1024;
+    for (int64_t steps = 0; left > 1; ++steps) {
+        int32_t C = steps > max_steps ? A : best_match(left, B);
         if (A == C) {
             int32_t na = ids[A], nb = ids[B];
             Box m;

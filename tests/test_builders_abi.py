@@ -272,3 +272,47 @@ def test_host_only_context_builds_on_host_threads_whatever_the_knob():
         a.close()
         b.close()
 
+
+
+def _tree_covers(nodes, tri, first, count):
+    assert sorted(tri.tolist()) == list(range(first, first + count))   # a permutation
+    covered = np.zeros(count, np.int32)
+    for lf in _leaves(nodes):
+        s, c = nodes["first"][lf], nodes["count"][lf]
+        covered[s:s + c] += 1
+    assert np.all(covered == 1)
+
+
+def test_host_builders_take_non_finite_positions():
+    """NaN and infinite vertex positions, and instance boxes whose merged
+    areas overflow: the reference's binning converts a NaN bin position to
+    an integer and its best-match search returns -1 when no merged area is
+    below infinity (as_builder.cuh:375-391); the host builders end with
+    trees that cover every triangle and instance instead."""
+    rng = np.random.default_rng(71)
+    p = rng.uniform(-1, 1, (40, 3, 3)).astype(np.float32)
+    p[7, 1, 2] = np.nan
+    p[19, 0, 0] = np.inf
+    p[23, 2, 1] = -np.inf
+    sc = _tiny_scene(p)
+    sc.instance_mesh = np.zeros(5, np.int32)
+    sc.instance_model = np.stack([scenes.trs(translate=(3.0 * i, 0, 0)) for i in range(5)])
+    ctx = Context(-2)
+    try:
+        ctx.upload(sc)
+        nodes, tri = ctx.blas(0)
+        _tree_covers(nodes, tri, 0, 40)
+        t = ctx.tlas()
+        assert len(t) == 2 * 5
+        assert sorted(t["blas"][1:][t["lr"][1:] == 0].tolist()) == list(range(5))
+        # finite meshes far apart: every merged area overflows to infinity
+        q = np.array([_tri_at(0.0, s=1e19), _tri_at(0.0, s=1e19)], np.float32)
+        sc2 = _tiny_scene(q)
+        sc2.instance_mesh = np.zeros(4, np.int32)
+        sc2.instance_model = np.stack([scenes.trs(translate=(1e19 * i, 0, 0)) for i in range(4)])
+        ctx.upload(sc2)
+        t = ctx.tlas()
+        assert len(t) == 2 * 4
+        assert sorted(t["blas"][1:][t["lr"][1:] == 0].tolist()) == list(range(4))
+    finally:
+        ctx.close()
