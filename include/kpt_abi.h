@@ -338,8 +338,9 @@ int kpt_last_build_timing(const kpt_ctx* ctx, double out[2]);
  * DebugData counters, blas.cuh:39-42). */
 int kpt_set_trace_counting(kpt_ctx* ctx, int32_t enable);
 /* Launch-shape knobs of the wavefront stages, chosen by the autotuner (no
- * reference counterpart): knobs[0] lane refill (0/1), [1] queue entries a
- * wave claims per atomic (64..65536), [2] idle lanes that trigger a refill
+ * reference counterpart): knobs[0] lane refill (0/1, default 0), [1] queue
+ * entries a wave claims per atomic (64..65536, default 64; without refill a
+ * wave traces its claim 64 rays at a time), [2] idle lanes that trigger a refill
  * (1..64), [3] persistent workgroups per compute unit (1..64), [4] sort
  * shading by material (0 no, 1 every bounce, 2 from bounce 1 on), [5] (when n >= 6) capture every frame as a
  * hipGraph and replay it (0/1; the per-launch trace times of kpt_last_timing
@@ -348,8 +349,8 @@ int kpt_set_trace_counting(kpt_ctx* ctx, int32_t enable);
  * (0/1; not applied inside captured graphs), [7] (when n >= 8) the bounce
  * from which the paths still alive finish in one persistent launch, a lane
  * running one path at a time vertex by vertex and taking the next queued path
- * when its own ends (0: never; 0..64; default 1; frames in which the
- * radiance cache renders or trains keep the stage kernels), [8] (when n >= 9)
+ * when its own ends (0: never; 0..64; default 1; on radiance-cache frames
+ * the tail also finishes the cached bounce-1 vertices), [8] (when n >= 9)
  * rebuild the TLAS after instance edits on the device (1: the same tree,
  * built and laid out on the stream without a host synchronisation; slower
  * than the host builder for 1,000 instances), refit it on the device (2: the

@@ -12,7 +12,9 @@ import os
 import numpy as np
 
 LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib")
-LIB_PATH = os.path.join(LIB_DIR, "libkpt_hip.so")
+# KPT_LIB_PATH selects another build of the same library (the host-code
+# sanitizer build the CPU tests can run against).
+LIB_PATH = os.environ.get("KPT_LIB_PATH") or os.path.join(LIB_DIR, "libkpt_hip.so")
 
 KPT_OK = 0
 KPT_ERR_INVALID_ARG = 1

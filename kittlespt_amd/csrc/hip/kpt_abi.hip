@@ -108,7 +108,7 @@ struct kpt_ctx {
     int32_t use_graph = 0;               // capture each frame as a hipGraph and replay it
     hipGraphExec_t graph_exec = nullptr;        // kpt_render's per-frame graph
     hipGraphExec_t views_graph_exec = nullptr;  // kpt_render_views' per-sample graph
-    TraceTuning tune{0, 128, 16, 8, 0, 1};   // the autotuner's usual choice on the 1080p workload
+    TraceTuning tune{0, 64, 16, 8, 0, 1};    // the autotuner's usual choice on the 1080p workload
     int32_t sort_by_material = 2;
     bool count_trace = false;
     unsigned long long* d_counts = nullptr;

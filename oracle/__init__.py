@@ -15,7 +15,8 @@ import numpy as np
 from kittlespt_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "libkpt_oracle.so")
+# KPT_ORACLE_LIB selects another build (the sanitizer build: make -C oracle sanitize)
+LIB_PATH = os.environ.get("KPT_ORACLE_LIB") or os.path.join(HERE, "_build", "libkpt_oracle.so")
 _lib = None
 
 MODE_CLOSEST, MODE_ANY, MODE_BRUTE_CLOSEST, MODE_BRUTE_ANY, MODE_CLOSEST_ENTRY_BOUND = 0, 1, 2, 3, 4

@@ -2100,7 +2100,7 @@ template <class R> BSDF<R> ProbeBSDF(const float* p) {
 template <class R> void BsdfEval(const float* params, const float* wo, const float* wi, int64_t n, float* of, float* op) {
     BSDF<R> b = ProbeBSDF<R>(params);
     for (int64_t i = 0; i < n; ++i) {
-        V3<R> o(R(wo[3 * i]), R(wo[3 * i + 1]), R(wo[3 * i + 2])), w(R(wi[3 * i]), R(wi[3 * i + 1]), R(wi[3 * i + 2]));
+        V3<R> o{R(wo[3 * i]), R(wo[3 * i + 1]), R(wo[3 * i + 2])}, w{R(wi[3 * i]), R(wi[3 * i + 1]), R(wi[3 * i + 2])};
         V3<R> f = b.f(o, w);
         of[3 * i] = (float)f.x; of[3 * i + 1] = (float)f.y; of[3 * i + 2] = (float)f.z;
         op[i] = (float)b.pdf(o, w);
@@ -2110,7 +2110,7 @@ template <class R> void BsdfSample(const float* params, const float* wo, const f
                                    float* op, int32_t* ofl) {
     BSDF<R> b = ProbeBSDF<R>(params);
     for (int64_t i = 0; i < n; ++i) {
-        V3<R> o(R(wo[3 * i]), R(wo[3 * i + 1]), R(wo[3 * i + 2]));
+        V3<R> o{R(wo[3 * i]), R(wo[3 * i + 1]), R(wo[3 * i + 2])};
         V2<R> u2, x2;
         u2.x = R(u[4 * i]); u2.y = R(u[4 * i + 1]); x2.x = R(u[4 * i + 2]); x2.y = R(u[4 * i + 3]);
         BSDFSample<R> s = b.sampleF(o, u2, x2);
