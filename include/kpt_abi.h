@@ -337,6 +337,17 @@ int kpt_last_build_timing(const kpt_ctx* ctx, double out[2]);
  * kpt_render.  No reference counterpart (the reference keeps per-ray
  * DebugData counters, blas.cuh:39-42). */
 int kpt_set_trace_counting(kpt_ctx* ctx, int32_t enable);
+/* DebugData heat map (blas.cuh:39-42 counts, kernels.cu:442-443 colours) of
+ * the last frame of the last kpt_render made with trace counting on and the
+ * wavefront integrator: per pixel, the BLAS node pops (blas_hits,
+ * blas.cuh:140) and TLAS node pops (tlas_hits, tlas.cu:87) of its path's
+ * closest-hit traces, shadow rays excluded (integrator.cuh:129).  BLASes are
+ * entered with the closest hit so far as their bound, so the counts are at
+ * most the reference's.  counts (or NULL): n_pixels x 2 int32; rgba (or
+ * NULL): n_pixels x 4 floats, (0, 0.02 * blas pops, 0.05 * tlas pops, 1).
+ * n_pixels must be that render's width x height; KPT_ERR_STATE when no
+ * counted render has run.  Synchronises the context stream. */
+int kpt_get_heat_map(kpt_ctx* ctx, int32_t* counts, float* rgba, int64_t n_pixels);
 /* Launch-shape knobs of the wavefront stages, chosen by the autotuner (no
  * reference counterpart): knobs[0] lane refill (0/1, default 0), [1] queue
  * entries a wave claims per atomic (64..65536, default 64; without refill a

@@ -262,6 +262,7 @@ SIGNATURES = [
     ("kpt_last_build_timing", C.c_int, [_P, C.POINTER(C.c_double)]),
     ("kpt_set_trace_counting", C.c_int, [_P, C.c_int32]),
     ("kpt_get_render_counters", C.c_int, [_P, C.POINTER(KptTraceCounters)]),
+    ("kpt_get_heat_map", C.c_int, [_P, _P, _P, C.c_int64]),
     ("kpt_get_stage_stats", C.c_int, [_P, C.c_int32, C.POINTER(KptStageStats)]),
     ("kpt_get_resolution", C.c_int, [_P, C.POINTER(C.c_int32)]),
     ("kpt_set_tuning", C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32]),

@@ -254,6 +254,20 @@ class Context:
 
     STAGES = {"trace": 0, "tail": 1, "shade": 2, "sky": 3, "cache_infer": 4, "cache_train": 5}
 
+    def heat_map(self, rgba=False):
+        """DebugData heat map of the last counted wavefront render's last
+        frame (blas.cuh:39-42): (H, W, 2) int32 BLAS and TLAS node pops of
+        each pixel's closest-hit traces, or with rgba=True the reference's
+        colours (kernels.cu:442-443) as (H, W, 4) float32."""
+        W, H = self.resolution()[:2]
+        if rgba:
+            out = np.zeros((H, W, 4), np.float32)
+            self._ck(self.L.kpt_get_heat_map(self.h, None, out.ctypes.data_as(C.c_void_p), W * H))
+        else:
+            out = np.zeros((H, W, 2), np.int32)
+            self._ck(self.L.kpt_get_heat_map(self.h, out.ctypes.data_as(C.c_void_p), None, W * H))
+        return out
+
     def stage_stats(self, stage):
         """Per-stage measurements of the last wavefront render: 'trace' (the
         closest-hit launches) or 'tail' (the path-per-lane tail): HIP-event

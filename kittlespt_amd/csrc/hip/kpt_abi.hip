@@ -112,6 +112,10 @@ struct kpt_ctx {
     int32_t sort_by_material = 2;
     bool count_trace = false;
     unsigned long long* d_counts = nullptr;
+    // DebugData heat map of the last counted render's last frame (2 int32 per pixel)
+    int32_t* d_heat = nullptr;
+    int64_t heat_pixels = 0;
+    bool heat_valid = false;
     kpt_trace_counters render_counts{};
     double timing[4] = {0, 0, 0, 0};
     kpt_stage_stats stages[6]{};         // closest-hit trace, path-per-lane tail, primary shade, sky pass,
@@ -781,6 +785,7 @@ int kpt_ctx_destroy(kpt_ctx* ctx) {
         if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
         if (ctx->views_graph_exec) (void)hipGraphExecDestroy(ctx->views_graph_exec);
         dev_free(&ctx->d_counts);
+        dev_free(&ctx->d_heat);
         dev_free(&ctx->d_inst_bounds);
         dev_free(&ctx->d_tlas_ref);
         dev_free(&ctx->d_tlas_status);
@@ -1258,6 +1263,7 @@ int kpt_render_views(kpt_ctx* ctx, const kpt_view* views, int32_t n, int32_t spp
     vb.out = (float4*)out;
     HIPCHK(launch_clear(ctx->stream, vb.out, total));
     SceneView S = scene_view(ctx);
+    ctx->ws.heat = nullptr;   // the heat map covers single-view renders only
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     // with the graph knob, each sample's launches are captured and replayed
     // as one graph (the library radix sort stays out of captured graphs)
@@ -1424,6 +1430,19 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
     }
     if (ctx->count_trace) {
         HIPCHK(hipMemsetAsync(ctx->d_counts, 0, 16 * sizeof(unsigned long long), ctx->stream));
+    }
+    // per-pixel DebugData of counted wavefront frames (cleared per frame by wavefront_frame)
+    ctx->heat_valid = false;
+    ctx->ws.heat = nullptr;
+    if (ctx->count_trace && ctx->settings.integrator == KPT_INTEGRATOR_WAVEFRONT && spp > 0) {
+        if (ctx->heat_pixels != npx) {
+            dev_free(&ctx->d_heat);
+            ctx->heat_pixels = 0;
+            HIPCHK(hipMalloc((void**)&ctx->d_heat, (size_t)npx * 2 * sizeof(int32_t)));
+            ctx->heat_pixels = npx;
+        }
+        ctx->ws.heat = ctx->d_heat;
+        ctx->heat_valid = true;
     }
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     double trace_ms = 0.0, tail_ms = 0.0;
@@ -1848,6 +1867,30 @@ int kpt_set_trace_counting(kpt_ctx* ctx, int32_t enable) {
 int kpt_get_render_counters(const kpt_ctx* ctx, kpt_trace_counters* out) {
     if (!ctx || !out) return KPT_ERR_INVALID_ARG;
     *out = ctx->render_counts;
+    return KPT_OK;
+}
+
+int kpt_get_heat_map(kpt_ctx* ctx, int32_t* counts, float* rgba, int64_t n_pixels) {
+    if (!ctx || n_pixels < 0 || (!counts && !rgba)) return KPT_ERR_INVALID_ARG;
+    if (ctx->host_only) return fail(ctx, KPT_ERR_NO_DEVICE, "host-only context");
+    if (!ctx->heat_valid || !ctx->d_heat)
+        return fail(ctx, KPT_ERR_STATE, "no counted wavefront render (kpt_set_trace_counting before kpt_render)");
+    if (n_pixels != ctx->heat_pixels)
+        return fail(ctx, KPT_ERR_INVALID_ARG, "n_pixels is not the counted render's width x height");
+    if (counts)
+        HIPCHK(hipMemcpyAsync(counts, ctx->d_heat, (size_t)n_pixels * 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    if (rgba) {
+        float4* d = nullptr;
+        HIPCHK(hipMalloc((void**)&d, (size_t)std::max<int64_t>(1, n_pixels) * sizeof(float4)));
+        hipError_t e = launch_heat_colour(ctx->stream, ctx->d_heat, n_pixels, d);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(rgba, d, (size_t)n_pixels * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        (void)hipFree(d);
+        HIPCHK(e);
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return KPT_OK;
 }
 

@@ -232,15 +232,23 @@ KD CacheVertex cache_vertex(const FrameConst& fc, const CacheBuffers& cb, const 
 // CACHE: the radiance cache's hooks at the path's cache vertex (cache_vertex
 // with path index `path`), as the wavefront's shade stage applies them; a
 // path that ends in the network returns false after its next-event estimate.
+// heat (COUNT with cnt): the path's pixel's DebugData pair, to which the
+// closest-hit trace's BLAS and TLAS node pops are added.
 template <bool COUNT = false, bool DEFER_SKY = false, bool CACHE = false>
 KD bool li_bounce(const SceneView& S, const FrameConst& fc, PathState& ps, uint32_t& seed, uint32_t* lds,
                   bool* ovf, VisOut* vs, TraceCounts* cnt = nullptr, bool* missed = nullptr,
-                  const CacheBuffers* cb = nullptr, const CacheFrame* cf = nullptr, int32_t path = -1) {
+                  const CacheBuffers* cb = nullptr, const CacheFrame* cf = nullptr, int32_t path = -1,
+                  int32_t* heat = nullptr) {
     const float eta_scale = 1.0f;
     seed = seed + (uint32_t)ps.depth;
     const bool first = (ps.depth == 0);
     bool o = false;
+    const uint32_t b0 = (COUNT && cnt) ? cnt->interior + cnt->leaf : 0u, t0 = (COUNT && cnt) ? cnt->tlas : 0u;
     Hit h = trace_closest<COUNT>(S, ps.ray, 0.0f, __builtin_inff(), lds, &o, cnt);
+    if (COUNT && cnt && heat) {
+        atomicAdd(&heat[0], (int32_t)(cnt->interior + cnt->leaf - b0));
+        atomicAdd(&heat[1], (int32_t)(cnt->tlas - t0));
+    }
     *ovf = *ovf || o;
     if (DEFER_SKY && h.slot < 0 && !first) {
         *missed = true;

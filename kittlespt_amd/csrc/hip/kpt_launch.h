@@ -52,6 +52,9 @@ struct WavefrontState {
     int32_t* rvals[2];
     void* rtemp;
     size_t rtemp_bytes;
+    // counting mode, one view: per pixel, the BLAS and TLAS node pops of the
+    // frame's closest-hit traces (DebugData, blas.cuh:39-42); null otherwise
+    int32_t* heat;
 };
 
 // Exposure state resident in device memory: the auto-exposure program's
@@ -190,6 +193,10 @@ struct CacheHooks {
 // Wavefront path tracer: one reference frame (one sample per pixel).  With
 // a cache, non-training paths end in it at the bounce-1 vertex and training
 // paths leave suffix targets for the caller's training steps.
+// DebugData heat map colour (kernels.cu:442-443): rgba[i] = (0, 0.02 *
+// blas pops, 0.05 * tlas pops, 1) of heat[2i], heat[2i + 1].
+hipError_t launch_heat_colour(hipStream_t s, const int32_t* heat, int64_t n, float4* rgba);
+
 hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& fc, int32_t frame,
                            const FrameBuffers& fb, WavefrontState& W, int32_t* overflow, int32_t num_materials,
                            int32_t sort_by_material, const TraceTuning& tune, WfInstrument* ins,

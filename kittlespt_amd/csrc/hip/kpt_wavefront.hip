@@ -166,6 +166,15 @@ __global__ __launch_bounds__(WF_BLOCK) void k_views_mean(float4* out, int64_t n,
     out[i] = make_float4(est.x, est.y, est.z, 1.0f);
 }
 
+// DebugData heat map colour, kernels.cu:442-443: green 0.02 per BLAS node
+// pop, blue 0.05 per TLAS node pop.
+__global__ __launch_bounds__(WF_BLOCK) void k_heat_colour(const int32_t* heat, int64_t n, float4* rgba) {
+    const int64_t i = (int64_t)blockIdx.x * WF_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const float b = (float)heat[2 * i], t = (float)heat[2 * i + 1];
+    rgba[i] = make_float4(0.0f, b * 0.02f, t * 0.05f, 1.0f);
+}
+
 // ---- spatial ray sort: rays with nearby origins (and, for continuation rays,
 // the same direction octant) end up in the same wave, so a wave's traversals
 // share nodes and cache lines.  Results do not depend on the order.
@@ -213,7 +222,16 @@ __global__ __launch_bounds__(WF_BLOCK) void k_shadow_keys(FrameConst fc, Wavefro
     }
 }
 
-// Closest hit of every live path.  COUNT accumulates traversal events.
+// Counting mode: one closest-hit trace's node pops added to its pixel's
+// DebugData, blas_hits and tlas_hits (blas.cuh:140, tlas.cu:87).
+__device__ __forceinline__ void heat_add(int32_t* heat, float pixel_bits, uint32_t blas_pops, uint32_t tlas_pops) {
+    int32_t* h = heat + 2 * (int64_t)fbits(pixel_bits);
+    atomicAdd(&h[0], (int32_t)blas_pops);
+    atomicAdd(&h[1], (int32_t)tlas_pops);
+}
+
+// Closest hit of every live path.  COUNT accumulates traversal events (and,
+// when W.heat is set, each pixel's node pops).
 template <bool COUNT>
 __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TRACE void k_trace(SceneView S, WavefrontState W, int parity, int pull_slot,
                                                     int32_t* overflow, unsigned long long* counters,
@@ -238,11 +256,13 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TRACE void k_trace(SceneView S, W
                 if (k >= count) continue;
                 const int p = q[k];
                 const float4 o = W.ray_o[p], d = W.ray_d[p];
+                const uint32_t b0 = c.interior + c.leaf, t0 = c.tlas;
                 ClosestTrav tr;
                 if (tr.init(S, make_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z)), 0.0f, __builtin_inff(), lds,
                             spill)) {
                     while (tr.template step<COUNT>(S, &c)) {}
                 }
+                if (COUNT && W.heat) heat_add(W.heat, o.w, c.interior + c.leaf - b0, c.tlas - t0);
                 ovf = ovf || tr.st.overflow;
                 W.hit[p] = make_float4(tr.h.t, bitsf((uint32_t)tr.h.slot), bitsf((uint32_t)tr.h.inst), tr.h.u);
                 W.hit_v[p] = tr.h.v;
@@ -254,12 +274,15 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TRACE void k_trace(SceneView S, W
         ClosestTrav tr;
         int p = -1;
         bool busy = false;
+        uint32_t b0 = 0u, t0 = 0u;   // counting mode: the lane's counts when its ray started
+        float pix = 0.0f;
         for (;;) {
             const int k = feeder.feed(ctr, count, !busy);
             bool done_now = false;
             if (k >= 0) {
                 p = q[k];
                 const float4 o = W.ray_o[p], d = W.ray_d[p];
+                if (COUNT) { b0 = c.interior + c.leaf; t0 = c.tlas; pix = o.w; }
                 busy = tr.init(S, make_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z)), 0.0f, __builtin_inff(), lds,
                                spill);
                 done_now = !busy;   // empty scene or root box missed
@@ -274,6 +297,7 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TRACE void k_trace(SceneView S, W
             }
             if (done_now) {
                 ovf = ovf || tr.st.overflow;
+                if (COUNT && W.heat) heat_add(W.heat, pix, c.interior + c.leaf - b0, c.tlas - t0);
                 W.hit[p] = make_float4(tr.h.t, bitsf((uint32_t)tr.h.slot), bitsf((uint32_t)tr.h.inst), tr.h.u);
                 W.hit_v[p] = tr.h.v;
             }
@@ -486,6 +510,7 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TAIL void k_tail(SceneView S, Fra
     VisOut vs;   // written only at depth 0, which never reaches this kernel
     TraceCounts c = {0, 0, 0, 0, 0};
     uint32_t n_paths = 0, n_vertices = 0;
+    int32_t* heat_px = nullptr;   // counting mode: the lane's pixel in W.heat
     for (;;) {
         // wave-uniform: claim one queue entry per idle lane
         const uint64_t idle = __ballot(!active);
@@ -509,6 +534,7 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TAIL void k_tail(SceneView S, Fra
                 seed = fbits(d4.w);
                 active = true;
                 if (COUNT) ++n_paths;
+                if (COUNT && W.heat) heat_px = W.heat + 2 * (int64_t)fbits(o4.w);
             }
         }
         if (!__any(active)) break;   // wave-uniform: queue drained, every lane idle
@@ -518,7 +544,7 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TAIL void k_tail(SceneView S, Fra
             if (COUNT && !is_zero(ps.beta)) ++n_vertices;
             const bool more = !is_zero(ps.beta) &&
                               li_bounce<COUNT, true, CACHE>(S, fc, ps, seed, &s_stack[0][threadIdx.x], &o, &vs, &c,
-                                                            &missed, &cb, &cf, p);
+                                                            &missed, &cb, &cf, p, heat_px);
             ovf = ovf || o;
             if (!more || is_zero(ps.beta)) {
                 W.L[p] = make_float4(ps.L.x, ps.L.y, ps.L.z, bitsf((uint32_t)ps.depth | (st & 0x10000u)));
@@ -676,6 +702,12 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
     }
     const int64_t npx = views ? views->total : (int64_t)fc.width * fc.height;
     const unsigned gblocks = (unsigned)((npx + WF_BLOCK - 1) / WF_BLOCK);
+    // the per-pixel DebugData: counted single-view frames only, cleared per frame
+    if (views || !(ins && ins->counters)) W.heat = nullptr;
+    if (W.heat) {
+        const int64_t nh = 2 * npx;
+        hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, s, W.heat, 0, (int)nh);
+    }
     // persistent grids: 8 workgroups of 256 lanes per CU
     const unsigned pgrid = (unsigned)std::max<int64_t>(
         1, std::min<int64_t>((int64_t)g_num_cus * std::max(1, tune.blocks_per_cu), gblocks));
@@ -816,6 +848,13 @@ hipError_t launch_views_mean(hipStream_t s, float4* out, int64_t n, int32_t spp)
     if (n <= 0 || spp <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_views_mean, dim3((unsigned)((n + WF_BLOCK - 1) / WF_BLOCK)), dim3(WF_BLOCK), 0, s, out, n,
                        spp);
+    return hipGetLastError();
+}
+
+hipError_t launch_heat_colour(hipStream_t s, const int32_t* heat, int64_t n, float4* rgba) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_heat_colour, dim3((unsigned)((n + WF_BLOCK - 1) / WF_BLOCK)), dim3(WF_BLOCK), 0, s, heat, n,
+                       rgba);
     return hipGetLastError();
 }
 

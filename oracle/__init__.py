@@ -527,6 +527,21 @@ OP_PHASES = ("traversal", "sky_primary", "sky_later", "sky_sun", "shade_primary"
 OP_KINDS = ("arith", "sqrt", "transcendental", "minmax")
 
 
+def heat_map(scene_handle, settings, camera, width, height, pixels, frame=0, entry_bound=True):
+    """Per-pixel DebugData of one sample (blas.cuh:39-42): an (n, 2) int64
+    array of BLAS node pops and TLAS node pops over the listed pixels'
+    closest-hit traces in frame `frame` (shadow rays not counted).
+    entry_bound: BLASes entered with the closest hit so far, as the GPU
+    kernels enter them; False: the reference's traversal."""
+    px = np.ascontiguousarray(pixels, np.int32)
+    out = np.zeros((len(px), 2), np.int64)
+    _sig("orc_heat_map", None, [C.c_void_p, C.POINTER(abi.KptSettings), C.POINTER(abi.KptCamera), C.c_int32,
+                                C.c_int32, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p])(
+        scene_handle.h, C.byref(settings), C.byref(camera), width, height, _ptr(px), len(px), frame,
+        1 if entry_bound else 0, _ptr(out))
+    return out
+
+
 def count_ops(scene_handle, settings, camera, width, height, pixels, frame=0):
     """Algorithmic FP32 op counts per sample (the oracle instantiated with a
     counting float): {phase: {kind: ops per sample}} over the listed pixels'

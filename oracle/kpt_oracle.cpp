@@ -1243,6 +1243,11 @@ struct FrameConst {
     M4<float> inv_view, inv_proj;
     V3<float> cam_pos;
     M4<float> curr_vp, prev_vp;
+    // DebugData (blas.cuh:39-42): when set, the node pops of every
+    // closest-hit trace of a path are added here (shadow rays use a dummy,
+    // integrator.cuh:129); entry_bound selects the GPU kernels' BLAS entry.
+    Counters* path_counts = nullptr;
+    bool entry_bound = false;
 };
 
 template <class R> struct SurfInt {  // interaction.cuh:15-67
@@ -1439,7 +1444,12 @@ template <class R> V3<R> Li(const FrameConst& fc, Ray<R> ray, Sampler<R>& sample
         sampler.seed = sampler.seed + (uint32_t)depth;
         bool first = (depth == 0);
         Counters cn;
-        Isect<R> it = TlasIntersect(S, ray, R(0), Inf<R>(), &cn);
+        Isect<R> it = TlasIntersect(S, ray, R(0), Inf<R>(), &cn, fc.entry_bound);
+        if (fc.path_counts) {   // dbg.blas_hits / dbg.tlas_hits (blas.cuh:140, tlas.cu:87)
+            fc.path_counts->interior += cn.interior;
+            fc.path_counts->leaf += cn.leaf;
+            fc.path_counts->tlas += cn.tlas;
+        }
         if (it.prim < 0) {
             SetPhase<R>(first ? PH_SKY0 : PH_SKY);
             V3<R> sky = SkyLi(Ray<R>(observer, ray.d), sun, emis);
@@ -2275,6 +2285,25 @@ void orc_count_ops(const orc_scene* s, const kpt_settings* settings, const kpt_c
     g_ops = nullptr;
     g_phase = PH_OTHER;
     std::memcpy(out, &c, sizeof(c));
+}
+
+// Per-pixel DebugData of one sample (kernels.cu:404-416, the commented-out
+// heat map at :442-443): out[2i] = BLAS node pops (blas_hits), out[2i+1] =
+// TLAS node pops (tlas_hits) of pixel pixels[i]'s closest-hit traces in frame
+// `frame`.  entry_bound: traversal as the GPU kernels enter BLASes.
+void orc_heat_map(const orc_scene* s, const kpt_settings* settings, const kpt_camera* cam, int32_t W, int32_t H,
+                  const int32_t* pixels, int64_t n, int32_t frame, int32_t entry_bound, int64_t* out) {
+    FrameConst fc = MakeFrame(s, settings, cam);
+    fc.entry_bound = entry_bound != 0;
+    for (int64_t i = 0; i < n; ++i) {
+        Counters cn;
+        fc.path_counts = &cn;
+        V3<float> smp;
+        VisSurf<float> vs;
+        RenderPixel<float>(fc, W, H, pixels[i], frame, &smp, &vs);
+        out[2 * i] = (int64_t)(cn.interior + cn.leaf);
+        out[2 * i + 1] = (int64_t)cn.tlas;
+    }
 }
 
 }  // extern "C"

@@ -70,6 +70,15 @@ void orc_render_temporal_pixels(const orc_scene* s, const kpt_settings* settings
                                 int32_t width, int32_t height, int32_t frame, const float* prev_accum,
                                 const uint32_t* prev_gbuffer, const int32_t* pixels, int64_t n,
                                 float* out_render, uint32_t* out_gbuffer, float* out_vbuffer, int32_t threads);
+/* FP32 op counts per phase and kind of the listed pixels' samples of frame
+ * `frame` (the oracle instantiated with a counting float); out[7 * 4]. */
+void orc_count_ops(const orc_scene* s, const kpt_settings* settings, const kpt_camera* cam, int32_t W, int32_t H,
+                   const int32_t* pixels, int64_t n, int32_t frame, uint64_t* out);
+/* Per-pixel traversal counts of one sample's closest-hit traces (DebugData,
+ * blas.cuh:39-42): out[2i] BLAS node pops, out[2i+1] TLAS node pops;
+ * entry_bound: BLASes entered as the GPU kernels enter them. */
+void orc_heat_map(const orc_scene* s, const kpt_settings* settings, const kpt_camera* cam, int32_t W, int32_t H,
+                  const int32_t* pixels, int64_t n, int32_t frame, int32_t entry_bound, int64_t* out);
 
 /* One sample (radiance before clamp/accumulation) per pixel for frame `frame`.
  * Used to check the accumulation kernels independently. */

@@ -42,6 +42,9 @@ def test_host_only_context_errors():
     with pytest.raises(abi.KptError) as e:
         c.upload(bad)
     assert e.value.code == abi.KPT_ERR_INVALID_ARG
+    out = np.zeros(8, np.int32)
+    assert abi.lib().kpt_get_heat_map(c.h, out.ctypes.data, None, 4) == abi.KPT_ERR_NO_DEVICE
+    assert abi.lib().kpt_get_heat_map(c.h, None, None, 4) == abi.KPT_ERR_INVALID_ARG
     c.close()
 
 
