@@ -373,7 +373,14 @@ int kpt_get_heat_map(kpt_ctx* ctx, int32_t* counts, float* rgba, int64_t n_pixel
  * only choice of a host-only context).  A scene the device builder cannot
  * take (non-finite vertex positions, more than 2^29 triangles) or a device
  * build that fails is built on host threads; kpt_last_build_timing says
- * which ran.  n >= 5. */
+ * which ran, [10] (when n >= 11) traversal stack entries per lane held in
+ * LDS by the wavefront's trace, shadow and tail kernels (8..32, default 16;
+ * deeper entries go to private memory, 128 entries in all whatever the
+ * split; the reference's stacks hold 16, tlas.cuh:10, blas.cuh:37), [11]
+ * (when n >= 12) lanes per workgroup of those kernels (64, 128 or 256,
+ * default 256; the reference launches 8x8 blocks, kernels.cu:32-36; the
+ * persistent grid keeps blocks_per_cu x 256 lanes per compute unit).
+ * n >= 5. */
 int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n);
 int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n);
 int kpt_get_render_counters(const kpt_ctx* ctx, kpt_trace_counters* out);
@@ -468,6 +475,29 @@ int kpt_cache_get_queries(kpt_ctx* ctx, float* pos, float* dir, float* beta, int
  * 8-bit, non-interlaced image with bpp bytes per pixel); out receives
  * height x stride bytes.  KPT_ERR_INVALID_ARG on an unknown filter type. */
 int kpt_png_unfilter(const uint8_t* raw, int64_t height, int64_t stride, int32_t bpp, uint8_t* out);
+/* PNG image (n bytes at data) to RGB8, as the reference's importer calls
+ * stbi_load_from_memory(..., 3) (SampleApp model_importer.hpp:287-296): grey
+ * replicated, alpha dropped, palette looked up.  8-bit non-interlaced images
+ * of colour type 0, 2, 3, 4 or 6 are read (others: KPT_ERR_UNSUPPORTED; a
+ * malformed stream: KPT_ERR_INVALID_ARG).  *width and *height receive the
+ * size; with rgb NULL nothing else is done, else rgb receives width x height
+ * x 3 bytes, rows top-down (capacity >= that). */
+int kpt_png_decode_rgb8(const uint8_t* data, int64_t n, int32_t* width, int32_t* height, uint8_t* rgb,
+                        int64_t capacity);
+/* ---- frame files (no reference counterpart: the reference shows frames
+ * through GL textures, renderer.cu:197-205) ---- */
+/* px: width x height pixels of `channels` (3 or 4) floats, row 0 = the frame
+ * buffers' bottom row.  kpt_write_png_rgba8: 8-bit RGBA PNG of values in
+ * [0, 1] (floor(v * 255 + 0.5) clamped, NaN -> 0, alpha 1 when absent), rows
+ * flipped to top-down when flip_y.  kpt_write_pfm: colour PFM, little-endian,
+ * rows as given (PFM is bottom-up).  kpt_write_exr: single-part scanline
+ * OpenEXR, uncompressed FLOAT A/B/G/R, rows flipped when flip_y.
+ * KPT_ERR_INVALID_ARG on bad arguments or a file that cannot be written. */
+int kpt_write_png_rgba8(const char* path, const float* px, int32_t width, int32_t height, int32_t channels,
+                        int32_t flip_y);
+int kpt_write_pfm(const char* path, const float* px, int32_t width, int32_t height, int32_t channels);
+int kpt_write_exr(const char* path, const float* px, int32_t width, int32_t height, int32_t channels,
+                  int32_t flip_y);
 
 #ifdef __cplusplus
 }

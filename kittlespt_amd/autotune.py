@@ -2,8 +2,10 @@
 
 Knobs (kpt_set_tuning): lane refill on/off, queue entries claimed per wave
 atomic (with or without refill), refill threshold, persistent workgroups per compute unit, material
-sort on/off, per-frame hipGraph replay on/off, spatial ray sort on/off, and
-the bounce from which the remaining paths finish in one path-per-lane launch.
+sort on/off, per-frame hipGraph replay on/off, spatial ray sort on/off, the
+bounce from which the remaining paths finish in one path-per-lane launch, the
+traversal stack entries per lane held in LDS (the rest in private memory) and
+the workgroup size of the trace, shadow and tail kernels.
 The tuner renders real frames of the caller's scene and camera
 and keeps the fastest setting by coordinate descent (each knob swept with the
 others fixed, two passes).  Results are cached in a JSON file keyed by the
@@ -24,8 +26,10 @@ SPACE = {
     "graph": [0, 1],
     "ray_sort": [0, 1],
     "tail": [0, 1, 2, 3, 4],
+    "stack": [8, 16, 24, 32],
+    "block": [64, 128, 256],
 }
-ORDER = ["tail", "ray_sort", "refill", "blocks_per_cu", "sort", "graph", "chunk", "refill_min"]
+ORDER = ["tail", "ray_sort", "refill", "blocks_per_cu", "stack", "block", "sort", "graph", "chunk", "refill_min"]
 
 
 def measure(ctx, cfg, spp=1, repeats=2):

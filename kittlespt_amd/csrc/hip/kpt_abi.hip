@@ -108,7 +108,7 @@ struct kpt_ctx {
     int32_t use_graph = 0;               // capture each frame as a hipGraph and replay it
     hipGraphExec_t graph_exec = nullptr;        // kpt_render's per-frame graph
     hipGraphExec_t views_graph_exec = nullptr;  // kpt_render_views' per-sample graph
-    TraceTuning tune{0, 64, 16, 8, 0, 1};    // the autotuner's usual choice on the 1080p workload
+    TraceTuning tune{0, 64, 16, 8, 0, 1, 16, 256};    // the autotuner's usual choice on the 1080p workload
     int32_t sort_by_material = 2;
     bool count_trace = false;
     unsigned long long* d_counts = nullptr;
@@ -1840,6 +1840,16 @@ int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n) {
         if (knobs[9] < 0 || knobs[9] > 1) return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
         ctx->blas_device = knobs[9];
     }
+    if (n >= 11) {
+        if (knobs[10] < LDS_STACK_MIN || knobs[10] > LDS_STACK_MAX)
+            return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
+        ctx->tune.stack = knobs[10];
+    }
+    if (n >= 12) {
+        if (knobs[11] != 64 && knobs[11] != 128 && knobs[11] != 256)
+            return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
+        ctx->tune.block = knobs[11];
+    }
     return KPT_OK;
 }
 
@@ -1855,6 +1865,8 @@ int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n) {
     if (n >= 8) knobs[7] = ctx->tune.tail;
     if (n >= 9) knobs[8] = ctx->tlas_device;
     if (n >= 10) knobs[9] = ctx->blas_device;
+    if (n >= 11) knobs[10] = ctx->tune.stack;
+    if (n >= 12) knobs[11] = ctx->tune.block;
     return KPT_OK;
 }
 

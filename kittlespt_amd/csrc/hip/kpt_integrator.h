@@ -235,7 +235,7 @@ KD CacheVertex cache_vertex(const FrameConst& fc, const CacheBuffers& cb, const 
 // heat (COUNT with cnt): the path's pixel's DebugData pair, to which the
 // closest-hit trace's BLAS and TLAS node pops are added.
 template <bool COUNT = false, bool DEFER_SKY = false, bool CACHE = false>
-KD bool li_bounce(const SceneView& S, const FrameConst& fc, PathState& ps, uint32_t& seed, uint32_t* lds,
+KD bool li_bounce(const SceneView& S, const FrameConst& fc, PathState& ps, uint32_t& seed, const StackShape& lds,
                   bool* ovf, VisOut* vs, TraceCounts* cnt = nullptr, bool* missed = nullptr,
                   const CacheBuffers* cb = nullptr, const CacheFrame* cf = nullptr, int32_t path = -1,
                   int32_t* heat = nullptr) {
@@ -324,7 +324,7 @@ KD bool li_bounce(const SceneView& S, const FrameConst& fc, PathState& ps, uint3
 
 // Li from a path state to the end of the path, in one thread (the megakernel
 // from depth 0).  ps.L holds the radiance on return.
-KD v3 li_loop(const SceneView& S, const FrameConst& fc, PathState& ps, uint32_t& seed, uint32_t* lds, bool* ovf,
+KD v3 li_loop(const SceneView& S, const FrameConst& fc, PathState& ps, uint32_t& seed, const StackShape& lds, bool* ovf,
               VisOut* vs) {
     while (!is_zero(ps.beta)) {
         if (!li_bounce(S, fc, ps, seed, lds, ovf, vs)) break;
@@ -333,7 +333,8 @@ KD v3 li_loop(const SceneView& S, const FrameConst& fc, PathState& ps, uint32_t&
 }
 
 // Megakernel form of Li: the whole path in one thread.
-KD v3 li_path(const SceneView& S, const FrameConst& fc, Ray ray, uint32_t& seed, uint32_t* lds, bool* ovf, VisOut* vs) {
+KD v3 li_path(const SceneView& S, const FrameConst& fc, Ray ray, uint32_t& seed, const StackShape& lds, bool* ovf,
+              VisOut* vs) {
     PathState ps;
     ps.ray = ray;
     ps.L = mk3(0.0f, 0.0f, 0.0f);

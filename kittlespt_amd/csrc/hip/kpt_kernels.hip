@@ -19,7 +19,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_closest(SceneView S, cons
         const kpt_ray r = rays[i];
         Ray w = make_ray(mk3(r.org[0], r.org[1], r.org[2]), mk3(r.dir[0], r.dir[1], r.dir[2]));
         bool ovf = false;
-        Hit h = trace_closest<COUNT>(S, w, r.tmin, r.tmax, &s_stack[0][threadIdx.x], &ovf, &c);
+        Hit h = trace_closest<COUNT>(S, w, r.tmin, r.tmax, StackShape{&s_stack[0][threadIdx.x], TRACE_BLOCK, LDS_STACK}, &ovf, &c);
         kpt_hit o;
         o.t = h.t;
         if (h.slot >= 0) {
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_any(SceneView S, const kp
     const kpt_ray r = rays[i];
     Ray w = make_ray(mk3(r.org[0], r.org[1], r.org[2]), mk3(r.dir[0], r.dir[1], r.dir[2]));
     bool ovf = false;
-    bool o = trace_any<false>(S, w, r.tmin, r.tmax, &s_stack[0][threadIdx.x], &ovf, nullptr);
+    bool o = trace_any<false>(S, w, r.tmin, r.tmax, StackShape{&s_stack[0][threadIdx.x], TRACE_BLOCK, LDS_STACK}, &ovf, nullptr);
     occluded[i] = o ? 1 : 0;
     if (ovf) atomicOr(overflow, 1);
 }
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_megakernel(SceneView S, FrameCo
     vs.albedo = mk3(0.0f, 0.0f, 0.0f); vs.normal = mk3(0.0f, 0.0f, 0.0f); vs.pos = mk3(0.0f, 0.0f, 0.0f);
     vs.depth = __builtin_inff(); vs.inst = -1; vs.prim = -1;
     bool ovf = false;
-    v3 L = li_path(S, fc, ray, seed, &s_stack[0][threadIdx.x], &ovf, &vs);
+    v3 L = li_path(S, fc, ray, seed, StackShape{&s_stack[0][threadIdx.x], TRACE_BLOCK, LDS_STACK}, &ovf, &vs);
     L = (mk3(1.0f, 1.0f, 1.0f) * 1.0f) * L;
     L = clamp_output(L);
     write_pixel(S, fc, fb, frame, idx, px, py, L, vs.albedo, vs.normal, vs.depth, vs.inst, vs.prim, vs.pos);

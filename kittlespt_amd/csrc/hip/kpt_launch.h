@@ -138,6 +138,8 @@ struct TraceTuning {
     int blocks_per_cu; // persistent grid size per compute unit
     int ray_sort;      // sort secondary and shadow rays by direction octant + Morton cell of the origin
     int tail;          // from this bounce on, live paths finish in one path-per-lane launch (0: never)
+    int stack;         // traversal stack entries per lane held in LDS (LDS_STACK_MIN..LDS_STACK_MAX)
+    int block;         // lanes per workgroup of the persistent trace, shadow and tail kernels (64, 128, 256)
 };
 
 // Events bracketing every closest-hit trace launch of a frame (recorded on the

@@ -41,9 +41,16 @@
 
 namespace kpt {
 
-constexpr int TRACE_BLOCK = 256;      // threads per workgroup of trace-bearing kernels
-constexpr int LDS_STACK = 16;         // stack entries per lane held in LDS
-constexpr int SPILL_STACK = 112;      // further entries in private memory
+constexpr int TRACE_BLOCK = 256;      // threads per workgroup of the fixed-shape trace kernels
+constexpr int LDS_STACK = 16;         // stack entries per lane held in LDS (fixed-shape kernels)
+constexpr int STACK_TOTAL = 128;      // entries per lane, LDS and private together; a push beyond overflows
+// The wavefront's persistent kernels take their LDS depth and workgroup size
+// from the tuning knobs (dynamic LDS of depth x workgroup words); the private
+// spill array is sized for the shallowest LDS part, so the total capacity is
+// STACK_TOTAL whatever the depth.
+constexpr int LDS_STACK_MIN = 8;
+constexpr int LDS_STACK_MAX = 32;
+constexpr int SPILL_STACK = STACK_TOTAL - LDS_STACK_MIN;
 constexpr uint32_t TLAS_BIT = 0x80000000u;
 #ifndef KPT_TRI_BATCH
 #define KPT_TRI_BATCH 4
@@ -119,25 +126,39 @@ KD bool tri_test(const TriPos& tp, const Ray& r, float tmin, float tmax, float* 
     return false;
 }
 
-// Per-lane stack: LDS part [LDS_STACK][TRACE_BLOCK] + a private spill array
-// owned by the caller (kept outside the traversal state so the state itself
-// stays in registers).
+// Per-lane stack: LDS part [depth][stride] (stride = the workgroup's lanes,
+// so the lanes of a wave hit distinct banks) + a private spill array owned by
+// the caller (kept outside the traversal state so the state itself stays in
+// registers).
 typedef __attribute__((address_space(3))) uint32_t lds_u32;   // a pointer that is known to be LDS
 
+// Where a lane's stack lives: its first LDS word, the word stride between
+// entries and the number of LDS entries (LDS_STACK_MIN..LDS_STACK_MAX, or
+// LDS_STACK for the fixed-shape kernels); all but the pointer are uniform.
+struct StackShape {
+    uint32_t* lds;
+    int stride;
+    int depth;
+};
+
 struct LaneStack {
-    lds_u32* lds;       // &s_stack[0][lane]
+    lds_u32* lds;       // the lane's entry 0
     uint32_t* spill;    // private array of SPILL_STACK entries
+    int stride, depth;
     int sp;
     bool overflow;
     KD void push(uint32_t e) {
-        if (sp < LDS_STACK) lds[sp * TRACE_BLOCK] = e;
-        else if (sp < LDS_STACK + SPILL_STACK) spill[sp - LDS_STACK] = e;
+        if (sp < depth) lds[sp * stride] = e;
+        else if (sp < STACK_TOTAL) spill[sp - depth] = e;
         else { overflow = true; return; }
         ++sp;
     }
     KD uint32_t pop() {
         --sp;
-        return sp < LDS_STACK ? lds[sp * TRACE_BLOCK] : spill[sp - LDS_STACK];
+        return sp < depth ? lds[sp * stride] : spill[sp - depth];
+    }
+    KD void reset(const StackShape& sh, uint32_t* spill_) {
+        lds = (lds_u32*)sh.lds; spill = spill_; stride = sh.stride; depth = sh.depth; sp = 0; overflow = false;
     }
 };
 
@@ -242,12 +263,12 @@ struct ClosestTrav {
 
     // TLAS::intersect entry: empty TLAS or a miss of the root box ends the
     // query at once (tlas.cu:62-69).  Returns false when nothing remains.
-    KD bool init(const SceneView& S, const Ray& ray, float t0, float t1, uint32_t* lds, uint32_t* spill) {
+    KD bool init(const SceneView& S, const Ray& ray, float t0, float t1, const StackShape& sh, uint32_t* spill) {
         w = ray; o = ray; tmin = t0; tmax = t1;
         h.t = __builtin_inff(); h.slot = -1; h.inst = -1; h.u = 0.0f; h.v = 0.0f;
         cur_inst = -1;
         la = 0u; lb = 0u;
-        st.lds = (lds_u32*)lds; st.spill = spill; st.sp = 0; st.overflow = false;
+        st.reset(sh, spill);
         if (S.tlas_root < 0) return false;
         float4 rlo, rhi;
         load_node(S.tlas_nodes, 0, &rlo, &rhi);
@@ -381,10 +402,10 @@ struct AnyTrav {
     uint32_t la, lb;
     LaneStack st;
 
-    KD bool init(const SceneView& S, const Ray& ray, float t0, float t1, uint32_t* lds, uint32_t* spill) {
+    KD bool init(const SceneView& S, const Ray& ray, float t0, float t1, const StackShape& sh, uint32_t* spill) {
         w = ray; o = ray; tmin = t0; tmax = t1; hit = false;
         la = 0u; lb = 0u;
-        st.lds = (lds_u32*)lds; st.spill = spill; st.sp = 0; st.overflow = false;
+        st.reset(sh, spill);
         if (S.tlas_root < 0) return false;
         float4 rlo, rhi;
         load_node(S.tlas_nodes, 0, &rlo, &rhi);
@@ -483,13 +504,13 @@ struct AnyTrav {
     }
 };
 
-// Whole-query forms (megakernel and trace entry points).
+// Whole-query forms (megakernel, trace entry points, the wavefront's tail).
 template <bool COUNT>
-KD Hit trace_closest(const SceneView& S, const Ray& w, float tmin, float tmax, uint32_t* lds_lane,
+KD Hit trace_closest(const SceneView& S, const Ray& w, float tmin, float tmax, const StackShape& sh,
                      bool* overflow, TraceCounts* cnt) {
     uint32_t spill[SPILL_STACK];
     ClosestTrav tr;
-    if (tr.init(S, w, tmin, tmax, lds_lane, spill)) {
+    if (tr.init(S, w, tmin, tmax, sh, spill)) {
         while (tr.template step<COUNT>(S, cnt)) {}
     }
     *overflow = tr.st.overflow;
@@ -497,11 +518,11 @@ KD Hit trace_closest(const SceneView& S, const Ray& w, float tmin, float tmax, u
 }
 
 template <bool COUNT>
-KD bool trace_any(const SceneView& S, const Ray& w, float tmin, float tmax, uint32_t* lds_lane, bool* overflow,
+KD bool trace_any(const SceneView& S, const Ray& w, float tmin, float tmax, const StackShape& sh, bool* overflow,
                   TraceCounts* cnt) {
     uint32_t spill[SPILL_STACK];
     AnyTrav tr;
-    if (tr.init(S, w, tmin, tmax, lds_lane, spill)) {
+    if (tr.init(S, w, tmin, tmax, sh, spill)) {
         while (tr.template step<COUNT>(S, cnt)) {}
     }
     *overflow = tr.st.overflow;

@@ -236,12 +236,12 @@ template <bool COUNT>
 __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TRACE void k_trace(SceneView S, WavefrontState W, int parity, int pull_slot,
                                                     int32_t* overflow, unsigned long long* counters,
                                                     TraceTuning tune, const int32_t* sorted) {
-    __shared__ uint32_t s_stack[LDS_STACK][TRACE_BLOCK];
+    extern __shared__ uint32_t s_dyn[];   // tune.stack x blockDim.x words
     uint32_t spill[SPILL_STACK];
     const int count = W.counters[C_QCOUNT + parity];
     const int32_t* q = sorted ? sorted : W.queue[parity];
     int32_t* ctr = &W.counters[pull_slot];
-    uint32_t* lds = &s_stack[0][threadIdx.x];
+    const StackShape lds{s_dyn + threadIdx.x, (int)blockDim.x, tune.stack};
     bool ovf = false;
     TraceCounts c = {0, 0, 0, 0, 0};
     if (!tune.refill) {
@@ -497,8 +497,9 @@ template <bool COUNT, bool CACHE>
 __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TAIL void k_tail(SceneView S, FrameConst fc, WavefrontState W, int parity,
                                                    int pull_slot, int32_t* overflow,
                                                    unsigned long long* counters, const int32_t* order,
-                                                   CacheBuffers cb, CacheFrame cf) {
-    __shared__ uint32_t s_stack[LDS_STACK][WF_BLOCK];
+                                                   CacheBuffers cb, CacheFrame cf, int stack_depth) {
+    extern __shared__ uint32_t s_dyn[];   // stack_depth x blockDim.x words
+    const StackShape lds{s_dyn + threadIdx.x, (int)blockDim.x, stack_depth};
     const int count = W.counters[C_QCOUNT + parity];
     const int32_t* q = order ? order : W.queue[parity];
     int32_t* ctr = &W.counters[pull_slot];
@@ -543,7 +544,7 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TAIL void k_tail(SceneView S, Fra
             bool o = false;
             if (COUNT && !is_zero(ps.beta)) ++n_vertices;
             const bool more = !is_zero(ps.beta) &&
-                              li_bounce<COUNT, true, CACHE>(S, fc, ps, seed, &s_stack[0][threadIdx.x], &o, &vs, &c,
+                              li_bounce<COUNT, true, CACHE>(S, fc, ps, seed, lds, &o, &vs, &c,
                                                             &missed, &cb, &cf, p, heat_px);
             ovf = ovf || o;
             if (!more || is_zero(ps.beta)) {
@@ -601,11 +602,11 @@ __device__ __forceinline__ void shadow_resolve(WavefrontState& W, float4* t_lbef
 
 __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TRACE void k_shadow(SceneView S, WavefrontState W, int pull_slot, int32_t* overflow,
                                                      TraceTuning tune, const int32_t* order, float4* t_lbefore) {
-    __shared__ uint32_t s_stack[LDS_STACK][TRACE_BLOCK];
+    extern __shared__ uint32_t s_dyn[];   // tune.stack x blockDim.x words
     uint32_t spill[SPILL_STACK];
     const int count = W.counters[C_SHADOW];
     int32_t* ctr = &W.counters[pull_slot];
-    uint32_t* lds = &s_stack[0][threadIdx.x];
+    const StackShape lds{s_dyn + threadIdx.x, (int)blockDim.x, tune.stack};
     bool ovf = false;
     if (!tune.refill) {
         const int chunk = (tune.chunk / WAVE) * WAVE;
@@ -708,9 +709,16 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
         const int64_t nh = 2 * npx;
         hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, s, W.heat, 0, (int)nh);
     }
-    // persistent grids: 8 workgroups of 256 lanes per CU
+    // persistent grids: blocks_per_cu workgroups of 256 lanes per CU; the
+    // trace-bearing kernels run the same number of lanes in workgroups of
+    // tune.block lanes, each with tune.stack LDS stack entries per lane
     const unsigned pgrid = (unsigned)std::max<int64_t>(
         1, std::min<int64_t>((int64_t)g_num_cus * std::max(1, tune.blocks_per_cu), gblocks));
+    const int tblock = tune.block;
+    const unsigned tgrid = (unsigned)std::max<int64_t>(
+        1, std::min<int64_t>((int64_t)g_num_cus * std::max(1, tune.blocks_per_cu) * (WF_BLOCK / tblock),
+                             (npx + tblock - 1) / tblock));
+    const size_t tlds = (size_t)tune.stack * (size_t)tblock * sizeof(uint32_t);
     const int nkeys = std::min(MAX_KEYS, num_materials + 1);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, W.counters, 0, C_PULL + 3 * MAX_BOUNCE_SLOTS);
     if (views)
@@ -740,17 +748,17 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
             const bool with_cache = cf.enabled || cf.train;
             unsigned long long* tc = (ins && ins->counters) ? ins->counters : nullptr;
             if (tc && with_cache)
-                hipLaunchKernelGGL((k_tail<true, true>), dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity,
-                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf);
+                hipLaunchKernelGGL((k_tail<true, true>), dim3(tgrid), dim3(tblock), tlds, s, S, fc, W, parity,
+                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf, tune.stack);
             else if (tc)
-                hipLaunchKernelGGL((k_tail<true, false>), dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity,
-                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf);
+                hipLaunchKernelGGL((k_tail<true, false>), dim3(tgrid), dim3(tblock), tlds, s, S, fc, W, parity,
+                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf, tune.stack);
             else if (with_cache)
-                hipLaunchKernelGGL((k_tail<false, true>), dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity,
-                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf);
+                hipLaunchKernelGGL((k_tail<false, true>), dim3(tgrid), dim3(tblock), tlds, s, S, fc, W, parity,
+                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf, tune.stack);
             else
-                hipLaunchKernelGGL((k_tail<false, false>), dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity,
-                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf);
+                hipLaunchKernelGGL((k_tail<false, false>), dim3(tgrid), dim3(tblock), tlds, s, S, fc, W, parity,
+                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf, tune.stack);
             if (ins) {
                 (void)hipEventRecord(ins->tail_end, s);
                 ins->tail_launches = 1;
@@ -779,10 +787,10 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
         const bool timed = ins && ins->launches < WF_MAX_TRACE_LAUNCHES;
         if (timed) (void)hipEventRecord(ins->begin[ins->launches], s);
         if (ins && ins->counters) {
-            hipLaunchKernelGGL(k_trace<true>, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, W, parity, C_PULL + 3 * bounce,
+            hipLaunchKernelGGL(k_trace<true>, dim3(tgrid), dim3(tblock), tlds, s, S, W, parity, C_PULL + 3 * bounce,
                                overflow, ins->counters, tune, trace_q);
         } else {
-            hipLaunchKernelGGL(k_trace<false>, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, W, parity, C_PULL + 3 * bounce,
+            hipLaunchKernelGGL(k_trace<false>, dim3(tgrid), dim3(tblock), tlds, s, S, W, parity, C_PULL + 3 * bounce,
                                overflow, (unsigned long long*)nullptr, tune, trace_q);
         }
         if (timed) (void)hipEventRecord(ins->end[ins->launches++], s);
@@ -813,7 +821,7 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
             if (e != hipSuccess) return e;
             shadow_order = W.rvals[1];
         }
-        hipLaunchKernelGGL(k_shadow, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, W, C_PULL + 3 * bounce + 2, overflow,
+        hipLaunchKernelGGL(k_shadow, dim3(tgrid), dim3(tblock), tlds, s, S, W, C_PULL + 3 * bounce + 2, overflow,
                            tune, shadow_order, cb.t_lbefore);
     }
     if (cf.enabled) {

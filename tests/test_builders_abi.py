@@ -62,6 +62,23 @@ def test_blas_device_knob():
     c.close()
 
 
+def test_stack_and_workgroup_knobs():
+    """LDS stack entries per lane (8..32) and workgroup size (64, 128, 256)
+    of the wavefront's trace kernels: defaults, accepted values, rejections."""
+    c = Context(-2)
+    t = c.tuning()
+    assert (t["stack"], t["block"]) == (16, 256)
+    for stack, block in ((8, 64), (32, 128), (24, 256)):
+        c.set_tuning(stack=stack, block=block)
+        assert (c.tuning()["stack"], c.tuning()["block"]) == (stack, block)
+    for bad in ({"stack": 7}, {"stack": 33}, {"block": 96}, {"block": 512}, {"block": 0}):
+        with pytest.raises(abi.KptError) as e:
+            c.set_tuning(**bad)
+        assert e.value.code == abi.KPT_ERR_INVALID_ARG
+    assert (c.tuning()["stack"], c.tuning()["block"]) == (24, 256)
+    c.close()
+
+
 def test_settings_validation_effects_and_upscale():
     from common import settings
     c = Context(-2)

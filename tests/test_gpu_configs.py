@@ -299,16 +299,21 @@ def test_deep_tlas_spills_the_stack_and_matches_oracle():
         assert np.array_equal(g["bary"][hit].view(np.uint32), o["bary"][hit].view(np.uint32))
         rays["tmax"] = np.random.default_rng(48).uniform(1.0, 1e4, len(rays)).astype(np.float32)
         assert np.array_equal(ctx.trace_any(rays), orc.trace(rays, oracle.MODE_ANY)[0])
-        for integ, tail in ((abi.INTEGRATOR_MEGAKERNEL, 0), (abi.INTEGRATOR_WAVEFRONT, 1),
-                            (abi.INTEGRATOR_WAVEFRONT, 0)):
-            ctx.set_tuning(tail=tail)
+        # the wavefront kernels' LDS share of the stack is a tuning knob: the
+        # shallowest and deepest LDS parts, at every workgroup size, give
+        # the same bits (the 128-entry capacity does not change)
+        for integ, tail, stack, block in ((abi.INTEGRATOR_MEGAKERNEL, 0, 16, 256), (abi.INTEGRATOR_WAVEFRONT, 1, 16, 256),
+                                          (abi.INTEGRATOR_WAVEFRONT, 0, 16, 256), (abi.INTEGRATOR_WAVEFRONT, 1, 8, 64),
+                                          (abi.INTEGRATOR_WAVEFRONT, 0, 32, 128), (abi.INTEGRATOR_WAVEFRONT, 1, 24, 256)):
+            ctx.set_tuning(tail=tail, stack=stack, block=block)
             st = settings(max_depth=4, integrator=integ)
             ctx.set_settings(st)
             ctx.resize(64, 64)
             p, v = view_proj(sc, 64, 64)
             ctx.set_view(p, v)
             cam = ctx.camera()
-            print(f"[deep] render 64x64 x 2 spp, integrator {integ}, tail {tail}", flush=True)
+            print(f"[deep] render 64x64 x 2 spp, integrator {integ}, tail {tail}, LDS stack {stack}, "
+                  f"workgroup {block}", flush=True)
             ctx.render(2)
             img = ctx.estimate().reshape(-1, 3)
             cam.prev_inv_view[:] = cam.inv_view[:]
@@ -344,6 +349,12 @@ def test_stack_overflow_is_reported_once():
         p, v = view_proj(sc, 32, 32)
         ctx.set_view(p, v)
         print("[overflow] render of the 139-level scene", flush=True)
+        with pytest.raises(abi.KptError) as ei:
+            ctx.render(1)
+        assert ei.value.code == abi.KPT_ERR_OVERFLOW
+        # the deepest LDS part keeps the 128-entry capacity
+        ctx.set_tuning(stack=32, block=64)
+        print("[overflow] render of the 139-level scene, LDS stack 32, workgroup 64", flush=True)
         with pytest.raises(abi.KptError) as ei:
             ctx.render(1)
         assert ei.value.code == abi.KPT_ERR_OVERFLOW
