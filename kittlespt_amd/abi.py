@@ -247,6 +247,10 @@ SIGNATURES = [
     ("kpt_trace_closest_counted", C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(KptTraceCounters)]),
     ("kpt_trace_closest_counted_noalloc", C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(KptTraceCounters)]),
     ("kpt_png_unfilter", C.c_int, [_P, C.c_int64, C.c_int64, C.c_int32, _P]),
+    ("kpt_png_decode_rgb8", C.c_int, [_P, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _P, C.c_int64]),
+    ("kpt_write_png_rgba8", C.c_int, [C.c_char_p, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    ("kpt_write_pfm", C.c_int, [C.c_char_p, _P, C.c_int32, C.c_int32, C.c_int32]),
+    ("kpt_write_exr", C.c_int, [C.c_char_p, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     ("kpt_set_settings", C.c_int, [_P, C.POINTER(KptSettings)]),
     ("kpt_get_settings", C.c_int, [_P, C.POINTER(KptSettings)]),
     ("kpt_set_view", C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),

@@ -42,7 +42,7 @@ def hipcc():
 def build(verbose=False):
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
     tmp = OUT + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp] + SOURCES + ["-lpthread"]
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp] + SOURCES + ["-lpthread", "-lz"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -217,3 +217,232 @@ def test_import_whole_file(tmp_path, binary):
     assert np.array_equal(sc.pixels.reshape(4, 3, 3), px)
     assert len(cams) == 1 and cams[0].yfov == pytest.approx(0.8)
     assert np.allclose(_apply(cams[0].view.reshape(4, 4), (0, 1, 5)), (0, 0, 0), atol=1e-6)
+
+
+# ---------------------------------------------------------------- C++ importer and C PNG decoder
+import ctypes  # noqa: E402
+import os  # noqa: E402
+import subprocess  # noqa: E402
+
+from kittlespt_amd import abi, build  # noqa: E402
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_APP_SRC = os.path.join(_HERE, "cpp", "gltf_import.cpp")
+_APP = os.path.join(_HERE, "cpp", "_build", "gltf_import")
+
+
+def _c_decode(png):
+    L = abi.lib()
+    w, h = ctypes.c_int32(), ctypes.c_int32()
+    buf = np.frombuffer(png, np.uint8)
+    rc = L.kpt_png_decode_rgb8(buf.ctypes.data, len(png), ctypes.byref(w), ctypes.byref(h), None, 0)
+    if rc != abi.KPT_OK:
+        return rc, None
+    out = np.zeros((h.value, w.value, 3), np.uint8)
+    rc = L.kpt_png_decode_rgb8(buf.ctypes.data, len(png), ctypes.byref(w), ctypes.byref(h), out.ctypes.data, out.size)
+    return rc, out
+
+
+@pytest.mark.parametrize("ctype,chans", [(2, 3), (6, 4), (0, 1), (4, 2)])
+def test_c_png_decoder_equals_python(ctype, chans):
+    rng = np.random.default_rng(10 + ctype)
+    px = rng.integers(0, 256, (9, 13, chans), dtype=np.uint8)
+    png = _encode_png(px, ctype, [0, 1, 2, 3, 4, 4, 3, 1])
+    rc, got = _c_decode(png)
+    assert rc == abi.KPT_OK and np.array_equal(got, gltf.decode_png(png))
+
+
+def test_c_png_decoder_palette_and_errors():
+    pal = np.array([[255, 0, 0], [0, 255, 0], [0, 0, 255]], np.uint8)
+    idx = np.array([[0, 1, 2], [2, 1, 0]], np.uint8)[..., None]
+    png = _encode_png(idx, 3, [1, 2], palette=pal)
+    rc, got = _c_decode(png)
+    assert rc == abi.KPT_OK and np.array_equal(got, pal[idx[..., 0]])
+    bad_index = _encode_png(np.full((2, 2, 1), 7, np.uint8), 3, [0], palette=pal)   # index past the palette
+    assert _c_decode(bad_index)[0] == abi.KPT_ERR_INVALID_ARG
+    assert _c_decode(b"\xff\xd8\xff\xe0JFIF")[0] == abi.KPT_ERR_INVALID_ARG
+    assert _c_decode(png[:40])[0] == abi.KPT_ERR_INVALID_ARG                     # IDAT cut short
+    sixteen = bytearray(png)
+    sixteen[24] = 16                                                              # IHDR bit depth
+    assert _c_decode(bytes(sixteen))[0] == abi.KPT_ERR_UNSUPPORTED
+
+
+def _app():
+    deps = [_APP_SRC, abi.LIB_PATH] + [os.path.join(build.INCLUDE, "kittlespt", f) for f in
+                                       ("model_importer.hpp", "pod_types.hpp")] + [os.path.join(build.INCLUDE,
+                                                                                                "kpt_abi.h")]
+    if not os.path.exists(_APP) or os.path.getmtime(_APP) < max(os.path.getmtime(d) for d in deps):
+        build.compile_cpp_app([_APP_SRC], _APP)
+    return _APP
+
+
+def _cpp_import(path, tmp_path):
+    dump = tmp_path / "dump.bin"
+    r = subprocess.run([_app(), str(path), str(dump)], capture_output=True, text=True, timeout=60)
+    if r.returncode != 0:
+        return None, r.stdout
+    b = dump.read_bytes()
+    pos = 0
+
+    def take(dt, n):
+        nonlocal pos
+        a = np.frombuffer(b, dt, n, pos)
+        pos += a.nbytes
+        return a
+    nm, nmat, ntex, ncam = take(np.int32, 4)
+    meshes = []
+    for _ in range(nm):
+        nt = int(take(np.int32, 1)[0])
+        model = take(np.float32, 16)
+        tris = []
+        for _ in range(nt):
+            tris.append((take(np.float32, 24).copy(), int(take(np.int32, 1)[0])))
+        meshes.append((model.copy(), tris))
+    mats = [take(np.float32, 18).copy() for _ in range(nmat)]
+    texs = []
+    for _ in range(ntex):
+        w, h, ch = (int(x) for x in take(np.uint32, 3))
+        texs.append((w, h, ch, take(np.uint8, w * h * ch).copy()))
+    cams = [(take(np.float32, 16).copy(), float(take(np.float32, 1)[0])) for _ in range(ncam)]
+    assert pos == len(b)
+    return (meshes, mats, texs, cams), r.stdout
+
+
+def _assert_same_scene(cpp, sc, cams):
+    meshes, mats, texs, ccams = cpp
+    assert len(meshes) == len(sc.mesh_first)
+    for k, (model, tris) in enumerate(meshes):
+        assert np.array_equal(model.view(np.uint32), np.asarray(sc.mesh_model[k], np.float32).view(np.uint32))
+        f, n = int(sc.mesh_first[k]), int(sc.mesh_count[k])
+        assert len(tris) == n
+        for j, (v, mat) in enumerate(tris):
+            t = f + j
+            want = np.concatenate([sc.positions[t].reshape(-1), sc.normals[t].reshape(-1),
+                                   sc.uvs[t].reshape(-1)]).astype(np.float32)
+            assert np.array_equal(v.view(np.uint32), want.view(np.uint32)), (k, j)
+            assert mat == int(sc.material_ids[t])
+    assert len(mats) == len(sc.materials)
+    for got, m in zip(mats, sc.materials):
+        want = np.array([m["albedo_texture_id"], *m["albedo"], m["orm_texture_id"], m["metallic"], m["roughness"],
+                         m["transmission_texture_id"], m["transmission"], m["ior"], m["emission_texture_id"],
+                         *m["emission"], m["emission_scale_nits"], m["normal_texture_id"], m["normal_scale"], 0.0],
+                        np.float32)
+        assert np.array_equal(got, want)
+    assert [(w, h, ch) for w, h, ch, _ in texs] == [(w, h, ch) for w, h, ch, _ in sc.textures]
+    for (w, h, ch, px), (_, _, _, off) in zip(texs, sc.textures):
+        assert np.array_equal(px, sc.pixels[off:off + w * h * ch])
+    assert len(ccams) == len(cams)
+    for (view, yfov), c in zip(ccams, cams):
+        assert yfov == np.float32(c.yfov)
+        # both invert in double and round to float: equal, or one float ulp apart
+        assert np.allclose(view, c.view, rtol=2e-7, atol=1e-7)
+
+
+@pytest.mark.parametrize("binary", [False, True])
+def test_cpp_importer_equals_python(tmp_path, binary):
+    path, _, _ = _write(tmp_path, binary)
+    cpp, log = _cpp_import(path, tmp_path)
+    assert cpp is not None, log
+    sc, cams = gltf.load_gltf(str(path))
+    _assert_same_scene(cpp, sc, cams)
+
+
+def _rich_files(tmp_path):
+    """A .gltf with an external buffer and external images: interleaved
+    position/normal (byte stride 24), 8- and 32-bit indices with accessor
+    offsets, a primitive without indices, texture coordinates or material,
+    a palette and a grey-alpha PNG, a camera with a non-normalised
+    quaternion and a mesh node with a full TRS."""
+    rng = np.random.default_rng(5)
+    nv = 11
+    P = rng.uniform(-3, 3, (nv, 3)).astype(np.float32)
+    N = rng.normal(size=(nv, 3)).astype(np.float32)
+    U = rng.uniform(0, 1, (nv, 2)).astype(np.float32)
+    inter = np.concatenate([P, N], axis=1).astype(np.float32).tobytes()        # stride 24
+    i8 = np.array([0, 1, 2, 3, 4, 5, 6, 7, 8], np.uint8).tobytes() + b"\0" * 3
+    i32 = np.array([9, 9, 9, 10, 2, 4, 1, 3, 5, 8, 7, 6], np.uint32).tobytes()
+    blob = inter + U.tobytes() + i8 + i32
+    o_u = len(inter)
+    o_i8 = o_u + U.nbytes
+    o_i32 = o_i8 + len(i8)
+    (tmp_path / "geo.bin").write_bytes(blob)
+    pal = rng.integers(0, 256, (6, 3), dtype=np.uint8)
+    (tmp_path / "pal.png").write_bytes(_encode_png(rng.integers(0, 6, (5, 4, 1), dtype=np.uint8), 3, [4, 1, 0],
+                                                   palette=pal))
+    ga = _encode_png(rng.integers(0, 256, (3, 6, 2), dtype=np.uint8), 4, [3, 2])
+    doc = {
+        "asset": {"version": "2.0"},
+        "scenes": [{"nodes": [0, 1, 2]}],
+        "nodes": [
+            {"mesh": 0, "translation": [0.5, -2, 7], "rotation": [0.1, 0.7, -0.2, 0.68], "scale": [1, 2, 0.5]},
+            {"camera": 0, "translation": [3, 1, -4], "rotation": [0.3, 0.2, 0.1, 2.0]},
+            {"mesh": 1},
+        ],
+        "meshes": [
+            {"name": "a", "primitives": [
+                {"attributes": {"POSITION": 0, "NORMAL": 1, "TEXCOORD_0": 2}, "indices": 3, "material": 1},
+                {"attributes": {"POSITION": 0, "NORMAL": 1}, "indices": 4}]},
+            {"name": "b", "primitives": [{"attributes": {"POSITION": 0, "NORMAL": 1, "TEXCOORD_0": 2}}]},
+        ],
+        "materials": [
+            {"name": "m0"},
+            {"name": "m1", "pbrMetallicRoughness": {"baseColorTexture": {"index": 1},
+                                                    "metallicRoughnessTexture": {"index": 0}},
+             "emissiveTexture": {"index": 1},
+             "extensions": {"KHR_materials_transmission": {"transmissionFactor": 0.75,
+                                                           "transmissionTexture": {"index": 0}}}},
+        ],
+        "textures": [{"source": 0}, {"source": 1}],
+        "images": [{"uri": "pal.png", "name": "pal"},
+                   {"uri": "data:image/png;base64," + base64.b64encode(ga).decode(), "name": "ga"}],
+        "cameras": [{"name": "cam", "type": "perspective", "perspective": {"yfov": 0.6}}],
+        "accessors": [
+            {"bufferView": 0, "componentType": 5126, "count": nv, "type": "VEC3"},
+            {"bufferView": 0, "byteOffset": 12, "componentType": 5126, "count": nv, "type": "VEC3"},
+            {"bufferView": 1, "componentType": 5126, "count": nv, "type": "VEC2"},
+            {"bufferView": 2, "componentType": 5121, "count": 9, "type": "SCALAR"},
+            {"bufferView": 3, "byteOffset": 12, "componentType": 5125, "count": 9, "type": "SCALAR"},
+        ],
+        "bufferViews": [
+            {"buffer": 0, "byteOffset": 0, "byteLength": len(inter), "byteStride": 24},
+            {"buffer": 0, "byteOffset": o_u, "byteLength": U.nbytes},
+            {"buffer": 0, "byteOffset": o_i8, "byteLength": 9},
+            {"buffer": 0, "byteOffset": o_i32, "byteLength": len(i32)},
+        ],
+        "buffers": [{"uri": "geo.bin", "byteLength": len(blob)}],
+    }
+    path = tmp_path / "rich.gltf"
+    path.write_text(json.dumps(doc, indent=1))
+    return path, doc
+
+
+def test_cpp_importer_equals_python_on_layouts_and_defaults(tmp_path):
+    path, _ = _rich_files(tmp_path)
+    sc, cams = gltf.load_gltf(str(path))
+    assert list(sc.mesh_count) == [6, 3] and len(sc.textures) == 2 and len(cams) == 1
+    cpp, log = _cpp_import(path, tmp_path)
+    assert cpp is not None, log
+    _assert_same_scene(cpp, sc, cams)
+
+
+def test_cpp_importer_reports_malformed_files(tmp_path):
+    path, doc = _rich_files(tmp_path)
+    cases = {
+        "truncated.glb": None,
+        "badjson.gltf": '{"asset": {"version": "2.0"}, "scenes": [',
+        "nomesh.gltf": json.dumps({**doc, "scenes": [{"nodes": [1]}]}),
+        "oob.gltf": json.dumps({**doc, "accessors": doc["accessors"][:3] + [
+            {"bufferView": 2, "byteOffset": 8, "componentType": 5121, "count": 9, "type": "SCALAR"}] +
+            doc["accessors"][4:]}),
+    }
+    glb_path, _, _ = _write(tmp_path, True)
+    for name, text in cases.items():
+        p = tmp_path / name
+        if text is None:
+            p.write_bytes(glb_path.read_bytes()[:200])
+        else:
+            p.write_text(text)
+        cpp, log = _cpp_import(p, tmp_path)
+        assert cpp is None and log.startswith("import failed: "), (name, log)
+        with pytest.raises(Exception):
+            gltf.load_gltf(str(p))

@@ -5,6 +5,7 @@ import struct
 import zlib
 
 import numpy as np
+import pytest
 
 from kittlespt_amd import imageio
 
@@ -96,3 +97,46 @@ def test_exr_round_trip(tmp_path):
     assert names == ["A", "B", "G", "R"]
     got = img[..., [3, 2, 1, 0]][::-1]                  # back to RGBA, bottom-up rows
     assert np.array_equal(got.view(np.uint32), rgba.view(np.uint32))
+
+
+# ---------------------------------------------------------------- the C-ABI writers (C++ callers)
+def _c_write(fn, path, a, *extra):
+    from kittlespt_amd import abi
+    a = np.ascontiguousarray(a, np.float32)
+    h, w, ch = a.shape
+    return getattr(abi.lib(), fn)(str(path).encode(), a.ctypes.data, w, h, ch, *extra)
+
+
+def _frame(seed, ch, h=7, w=9):
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(-0.2, 1.3, (h, w, ch)).astype(np.float32)
+    a[0, 0, 0], a[1, 2, 1], a[2, 3, 2], a[3, 4, 0] = np.nan, np.inf, -np.inf, 0.5 / 255.0
+    return a
+
+
+@pytest.mark.parametrize("ch", [3, 4])
+def test_c_writers_equal_python(tmp_path, ch):
+    """kpt_write_exr and kpt_write_pfm write the Python writers' bytes;
+    kpt_write_png_rgba8 the same pixels (NaN, +-inf and a value on a rounding
+    boundary included) in a PNG the reader takes."""
+    from kittlespt_amd import abi
+    a = _frame(ch, ch)
+    for flip in (1, 0):
+        imageio.write_exr(str(tmp_path / "p.exr"), a, flip_y=bool(flip))
+        assert _c_write("kpt_write_exr", tmp_path / "c.exr", a, flip) == abi.KPT_OK
+        assert (tmp_path / "p.exr").read_bytes() == (tmp_path / "c.exr").read_bytes()
+        imageio.write_png(str(tmp_path / "p.png"), a, flip_y=bool(flip))
+        assert _c_write("kpt_write_png_rgba8", tmp_path / "c.png", a, flip) == abi.KPT_OK
+        assert np.array_equal(imageio.read_png_rgba8(str(tmp_path / "p.png")),
+                              imageio.read_png_rgba8(str(tmp_path / "c.png")))
+    imageio.write_pfm(str(tmp_path / "p.pfm"), a)
+    assert _c_write("kpt_write_pfm", tmp_path / "c.pfm", a) == abi.KPT_OK
+    assert (tmp_path / "p.pfm").read_bytes() == (tmp_path / "c.pfm").read_bytes()
+
+
+def test_c_writers_reject_bad_arguments(tmp_path):
+    from kittlespt_amd import abi
+    a = np.zeros((4, 5, 2), np.float32)
+    assert _c_write("kpt_write_png_rgba8", tmp_path / "x.png", a, 1) == abi.KPT_ERR_INVALID_ARG   # 2 channels
+    b = _frame(2, 3)
+    assert _c_write("kpt_write_exr", tmp_path / "no" / "such" / "dir.exr", b, 1) == abi.KPT_ERR_INVALID_ARG

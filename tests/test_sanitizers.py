@@ -60,7 +60,7 @@ def build_libraries():
         flags = [f for f in build.HIPCC_FLAGS if f != "-O3"] + ["-O1", "-g"]
         for f in SAN:                      # host pass only
             flags += ["-Xarch_host", f]
-        _run([build.hipcc()] + flags + ["-o", HIP_SAN] + build.SOURCES + ["-lpthread"])
+        _run([build.hipcc()] + flags + ["-o", HIP_SAN] + build.SOURCES + ["-lpthread", "-lz"])
     oracle_deps = ORACLE_SRCS + _headers(ORACLE_DIR, build.INCLUDE, os.path.join(build.CSRC, "common"))
     if _stale(ORACLE_SAN, oracle_deps):
         os.makedirs(os.path.dirname(ORACLE_SAN), exist_ok=True)
