@@ -284,8 +284,45 @@ KD bool li_bounce(const SceneView& S, const FrameConst& fc, PathState& ps, uint3
             cv.trec = -1;
         }
     }
+    // The next-event estimate's random numbers are drawn here, before the
+    // continuation's, as the reference draws them; its shadow ray is traced
+    // after the continuation is sampled (its answer only picks the term added
+    // to L, and nothing else touches L in between), so the surface and BSDF
+    // are no longer live while it runs.
+    NeeResult ne;
+    ne.trace = false;
+    bool nee = false;
     if (!(CACHE && cv.to_cache && !cf->nee) && !(b.alpha < 0.001f)) {
-        NeeResult ne = next_event(S, fc, s, b, ps.ray.d, ps.beta, seed, cv.sun_p);
+        ne = next_event(S, fc, s, b, ps.ray.d, ps.beta, seed, cv.sun_p);
+        nee = true;
+    }
+    bool more = true;
+    if (ps.depth++ == fc.max_depth) more = false;
+    else if (CACHE && cv.to_cache) more = false;
+    else {
+        v3 wo = -ps.ray.d;
+        v2 u2, X2;
+        u2.x = rng1(seed); u2.y = rng1(seed);
+        X2.x = rng1(seed); X2.y = rng1(seed);
+        BSample bs = bsdf_sample(b, wo, u2, X2);
+        if (is_zero(bs.f)) more = false;
+        else {
+            ps.beta = ps.beta * (bs.f * absdot(s.n, bs.wi) / bs.pdf);
+            ps.p_b = bsdf_pdf(b, wo, bs.wi);
+            ps.any_nonspec = ps.any_nonspec || !((bs.scatter & SC_SPECULAR) != 0);
+            if (CACHE) ps.wide_in = ((bs.scatter & SC_DIFFUSE) != 0) || b.roughness >= cf->wide_roughness;
+            ps.prev_pos = s.pos;
+            v3 org = (bs.scatter & SC_TRANSMITTED) ? s.pos - (s.n * KPT_HIT_EPS) : s.pos + (s.n * KPT_HIT_EPS);
+            ps.ray = make_ray(org, bs.wi);
+            v3 rr = ps.beta * eta_scale;
+            if (max_comp(rr) < 1.0f && ps.depth > 1) {
+                float q = fmaxf_(0.0f, 1.0f - max_comp(rr));
+                if (rng1(seed) < q) more = false;
+                else ps.beta = ps.beta / (1.0f - q);
+            }
+        }
+    }
+    if (nee) {
         if (ne.trace) {
             bool oo = false;
             Ray sr = make_ray(ne.org, ne.dir);
@@ -298,28 +335,7 @@ KD bool li_bounce(const SceneView& S, const FrameConst& fc, PathState& ps, uint3
     }
     // a training record's radiance so far includes the vertex's estimate
     if (CACHE && cv.trec >= 0) cb->t_lbefore[cv.trec] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
-    if (ps.depth++ == fc.max_depth) return false;
-    if (CACHE && cv.to_cache) return false;
-    v3 wo = -ps.ray.d;
-    v2 u2, X2;
-    u2.x = rng1(seed); u2.y = rng1(seed);
-    X2.x = rng1(seed); X2.y = rng1(seed);
-    BSample bs = bsdf_sample(b, wo, u2, X2);
-    if (is_zero(bs.f)) return false;
-    ps.beta = ps.beta * (bs.f * absdot(s.n, bs.wi) / bs.pdf);
-    ps.p_b = bsdf_pdf(b, wo, bs.wi);
-    ps.any_nonspec = ps.any_nonspec || !((bs.scatter & SC_SPECULAR) != 0);
-    if (CACHE) ps.wide_in = ((bs.scatter & SC_DIFFUSE) != 0) || b.roughness >= cf->wide_roughness;
-    ps.prev_pos = s.pos;
-    v3 org = (bs.scatter & SC_TRANSMITTED) ? s.pos - (s.n * KPT_HIT_EPS) : s.pos + (s.n * KPT_HIT_EPS);
-    ps.ray = make_ray(org, bs.wi);
-    v3 rr = ps.beta * eta_scale;
-    if (max_comp(rr) < 1.0f && ps.depth > 1) {
-        float q = fmaxf_(0.0f, 1.0f - max_comp(rr));
-        if (rng1(seed) < q) return false;
-        ps.beta = ps.beta / (1.0f - q);
-    }
-    return true;
+    return more;
 }
 
 // Li from a path state to the end of the path, in one thread (the megakernel

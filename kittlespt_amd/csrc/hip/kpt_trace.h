@@ -158,7 +158,12 @@ struct LaneStack {
         return sp < depth ? lds[sp * stride] : spill[sp - depth];
     }
     KD void reset(const StackShape& sh, uint32_t* spill_) {
-        lds = (lds_u32*)sh.lds; spill = spill_; stride = sh.stride; depth = sh.depth; sp = 0; overflow = false;
+        // stride and depth are the same for every lane: read them as such, so
+        // they stay in scalar registers wherever the traversal runs
+        lds = (lds_u32*)sh.lds; spill = spill_;
+        stride = __builtin_amdgcn_readfirstlane(sh.stride);
+        depth = __builtin_amdgcn_readfirstlane(sh.depth);
+        sp = 0; overflow = false;
     }
 };
 
