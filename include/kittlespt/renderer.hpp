@@ -255,6 +255,10 @@ public:
         return out;
     }
     void setSeedMode(int mode) { m_settings.seed_mode = mode; push(); }
+    // The reference chooses its upscaler at compile time (USE_FSR,
+    // kernels.cuh:3; off by default): with upscale_enable, true selects FSR
+    // 1.0 (EASU + RCAS) instead of the bilinear upscale.
+    void setFsrUpscale(bool fsr) { m_settings.upscale_fsr = fsr ? 1 : 0; push(); }
 
 private:
     // buffers 0-3 at the render size, 4 (output) at the output size

@@ -160,7 +160,13 @@ typedef struct kpt_settings {
      * upscale.cuh:260-298 without USE_FSR); the render size
      * takes effect at the next kpt_resize (the reference's resizeResolution). */
     int32_t upscale_enable;
-    int32_t reserved;
+    /* With upscale_enable: FSR 1.0 instead of the bilinear upscale — EASU
+     * from the render size to the output size, then RCAS (sharpness 0), as
+     * the reference's USE_FSR build does (upscale.cuh:13-408, kernels.cu:
+     * 84-103, renderer.cu:138-147; compiled out there by default,
+     * kernels.cuh:3); 0/1.  The reference's inversesqrt (rsqrtf) is taken
+     * as the IEEE 1/sqrtf. */
+    int32_t upscale_fsr;
 } kpt_settings;
 
 /* Camera exposure as Renderer::setExposure takes it (ExposureValues,

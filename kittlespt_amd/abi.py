@@ -145,7 +145,7 @@ class KptSettings(C.Structure):
         ("integrator", C.c_int32),
         ("postprocess_enable_effects", C.c_int32),
         ("upscale_enable", C.c_int32),
-        ("reserved", C.c_int32),
+        ("upscale_fsr", C.c_int32),
     ]
 
 

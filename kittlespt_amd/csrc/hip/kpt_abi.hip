@@ -1203,7 +1203,7 @@ int kpt_set_settings(kpt_ctx* ctx, const kpt_settings* s) {
     if (!ctx || !s) return KPT_ERR_INVALID_ARG;
     if (s->max_depth < 0 || s->seed_mode < 0 || s->seed_mode > 1 || s->integrator < 0 || s->integrator > 1 ||
         s->postprocess_enable_effects < 0 || s->postprocess_enable_effects > 1 || s->upscale_enable < 0 ||
-        s->upscale_enable > 1)
+        s->upscale_enable > 1 || s->upscale_fsr < 0 || s->upscale_fsr > 1)
         return fail(ctx, KPT_ERR_INVALID_ARG, "invalid settings");
     ctx->settings = *s;
     return kpt_reset_accumulation(ctx);   // renderer.cu:301-305
@@ -1543,6 +1543,7 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
             pp.out_height = ctx->out_height;
             pp.upscale = (ctx->settings.upscale_enable || ctx->out_width != ctx->width ||
                           ctx->out_height != ctx->height) ? 1 : 0;
+            pp.fsr = (pp.upscale && ctx->settings.upscale_fsr) ? 1 : 0;
             e = launch_post_frame(ctx->stream, ctx->fb, ctx->pb, pp, ctx->num_cus);
             if (e != hipSuccess) return e;
             // end of executeRendering: gbuffer -> prev_gbuffer (renderer.cu:186)

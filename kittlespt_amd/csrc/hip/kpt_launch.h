@@ -88,6 +88,7 @@ struct PostParams {
     int32_t effects;                // fisheye + chromatic aberration from pb.back into fb.output
     int32_t out_width, out_height;  // output size (width x height is the render size)
     int32_t upscale;                // bilinear upscale render -> pb.up, which bloom and post read
+    int32_t fsr;                    // with upscale: FSR 1.0 (EASU render -> pb.back, RCAS -> pb.up)
 };
 // executeRendering after the path trace (renderer.cu:117-170): modulate,
 // auto exposure, bloom, post.  Everything stays on the stream.

@@ -335,6 +335,158 @@ __global__ __launch_bounds__(POST_BLOCK) void k_upscale(FrameBuffers fb, PostBuf
     if (p.bloom) pb.mip[0][i] = c;
 }
 
+// ------------------------------------------------------------ FSR 1.0
+// The reference's USE_FSR build of the upscale (upscale.cuh:13-258 EASU,
+// 302-378 RCAS; kernels.cu:84-103 runs EASU render -> output and RCAS output
+// -> back buffer, renderer.cu:138-147 copies the back buffer to the output).
+// Here EASU writes pb.back (free until the post pass) and RCAS writes the
+// upscaled output texture pb.up (and bloom's mip 0).  Float vector
+// arithmetic as vector_types_extension.cuh defines it, the reference's double
+// literals promoted as written, inversesqrt taken as the IEEE 1/sqrtf, RCAS
+// with sharpness 0 (con = 1) and no denoise term; one thread per output
+// pixel, each tap a bilinear read of the clamped source.
+struct f2 {
+    float x, y;
+};
+__device__ __forceinline__ c3 easu_cf(float px, float py, const Tex& t) {   // textureReadBilinearUV(p, true)
+    const c4 c = t.bilinear(px * (float)t.w, py * (float)t.h);
+    return C3(c.x, c.y, c.z);
+}
+__device__ __forceinline__ float easu_luma(c3 c) { return (float)((double)c.y + 0.5 * (double)(c.x + c.z)); }
+__device__ __forceinline__ float hm_clamp(float f, float a, float b) { return fmaxf(a, fminf(f, b)); }
+// FsrEasuSetF (upscale.cuh:103-130)
+__device__ __forceinline__ void easu_set(f2& dir, float& len, float w, float lA, float lB, float lC, float lD,
+                                         float lE) {
+    float lenX = fmaxf(fabsf(lD - lC), fabsf(lC - lB));
+    const float dirX = lD - lB;
+    dir.x += dirX * w;
+    lenX = hm_clamp(fabsf(dirX) / lenX, 0.0f, 1.0f);
+    lenX *= lenX;
+    len += lenX * w;
+    float lenY = fmaxf(fabsf(lE - lC), fabsf(lC - lA));
+    const float dirY = lE - lA;
+    dir.y += dirY * w;
+    lenY = hm_clamp(fabsf(dirY) / lenY, 0.0f, 1.0f);
+    lenY *= lenY;
+    len += lenY * w;
+}
+// FsrEasuTapF (upscale.cuh:64-101)
+__device__ __forceinline__ void easu_tap(c3& aC, float& aW, f2 off, f2 dir, f2 len, float lob, float clp, c3 c) {
+    float vx = off.x * dir.x + off.y * dir.y, vy = off.x * -dir.y + off.y * dir.x;
+    vx *= len.x;
+    vy *= len.y;
+    const float d2 = fminf(vx * vx + vy * vy, clp);
+    float wB = (float)(0.4 * (double)d2 - 1.0);
+    float wA = (float)((double)(lob * d2) - 1.0);
+    wB *= wB;
+    wA *= wA;
+    wB = (float)(1.5625 * (double)wB - 0.5625);
+    const float w = wB * wA;
+    aC = C3(aC.x + c.x * w, aC.y + c.y * w, aC.z + c.z * w);
+    aW += w;
+}
+
+// FsrEasuCon + FsrEasuF (upscale.cuh:18-62, 132-258): render (width x height)
+// -> pb.back (out_width x out_height).
+__global__ __launch_bounds__(POST_BLOCK) void k_fsr_easu(FrameBuffers fb, PostBuffers pb, PostParams p) {
+    const int64_t npx = (int64_t)p.out_width * p.out_height;
+    const int64_t i = (int64_t)blockIdx.x * POST_BLOCK + threadIdx.x;
+    if (i >= npx) return;
+    const int x = (int)(i % p.out_width), y = (int)(i / p.out_width);
+    const Tex t{fb.render, p.width, p.height};
+    const float iw = (float)p.width, ih = (float)p.height, ow = (float)p.out_width, oh = (float)p.out_height;
+    const float c0x = iw / ow, c0y = ih / oh;
+    const float c0z = (float)(0.5 * (double)iw / (double)ow - 0.5), c0w = (float)(0.5 * (double)ih / (double)oh - 0.5);
+    const float c1x = 1.0f / iw, c1y = 1.0f / ih, c1z = 1.0f / iw, c1w = -1.0f / ih;
+    const float c2x = -1.0f / iw, c2y = 2.0f / ih, c2z = 1.0f / iw, c2w = 2.0f / ih;
+    const float c3x = 0.0f / iw, c3y = 4.0f / ih;
+    f2 pp{(float)x * c0x + c0z, (float)y * c0y + c0w};
+    const f2 fp{floorf(pp.x), floorf(pp.y)};
+    pp = f2{pp.x - fp.x, pp.y - fp.y};
+    const f2 p0{fp.x * c1x + c1z, fp.y * c1y + c1w};
+    const f2 p1{p0.x + c2x, p0.y + c2y}, p2{p0.x + c2z, p0.y + c2w}, p3{p0.x + c3x, p0.y + c3y};
+    const float ox = -0.5f * c1x, oy = 0.5f * c1x, oz = -0.5f * c1y, ow_ = 0.5f * c1y;
+    const c3 bC = easu_cf(p0.x + ox, p0.y + ow_, t), cC = easu_cf(p0.x + oy, p0.y + ow_, t);
+    const c3 iC = easu_cf(p1.x + ox, p1.y + ow_, t), jC = easu_cf(p1.x + oy, p1.y + ow_, t);
+    const c3 fC = easu_cf(p1.x + oy, p1.y + oz, t), eC = easu_cf(p1.x + ox, p1.y + oz, t);
+    const c3 kC = easu_cf(p2.x + ox, p2.y + ow_, t), lC = easu_cf(p2.x + oy, p2.y + ow_, t);
+    const c3 hC = easu_cf(p2.x + oy, p2.y + oz, t), gC = easu_cf(p2.x + ox, p2.y + oz, t);
+    const c3 oC = easu_cf(p3.x + oy, p3.y + oz, t), nC = easu_cf(p3.x + ox, p3.y + oz, t);
+    const float bL = easu_luma(bC), cL = easu_luma(cC), iL = easu_luma(iC), jL = easu_luma(jC),
+                fL = easu_luma(fC), eL = easu_luma(eC), kL = easu_luma(kC), lL = easu_luma(lC),
+                hL = easu_luma(hC), gL = easu_luma(gC), oL = easu_luma(oC), nL = easu_luma(nC);
+    f2 dir{0.0f, 0.0f};
+    float len = 0.0f;
+    easu_set(dir, len, (float)((1.0 - (double)pp.x) * (1.0 - (double)pp.y)), bL, eL, fL, gL, jL);
+    easu_set(dir, len, (float)((double)pp.x * (1.0 - (double)pp.y)), cL, fL, gL, hL, kL);
+    easu_set(dir, len, (float)((1.0 - (double)pp.x) * (double)pp.y), fL, iL, jL, kL, nL);
+    easu_set(dir, len, pp.x * pp.y, gL, jL, kL, lL, oL);
+    float dirR = dir.x * dir.x + dir.y * dir.y;
+    const bool zro = (double)dirR < (1.0 / 32768.0);
+    dirR = 1.0f / sqrtf(dirR);
+    dirR = zro ? 1.0f : dirR;
+    dir.x = zro ? 1.0f : dir.x;
+    dir = f2{dir.x * dirR, dir.y * dirR};
+    len = (float)((double)len * 0.5);
+    len *= len;
+    const float stretch = (dir.x * dir.x + dir.y * dir.y) / fmaxf(fabsf(dir.x), fabsf(dir.y));
+    const f2 len2{(float)(1.0 + ((double)stretch - 1.0) * (double)len), (float)(1.0 - 0.5 * (double)len)};
+    const float lob = (float)(0.5 - 0.29 * (double)len);
+    const float clp = (float)(1.0 / (double)lob);
+    const c3 min4 = C3(fminf(fminf(fC.x, gC.x), fminf(jC.x, kC.x)), fminf(fminf(fC.y, gC.y), fminf(jC.y, kC.y)),
+                       fminf(fminf(fC.z, gC.z), fminf(jC.z, kC.z)));
+    const c3 max4 = C3(fmaxf(fmaxf(fC.x, gC.x), fmaxf(jC.x, kC.x)), fmaxf(fmaxf(fC.y, gC.y), fmaxf(jC.y, kC.y)),
+                       fmaxf(fmaxf(fC.z, gC.z), fmaxf(jC.z, kC.z)));
+    c3 aC = C3(0.0f, 0.0f, 0.0f);
+    float aW = 0.0f;
+    easu_tap(aC, aW, f2{0.0f - pp.x, -1.0f - pp.y}, dir, len2, lob, clp, bC);
+    easu_tap(aC, aW, f2{1.0f - pp.x, -1.0f - pp.y}, dir, len2, lob, clp, cC);
+    easu_tap(aC, aW, f2{-1.0f - pp.x, 1.0f - pp.y}, dir, len2, lob, clp, iC);
+    easu_tap(aC, aW, f2{0.0f - pp.x, 1.0f - pp.y}, dir, len2, lob, clp, jC);
+    easu_tap(aC, aW, f2{0.0f - pp.x, 0.0f - pp.y}, dir, len2, lob, clp, fC);
+    easu_tap(aC, aW, f2{-1.0f - pp.x, 0.0f - pp.y}, dir, len2, lob, clp, eC);
+    easu_tap(aC, aW, f2{1.0f - pp.x, 1.0f - pp.y}, dir, len2, lob, clp, kC);
+    easu_tap(aC, aW, f2{2.0f - pp.x, 1.0f - pp.y}, dir, len2, lob, clp, lC);
+    easu_tap(aC, aW, f2{2.0f - pp.x, 0.0f - pp.y}, dir, len2, lob, clp, hC);
+    easu_tap(aC, aW, f2{1.0f - pp.x, 0.0f - pp.y}, dir, len2, lob, clp, gC);
+    easu_tap(aC, aW, f2{1.0f - pp.x, 2.0f - pp.y}, dir, len2, lob, clp, oC);
+    easu_tap(aC, aW, f2{0.0f - pp.x, 2.0f - pp.y}, dir, len2, lob, clp, nC);
+    const float rx = aC.x / aW, ry = aC.y / aW, rz = aC.z / aW;
+    pb.back[i] = make_float4(fminf(max4.x, fmaxf(min4.x, rx)), fminf(max4.y, fmaxf(min4.y, ry)),
+                             fminf(max4.z, fmaxf(min4.z, rz)), 1.0f);
+}
+
+// FsrRcasF with con = exp2(-0) = 1 (upscale.cuh:316-378, 389-408): pb.back ->
+// pb.up (and bloom's mip 0), output size.
+__global__ __launch_bounds__(POST_BLOCK) void k_fsr_rcas(PostBuffers pb, PostParams p) {
+    const int64_t npx = (int64_t)p.out_width * p.out_height;
+    const int64_t i = (int64_t)blockIdx.x * POST_BLOCK + threadIdx.x;
+    if (i >= npx) return;
+    const int x = (int)(i % p.out_width), y = (int)(i / p.out_width);
+    const Tex t{pb.back, p.out_width, p.out_height};
+    const float w = (float)p.out_width, h = (float)p.out_height;
+    auto load = [&](float px, float py) {   // textureReadBilinearUV(p / dimensions, true)
+        const float u = px / w, v = py / h;
+        const c4 c = t.bilinear(u * w, v * h);
+        return C3(c.x, c.y, c.z);
+    };
+    const float sx = (float)x, sy = (float)y;
+    const c3 b = load(sx + 0.0f, sy + -1.0f), d = load(sx + -1.0f, sy + 0.0f), e = load(sx, sy),
+             f = load(sx + 1.0f, sy + 0.0f), hh = load(sx + 0.0f, sy + 1.0f);
+    const c3 mn4 = C3(fminf(b.x, fminf(f.x, hh.x)), fminf(b.y, fminf(f.y, hh.y)), fminf(b.z, fminf(f.z, hh.z)));
+    const c3 mx4 = C3(fmaxf(b.x, fmaxf(f.x, hh.x)), fmaxf(b.y, fmaxf(f.y, hh.y)), fmaxf(b.z, fmaxf(f.z, hh.z)));
+    const c3 hitMin = C3(mn4.x / (4.0f * mx4.x), mn4.y / (4.0f * mx4.y), mn4.z / (4.0f * mx4.z));
+    const c3 hitMax = C3((1.0f - mx4.x) / (4.0f * mn4.x + -4.0f), (1.0f - mx4.y) / (4.0f * mn4.y + -4.0f),
+                         (1.0f - mx4.z) / (4.0f * mn4.z + -4.0f));
+    const float m = fmaxf(fmaxf(-hitMin.x, hitMax.x), fmaxf(fmaxf(-hitMin.y, hitMax.y), fmaxf(-hitMin.z, hitMax.z)));
+    const float lobe = (float)(fmax(-(0.25 - (1.0 / 16.0)), fmin((double)m, 0.0)) * 1.0);
+    const c3 s = C3(((b.x + d.x) + hh.x) + f.x, ((b.y + d.y) + hh.y) + f.y, ((b.z + d.z) + hh.z) + f.z);
+    const float den = (float)(4.0 * (double)lobe + 1.0);
+    const float4 o = make_float4((lobe * s.x + e.x) / den, (lobe * s.y + e.y) / den, (lobe * s.z + e.z) / den, 1.0f);
+    pb.up[i] = o;
+    if (p.bloom) pb.mip[0][i] = o;
+}
+
 // ------------------------------------------------------------ post
 // computePostProcess (kernels.cu:475-504): bloom blend, exposure in Yxy, AgX,
 // at the output size over the output texture (the render buffer, or its
@@ -440,7 +592,12 @@ hipError_t launch_post_frame(hipStream_t s, const FrameBuffers& fb, const PostBu
     // from here on the output size: bloom, post and effects
     const int64_t onpx = (int64_t)p.out_width * p.out_height;
     const unsigned oblocks = (unsigned)((onpx + POST_BLOCK - 1) / POST_BLOCK);
-    if (p.upscale) hipLaunchKernelGGL(k_upscale, dim3(oblocks), dim3(POST_BLOCK), 0, s, fb, pb, p);
+    if (p.upscale && p.fsr) {
+        hipLaunchKernelGGL(k_fsr_easu, dim3(oblocks), dim3(POST_BLOCK), 0, s, fb, pb, p);
+        hipLaunchKernelGGL(k_fsr_rcas, dim3(oblocks), dim3(POST_BLOCK), 0, s, pb, p);
+    } else if (p.upscale) {
+        hipLaunchKernelGGL(k_upscale, dim3(oblocks), dim3(POST_BLOCK), 0, s, fb, pb, p);
+    }
     const int OW = p.out_width, OH = p.out_height;
     if (p.bloom) {
         for (int k = 0; k < pb.levels; ++k) {

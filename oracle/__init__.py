@@ -313,6 +313,28 @@ def post_frame(estimate, gbuffer, settings, exposure, film, frame_delta_ms=16.0,
     return render, out, bloom, hist
 
 
+def upscale_fsr(src, out_w, out_h, easu=False):
+    """FSR 1.0 (the reference's USE_FSR build, upscale.cuh:13-408) of an
+    [H, W, 4] image to [out_h, out_w, 4]: EASU then RCAS; with easu=True also
+    returns the EASU pass alone."""
+    s = _f32(src)
+    H, W = s.shape[:2]
+    out = np.zeros((out_h, out_w, 4), np.float32)
+    e = np.zeros((out_h, out_w, 4), np.float32) if easu else None
+    _sig("orc_upscale_fsr", None, [C.c_void_p] + [C.c_int32] * 4 + [C.c_void_p, C.c_void_p])(
+        _ptr(s), W, H, out_w, out_h, _ptr(out), _ptr(e) if easu else None)
+    return (out, e) if easu else out
+
+
+def fsr_rcas(src):
+    """The RCAS pass alone (upscale.cuh:316-378, sharpness 0) of an [H, W, 4] image."""
+    s = _f32(src)
+    H, W = s.shape[:2]
+    out = np.zeros((H, W, 4), np.float32)
+    _sig("orc_fsr_rcas", None, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p])(_ptr(s), W, H, _ptr(out))
+    return out
+
+
 def upscale_bilinear(src, out_w, out_h):
     """upscale (upscale.cuh:287-292) of an [H, W, 4] image to [out_h, out_w, 4]."""
     s = _f32(src)

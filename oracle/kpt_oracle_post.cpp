@@ -368,6 +368,162 @@ using namespace orp;
 
 extern "C" {
 
+// ----------------------------------------------------------------- FSR 1.0
+// The reference's USE_FSR build of the upscale pass (upscale.cuh:13-258 EASU,
+// 260-298 upscale, 302-378 RCAS, 389-408 sharpen; kernels.cu:84-103 runs EASU
+// render -> output, RCAS output -> back buffer; renderer.cu:138-147 copies the
+// back buffer to the output), restated expression by expression: float
+// vector arithmetic as vector_types_extension.cuh defines it, the double
+// literals' promotions kept, inversesqrt as the IEEE 1/sqrtf (the reference's
+// rsqrtf is an approximation), RCAS without FSR_RCAS_DENOISE and sharpness 0
+// (con = exp2(-0) = 1).
+struct F2 {
+    float x, y;
+};
+static F3 EasuCF(F2 p, const Img& t) {   // textureReadBilinearUV(p, true): p * dimensions
+    const F4 c = t.bilinear(p.x * (float)t.w, p.y * (float)t.h, true);
+    return F3{c.x, c.y, c.z};
+}
+static float EasuLuma(F3 c) { return (float)((double)c.y + 0.5 * (double)(c.x + c.z)); }
+static float HmClamp(float f, float a, float b) { return std::fmax(a, std::fmin(f, b)); }   // helper clamp
+// FsrEasuSetF (upscale.cuh:103-130)
+static void EasuSet(F2& dir, float& len, float w, float lA, float lB, float lC, float lD, float lE) {
+    float lenX = std::fmax(std::fabs(lD - lC), std::fabs(lC - lB));
+    const float dirX = lD - lB;
+    dir.x += dirX * w;
+    lenX = HmClamp(std::fabs(dirX) / lenX, 0.0f, 1.0f);
+    lenX *= lenX;
+    len += lenX * w;
+    float lenY = std::fmax(std::fabs(lE - lC), std::fabs(lC - lA));
+    const float dirY = lE - lA;
+    dir.y += dirY * w;
+    lenY = HmClamp(std::fabs(dirY) / lenY, 0.0f, 1.0f);
+    lenY *= lenY;
+    len += lenY * w;
+}
+// FsrEasuTapF (upscale.cuh:64-101)
+static void EasuTap(F3& aC, float& aW, F2 off, F2 dir, F2 len, float lob, float clp, F3 c) {
+    F2 v{off.x * dir.x + off.y * dir.y, off.x * -dir.y + off.y * dir.x};
+    v.x *= len.x;
+    v.y *= len.y;
+    const float d2 = std::fmin(v.x * v.x + v.y * v.y, clp);
+    float wB = (float)(0.4 * (double)d2 - 1.0);
+    float wA = (float)((double)(lob * d2) - 1.0);
+    wB *= wB;
+    wA *= wA;
+    wB = (float)(1.5625 * (double)wB - 0.5625);
+    const float w = wB * wA;
+    aC = F3{aC.x + c.x * w, aC.y + c.y * w, aC.z + c.z * w};
+    aW += w;
+}
+// FsrEasuCon + FsrEasuF for output pixel (x, y) of an OW x OH upscale of t.
+static F3 EasuPixel(const Img& t, int OW, int OH, int x, int y) {
+    const float iw = (float)t.w, ih = (float)t.h, ow = (float)OW, oh = (float)OH;
+    const float c0x = iw / ow, c0y = ih / oh;
+    const float c0z = (float)(0.5 * (double)iw / (double)ow - 0.5), c0w = (float)(0.5 * (double)ih / (double)oh - 0.5);
+    const float c1x = 1.0f / iw, c1y = 1.0f / ih, c1z = 1.0f / iw, c1w = -1.0f / ih;
+    const float c2x = -1.0f / iw, c2y = 2.0f / ih, c2z = 1.0f / iw, c2w = 2.0f / ih;
+    const float c3x = 0.0f / iw, c3y = 4.0f / ih;
+    F2 pp{(float)x * c0x + c0z, (float)y * c0y + c0w};
+    const F2 fp{std::floor(pp.x), std::floor(pp.y)};
+    pp = F2{pp.x - fp.x, pp.y - fp.y};
+    const F2 p0{fp.x * c1x + c1z, fp.y * c1y + c1w};
+    const F2 p1{p0.x + c2x, p0.y + c2y}, p2{p0.x + c2z, p0.y + c2w}, p3{p0.x + c3x, p0.y + c3y};
+    const float ox = -0.5f * c1x, oy = 0.5f * c1x, oz = -0.5f * c1y, ow_ = 0.5f * c1y;
+    const F3 bC = EasuCF(F2{p0.x + ox, p0.y + ow_}, t), cC = EasuCF(F2{p0.x + oy, p0.y + ow_}, t);
+    const F3 iC = EasuCF(F2{p1.x + ox, p1.y + ow_}, t), jC = EasuCF(F2{p1.x + oy, p1.y + ow_}, t);
+    const F3 fC = EasuCF(F2{p1.x + oy, p1.y + oz}, t), eC = EasuCF(F2{p1.x + ox, p1.y + oz}, t);
+    const F3 kC = EasuCF(F2{p2.x + ox, p2.y + ow_}, t), lC = EasuCF(F2{p2.x + oy, p2.y + ow_}, t);
+    const F3 hC = EasuCF(F2{p2.x + oy, p2.y + oz}, t), gC = EasuCF(F2{p2.x + ox, p2.y + oz}, t);
+    const F3 oC = EasuCF(F2{p3.x + oy, p3.y + oz}, t), nC = EasuCF(F2{p3.x + ox, p3.y + oz}, t);
+    const float bL = EasuLuma(bC), cL = EasuLuma(cC), iL = EasuLuma(iC), jL = EasuLuma(jC), fL = EasuLuma(fC),
+                eL = EasuLuma(eC), kL = EasuLuma(kC), lL = EasuLuma(lC), hL = EasuLuma(hC), gL = EasuLuma(gC),
+                oL = EasuLuma(oC), nL = EasuLuma(nC);
+    F2 dir{0.0f, 0.0f};
+    float len = 0.0f;
+    EasuSet(dir, len, (float)((1.0 - (double)pp.x) * (1.0 - (double)pp.y)), bL, eL, fL, gL, jL);
+    EasuSet(dir, len, (float)((double)pp.x * (1.0 - (double)pp.y)), cL, fL, gL, hL, kL);
+    EasuSet(dir, len, (float)((1.0 - (double)pp.x) * (double)pp.y), fL, iL, jL, kL, nL);
+    EasuSet(dir, len, pp.x * pp.y, gL, jL, kL, lL, oL);
+    const F2 dir2{dir.x * dir.x, dir.y * dir.y};
+    float dirR = dir2.x + dir2.y;
+    const bool zro = (double)dirR < (1.0 / 32768.0);
+    dirR = 1.0f / std::sqrt(dirR);
+    dirR = zro ? 1.0f : dirR;
+    dir.x = zro ? 1.0f : dir.x;
+    dir = F2{dir.x * dirR, dir.y * dirR};
+    len = (float)((double)len * 0.5);
+    len *= len;
+    const float stretch = (dir.x * dir.x + dir.y * dir.y) / std::fmax(std::fabs(dir.x), std::fabs(dir.y));
+    const F2 len2{(float)(1.0 + ((double)stretch - 1.0) * (double)len), (float)(1.0 - 0.5 * (double)len)};
+    const float lob = (float)(0.5 - 0.29 * (double)len);
+    const float clp = (float)(1.0 / (double)lob);
+    const F3 min4{std::fmin(std::fmin(fC.x, gC.x), std::fmin(jC.x, kC.x)),
+                  std::fmin(std::fmin(fC.y, gC.y), std::fmin(jC.y, kC.y)),
+                  std::fmin(std::fmin(fC.z, gC.z), std::fmin(jC.z, kC.z))};
+    const F3 max4{std::fmax(std::fmax(fC.x, gC.x), std::fmax(jC.x, kC.x)),
+                  std::fmax(std::fmax(fC.y, gC.y), std::fmax(jC.y, kC.y)),
+                  std::fmax(std::fmax(fC.z, gC.z), std::fmax(jC.z, kC.z))};
+    F3 aC{0.0f, 0.0f, 0.0f};
+    float aW = 0.0f;
+    EasuTap(aC, aW, F2{0.0f - pp.x, -1.0f - pp.y}, dir, len2, lob, clp, bC);
+    EasuTap(aC, aW, F2{1.0f - pp.x, -1.0f - pp.y}, dir, len2, lob, clp, cC);
+    EasuTap(aC, aW, F2{-1.0f - pp.x, 1.0f - pp.y}, dir, len2, lob, clp, iC);
+    EasuTap(aC, aW, F2{0.0f - pp.x, 1.0f - pp.y}, dir, len2, lob, clp, jC);
+    EasuTap(aC, aW, F2{0.0f - pp.x, 0.0f - pp.y}, dir, len2, lob, clp, fC);
+    EasuTap(aC, aW, F2{-1.0f - pp.x, 0.0f - pp.y}, dir, len2, lob, clp, eC);
+    EasuTap(aC, aW, F2{1.0f - pp.x, 1.0f - pp.y}, dir, len2, lob, clp, kC);
+    EasuTap(aC, aW, F2{2.0f - pp.x, 1.0f - pp.y}, dir, len2, lob, clp, lC);
+    EasuTap(aC, aW, F2{2.0f - pp.x, 0.0f - pp.y}, dir, len2, lob, clp, hC);
+    EasuTap(aC, aW, F2{1.0f - pp.x, 0.0f - pp.y}, dir, len2, lob, clp, gC);
+    EasuTap(aC, aW, F2{1.0f - pp.x, 2.0f - pp.y}, dir, len2, lob, clp, oC);
+    EasuTap(aC, aW, F2{0.0f - pp.x, 2.0f - pp.y}, dir, len2, lob, clp, nC);
+    const F3 r{aC.x / aW, aC.y / aW, aC.z / aW};
+    return F3{std::fmin(max4.x, std::fmax(min4.x, r.x)), std::fmin(max4.y, std::fmax(min4.y, r.y)),
+              std::fmin(max4.z, std::fmax(min4.z, r.z))};
+}
+// FsrRcasF (upscale.cuh:316-378) at output pixel (x, y) of t, con = 1.
+static F3 RcasPixel(const Img& t, int x, int y) {
+    const float w = (float)t.w, h = (float)t.h;
+    auto load = [&](float px, float py) {   // textureReadBilinearUV(p / dimensions, true)
+        const float u = px / w, v = py / h;
+        const F4 c = t.bilinear(u * w, v * h, true);
+        return F3{c.x, c.y, c.z};
+    };
+    const float sx = (float)x, sy = (float)y;
+    const F3 b = load(sx + 0.0f, sy + -1.0f), d = load(sx + -1.0f, sy + 0.0f), e = load(sx, sy),
+             f = load(sx + 1.0f, sy + 0.0f), hh = load(sx + 0.0f, sy + 1.0f);
+    const F3 mn4{std::fmin(b.x, std::fmin(f.x, hh.x)), std::fmin(b.y, std::fmin(f.y, hh.y)),
+                 std::fmin(b.z, std::fmin(f.z, hh.z))};
+    const F3 mx4{std::fmax(b.x, std::fmax(f.x, hh.x)), std::fmax(b.y, std::fmax(f.y, hh.y)),
+                 std::fmax(b.z, std::fmax(f.z, hh.z))};
+    const F3 hitMin{mn4.x / (4.0f * mx4.x), mn4.y / (4.0f * mx4.y), mn4.z / (4.0f * mx4.z)};
+    const F3 hitMax{(1.0f - mx4.x) / (4.0f * mn4.x + -4.0f), (1.0f - mx4.y) / (4.0f * mn4.y + -4.0f),
+                    (1.0f - mx4.z) / (4.0f * mn4.z + -4.0f)};
+    const F3 lobeRGB{std::fmax(-hitMin.x, hitMax.x), std::fmax(-hitMin.y, hitMax.y), std::fmax(-hitMin.z, hitMax.z)};
+    const float m = std::fmax(lobeRGB.x, std::fmax(lobeRGB.y, lobeRGB.z));
+    const float lobe = (float)(std::fmax(-(0.25 - (1.0 / 16.0)), std::fmin((double)m, 0.0)) * 1.0);
+    const F3 s{((b.x + d.x) + hh.x) + f.x, ((b.y + d.y) + hh.y) + f.y, ((b.z + d.z) + hh.z) + f.z};
+    const float den = (float)(4.0 * (double)lobe + 1.0);
+    return F3{(lobe * s.x + e.x) / den, (lobe * s.y + e.y) / den, (lobe * s.z + e.z) / den};
+}
+// EASU of src (W x H) to OW x OH, then RCAS of that; out OW x OH RGBA (alpha 1).
+static void FsrUpscale(const Img& src, int OW, int OH, std::vector<F4>& out) {
+    std::vector<F4> easu((size_t)OW * OH);
+    for (int y = 0; y < OH; ++y)
+        for (int x = 0; x < OW; ++x) {
+            const F3 c = EasuPixel(src, OW, OH, x, y);
+            easu[(size_t)y * OW + x] = F4{c.x, c.y, c.z, 1.0f};
+        }
+    const Img e{easu.data(), OW, OH};
+    out.resize((size_t)OW * OH);
+    for (int y = 0; y < OH; ++y)
+        for (int x = 0; x < OW; ++x) {
+            const F3 c = RcasPixel(e, x, y);
+            out[(size_t)y * OW + x] = F4{c.x, c.y, c.z, 1.0f};
+        }
+}
+
 // executeRendering's post chain with the render size W x H and the output
 // size OW x OH: modulation and auto exposure at the render size; the output
 // texture is the render buffer (same size, no upscale) or its bilinear upscale
@@ -402,7 +558,10 @@ void orc_post_frame_sized(const float* estimate, const uint32_t* gbuffer, int32_
     const bool up = upscale || OW != W || OH != H;
     const size_t no = (size_t)OW * OH;
     std::vector<F4> outtex(no);
-    if (up) {
+    if (up && s->upscale_fsr) {
+        Img src{render.data(), W, H};
+        FsrUpscale(src, OW, OH, outtex);
+    } else if (up) {
         Img src{render.data(), W, H};
         for (int y = 0; y < OH; ++y)
             for (int x = 0; x < OW; ++x) {
@@ -432,6 +591,37 @@ void orc_post_frame(const float* estimate, const uint32_t* gbuffer, int32_t W, i
                     float* out_output, float* out_bloom, uint64_t* out_hist) {
     orc_post_frame_sized(estimate, gbuffer, W, H, W, H, 0, s, frame_delta_ms, exposure, film, out_render, out_output,
                          out_bloom, out_hist);
+}
+
+// FSR 1.0 alone (EASU then RCAS, the USE_FSR build) of a W x H RGBA image to
+// OW x OH; out_easu (optional) receives the EASU pass before sharpening.
+void orc_upscale_fsr(const float* src, int32_t W, int32_t H, int32_t OW, int32_t OH, float* out, float* out_easu) {
+    std::vector<F4> s4((size_t)W * H);
+    std::memcpy(s4.data(), src, s4.size() * 16);
+    Img img{s4.data(), W, H};
+    std::vector<F4> o;
+    FsrUpscale(img, OW, OH, o);
+    std::memcpy(out, o.data(), o.size() * 16);
+    if (out_easu)
+        for (int y = 0; y < OH; ++y)
+            for (int x = 0; x < OW; ++x) {
+                const F3 c = EasuPixel(img, OW, OH, x, y);
+                const float v[4] = {c.x, c.y, c.z, 1.0f};
+                std::memcpy(out_easu + 4 * ((size_t)y * OW + x), v, 16);
+            }
+}
+
+// The RCAS pass alone (upscale.cuh:316-378, con = 1) of a W x H RGBA image.
+void orc_fsr_rcas(const float* src, int32_t W, int32_t H, float* out) {
+    std::vector<F4> s4((size_t)W * H);
+    std::memcpy(s4.data(), src, s4.size() * 16);
+    Img img{s4.data(), W, H};
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const F3 c = RcasPixel(img, x, y);
+            const float v[4] = {c.x, c.y, c.z, 1.0f};
+            std::memcpy(out + 4 * ((size_t)y * W + x), v, 16);
+        }
 }
 
 // The upscale alone (upscale.cuh:287-292) of a W x H RGBA image to OW x OH.

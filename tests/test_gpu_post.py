@@ -97,17 +97,19 @@ def test_effects_pass_parity(plain_ctx, W, H):
     assert not np.array_equal(outs[1], outs[0])          # the pass changed the image
 
 
-@pytest.mark.parametrize("ae,bloom", [(0, 0), (0, 1), (1, 1)])
-def test_upscale_frames_parity(plain_ctx, ae, bloom):
+@pytest.mark.parametrize("ae,bloom,fsr", [(0, 0, 0), (0, 1, 0), (1, 1, 0), (0, 0, 1), (1, 1, 1)])
+def test_upscale_frames_parity(plain_ctx, ae, bloom, fsr):
     """upscale_enable: the path trace, modulation and histogram at
-    int(W/2.5) x int(H/2.5), the bilinear upscale to W x H, then bloom and the
-    post pass at the output size — against the oracle's post chain at the two
-    sizes, frame after frame."""
+    int(W/2.5) x int(H/2.5), the bilinear upscale (or, with upscale_fsr, FSR
+    1.0's EASU + RCAS) to W x H, then bloom and the post pass at the output
+    size — against the oracle's post chain at the two sizes, frame after
+    frame."""
     sc, ctx = plain_ctx
     W, H = 160, 96
     st = settings(max_depth=3, integrator=abi.INTEGRATOR_WAVEFRONT, sun_theta=sc.sun_theta)
     st.enable_auto_exposure, st.bloom_generate_bloom = ae, bloom
     st.upscale_enable = 1
+    st.upscale_fsr = fsr
     ctx.set_settings(st)
     ctx.resize(W, H)
     rw, rh, ow, oh = ctx.resolution()
@@ -118,7 +120,7 @@ def test_upscale_frames_parity(plain_ctx, ae, bloom):
     ctx.set_frame_delta(16.0)
     e, film = ctx.exposure(), ctx.film()
     for f in range(3):
-        print(f"[upscale] ae={ae} bloom={bloom} frame {f}", flush=True)
+        print(f"[upscale] ae={ae} bloom={bloom} fsr={fsr} frame {f}", flush=True)
         ctx.render(1)
         est = np.zeros((rh, rw, 4), np.float32)
         est[..., :3] = ctx.estimate()
@@ -130,6 +132,7 @@ def test_upscale_frames_parity(plain_ctx, ae, bloom):
         diff = (out_g.view(np.uint32) != out_o.view(np.uint32)).any(-1)
         assert not diff.any(), f"{int(diff.sum())} output pixels differ"
     st.upscale_enable = 0
+    st.upscale_fsr = 0
     ctx.set_settings(st)
 
 

@@ -359,6 +359,99 @@ def test_upscale_reproduces_a_linear_ramp():
     assert np.allclose(out[..., 3], 1.0)
 
 
+# ---------------------------------------------------------------- FSR 1.0
+# The reference's USE_FSR build (upscale.cuh:13-408): EASU then RCAS with
+# sharpness 0.  Pinned by what the algorithm fixes in closed form.
+
+@pytest.mark.parametrize("size", [(7, 5, 17, 12), (16, 9, 40, 22), (8, 8, 8, 8)])
+def test_fsr_constant_image_is_unchanged(size):
+    # EASU normalises its weights and clamps to the 4 nearest taps; RCAS on a
+    # constant ring: lobe = max(-3/16, min(max(-1/4, -1/4), 0)) = -3/16 and
+    # (lobe * 4c + c) / (4 lobe + 1) = c
+    W, H, OW, OH = size
+    src = np.full((H, W, 4), 0.375, np.float32)
+    out, easu = oracle.upscale_fsr(src, OW, OH, easu=True)
+    assert np.array_equal(easu[..., :3], np.full((OH, OW, 3), 0.375, np.float32))
+    assert np.array_equal(out[..., :3], np.full((OH, OW, 3), 0.375, np.float32))
+    assert np.array_equal(out[..., 3], np.ones((OH, OW), np.float32))
+
+
+def test_fsr_rcas_impulse_closed_form():
+    # a pixel of value v with a black ring: mn4 = mx4 = 0 -> hitMin = 0/0 (NaN,
+    # dropped by fmaxf), hitMax = (1 - 0) / (0 - 4) = -1/4, lobe = -3/16, out =
+    # v / (1 - 3/4) = 4v; the pixel above sees mn4 = 0, mx4 = v -> hitMin = 0,
+    # hitMax = (1 - v) / -4 < 0, lobe = 0, out = its own 0.  8 x 8 keeps the
+    # reads' p / 8 * 8 exact.
+    v = 0.1875
+    src = np.zeros((8, 8, 4), np.float32)
+    src[3, 3, :3] = v
+    src[..., 3] = 1.0
+    out = oracle.fsr_rcas(src)
+    assert np.array_equal(out[3, 3, :3], np.full(3, 4 * v, np.float32))
+    assert np.array_equal(out[2, 3, :3], np.zeros(3, np.float32))
+    assert np.array_equal(out[4, 3, :3], np.zeros(3, np.float32))
+    assert np.array_equal(out[0, 0, :3], np.zeros(3, np.float32))
+
+
+def test_fsr_easu_at_one_to_one_centres_on_the_half_pixel():
+    # at 1:1, con0 = (1, 1, 0, 0): pp = (x, y) has no fraction, and the four
+    # nearest taps read textureReadBilinear at (x + 1/2, y + 1/2) ('f'), (x +
+    # 3/2, y + 1/2) ('g'), (x + 1/2, y + 3/2) ('j'), (x + 3/2, y + 3/2) ('k')
+    # (p0 = ((x+1)/W, (y-1)/H), p1 = p0 + (-1/W, 2/H), p2 = p0 + (1/W, 2/H),
+    # the gather offsets -+1/2W, -+1/2H), i.e. with the truncating bilinear
+    # the pixels x..x+2, y..y+2.  An image constant on 4 x 4 blocks therefore
+    # has f = g = j = k at x, y = 0 mod 4, and the de-ring clamp returns that
+    # block's value exactly; one block further right it is the next block's.
+    W = H = 16
+    rng = np.random.default_rng(8)
+    blocks = rng.random((H // 4, W // 4)).astype(np.float32) * 0.5 + 0.25
+    img = np.repeat(np.repeat(blocks, 4, 0), 4, 1)
+    src = np.stack([img] * 3 + [np.ones_like(img)], -1)
+    _, easu = oracle.upscale_fsr(src, W, H, easu=True)
+    for y in range(0, H - 3, 4):
+        for x in range(0, W - 3, 4):
+            assert easu[y, x, 0] == img[y, x], (y, x)
+            assert easu[y, x + 1, 0] == img[y, x], (y, x)       # pixels x+1..x+3 still inside
+
+
+def test_fsr_easu_stays_within_the_nearest_taps_and_keeps_a_step_edge():
+    # a vertical step edge 0 | 1 upscaled 2.5x: every output lies in [0, 1]
+    # (dering clamp to the 4 nearest taps), every row is monotone across the
+    # edge, the plateaus are exact, and the edge stays sharp: at most 4 output
+    # pixels per row lie strictly between 0 and 1 (a bilinear upscale makes
+    # 2.5 x 1 + 1 of them; a Gaussian-like kernel of EASU's support, more)
+    W, H, OW, OH = 12, 6, 30, 15
+    src = np.zeros((H, W, 4), np.float32)
+    src[:, W // 2:, :3] = 1.0
+    src[..., 3] = 1.0
+    _, easu = oracle.upscale_fsr(src, OW, OH, easu=True)
+    r = easu[..., 0]
+    assert r.min() >= 0.0 and r.max() <= 1.0
+    assert np.all(np.diff(r.astype(np.float64), axis=1) >= 0)
+    assert np.all(r[:, :8] == 0.0) and np.all(r[:, -8:] == 1.0)
+    assert np.all(((r > 0) & (r < 1)).sum(axis=1) <= 4)
+
+
+def test_fsr_rcas_sharpens_the_knees_of_a_ramp():
+    # a ramp 0.25 -> 0.75 along x (16 x 16: exact reads).  On its linear
+    # interior the ring averages to the centre, so (lobe * 4e + e) / (4 lobe
+    # + 1) = e; at the dark knee (e = d = b = h = 0.25, f = 0.375): mn4 = 0.25,
+    # mx4 = 0.375, hitMin = 1/6, hitMax = 0.625 / -3, lobe = -1/6, out =
+    # (-1/6 * 1.125 + 0.25) / (1/3) = 0.1875; at the bright knee (e = f = b = h
+    # = 0.75, d = 0.625): mn4 = mx4 = 0.75, hitMin = 1/4, hitMax = 0.25 / -1,
+    # lobe = -3/16, out = (-3/16 * 2.875 + 0.75) / (1/4) = 0.84375; plateaus
+    # unchanged.
+    W = H = 16
+    x = np.clip((np.arange(W, dtype=np.float32) - 6.0) / 4.0, 0.0, 1.0) * 0.5 + 0.25
+    src = np.zeros((H, W, 4), np.float32)
+    src[..., :3] = x[None, :, None]
+    src[..., 3] = 1.0
+    row = oracle.fsr_rcas(src)[8, :, 0].astype(np.float64)
+    assert np.isclose(row[6], 0.1875, rtol=1e-6) and np.isclose(row[10], 0.84375, rtol=1e-6)
+    assert np.allclose(row[7:10], x[7:10], rtol=1e-6)
+    assert np.all(row[:6] == 0.25) and np.all(row[11:] == 0.75)
+
+
 def test_post_frame_at_the_render_size_is_unchanged_by_the_sized_form():
     # out_size equal to the render size and no upscale: the same frame as before
     H, W = 12, 20
