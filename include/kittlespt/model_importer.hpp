@@ -75,7 +75,8 @@ struct Json {
     }
     long long integer(const char* key, long long def) const {
         const Json* v = get(key);
-        return (v && v->kind == Number) ? (long long)v->num : def;
+        // out-of-range values (beyond what any index or count can be) read as the default
+        return (v && v->kind == Number && std::fabs(v->num) < 9.0e18) ? (long long)v->num : def;
     }
     std::string string(const char* key) const {
         const Json* v = get(key);
@@ -373,6 +374,11 @@ private:
     std::string base_;
 
     [[noreturn]] static void fail(const std::string& what) { throw std::runtime_error(what); }
+    // An index-valued member (buffer, bufferView, accessor, node, ...).
+    static long long index_of(const Json& j) {
+        if (j.kind != Json::Number || !(std::fabs(j.num) < 9.0e18)) fail("index is not a number");
+        return (long long)j.num;
+    }
 
     const Json& member(const Json& j, const char* key) const {
         const Json* v = j.get(key);
@@ -439,13 +445,15 @@ private:
     // [ptr, len) of a buffer view, and its byte stride (0: packed).
     const uint8_t* view_bytes(long long vi, size_t* len, size_t* stride) const {
         const Json& v = element("bufferViews", vi);
-        const long long bi = (long long)member(v, "buffer").num;
+        const long long bi = index_of(member(v, "buffer"));
         if (bi < 0 || (size_t)bi >= buffers_.size()) fail("buffer index out of range");
         const std::vector<uint8_t>& buf = buffers_[(size_t)bi];
         const double off = v.number("byteOffset", 0.0), n = member(v, "byteLength").num;
-        if (off < 0 || n < 0 || off + n > (double)buf.size()) fail("buffer view runs past its buffer");
+        if (!(off >= 0) || !(n >= 0) || off + n > (double)buf.size()) fail("buffer view runs past its buffer");
+        const long long st = v.integer("byteStride", 0);
+        if (st != 0 && (st < 4 || st > 252)) fail("buffer view byteStride outside 4..252");
         *len = (size_t)n;
-        *stride = (size_t)v.integer("byteStride", 0);
+        *stride = (size_t)st;
         return buf.data() + (size_t)off;
     }
 
@@ -453,7 +461,7 @@ private:
     // accessor's count read 0), values cast from their component type.
     std::vector<float> accessor_f(long long ai, size_t want, size_t* count_out) const {
         const Json& a = element("accessors", ai);
-        const int ct = (int)member(a, "componentType").num;
+        const int ct = (int)index_of(member(a, "componentType"));
         const std::string type = member(a, "type").str;
         const size_t nc = type == "SCALAR" ? 1 : type == "VEC2" ? 2 : type == "VEC3" ? 3 : type == "VEC4" ? 4
                         : type == "MAT4" ? 16 : 0;
@@ -473,8 +481,10 @@ private:
         const Json* bv = a.get("bufferView");
         if (!bv) return out;   // no buffer view: zeros (glTF sparse accessors are not read)
         size_t len = 0, stride = 0;
-        const uint8_t* base = view_bytes((long long)bv->num, &len, &stride);
-        const size_t off = (size_t)a.number("byteOffset", 0.0), elem = csize * nc;
+        const uint8_t* base = view_bytes(index_of(*bv), &len, &stride);
+        const double aoff = a.number("byteOffset", 0.0);
+        if (!(aoff >= 0) || aoff > (double)len) fail("accessor offset outside its buffer view");
+        const size_t off = (size_t)aoff, elem = csize * nc;
         if (stride == 0) stride = elem;
         if (count > 0 && off + (count - 1) * stride + elem > len) fail("accessor runs past its buffer view");
         for (size_t i = 0; i < count; ++i) {
@@ -498,7 +508,7 @@ private:
 
     std::vector<int64_t> indices(long long ai) const {
         const Json& a = element("accessors", ai);
-        const int ct = (int)member(a, "componentType").num;
+        const int ct = (int)index_of(member(a, "componentType"));
         const double cnt = member(a, "count").num;
         if (cnt < 0 || cnt > 1e9) fail("bad accessor count");
         const size_t count = (size_t)cnt;
@@ -509,8 +519,10 @@ private:
         if (csize == 0) fail("index accessor is not an unsigned integer type");
         if (member(a, "type").str != "SCALAR") fail("index accessor is not SCALAR");
         size_t len = 0, stride = 0;
-        const uint8_t* base = view_bytes((long long)bv->num, &len, &stride);
-        const size_t off = (size_t)a.number("byteOffset", 0.0);
+        const uint8_t* base = view_bytes(index_of(*bv), &len, &stride);
+        const double aoff = a.number("byteOffset", 0.0);
+        if (!(aoff >= 0) || aoff > (double)len) fail("accessor offset outside its buffer view");
+        const size_t off = (size_t)aoff;
         if (stride == 0) stride = csize;
         if (count > 0 && off + (count - 1) * stride + csize > len) fail("accessor runs past its buffer view");
         for (size_t i = 0; i < count; ++i) {
@@ -530,7 +542,7 @@ private:
             std::vector<uint8_t> bytes;
             if (const Json* bv = im.get("bufferView")) {
                 size_t len = 0, stride = 0;
-                const uint8_t* p = view_bytes((long long)bv->num, &len, &stride);
+                const uint8_t* p = view_bytes(index_of(*bv), &len, &stride);
                 bytes.assign(p, p + len);
             } else {
                 const std::string uri = im.string("uri");
@@ -653,7 +665,7 @@ private:
         const Json* nodes = s0 ? s0->get("nodes") : nullptr;   // scene 0, as the reference
         size_t meshes = 0;
         for (size_t k = 0; nodes && k < nodes->size(); ++k) {
-            const Json& node = element("nodes", (long long)nodes->arr[k].num);
+            const Json& node = element("nodes", index_of(nodes->arr[k]));
             if (node.integer("camera", -1) >= 0) {
                 const Json& c = element("cameras", node.integer("camera", -1));
                 const Json* persp = c.get("perspective");
@@ -670,14 +682,14 @@ private:
                 if (prim.integer("mode", 4) != 4) fail("only triangle primitives are read");
                 const Json& at = member(prim, "attributes");
                 size_t np = 0, nn = 0, nu = 0;
-                const std::vector<float> P = accessor_f((long long)member(at, "POSITION").num, 3, &np);
-                const std::vector<float> N = accessor_f((long long)member(at, "NORMAL").num, 3, &nn);
+                const std::vector<float> P = accessor_f(index_of(member(at, "POSITION")), 3, &np);
+                const std::vector<float> N = accessor_f(index_of(member(at, "NORMAL")), 3, &nn);
                 std::vector<float> U;
-                if (const Json* uv = at.get("TEXCOORD_0")) U = accessor_f((long long)uv->num, 2, &nu);
+                if (const Json* uv = at.get("TEXCOORD_0")) U = accessor_f(index_of(*uv), 2, &nu);
                 else U.assign(np * 2, 0.0f);
                 if (nn < np || (at.get("TEXCOORD_0") && nu < np)) fail("vertex attribute counts differ");
                 std::vector<int64_t> idx;
-                if (const Json* ia = prim.get("indices")) idx = indices((long long)ia->num);
+                if (const Json* ia = prim.get("indices")) idx = indices(index_of(*ia));
                 else for (size_t i = 0; i < np; ++i) idx.push_back((int64_t)i);
                 const int32_t mat = (int32_t)prim.integer("material", 0);
                 for (size_t t = 0; t + 2 < idx.size(); t += 3) {

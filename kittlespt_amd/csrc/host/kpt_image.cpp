@@ -116,7 +116,9 @@ int png_parse(const uint8_t* d, int64_t n, PngInfo* info) {
         case 6: info->chans = 4; break;
         default: return KPT_ERR_UNSUPPORTED;
     }
-    if (info->w == 0 || info->h == 0 || (uint64_t)info->w * info->h > (1ull << 30)) return KPT_ERR_UNSUPPORTED;
+    // at most 2^28 pixels: the inflated stream (<= 4 bytes + 1 per pixel row
+    // byte) then fits zlib's 32-bit buffer lengths
+    if (info->w == 0 || info->h == 0 || (uint64_t)info->w * info->h > (1ull << 28)) return KPT_ERR_UNSUPPORTED;
     return KPT_OK;
 }
 
@@ -140,6 +142,7 @@ extern "C" int kpt_png_decode_rgb8(const uint8_t* data, int64_t n, int32_t* widt
     z_stream zs;
     std::memset(&zs, 0, sizeof(zs));
     if (inflateInit(&zs) != Z_OK) return KPT_ERR_OUT_OF_MEMORY;
+    if (info.idat.size() > 0xFFFFFFFFull) return KPT_ERR_UNSUPPORTED;
     zs.next_in = info.idat.data();
     zs.avail_in = (uInt)info.idat.size();
     zs.next_out = raw.data();
