@@ -154,6 +154,28 @@ def op_counts(scene, st, cam, W, H, n):
     return ops
 
 
+def bytes_per_sample(scene, st, cam, W, H, n):
+    """Algorithmic traversal bytes of one sample (all its closest-hit and
+    shadow rays, reference layout: BYTES_PER_EVENT per event) over a random
+    pixel sample, counted by the oracle: mean, p50, p99 and rays per sample
+    (the yardstick of the traversal roofline's per-unit figure, never the
+    thing timed)."""
+    import oracle
+    orc = oracle_scene(scene)
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    pix = np.random.default_rng(2468).choice(W * H, size=n, replace=False).astype(np.int32)
+    t0 = time.time()
+    ev = oracle.sample_events(orc, st, cam, W, H, pix, frame=0, threads=cores)
+    log(f"[bytes] counted {n} samples in {time.time() - t0:.1f}s")
+    w = np.array([BYTES_PER_EVENT[k] for k in ("blas_interior_pops", "blas_leaf_pops", "tri_tests", "blas_entries",
+                                               "tlas_pops")], np.float64)
+    b = ev[:, :5].astype(np.float64) @ w
+    return {"mean": round(float(b.mean()), 1), "p50": round(float(np.percentile(b, 50)), 1),
+            "p99": round(float(np.percentile(b, 99)), 1), "rays_per_sample": round(float(ev[:, 5].mean()), 4),
+            "sample": f"{n} random pixels of the frame, 1 sample each, frame 0, the reference's traversal",
+            "per_event_bytes": dict(BYTES_PER_EVENT)}
+
+
 def measured_traffic(stage, W, H, instances, depth, spp, tail):
     """HBM bytes per launch of a stage's kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE passes of this workload (scaled by the
@@ -365,6 +387,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--op-count-pixels", type=int, default=2048,
                     help="pixel sample of the FP32 op count behind the shading stages' VALU rooflines (0: skip)")
+    ap.add_argument("--bytes-pixels", type=int, default=4096,
+                    help="pixel sample of the per-sample traversal bytes distribution (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", default="", help="fixed knobs, e.g. refill=0,blocks_per_cu=8,sort=1")
@@ -517,10 +541,25 @@ def main():
                                                     "traffic")}
                               for k, v in stage_roofs.items()}
 
+    # per-sample algorithmic traversal bytes (distribution) and rays per sample
+    bps = None
+    if integ == abi.INTEGRATOR_WAVEFRONT and args.bytes_pixels > 0:
+        try:
+            bps = bytes_per_sample(scene, st, cam, W, H, args.bytes_pixels)
+            log(f"[bytes] per sample: {bps}")
+            if roof is not None:
+                roof["bytes_per_sample"] = bps
+        except Exception as e:  # a reported figure, never fatal
+            log(f"[bytes] count failed: {e}")
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
             cpu = cpu_baseline(scene, st, cam, W, H, args.cpu_budget)
+            if bps is not None:
+                # rays per sample from the counted pixel sample above
+                cpu["mrays_per_s"] = round(cpu["value"] * bps["rays_per_sample"], 4)
+                cpu["mrays_note"] = "Msamples/s x rays per sample (closest-hit + shadow) of the counted pixel sample"
         except Exception as e:  # the baseline is reported, never fatal
             log(f"[cpu] baseline failed: {e}")
 

@@ -549,6 +549,20 @@ OP_PHASES = ("traversal", "sky_primary", "sky_later", "sky_sun", "shade_primary"
 OP_KINDS = ("arith", "sqrt", "transcendental", "minmax")
 
 
+def sample_events(scene_handle, settings, camera, width, height, pixels, frame=0, threads=None):
+    """Traversal events of one sample of each listed pixel in frame `frame`,
+    every closest-hit and shadow ray of its path, traversed as the reference
+    does: an (n, 6) int64 array of BLAS interior pops, BLAS leaf pops,
+    triangle tests, BLAS entries, TLAS pops and rays traced."""
+    px = np.ascontiguousarray(pixels, np.int32)
+    out = np.zeros((len(px), 6), np.int64)
+    _sig("orc_sample_events", None, [C.c_void_p, C.POINTER(abi.KptSettings), C.POINTER(abi.KptCamera), C.c_int32,
+                                     C.c_int32, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p])(
+        scene_handle.h, C.byref(settings), C.byref(camera), width, height, _ptr(px), len(px), frame,
+        threads or threads_default(), _ptr(out))
+    return out
+
+
 def heat_map(scene_handle, settings, camera, width, height, pixels, frame=0, entry_bound=True):
     """Per-pixel DebugData of one sample (blas.cuh:39-42): an (n, 2) int64
     array of BLAS node pops and TLAS node pops over the listed pixels'

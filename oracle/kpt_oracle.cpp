@@ -685,6 +685,21 @@ struct Counters {
     int max_stack = 0, tlas_depth = 0;
     void depth(int d) { if (d > max_stack) max_stack = d; }
 };
+// Traversal events of every trace a path makes while g_events is set
+// (orc_sample_events): closest-hit and shadow rays alike, and the ray count.
+struct SampleEvents {
+    uint64_t interior = 0, leaf = 0, tri = 0, entries = 0, tlas = 0, rays = 0;
+};
+thread_local SampleEvents* g_events = nullptr;
+inline void AddEvents(const Counters& c) {
+    if (!g_events) return;
+    g_events->interior += c.interior;
+    g_events->leaf += c.leaf;
+    g_events->tri += c.tri;
+    g_events->entries += c.entries;
+    g_events->tlas += c.tlas;
+    g_events->rays += 1;
+}
 
 // BLAS2::intersect, blas.cuh:113-188 (unbounded stack: the reference's 16
 // entries overflow on deep trees, which is undefined behaviour there).
@@ -1402,7 +1417,9 @@ template <class R> bool Unoccluded(const orc_scene* S, const SurfInt<R>& si, V3<
     Ray<R> sr(org, Normalize(target - org));
     R tmax = Distance(target, sr.o) - R(0.11f);
     Counters cn;
-    return !TlasIntersectP(S, sr, R(0), tmax, &cn);
+    const bool occluded = TlasIntersectP(S, sr, R(0), tmax, &cn);
+    AddEvents(cn);
+    return !occluded;
 }
 
 template <class R> V3<R> SunDir(const kpt_settings& st) {  // integrator.cuh:222-230
@@ -1445,6 +1462,7 @@ template <class R> V3<R> Li(const FrameConst& fc, Ray<R> ray, Sampler<R>& sample
         bool first = (depth == 0);
         Counters cn;
         Isect<R> it = TlasIntersect(S, ray, R(0), Inf<R>(), &cn, fc.entry_bound);
+        AddEvents(cn);
         if (fc.path_counts) {   // dbg.blas_hits / dbg.tlas_hits (blas.cuh:140, tlas.cu:87)
             fc.path_counts->interior += cn.interior;
             fc.path_counts->leaf += cn.leaf;
@@ -2304,6 +2322,26 @@ void orc_heat_map(const orc_scene* s, const kpt_settings* settings, const kpt_ca
         out[2 * i] = (int64_t)(cn.interior + cn.leaf);
         out[2 * i + 1] = (int64_t)cn.tlas;
     }
+}
+
+// Traversal events of one sample of each pixel pixels[i] in frame `frame`
+// (every closest-hit and shadow ray of its path, traversed as the reference
+// does): out[6i .. 6i+5] = BLAS interior pops, BLAS leaf pops, triangle tests,
+// BLAS entries, TLAS pops, rays traced.
+void orc_sample_events(const orc_scene* s, const kpt_settings* settings, const kpt_camera* cam, int32_t W, int32_t H,
+                       const int32_t* pixels, int64_t n, int32_t frame, int32_t threads, int64_t* out) {
+    FrameConst fc = MakeFrame(s, settings, cam);
+    ParallelFor(n, threads, [&](int64_t i) {
+        SampleEvents ev;
+        g_events = &ev;
+        V3<float> smp;
+        VisSurf<float> vs;
+        RenderPixel<float>(fc, W, H, pixels[i], frame, &smp, &vs);
+        g_events = nullptr;
+        const int64_t v[6] = {(int64_t)ev.interior, (int64_t)ev.leaf, (int64_t)ev.tri, (int64_t)ev.entries,
+                              (int64_t)ev.tlas, (int64_t)ev.rays};
+        std::memcpy(out + 6 * i, v, sizeof(v));
+    });
 }
 
 }  // extern "C"

@@ -670,3 +670,38 @@ def test_op_counts_follow_the_path_structure(furnace):
     for ph in oracle.OP_PHASES:
         for k in oracle.OP_KINDS:
             assert (a[ph][k] + b[ph][k]) / 2 == ab[ph][k], (ph, k)
+
+
+def test_sample_events_count_every_trace_of_the_path():
+    """orc_sample_events (the per-sample traversal bytes behind the traversal
+    roofline's mean, p50 and p99): with the sun switched off and no area
+    lights no shadow ray is traced, so a sample's BLAS pops and TLAS pops are
+    exactly its closest-hit traces' DebugData (orc_heat_map in the
+    reference's traversal) and its rays are its path vertices; with the sun
+    on, a depth-0 sample traces its camera ray and at most one shadow ray,
+    and its events can only grow."""
+    sc = scenes.scene_instanced_small()
+    orc = oracle.OracleScene(sc)
+    try:
+        W, H = 24, 16
+        cam = host_camera(sc, W, H)
+        px = np.arange(W * H, dtype=np.int32)
+        st = settings(max_depth=3, sun_theta=sc.sun_theta)
+        st.sun_emission_factor = 0.0
+        ev = oracle.sample_events(orc, st, cam, W, H, px)
+        heat = oracle.heat_map(orc, st, cam, W, H, px, entry_bound=False)
+        assert np.array_equal(ev[:, 0] + ev[:, 1], heat[:, 0])
+        assert np.array_equal(ev[:, 4], heat[:, 1])
+        assert ev[:, 5].min() >= 1 and ev[:, 5].max() <= 4            # one closest-hit trace per vertex
+        assert (ev[:, 2] > 0).any() and (ev[:, 3] > 0).any()
+        st0 = settings(max_depth=0, sun_theta=sc.sun_theta)
+        on = oracle.sample_events(orc, st0, cam, W, H, px)
+        st0.sun_emission_factor = 0.0
+        off = oracle.sample_events(orc, st0, cam, W, H, px)
+        assert np.array_equal(off[:, 5], np.ones(W * H, np.int64))
+        assert set(np.unique(on[:, 5])) <= {1, 2} and (on[:, 5] == 2).any()
+        assert (on[:, :5] >= off[:, :5]).all()
+        a = oracle.sample_events(orc, st, cam, W, H, px[:7])
+        assert np.array_equal(a, ev[:7])                                 # per-pixel, independent of the batch
+    finally:
+        orc.close()
