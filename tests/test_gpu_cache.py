@@ -100,9 +100,9 @@ def test_cache_learns_a_smooth_function(ctx):
     assert mse < 0.25 * base
 
 
-@pytest.mark.parametrize("tail", [0, 1])
+@pytest.mark.parametrize("tail,shape", [(0, {}), (1, {}), (1, {"stack": 8, "block": 64})])
 @pytest.mark.parametrize("nee,wide", [(0, 0.0), (1, 0.0), (0, 0.7), (1, 0.7)])
-def test_cached_render_matches_oracle(nee, wide, tail):
+def test_cached_render_matches_oracle(nee, wide, tail, shape):
     """One frame in which every path ends in the cache at its cache vertex
     (the bounce-1 vertex, or with wide > 0 the first vertex reached through a
     diffuse sample or a lobe at least that rough; with and without that
@@ -112,7 +112,7 @@ def test_cached_render_matches_oracle(nee, wide, tail):
     bit-exact, and the cached pixels equal radiance-so-far + throughput *
     scale * max(net(query), 0) within the network's tolerance.  The bounces
     after the primary run as stage kernels (tail 0) or in the path-per-lane
-    tail (tail 1)."""
+    tail (tail 1), also with the shallowest LDS stack in 64-lane workgroups."""
     from kittlespt_amd.context import Context
     sc = scenes.scene_plain(subdiv=3)
     W, H = 64, 48
@@ -123,7 +123,7 @@ def test_cached_render_matches_oracle(nee, wide, tail):
     orc = oracle.OracleScene(sc)
     try:
         ctx.upload(sc)
-        ctx.set_tuning(tail=tail)
+        ctx.set_tuning(tail=tail, **shape)
         ctx.set_settings(st)
         ctx.resize(W, H)
         p, v = view_proj(sc, W, H)
@@ -132,7 +132,8 @@ def test_cached_render_matches_oracle(nee, wide, tail):
         ctx.cache_configure(enabled=True, train=False, seed=7, bounds=box, radiance_scale=1e4, nee_at_vertex=nee,
                             wide_roughness=wide)
         ctx.cache_set_weights(w)
-        print(f"[cache] one cached frame 64x48, nee_at_vertex={nee}, wide_roughness={wide}, tail {tail}", flush=True)
+        print(f"[cache] one cached frame 64x48, nee_at_vertex={nee}, wide_roughness={wide}, tail {tail}, {shape}",
+              flush=True)
         ctx.render(1)
         gp, gd, gb = ctx.cache_queries()
         est = ctx.estimate().reshape(-1, 3)
