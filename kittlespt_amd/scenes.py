@@ -232,6 +232,58 @@ class Scene:
         return d, keep
 
 
+# ---------------------------------------------------------------- on-disk scenes
+SCENE_FORMAT = "kittlespt-scene/1"
+_ARRAYS = ("positions", "normals", "uvs", "material_ids", "mesh_first", "mesh_count", "mesh_model", "pixels")
+_DTYPES = {"positions": np.float32, "normals": np.float32, "uvs": np.float32, "material_ids": np.int32,
+           "mesh_first": np.int32, "mesh_count": np.int32, "mesh_model": np.float32, "pixels": np.uint8,
+           "instance_mesh": np.int32, "instance_model": np.float32}
+
+
+def save_scene(scene: Scene, path: str) -> None:
+    """Writes a scene as one uncompressed .npz: the flat arrays kpt_scene_desc
+    takes (float32 / int32 / uint8, the layout loadScene receives) and a JSON
+    header (format tag, materials, texture descriptors, camera, sun).  Loading
+    needs no pickling (numpy.load with allow_pickle off)."""
+    import json
+    header = {"format": SCENE_FORMAT, "materials": [{k: (list(v) if isinstance(v, tuple) else v)
+                                                     for k, v in m.items()} for m in scene.materials],
+              "textures": [list(map(int, t)) for t in scene.textures], "camera_eye": list(scene.camera_eye),
+              "camera_target": list(scene.camera_target), "camera_fov": float(scene.camera_fov),
+              "sun_theta": float(scene.sun_theta), "instanced": scene.instance_mesh is not None}
+    arrays = {k: np.ascontiguousarray(getattr(scene, k), _DTYPES[k]) for k in _ARRAYS}
+    if scene.instance_mesh is not None:
+        arrays["instance_mesh"] = np.ascontiguousarray(scene.instance_mesh, np.int32)
+        arrays["instance_model"] = np.ascontiguousarray(scene.instance_model, np.float32)
+    text = json.dumps(header, default=lambda o: o.item() if isinstance(o, np.generic) else o)
+    arrays["header"] = np.frombuffer(text.encode("utf-8"), np.uint8)
+    with open(path, "wb") as f:
+        np.savez(f, **arrays)
+
+
+def load_scene(path: str) -> Scene:
+    """Reads a scene written by save_scene (ValueError on another format)."""
+    import json
+    with np.load(path, allow_pickle=False) as z:
+        if "header" not in z.files:
+            raise ValueError(f"{path}: not a scene file (no header)")
+        header = json.loads(bytes(z["header"]).decode("utf-8"))
+        if header.get("format") != SCENE_FORMAT:
+            raise ValueError(f"{path}: scene format {header.get('format')!r} is not {SCENE_FORMAT!r}")
+        arrays = {}
+        for k in _ARRAYS:
+            if k not in z.files:
+                raise ValueError(f"{path}: array {k} missing")
+            arrays[k] = np.asarray(z[k], _DTYPES[k])
+        inst = {}
+        if header.get("instanced"):
+            inst = {k: np.asarray(z[k], _DTYPES[k]) for k in ("instance_mesh", "instance_model")}
+    mats = [{k: (tuple(v) if isinstance(v, list) else v) for k, v in m.items()} for m in header["materials"]]
+    return Scene(materials=mats, textures=[tuple(t) for t in header["textures"]],
+                 camera_eye=tuple(header["camera_eye"]), camera_target=tuple(header["camera_target"]),
+                 camera_fov=header["camera_fov"], sun_theta=header["sun_theta"], **arrays, **inst)
+
+
 def material(albedo=(0.8, 0.8, 0.8), metallic=0.0, roughness=0.5, transmission=0.0, ior=1.45,
              albedo_texture_id=-1, emission=(1.0, 1.0, 1.0), emission_scale_nits=0.0, orm_texture_id=-1,
              transmission_texture_id=-1, emission_texture_id=-1, normal_texture_id=-1, normal_scale=1.0):

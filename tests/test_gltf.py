@@ -446,3 +446,33 @@ def test_cpp_importer_reports_malformed_files(tmp_path):
         assert cpp is None and log.startswith("import failed: "), (name, log)
         with pytest.raises(Exception):
             gltf.load_gltf(str(p))
+
+
+# ---------------------------------------------------------------- on-disk scenes
+def test_scene_file_round_trip(tmp_path):
+    """save_scene / load_scene: every array bit for bit, materials, texture
+    descriptors, camera and sun; instanced and plain scenes; a file of
+    another kind is refused."""
+    from kittlespt_amd import scenes
+    for sc in (scenes.scene_instanced_small(), scenes.scene_plain(subdiv=2), gltf.load_gltf(str(_write(tmp_path,
+                                                                                                          True)[0]))[0]):
+        p = tmp_path / "s.npz"
+        scenes.save_scene(sc, str(p))
+        back = scenes.load_scene(str(p))
+        for k in ("positions", "normals", "uvs", "material_ids", "mesh_first", "mesh_count", "mesh_model", "pixels"):
+            a, b = np.asarray(getattr(sc, k)), getattr(back, k)
+            assert a.shape == b.shape and np.array_equal(a.view(np.uint8) if a.dtype.kind == "f" else a,
+                                                         b.view(np.uint8) if b.dtype.kind == "f" else b), k
+        assert (sc.instance_mesh is None) == (back.instance_mesh is None)
+        if sc.instance_mesh is not None:
+            assert np.array_equal(sc.instance_mesh, back.instance_mesh)
+            assert np.array_equal(np.asarray(sc.instance_model, np.float32).view(np.uint32),
+                                  back.instance_model.view(np.uint32))
+        assert back.materials == [{k: (tuple(v) if isinstance(v, list) else v) for k, v in m.items()}
+                                  for m in sc.materials]
+        assert back.textures == [tuple(int(x) for x in t) for t in sc.textures]
+        assert back.camera_eye == tuple(sc.camera_eye) and back.sun_theta == sc.sun_theta
+    bad = tmp_path / "bad.npz"
+    np.savez(bad, positions=np.zeros((1, 3, 3), np.float32))
+    with pytest.raises(ValueError):
+        scenes.load_scene(str(bad))
