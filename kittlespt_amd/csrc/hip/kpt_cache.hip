@@ -124,11 +124,33 @@ __global__ __launch_bounds__(256) void k_rc_infer(CacheBuffers cb, CacheEncode e
     uint16_t* act = s_act[wave];
     uint16_t* row = act + r * RC_HID;
     const int64_t tiles = (n + 15) / 16;
-    for (int64_t tile = (int64_t)blockIdx.x * 4 + wave; tile < tiles; tile += (int64_t)gridDim.x * 4) {
+    const int64_t stride = (int64_t)gridDim.x * 4;
+    const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // the next tile's query (and, adding into paths, its throughput and the
+    // path's radiance) is loaded while the current tile runs its layers
+    int64_t tile = (int64_t)blockIdx.x * 4 + wave;
+    float4 p4n = z4, d4n = z4, ben = z4, lpn = z4;
+    if (tile < tiles && tile * 16 + r < n) {
+        p4n = qpos[tile * 16 + r];
+        d4n = qdir[tile * 16 + r];
+        if (MODE == 0) {
+            ben = qbeta[tile * 16 + r];
+            lpn = dst[(int)__builtin_bit_cast(uint32_t, d4n.w)];
+        }
+    }
+    for (; tile < tiles; tile += stride) {
         const int64_t qi = tile * 16 + r;
         const bool valid = qi < n;
-        const float4 p4 = valid ? qpos[qi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        const float4 d4 = valid ? qdir[qi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float4 p4 = p4n, d4 = d4n, be = ben, lp = lpn;
+        const int64_t ni = (tile + stride) * 16 + r;
+        if (tile + stride < tiles && ni < n) {
+            p4n = qpos[ni];
+            d4n = qdir[ni];
+            if (MODE == 0) {
+                ben = qbeta[ni];
+                lpn = dst[(int)__builtin_bit_cast(uint32_t, d4n.w)];
+            }
+        }
         uint16_t h[8];
         const bf16x8 x = rc_features8(g, p4, d4, enc, h);
         // layer 0 (K = 32)
@@ -163,8 +185,6 @@ __global__ __launch_bounds__(256) void k_rc_infer(CacheBuffers cb, CacheEncode e
         if (valid && g == 0) {
             if (MODE == 0) {
                 const int p = (int)__builtin_bit_cast(uint32_t, d4.w);
-                const float4 be = qbeta[qi];
-                const float4 lp = dst[p];
                 dst[p] = make_float4(lp.x + be.x * (enc.scale * fmaxf(o[0], 0.0f)),
                                      lp.y + be.y * (enc.scale * fmaxf(o[1], 0.0f)),
                                      lp.z + be.z * (enc.scale * fmaxf(o[2], 0.0f)), lp.w);
@@ -413,10 +433,26 @@ hipError_t launch_cache_publish(hipStream_t s, const CacheBuffers& cb) {
     return hipGetLastError();
 }
 
+// Workgroups of k_rc_infer resident per CU (its 48 KB of LDS allow 3): the
+// persistent grid is sized to what runs at once, so no workgroup waits for
+// another to finish before it starts its share of the tiles.
+int infer_blocks_per_cu() {
+    static int nb = 0;
+    if (nb == 0) {
+        int v = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, reinterpret_cast<const void*>(&k_rc_infer<0>), 256, 0) !=
+                hipSuccess || v < 1)
+            v = 1;
+        nb = v;
+    }
+    return nb;
+}
+
 hipError_t launch_cache_infer_paths(hipStream_t s, const CacheBuffers& cb, const CacheEncode& enc, float4* L,
                                     int64_t max_queries, int num_cus) {
     const int64_t tiles = (max_queries + 15) / 16;
-    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, (int64_t)num_cus * 4));
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4,
+                                                                           (int64_t)num_cus * infer_blocks_per_cu()));
     hipLaunchKernelGGL(k_rc_infer<0>, dim3(grid), dim3(256), 0, s, cb, enc, cb.q_pos, cb.q_dir, cb.q_beta,
                        cb.counters, (int64_t)0, cb.query_cap, L);
     return hipGetLastError();
@@ -426,7 +462,8 @@ hipError_t launch_cache_infer_raw(hipStream_t s, const CacheBuffers& cb, const C
                                   const float4* dir, int64_t n, float4* out, int num_cus) {
     if (n <= 0) return hipSuccess;
     const int64_t tiles = (n + 15) / 16;
-    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, (int64_t)num_cus * 4));
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4,
+                                                                           (int64_t)num_cus * infer_blocks_per_cu()));
     hipLaunchKernelGGL(k_rc_infer<1>, dim3(grid), dim3(256), 0, s, cb, enc, pos, dir, (const float4*)nullptr,
                        (const int32_t*)nullptr, n, n, out);
     return hipGetLastError();
