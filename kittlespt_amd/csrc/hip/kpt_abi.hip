@@ -640,7 +640,7 @@ int alloc_wavefront(kpt_ctx* ctx, int64_t n) {
     HIPCHK(hipMalloc((void**)&w.shadow_o, f4)); HIPCHK(hipMalloc((void**)&w.shadow_d, f4));
     HIPCHK(hipMalloc((void**)&w.shadow_c, f4)); HIPCHK(hipMalloc((void**)&w.shadow_z, f4));
     HIPCHK(hipMalloc((void**)&w.sort_keys, i4)); HIPCHK(hipMalloc((void**)&w.sort_tmp, i4));
-    HIPCHK(hipMalloc((void**)&w.counters, 256 * sizeof(int32_t)));
+    HIPCHK(hipMalloc((void**)&w.counters, (size_t)WF_COUNTER_WORDS * sizeof(int32_t)));
     HIPCHK(hipMalloc((void**)&w.hist, (size_t)64 * ((n + 255) / 256) * sizeof(int32_t)));
     for (int k = 0; k < 2; ++k) {
         HIPCHK(hipMalloc((void**)&w.rkeys[k], i4));

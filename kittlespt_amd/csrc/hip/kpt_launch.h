@@ -22,6 +22,12 @@ struct FrameBuffers {
 };
 
 // Per-path state of the wavefront tracer (struct of arrays in HBM).
+// W.counters: WF_COUNTER_SLOTS queue counters, WF_COUNTER_STRIDE words
+// (256 bytes) apart
+constexpr int WF_COUNTER_SLOTS = 128;
+constexpr int WF_COUNTER_STRIDE = 64;
+constexpr int WF_COUNTER_WORDS = WF_COUNTER_SLOTS * WF_COUNTER_STRIDE;
+
 struct WavefrontState {
     int64_t capacity;
     // path records
@@ -45,7 +51,7 @@ struct WavefrontState {
     float4* shadow_z;     // contribution when occluded ((0*beta)/p)
     int32_t* sort_keys;
     int32_t* sort_tmp;
-    int32_t* counters;    // [0] = next queue size, [1] = shadow count, [2] = overflow
+    int32_t* counters;    // queue sizes and pull counters (WF_COUNTER_WORDS words)
     int32_t* hist;        // material histogram for the shading sort
     // spatial ray sort: keys and path/shadow indices (in, out), sort workspace
     uint32_t* rkeys[2];

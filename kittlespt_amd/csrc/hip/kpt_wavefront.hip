@@ -41,6 +41,12 @@ constexpr int C_QCOUNT = 0;      // + bounce parity: [0],[1] live-queue sizes
 constexpr int C_SHADOW = 2;      // shadow-queue size of the current bounce
 constexpr int C_PULL = 4;        // 4..(4+3*MAX_BOUNCE): pull counters per stage and bounce
 constexpr int MAX_BOUNCE_SLOTS = 40;
+static_assert(C_PULL + 3 * MAX_BOUNCE_SLOTS <= WF_COUNTER_SLOTS, "counter slots");
+// A slot's word in W.counters: slots lie WF_COUNTER_STRIDE words apart, so
+// the queue claims and appends of one kernel (up to three counters, each
+// taking an atomic per wave and chunk) go to different cache lines and L2
+// channels instead of queueing behind one another on one line.
+KD int cs(int slot) { return slot * WF_COUNTER_STRIDE; }
 
 KD uint32_t fbits(float f) { return __builtin_bit_cast(uint32_t, f); }
 KD float bitsf(uint32_t u) { return __builtin_bit_cast(float, u); }
@@ -115,7 +121,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_generate(FrameConst fc, int32_t fr
         slot = tile * 64 + (py & 7) * 8 + (px & 7);
     }
     W.queue[0][slot] = (int32_t)i;
-    if (i == 0) W.counters[C_QCOUNT] = (int32_t)npx;
+    if (i == 0) W.counters[cs(C_QCOUNT)] = (int32_t)npx;
 }
 
 // Multi-view generation: path i belongs to the view whose pixel range holds
@@ -143,7 +149,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_generate_views(FrameConst fc, int3
     W.L[i] = make_float4(0.0f, 0.0f, 0.0f, bitsf(0u));
     W.prev_pos[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     W.queue[0][i] = (int32_t)i;
-    if (i == 0) W.counters[C_QCOUNT] = (int32_t)vb.total;
+    if (i == 0) W.counters[cs(C_QCOUNT)] = (int32_t)vb.total;
 }
 
 // Multi-view finish: clampOutput and the running sum of the samples.
@@ -198,7 +204,7 @@ KD uint32_t ray_key(const FrameConst& fc, float4 o, float4 d, bool with_dir) {
 __global__ __launch_bounds__(WF_BLOCK) void k_ray_keys(FrameConst fc, WavefrontState W, int parity, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * WF_BLOCK + threadIdx.x;
     if (i >= n) return;
-    const int count = W.counters[C_QCOUNT + parity];
+    const int count = W.counters[cs(C_QCOUNT + parity)];
     if (i < count) {
         const int p = W.queue[parity][i];
         W.rkeys[0][i] = ray_key(fc, W.ray_o[p], W.ray_d[p], true);
@@ -212,7 +218,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_ray_keys(FrameConst fc, WavefrontS
 __global__ __launch_bounds__(WF_BLOCK) void k_shadow_keys(FrameConst fc, WavefrontState W, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * WF_BLOCK + threadIdx.x;
     if (i >= n) return;
-    const int count = W.counters[C_SHADOW];
+    const int count = W.counters[cs(C_SHADOW)];
     if (i < count) {
         W.rkeys[0][i] = ray_key(fc, W.shadow_o[i], W.shadow_d[i], true);
         W.rvals[0][i] = (int32_t)i;
@@ -238,9 +244,9 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TRACE void k_trace(SceneView S, W
                                                     TraceTuning tune, const int32_t* sorted) {
     extern __shared__ uint32_t s_dyn[];   // tune.stack x blockDim.x words
     uint32_t spill[SPILL_STACK];
-    const int count = W.counters[C_QCOUNT + parity];
+    const int count = W.counters[cs(C_QCOUNT + parity)];
     const int32_t* q = sorted ? sorted : W.queue[parity];
-    int32_t* ctr = &W.counters[pull_slot];
+    int32_t* ctr = &W.counters[cs(pull_slot)];
     const StackShape lds{s_dyn + threadIdx.x, (int)blockDim.x, tune.stack};
     bool ovf = false;
     TraceCounts c = {0, 0, 0, 0, 0};
@@ -331,7 +337,7 @@ __device__ __forceinline__ int path_key(const SceneView& S, const WavefrontState
 // W.hist layout: [0, MAX_KEYS) totals, [MAX_KEYS, 2*MAX_KEYS) running cursors.
 __global__ __launch_bounds__(WF_BLOCK) void k_sort_hist(SceneView S, WavefrontState W, int parity, int nkeys) {
     __shared__ int h[MAX_KEYS];
-    const int count = W.counters[C_QCOUNT + parity];
+    const int count = W.counters[cs(C_QCOUNT + parity)];
     for (int k = threadIdx.x; k < nkeys; k += WF_BLOCK) h[k] = 0;
     __syncthreads();
     for (int i = blockIdx.x * WF_BLOCK + threadIdx.x; i < count; i += gridDim.x * WF_BLOCK)
@@ -351,7 +357,7 @@ __global__ void k_sort_offsets(WavefrontState W, int nkeys) {
 __global__ __launch_bounds__(WF_BLOCK) void k_sort_scatter(SceneView S, WavefrontState W, int parity, int nkeys) {
     __shared__ int h[MAX_KEYS];
     __shared__ int base[MAX_KEYS];
-    const int count = W.counters[C_QCOUNT + parity];
+    const int count = W.counters[cs(C_QCOUNT + parity)];
     for (int b0 = blockIdx.x * WF_BLOCK; b0 < count; b0 += gridDim.x * WF_BLOCK) {
         for (int k = threadIdx.x; k < nkeys; k += WF_BLOCK) h[k] = 0;
         __syncthreads();
@@ -371,10 +377,10 @@ __global__ __launch_bounds__(WF_BLOCK) void k_sort_scatter(SceneView S, Wavefron
 // One bounce of Li for every live path (integrator.cuh:266-398).
 __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_SHADE void k_shade(SceneView S, FrameConst fc, WavefrontState W, int parity,
                                                     int pull_slot, const int32_t* q, CacheBuffers cb, CacheFrame cf) {
-    const int count = W.counters[C_QCOUNT + parity];
+    const int count = W.counters[cs(C_QCOUNT + parity)];
     int32_t* nq = W.queue[parity ^ 1];
     for (;;) {
-        const int base = wave_claim(&W.counters[pull_slot], WAVE);
+        const int base = wave_claim(&W.counters[cs(pull_slot)], WAVE);
         if (base >= count) break;
         const int k = base + lane_id();
         if (k >= count) continue;
@@ -430,7 +436,7 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_SHADE void k_shade(SceneView S, F
             if (!(to_cache && !cf.nee) && !(b.alpha < 0.001f)) {
                 NeeResult ne = next_event(S, fc, s, b, ray.d, beta, seed, sun_p);
                 if (ne.trace) {
-                    int sidx = atomicAdd(&W.counters[C_SHADOW], 1);
+                    int sidx = atomicAdd(&W.counters[cs(C_SHADOW)], 1);
                     W.shadow_queue[sidx] = p;
                     W.shadow_o[sidx] = make_float4(ne.org.x, ne.org.y, ne.org.z, ne.tmax);
                     W.shadow_d[sidx] = make_float4(ne.dir.x, ne.dir.y, ne.dir.z, bitsf((uint32_t)trec));
@@ -475,7 +481,7 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_SHADE void k_shade(SceneView S, F
         if (alive) {
             W.ray_o[p] = make_float4(ray.o.x, ray.o.y, ray.o.z, o4.w);
             W.beta[p] = make_float4(beta.x, beta.y, beta.z, p_b);
-            int ni = atomicAdd(&W.counters[C_QCOUNT + (parity ^ 1)], 1);
+            int ni = atomicAdd(&W.counters[cs(C_QCOUNT + (parity ^ 1))], 1);
             nq[ni] = p;
         }
     }
@@ -500,9 +506,9 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TAIL void k_tail(SceneView S, Fra
                                                    CacheBuffers cb, CacheFrame cf, int stack_depth) {
     extern __shared__ uint32_t s_dyn[];   // stack_depth x blockDim.x words
     const StackShape lds{s_dyn + threadIdx.x, (int)blockDim.x, stack_depth};
-    const int count = W.counters[C_QCOUNT + parity];
+    const int count = W.counters[cs(C_QCOUNT + parity)];
     const int32_t* q = order ? order : W.queue[parity];
-    int32_t* ctr = &W.counters[pull_slot];
+    int32_t* ctr = &W.counters[cs(pull_slot)];
     bool ovf = false;
     bool active = false, drained = false;   // drained: wave-uniform
     int p = -1;
@@ -555,7 +561,7 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TAIL void k_tail(SceneView S, Fra
         // wave-uniform: one atomic per wave for the lanes whose ray escaped
         const uint64_t mm = __ballot(missed);
         if (mm != 0ull) {
-            const int base = wave_claim(&W.counters[C_SHADOW], __popcll(mm));
+            const int base = wave_claim(&W.counters[cs(C_SHADOW)], __popcll(mm));
             if (missed) {
                 const int k = base + __popcll(mm & lanes_below());
                 W.shadow_queue[k] = p;
@@ -579,7 +585,7 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TAIL void k_tail(SceneView S, Fra
 // operations li_bounce makes at the miss, with all lanes of a wave busy.
 __global__ __launch_bounds__(WF_BLOCK) void k_sky(FrameConst fc, WavefrontState W) {
     const int i = (int)(blockIdx.x * WF_BLOCK + threadIdx.x);
-    if (i >= W.counters[C_SHADOW]) return;
+    if (i >= W.counters[cs(C_SHADOW)]) return;
     const int p = W.shadow_queue[i];
     const float4 d = W.shadow_d[i], b = W.shadow_c[i], l = W.L[p];
     const v3 sky = miss_radiance(fc, mk3(d.x, d.y, d.z), false);
@@ -604,8 +610,8 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TRACE void k_shadow(SceneView S, 
                                                      TraceTuning tune, const int32_t* order, float4* t_lbefore) {
     extern __shared__ uint32_t s_dyn[];   // tune.stack x blockDim.x words
     uint32_t spill[SPILL_STACK];
-    const int count = W.counters[C_SHADOW];
-    int32_t* ctr = &W.counters[pull_slot];
+    const int count = W.counters[cs(C_SHADOW)];
+    int32_t* ctr = &W.counters[cs(pull_slot)];
     const StackShape lds{s_dyn + threadIdx.x, (int)blockDim.x, tune.stack};
     bool ovf = false;
     if (!tune.refill) {
@@ -720,7 +726,8 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
                              (npx + tblock - 1) / tblock));
     const size_t tlds = (size_t)tune.stack * (size_t)tblock * sizeof(uint32_t);
     const int nkeys = std::min(MAX_KEYS, num_materials + 1);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, W.counters, 0, C_PULL + 3 * MAX_BOUNCE_SLOTS);
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((WF_COUNTER_WORDS + 255) / 256)), dim3(256), 0, s, W.counters, 0,
+                       WF_COUNTER_WORDS);
     if (views)
         hipLaunchKernelGGL(k_generate_views, dim3(gblocks), dim3(WF_BLOCK), 0, s, fc, frame, W, *views);
     else
@@ -743,7 +750,7 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
                 if (e != hipSuccess) return e;
                 tail_q = W.rvals[1];
             }
-            hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, W.counters + C_SHADOW, 0, 1);
+            hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, W.counters + cs(C_SHADOW), 0, 1);
             if (ins) (void)hipEventRecord(ins->tail_begin, s);
             const bool with_cache = cf.enabled || cf.train;
             unsigned long long* tc = (ins && ins->counters) ? ins->counters : nullptr;
@@ -772,8 +779,8 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
             break;
         }
         // next queue and shadow count start at zero
-        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, W.counters + C_QCOUNT + (parity ^ 1), 0, 1);
-        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, W.counters + C_SHADOW, 0, 1);
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, W.counters + cs(C_QCOUNT + (parity ^ 1)), 0, 1);
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, W.counters + cs(C_SHADOW), 0, 1);
         // continuation rays (not the pixel-ordered camera rays) in spatial order
         const int32_t* trace_q = nullptr;
         if (tune.ray_sort && bounce > 0 && W.rtemp) {
