@@ -374,17 +374,25 @@ __global__ __launch_bounds__(WF_BLOCK) void k_sort_scatter(SceneView S, Wavefron
     }
 }
 
-// One bounce of Li for every live path (integrator.cuh:266-398).
+// One bounce of Li for every live path (integrator.cuh:266-398), one queue
+// entry per lane.  A workgroup reserves its shadow rays and its surviving
+// paths with one atomic per queue after its lanes have shaded (a prefix over
+// its waves' ballots in LDS): the appends of every workgroup target the same
+// two counters, and one atomic per 256 lanes instead of one per wave and
+// queue keeps that address's queue in L2 short.
 __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_SHADE void k_shade(SceneView S, FrameConst fc, WavefrontState W, int parity,
-                                                    int pull_slot, const int32_t* q, CacheBuffers cb, CacheFrame cf) {
+                                                    const int32_t* q, CacheBuffers cb, CacheFrame cf) {
+    constexpr int NW = WF_BLOCK / WAVE;
+    __shared__ int s_off[2][NW + 1];   // per queue: the waves' offsets, then the workgroup's base
     const int count = W.counters[cs(C_QCOUNT + parity)];
-    int32_t* nq = W.queue[parity ^ 1];
-    for (;;) {
-        const int base = wave_claim(&W.counters[cs(pull_slot)], WAVE);
-        if (base >= count) break;
-        const int k = base + lane_id();
-        if (k >= count) continue;
-        const int p = q[k];
+    const int first_k = (int)blockIdx.x * WF_BLOCK;
+    if (first_k >= count) return;   // workgroup-uniform
+    const int k = first_k + (int)threadIdx.x;
+    bool want_shadow = false, alive = false;
+    int p = -1;
+    float4 sh_o, sh_d, sh_c, sh_z;
+    if (k < count) {
+        p = q[k];
         float4 o4 = W.ray_o[p], d4 = W.ray_d[p], b4 = W.beta[p], l4 = W.L[p];
         Ray ray = make_ray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z));
         uint32_t seed = fbits(d4.w);
@@ -401,7 +409,6 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_SHADE void k_shade(SceneView S, F
         float4 h4 = W.hit[p];
         Hit h;
         h.t = h4.x; h.slot = (int32_t)fbits(h4.y); h.inst = (int32_t)fbits(h4.z); h.u = h4.w; h.v = W.hit_v[p];
-        bool alive = false;
         if (h.slot < 0) {
             v3 sky = miss_radiance(fc, ray.d, first);
             if (first) {
@@ -436,12 +443,11 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_SHADE void k_shade(SceneView S, F
             if (!(to_cache && !cf.nee) && !(b.alpha < 0.001f)) {
                 NeeResult ne = next_event(S, fc, s, b, ray.d, beta, seed, sun_p);
                 if (ne.trace) {
-                    int sidx = atomicAdd(&W.counters[cs(C_SHADOW)], 1);
-                    W.shadow_queue[sidx] = p;
-                    W.shadow_o[sidx] = make_float4(ne.org.x, ne.org.y, ne.org.z, ne.tmax);
-                    W.shadow_d[sidx] = make_float4(ne.dir.x, ne.dir.y, ne.dir.z, bitsf((uint32_t)trec));
-                    W.shadow_c[sidx] = make_float4(ne.unoccl.x, ne.unoccl.y, ne.unoccl.z, 0.0f);
-                    W.shadow_z[sidx] = make_float4(ne.occl.x, ne.occl.y, ne.occl.z, 0.0f);
+                    want_shadow = true;
+                    sh_o = make_float4(ne.org.x, ne.org.y, ne.org.z, ne.tmax);
+                    sh_d = make_float4(ne.dir.x, ne.dir.y, ne.dir.z, bitsf((uint32_t)trec));
+                    sh_c = make_float4(ne.unoccl.x, ne.unoccl.y, ne.unoccl.z, 0.0f);
+                    sh_z = make_float4(ne.occl.x, ne.occl.y, ne.occl.z, 0.0f);
                 } else {
                     L = L + ne.occl;
                 }
@@ -481,10 +487,35 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_SHADE void k_shade(SceneView S, F
         if (alive) {
             W.ray_o[p] = make_float4(ray.o.x, ray.o.y, ray.o.z, o4.w);
             W.beta[p] = make_float4(beta.x, beta.y, beta.z, p_b);
-            int ni = atomicAdd(&W.counters[cs(C_QCOUNT + (parity ^ 1))], 1);
-            nq[ni] = p;
         }
     }
+    const uint64_t m_sh = __ballot(want_shadow), m_al = __ballot(alive);
+    const int wave = (int)(threadIdx.x / WAVE);
+    if (lane_id() == 0) {
+        s_off[0][wave] = __popcll(m_sh);
+        s_off[1][wave] = __popcll(m_al);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int j = (int)threadIdx.x;
+        int run = 0;
+        for (int w = 0; w < NW; ++w) {
+            const int c = s_off[j][w];
+            s_off[j][w] = run;
+            run += c;
+        }
+        s_off[j][NW] = run ? atomicAdd(&W.counters[cs(j == 0 ? C_SHADOW : C_QCOUNT + (parity ^ 1))], run) : 0;
+    }
+    __syncthreads();
+    if (want_shadow) {
+        const int sidx = s_off[0][NW] + s_off[0][wave] + __popcll(m_sh & lanes_below());
+        W.shadow_queue[sidx] = p;
+        W.shadow_o[sidx] = sh_o;
+        W.shadow_d[sidx] = sh_d;
+        W.shadow_c[sidx] = sh_c;
+        W.shadow_z[sidx] = sh_z;
+    }
+    if (alive) W.queue[parity ^ 1][s_off[1][NW] + s_off[1][wave] + __popcll(m_al & lanes_below())] = p;
 }
 
 // The rest of every path still alive, one path per lane (li_bounce from the
@@ -813,8 +844,7 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
         }
         const bool time_shade = ins && bounce == 0;
         if (time_shade) (void)hipEventRecord(ins->xbegin[WF_X_SHADE0], s);
-        hipLaunchKernelGGL(k_shade, dim3(pgrid), dim3(WF_BLOCK), 0, s, S, fc, W, parity, C_PULL + 3 * bounce + 1,
-                           shade_q, cb, cf);
+        hipLaunchKernelGGL(k_shade, dim3(gblocks), dim3(WF_BLOCK), 0, s, S, fc, W, parity, shade_q, cb, cf);
         if (time_shade) {
             (void)hipEventRecord(ins->xend[WF_X_SHADE0], s);
             ins->xlaunches[WF_X_SHADE0] = 1;
