@@ -385,7 +385,13 @@ int kpt_get_heat_map(kpt_ctx* ctx, int32_t* counts, float* rgba, int64_t n_pixel
  * split; the reference's stacks hold 16, tlas.cuh:10, blas.cuh:37), [11]
  * (when n >= 12) lanes per workgroup of those kernels (64, 128 or 256,
  * default 256; the reference launches 8x8 blocks, kernels.cu:32-36; the
- * persistent grid keeps blocks_per_cu x 256 lanes per compute unit).
+ * persistent grid keeps blocks_per_cu x 256 lanes per compute unit), [12]
+ * (when n >= 13) the tail's parking threshold (0..64, default 16; 0 never
+ * parks): once fewer than this many lanes of a tail wave are still in a
+ * vertex's closest-hit trace, and the wave still claims new paths, the
+ * unfinished traces are parked (their path state in the path arrays, their
+ * traversal state and stack, up to 48 entries, in a per-wave pool) and taken
+ * back by the wave's next idle lanes; the image does not change.
  * n >= 5. */
 int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n);
 int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n);

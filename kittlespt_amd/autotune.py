@@ -28,8 +28,10 @@ SPACE = {
     "tail": [0, 1, 2, 3, 4],
     "stack": [8, 16, 24, 32],
     "block": [64, 128, 256],
+    "suspend": [0, 8, 16, 24],
 }
-ORDER = ["tail", "ray_sort", "refill", "blocks_per_cu", "stack", "block", "sort", "graph", "chunk", "refill_min"]
+ORDER = ["tail", "suspend", "ray_sort", "refill", "blocks_per_cu", "stack", "block", "sort", "graph", "chunk",
+         "refill_min"]
 
 
 def measure(ctx, cfg, spp=1, repeats=2):

@@ -108,7 +108,7 @@ struct kpt_ctx {
     int32_t use_graph = 0;               // capture each frame as a hipGraph and replay it
     hipGraphExec_t graph_exec = nullptr;        // kpt_render's per-frame graph
     hipGraphExec_t views_graph_exec = nullptr;  // kpt_render_views' per-sample graph
-    TraceTuning tune{0, 64, 16, 8, 0, 1, 16, 256};    // the autotuner's usual choice on the 1080p workload
+    TraceTuning tune{0, 64, 16, 8, 0, 1, 16, 256, 16};    // the autotuner's usual choice on the 1080p workload
     int32_t sort_by_material = 2;
     bool count_trace = false;
     unsigned long long* d_counts = nullptr;
@@ -617,7 +617,8 @@ void free_wavefront(kpt_ctx* c) {
     dev_free(&w.vis_a); dev_free(&w.vis_n); dev_free(&w.vis_p); dev_free(&w.hit); dev_free(&w.hit_v);
     dev_free(&w.queue[0]); dev_free(&w.queue[1]); dev_free(&w.shadow_queue); dev_free(&w.shadow_o);
     dev_free(&w.shadow_d); dev_free(&w.shadow_c); dev_free(&w.shadow_z); dev_free(&w.sort_keys);
-    dev_free(&w.sort_tmp); dev_free(&w.counters); dev_free(&w.hist);
+    dev_free(&w.sort_tmp); dev_free(&w.counters); dev_free(&w.hist); dev_free(&w.tpool);
+    w.tpool_waves = 0;
     for (int k = 0; k < 2; ++k) { dev_free(&w.rkeys[k]); dev_free(&w.rvals[k]); }
     if (w.rtemp) { (void)hipFree(w.rtemp); w.rtemp = nullptr; }
     w.rtemp_bytes = 0;
@@ -641,6 +642,17 @@ int alloc_wavefront(kpt_ctx* ctx, int64_t n) {
     HIPCHK(hipMalloc((void**)&w.shadow_c, f4)); HIPCHK(hipMalloc((void**)&w.shadow_z, f4));
     HIPCHK(hipMalloc((void**)&w.sort_keys, i4)); HIPCHK(hipMalloc((void**)&w.sort_tmp, i4));
     HIPCHK(hipMalloc((void**)&w.counters, (size_t)WF_COUNTER_WORDS * sizeof(int32_t)));
+    // parked tail traces: the waves of the default tail grid (8 workgroups of
+    // 256 lanes per CU), 256 bytes per slot; larger grids park in these only
+    {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                                                       dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        const int64_t waves = (int64_t)cus * 8 * 4;
+        HIPCHK(hipMalloc((void**)&w.tpool, (size_t)waves * TAIL_POOL_BYTES));
+        w.tpool_waves = waves;
+    }
     HIPCHK(hipMalloc((void**)&w.hist, (size_t)64 * ((n + 255) / 256) * sizeof(int32_t)));
     for (int k = 0; k < 2; ++k) {
         HIPCHK(hipMalloc((void**)&w.rkeys[k], i4));
@@ -1851,6 +1863,10 @@ int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n) {
             return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
         ctx->tune.block = knobs[11];
     }
+    if (n >= 13) {
+        if (knobs[12] < 0 || knobs[12] > 64) return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
+        ctx->tune.suspend = knobs[12];
+    }
     return KPT_OK;
 }
 
@@ -1868,6 +1884,7 @@ int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n) {
     if (n >= 10) knobs[9] = ctx->blas_device;
     if (n >= 11) knobs[10] = ctx->tune.stack;
     if (n >= 12) knobs[11] = ctx->tune.block;
+    if (n >= 13) knobs[12] = ctx->tune.suspend;
     return KPT_OK;
 }
 

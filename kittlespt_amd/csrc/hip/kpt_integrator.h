@@ -234,14 +234,20 @@ KD CacheVertex cache_vertex(const FrameConst& fc, const CacheBuffers& cb, const 
 // path that ends in the network returns false after its next-event estimate.
 // heat (COUNT with cnt): the path's pixel's DebugData pair, to which the
 // closest-hit trace's BLAS and TLAS node pops are added.
+// li_shade is the vertex after its closest-hit trace (h, from ps.ray): the
+// part li_bounce and the wavefront tail share.
+template <bool COUNT = false, bool DEFER_SKY = false, bool CACHE = false>
+KD bool li_shade(const SceneView& S, const FrameConst& fc, PathState& ps, uint32_t& seed, const Hit& h,
+                 const StackShape& lds, bool* ovf, VisOut* vs, TraceCounts* cnt = nullptr, bool* missed = nullptr,
+                 const CacheBuffers* cb = nullptr, const CacheFrame* cf = nullptr, int32_t path = -1,
+                 uint32_t* spill = nullptr);
+
 template <bool COUNT = false, bool DEFER_SKY = false, bool CACHE = false>
 KD bool li_bounce(const SceneView& S, const FrameConst& fc, PathState& ps, uint32_t& seed, const StackShape& lds,
                   bool* ovf, VisOut* vs, TraceCounts* cnt = nullptr, bool* missed = nullptr,
                   const CacheBuffers* cb = nullptr, const CacheFrame* cf = nullptr, int32_t path = -1,
                   int32_t* heat = nullptr) {
-    const float eta_scale = 1.0f;
     seed = seed + (uint32_t)ps.depth;
-    const bool first = (ps.depth == 0);
     bool o = false;
     const uint32_t b0 = (COUNT && cnt) ? cnt->interior + cnt->leaf : 0u, t0 = (COUNT && cnt) ? cnt->tlas : 0u;
     Hit h = trace_closest<COUNT>(S, ps.ray, 0.0f, __builtin_inff(), lds, &o, cnt);
@@ -250,6 +256,15 @@ KD bool li_bounce(const SceneView& S, const FrameConst& fc, PathState& ps, uint3
         atomicAdd(&heat[1], (int32_t)(cnt->tlas - t0));
     }
     *ovf = *ovf || o;
+    return li_shade<COUNT, DEFER_SKY, CACHE>(S, fc, ps, seed, h, lds, ovf, vs, cnt, missed, cb, cf, path);
+}
+
+template <bool COUNT, bool DEFER_SKY, bool CACHE>
+KD bool li_shade(const SceneView& S, const FrameConst& fc, PathState& ps, uint32_t& seed, const Hit& h,
+                 const StackShape& lds, bool* ovf, VisOut* vs, TraceCounts* cnt, bool* missed,
+                 const CacheBuffers* cb, const CacheFrame* cf, int32_t path, uint32_t* spill) {
+    const float eta_scale = 1.0f;
+    const bool first = (ps.depth == 0);
     if (DEFER_SKY && h.slot < 0 && !first) {
         *missed = true;
         return false;
@@ -326,7 +341,9 @@ KD bool li_bounce(const SceneView& S, const FrameConst& fc, PathState& ps, uint3
         if (ne.trace) {
             bool oo = false;
             Ray sr = make_ray(ne.org, ne.dir);
-            bool occluded = trace_any<COUNT>(S, sr, 0.0f, ne.tmax, lds, &oo, cnt);
+            // the caller's stack spill area when it has one (its closest-hit trace has ended)
+            bool occluded = spill ? trace_any_in<COUNT>(S, sr, 0.0f, ne.tmax, lds, spill, &oo, cnt)
+                                  : trace_any<COUNT>(S, sr, 0.0f, ne.tmax, lds, &oo, cnt);
             *ovf = *ovf || oo;
             ps.L = ps.L + (occluded ? ne.occl : ne.unoccl);
         } else {

@@ -27,6 +27,8 @@ struct FrameBuffers {
 constexpr int WF_COUNTER_SLOTS = 128;
 constexpr int WF_COUNTER_STRIDE = 64;
 constexpr int WF_COUNTER_WORDS = WF_COUNTER_SLOTS * WF_COUNTER_STRIDE;
+// W.tpool: bytes per wave (64 parked traces of 256 bytes)
+constexpr int TAIL_POOL_BYTES = 64 * 256;
 
 struct WavefrontState {
     int64_t capacity;
@@ -58,6 +60,10 @@ struct WavefrontState {
     int32_t* rvals[2];
     void* rtemp;
     size_t rtemp_bytes;
+    // the tail's parked closest-hit traces: TAIL_POOL slots per wave for the
+    // first tpool_waves waves of its grid (null: nothing is parked)
+    float4* tpool;
+    int64_t tpool_waves;
     // counting mode, one view: per pixel, the BLAS and TLAS node pops of the
     // frame's closest-hit traces (DebugData, blas.cuh:39-42); null otherwise
     int32_t* heat;
@@ -147,6 +153,8 @@ struct TraceTuning {
     int tail;          // from this bounce on, live paths finish in one path-per-lane launch (0: never)
     int stack;         // traversal stack entries per lane held in LDS (LDS_STACK_MIN..LDS_STACK_MAX)
     int block;         // lanes per workgroup of the persistent trace, shadow and tail kernels (64, 128, 256)
+    int suspend;       // tail: park a vertex's closest-hit traces still running when fewer than this many
+                       // lanes of the wave trace (0: never)
 };
 
 // Events bracketing every closest-hit trace launch of a frame (recorded on the

@@ -157,6 +157,12 @@ struct LaneStack {
         --sp;
         return sp < depth ? lds[sp * stride] : spill[sp - depth];
     }
+    // entry i (< STACK_TOTAL) read or written in place (a parked trace's copy)
+    KD uint32_t at(int i) const { return i < depth ? lds[i * stride] : spill[i - depth]; }
+    KD void set(int i, uint32_t e) {
+        if (i < depth) lds[i * stride] = e;
+        else spill[i - depth] = e;
+    }
     KD void reset(const StackShape& sh, uint32_t* spill_) {
         // stride and depth are the same for every lane: read them as such, so
         // they stay in scalar registers wherever the traversal runs
@@ -520,6 +526,17 @@ KD Hit trace_closest(const SceneView& S, const Ray& w, float tmin, float tmax, c
     }
     *overflow = tr.st.overflow;
     return tr.h;
+}
+
+template <bool COUNT>
+KD bool trace_any_in(const SceneView& S, const Ray& w, float tmin, float tmax, const StackShape& sh, uint32_t* spill,
+                     bool* overflow, TraceCounts* cnt) {
+    AnyTrav tr;
+    if (tr.init(S, w, tmin, tmax, sh, spill)) {
+        while (tr.template step<COUNT>(S, cnt)) {}
+    }
+    *overflow = tr.st.overflow;
+    return tr.hit;
 }
 
 template <bool COUNT>

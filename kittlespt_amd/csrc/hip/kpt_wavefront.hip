@@ -529,17 +529,40 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_SHADE void k_shade(SceneView S, F
 // (path, direction, throughput) to the sky queue (the shadow-ray arrays,
 // unused while the tail runs; count in C_SHADOW, cleared before the launch)
 // instead of evaluating the sky with the few lanes that missed.
-// CACHE: the radiance cache's hooks at each path's cache vertex (li_bounce).
+// Parked traces (suspend > 0): a vertex's closest-hit trace holds the whole
+// wave until its longest ray ends, while most lanes have finished.  Once
+// fewer than `suspend` lanes are still tracing (and the wave can still claim
+// new paths), the wave stops the loop: each unfinished trace is parked — its
+// path state written back to the path arrays as a queued path's, its
+// traversal state and stack to one of the wave's slots in W.tpool — and its
+// lane goes on as an idle lane.  Idle lanes take parked traces before new
+// paths, load the path as a claimed one and continue the traversal from the
+// state it stopped in, so every ray runs the same steps in the same order and
+// the image is unchanged.  Each parked trace has run at least one step since
+// it was begun or taken back, and a wave that has drained the queue parks
+// nothing, so every trace ends.
+// CACHE: the radiance cache's hooks at each path's cache vertex (li_shade).
+constexpr int TAIL_POOL = 64;         // parked traces per wave
+constexpr int TAIL_SLOT_HEAD = 4;     // float4 of a parked trace's state before its stack
+constexpr int TAIL_SAVE_STACK = 48;   // stack entries a parked trace may hold
+constexpr int TAIL_SLOT_F4 = TAIL_SLOT_HEAD + TAIL_SAVE_STACK / 4;   // float4 per parked trace
+static_assert(TAIL_POOL * TAIL_SLOT_F4 * 16 == TAIL_POOL_BYTES, "pool layout");
+constexpr int TAIL_CHECK = 16;        // trace steps per lane between parking checks
+
 template <bool COUNT, bool CACHE>
 __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TAIL void k_tail(SceneView S, FrameConst fc, WavefrontState W, int parity,
                                                    int pull_slot, int32_t* overflow,
                                                    unsigned long long* counters, const int32_t* order,
-                                                   CacheBuffers cb, CacheFrame cf, int stack_depth) {
+                                                   CacheBuffers cb, CacheFrame cf, int stack_depth, int suspend) {
     extern __shared__ uint32_t s_dyn[];   // stack_depth x blockDim.x words
     const StackShape lds{s_dyn + threadIdx.x, (int)blockDim.x, stack_depth};
     const int count = W.counters[cs(C_QCOUNT + parity)];
     const int32_t* q = order ? order : W.queue[parity];
     int32_t* ctr = &W.counters[cs(pull_slot)];
+    const int64_t gwave = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    float4* pool = (suspend > 0 && W.tpool && gwave < W.tpool_waves)
+                       ? W.tpool + gwave * (int64_t)(TAIL_POOL * TAIL_SLOT_F4) : nullptr;
+    int npool = 0;   // wave-uniform: parked traces in the wave's slots [0, npool)
     bool ovf = false;
     bool active = false, drained = false;   // drained: wave-uniform
     int p = -1;
@@ -549,46 +572,138 @@ __global__ __launch_bounds__(WF_BLOCK) KPT_WPE_TAIL void k_tail(SceneView S, Fra
     TraceCounts c = {0, 0, 0, 0, 0};
     uint32_t n_paths = 0, n_vertices = 0;
     int32_t* heat_px = nullptr;   // counting mode: the lane's pixel in W.heat
+    uint32_t spill[SPILL_STACK];
     for (;;) {
+        uint64_t idle = __ballot(!active);
+        bool fresh = false, resumed = false;
+        int slot = 0;
+        // wave-uniform: parked traces first, the newest first
+        if (idle != 0ull && npool > 0) {
+            const int r = min(npool, __popcll(idle));
+            const int rank = __popcll(idle & lanes_below());
+            if (!active && rank < r) {
+                slot = npool - 1 - rank;
+                p = (int)fbits(pool[slot * TAIL_SLOT_F4].x);
+                resumed = true;
+            }
+            npool -= r;
+            idle = __ballot(!active && !resumed);
+        }
         // wave-uniform: claim one queue entry per idle lane
-        const uint64_t idle = __ballot(!active);
         if (idle != 0ull && !drained) {
             const int base = wave_claim(ctr, __popcll(idle));
             if (base + __popcll(idle) >= count) drained = true;
             const int k = base + __popcll(idle & lanes_below());
-            if (!active && k < count) {
+            if (!active && !resumed && k < count) {
                 p = q[k];
-                const float4 o4 = W.ray_o[p], d4 = W.ray_d[p], b4 = W.beta[p], l4 = W.L[p], pp = W.prev_pos[p];
-                ps.ray = make_ray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z));
-                ps.L = mk3(l4.x, l4.y, l4.z);
-                ps.beta = mk3(b4.x, b4.y, b4.z);
-                ps.p_b = b4.w;
-                st = fbits(l4.w);
-                ps.depth = (int32_t)(st & 0xFFFFu);
-                ps.any_nonspec = (st >> 16) & 1u;
-                ps.wide_in = (st >> 17) & 1u;
-                ps.cached = (st >> 18) & 1u;
-                ps.prev_pos = mk3(pp.x, pp.y, pp.z);
-                seed = fbits(d4.w);
-                active = true;
+                fresh = true;
                 if (COUNT) ++n_paths;
-                if (COUNT && W.heat) heat_px = W.heat + 2 * (int64_t)fbits(o4.w);
             }
         }
-        if (!__any(active)) break;   // wave-uniform: queue drained, every lane idle
-        bool missed = false;
+        if (fresh || resumed) {
+            const float4 o4 = W.ray_o[p], d4 = W.ray_d[p], b4 = W.beta[p], l4 = W.L[p], pp = W.prev_pos[p];
+            ps.ray = make_ray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z));
+            ps.L = mk3(l4.x, l4.y, l4.z);
+            ps.beta = mk3(b4.x, b4.y, b4.z);
+            ps.p_b = b4.w;
+            st = fbits(l4.w);
+            ps.depth = (int32_t)(st & 0xFFFFu);
+            ps.any_nonspec = (st >> 16) & 1u;
+            ps.wide_in = (st >> 17) & 1u;
+            ps.cached = (st >> 18) & 1u;
+            ps.prev_pos = mk3(pp.x, pp.y, pp.z);
+            seed = fbits(d4.w);
+            active = true;
+            if (COUNT && W.heat) heat_px = W.heat + 2 * (int64_t)fbits(o4.w);
+        }
+        if (!__any(active)) break;   // wave-uniform: queue drained, nothing parked, every lane idle
+        // the vertex's closest-hit trace: begun here, or taken back where it was parked
+        bool missed = false, parked = false;
         if (active) {
-            bool o = false;
-            if (COUNT && !is_zero(ps.beta)) ++n_vertices;
-            const bool more = !is_zero(ps.beta) &&
-                              li_bounce<COUNT, true, CACHE>(S, fc, ps, seed, lds, &o, &vs, &c,
-                                                            &missed, &cb, &cf, p, heat_px);
+            bool o = false, more = false;
+            if (resumed || !is_zero(ps.beta)) {
+                ClosestTrav tr;
+                bool busy = true;
+                if (resumed) {
+                    // the world ray is the path's ray (as the trace was begun with it), the
+                    // object ray recomputed from it as the instance step computed it
+                    const float4* sl = pool + slot * TAIL_SLOT_F4;
+                    const float4 a0 = sl[0], a1 = sl[1], a2 = sl[2], a3 = sl[3];
+                    tr.w = ps.ray;
+                    tr.tmin = a0.y; tr.tmax = a0.z; tr.h.t = a0.w;
+                    tr.h.slot = (int32_t)fbits(a1.x); tr.h.inst = (int32_t)fbits(a1.y); tr.h.u = a1.z; tr.h.v = a1.w;
+                    tr.cur_inst = (int32_t)fbits(a2.x); tr.cur = fbits(a2.y); tr.la = fbits(a2.z); tr.lb = fbits(a2.w);
+                    const int sp = (int)fbits(a3.x);
+                    tr.o = tr.cur_inst >= 0 ? object_ray(S.inst_trav[tr.cur_inst], tr.w) : tr.w;
+                    tr.st.reset(lds, spill);
+                    for (int i = 0; i < sp; i += 4) {
+                        const float4 e = sl[TAIL_SLOT_HEAD + i / 4];
+                        tr.st.set(i, fbits(e.x));
+                        if (i + 1 < sp) tr.st.set(i + 1, fbits(e.y));
+                        if (i + 2 < sp) tr.st.set(i + 2, fbits(e.z));
+                        if (i + 3 < sp) tr.st.set(i + 3, fbits(e.w));
+                    }
+                    tr.st.sp = sp;
+                } else {
+                    if (COUNT) ++n_vertices;
+                    seed = seed + (uint32_t)ps.depth;
+                    busy = tr.init(S, ps.ray, 0.0f, __builtin_inff(), lds, spill);
+                }
+                const uint32_t b0 = COUNT ? c.interior + c.leaf : 0u, t0 = COUNT ? c.tlas : 0u;
+                // TAIL_CHECK steps per lane between looks at how many lanes still trace
+                for (;;) {
+                    for (int j = 0; j < TAIL_CHECK && busy; ++j) busy = tr.template step<COUNT>(S, &c);
+                    const uint64_t m = __ballot(busy);   // the same for every lane in here
+                    if (m == 0ull) break;
+                    if (pool && !drained && __popcll(m) < suspend && __popcll(m) <= TAIL_POOL - npool &&
+                        __ballot(busy && tr.st.sp > TAIL_SAVE_STACK) == 0ull)
+                        break;
+                }
+                if (COUNT && heat_px) {
+                    atomicAdd(&heat_px[0], (int32_t)(c.interior + c.leaf - b0));
+                    atomicAdd(&heat_px[1], (int32_t)(c.tlas - t0));
+                }
+                // the traces still running are parked in the wave's slots from npool on
+                const uint64_t mp = __ballot(busy);
+                if (busy) {
+                    // the path as a queued one (seed already advanced: its trace has begun); the
+                    // pixel in the origin's w stays
+                    *reinterpret_cast<float3*>(&W.ray_o[p]) = make_float3(ps.ray.o.x, ps.ray.o.y, ps.ray.o.z);
+                    W.ray_d[p] = make_float4(ps.ray.d.x, ps.ray.d.y, ps.ray.d.z, bitsf(seed));
+                    W.beta[p] = make_float4(ps.beta.x, ps.beta.y, ps.beta.z, ps.p_b);
+                    W.L[p] = make_float4(ps.L.x, ps.L.y, ps.L.z,
+                                         bitsf((uint32_t)ps.depth | (ps.any_nonspec ? 0x10000u : 0u) |
+                                               (ps.wide_in ? 0x20000u : 0u) | (ps.cached ? 0x40000u : 0u)));
+                    W.prev_pos[p] = make_float4(ps.prev_pos.x, ps.prev_pos.y, ps.prev_pos.z, 0.0f);
+                    float4* sl = pool + (npool + __popcll(mp & lanes_below())) * TAIL_SLOT_F4;
+                    const int sp = tr.st.sp;
+                    sl[0] = make_float4(bitsf((uint32_t)p), tr.tmin, tr.tmax, tr.h.t);
+                    sl[1] = make_float4(bitsf((uint32_t)tr.h.slot), bitsf((uint32_t)tr.h.inst), tr.h.u, tr.h.v);
+                    sl[2] = make_float4(bitsf((uint32_t)tr.cur_inst), bitsf(tr.cur), bitsf(tr.la), bitsf(tr.lb));
+                    sl[3] = make_float4(bitsf((uint32_t)sp), 0.0f, 0.0f, 0.0f);
+                    for (int i = 0; i < sp; i += 4) {
+                        const uint32_t e0 = tr.st.at(i);
+                        const uint32_t e1 = i + 1 < sp ? tr.st.at(i + 1) : 0u;
+                        const uint32_t e2 = i + 2 < sp ? tr.st.at(i + 2) : 0u;
+                        const uint32_t e3 = i + 3 < sp ? tr.st.at(i + 3) : 0u;
+                        sl[TAIL_SLOT_HEAD + i / 4] = make_float4(bitsf(e0), bitsf(e1), bitsf(e2), bitsf(e3));
+                    }
+                    parked = true;
+                } else {
+                    o = tr.st.overflow;
+                    more = li_shade<COUNT, true, CACHE>(S, fc, ps, seed, tr.h, lds, &o, &vs, &c, &missed, &cb, &cf,
+                                                        p, spill);
+                }
+            }
             ovf = ovf || o;
-            if (!more || is_zero(ps.beta)) {
+            if (parked) {
+                active = false;
+            } else if (!more || is_zero(ps.beta)) {
                 W.L[p] = make_float4(ps.L.x, ps.L.y, ps.L.z, bitsf((uint32_t)ps.depth | (st & 0x10000u)));
                 active = false;
             }
         }
+        npool += __popcll(__ballot(parked));   // wave-uniform
         // wave-uniform: one atomic per wave for the lanes whose ray escaped
         const uint64_t mm = __ballot(missed);
         if (mm != 0ull) {
@@ -787,16 +902,16 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
             unsigned long long* tc = (ins && ins->counters) ? ins->counters : nullptr;
             if (tc && with_cache)
                 hipLaunchKernelGGL((k_tail<true, true>), dim3(tgrid), dim3(tblock), tlds, s, S, fc, W, parity,
-                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf, tune.stack);
+                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf, tune.stack, tune.suspend);
             else if (tc)
                 hipLaunchKernelGGL((k_tail<true, false>), dim3(tgrid), dim3(tblock), tlds, s, S, fc, W, parity,
-                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf, tune.stack);
+                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf, tune.stack, tune.suspend);
             else if (with_cache)
                 hipLaunchKernelGGL((k_tail<false, true>), dim3(tgrid), dim3(tblock), tlds, s, S, fc, W, parity,
-                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf, tune.stack);
+                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf, tune.stack, tune.suspend);
             else
                 hipLaunchKernelGGL((k_tail<false, false>), dim3(tgrid), dim3(tblock), tlds, s, S, fc, W, parity,
-                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf, tune.stack);
+                                   C_PULL + 3 * bounce, overflow, tc, tail_q, cb, cf, tune.stack, tune.suspend);
             if (ins) {
                 (void)hipEventRecord(ins->tail_end, s);
                 ins->tail_launches = 1;

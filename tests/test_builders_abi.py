@@ -64,7 +64,8 @@ def test_blas_device_knob():
 
 def test_stack_and_workgroup_knobs():
     """LDS stack entries per lane (8..32) and workgroup size (64, 128, 256)
-    of the wavefront's trace kernels: defaults, accepted values, rejections."""
+    of the wavefront's trace kernels, and the tail's parking threshold
+    (0..64): defaults, accepted values, rejections."""
     c = Context(-2)
     t = c.tuning()
     assert (t["stack"], t["block"]) == (16, 256)
@@ -76,6 +77,15 @@ def test_stack_and_workgroup_knobs():
             c.set_tuning(**bad)
         assert e.value.code == abi.KPT_ERR_INVALID_ARG
     assert (c.tuning()["stack"], c.tuning()["block"]) == (24, 256)
+    assert c.tuning()["suspend"] == 16
+    for v in (0, 8, 64):
+        c.set_tuning(suspend=v)
+        assert c.tuning()["suspend"] == v
+    for bad in (-1, 65):
+        with pytest.raises(abi.KptError) as e:
+            c.set_tuning(suspend=bad)
+        assert e.value.code == abi.KPT_ERR_INVALID_ARG
+    assert c.tuning()["suspend"] == 64
     c.close()
 
 

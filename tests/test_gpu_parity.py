@@ -174,7 +174,9 @@ def test_wavefront_equals_megakernel(plain):
 @pytest.mark.parametrize("knobs", [{"ray_sort": 1}, {"ray_sort": 1, "refill": 1}, {"sort": 0, "blocks_per_cu": 3},
                                    {"tail": 1}, {"tail": 2}, {"tail": 4, "ray_sort": 1}, {"sort": 2, "tail": 2},
                                    {"stack": 8, "block": 64}, {"stack": 32, "block": 128, "tail": 1},
-                                   {"stack": 24, "block": 64, "refill": 1, "tail": 2}])
+                                   {"stack": 24, "block": 64, "refill": 1, "tail": 2},
+                                   {"tail": 1, "suspend": 0}, {"tail": 1, "suspend": 64, "blocks_per_cu": 1},
+                                   {"tail": 1, "suspend": 64, "blocks_per_cu": 1, "stack": 8, "block": 64}])
 def test_launch_shapes_do_not_change_the_image(inst_small, knobs):
     """Every tuning knob only reorders work: the image and G-buffer are the
     default launch shape's, bit for bit (80x64: the camera rays run in 8x8
@@ -184,7 +186,7 @@ def test_launch_shapes_do_not_change_the_image(inst_small, knobs):
     out = []
     for kn in ({}, knobs):
         ctx.set_tuning(refill=0, chunk=128, refill_min=16, blocks_per_cu=8, sort=1, graph=0, ray_sort=0, tail=0,
-                       stack=16, block=256)
+                       stack=16, block=256, suspend=16)
         ctx.set_tuning(**kn)
         ctx.set_settings(settings(max_depth=6, integrator=abi.INTEGRATOR_WAVEFRONT, sun_theta=sc.sun_theta))
         ctx.resize(W, H)
@@ -194,7 +196,7 @@ def test_launch_shapes_do_not_change_the_image(inst_small, knobs):
         ctx.render(3)
         out.append((ctx.readback(1), ctx.readback(2)))
     ctx.set_tuning(refill=0, chunk=128, refill_min=16, blocks_per_cu=8, sort=1, graph=0, ray_sort=0, tail=0,
-                   stack=16, block=256)
+                   stack=16, block=256, suspend=16)
     assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
     assert np.array_equal(out[0][1], out[1][1])
 
