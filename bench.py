@@ -410,10 +410,24 @@ def main():
                     help="only render this many cached 1-spp frames of the workload (for rocprofv3)")
     ap.add_argument("--blas-build-only", action="store_true",
                     help="only build the instanced scene's BLASes on host threads and on the device, print one JSON line")
+    ap.add_argument("--calibrate-gather", type=int, default=0,
+                    help="only run this many scattered 16-byte reads of a 1 GiB table (HBM counter calibration)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
 
+    if args.calibrate_gather:
+        # scattered 16-byte reads of a 1 GiB table: FETCH_SIZE per read, against
+        # the copy's streaming calibration
+        from kittlespt_amd.context import DeviceBuffer
+        records = 1 << 26
+        src, dst = DeviceBuffer(records * 16), DeviceBuffer(args.calibrate_gather * 16)
+        idx = DeviceBuffer.from_array(np.random.default_rng(7).integers(0, records, args.calibrate_gather,
+                                                                         dtype=np.int32))
+        rc = abi.lib().kpt_measure_gather16(src.ptr, records, idx.ptr, args.calibrate_gather, dst.ptr)
+        print(json.dumps({"calibrate_gather_reads": args.calibrate_gather, "table_bytes": records * 16, "rc": rc}),
+              flush=True)
+        return
     if args.calibrate_copy:
         from kittlespt_amd.context import DeviceBuffer, hip
         a, b = DeviceBuffer(args.calibrate_copy), DeviceBuffer(args.calibrate_copy)

@@ -511,6 +511,17 @@ int kpt_write_pfm(const char* path, const float* px, int32_t width, int32_t heig
 int kpt_write_exr(const char* path, const float* px, int32_t width, int32_t height, int32_t channels,
                   int32_t flip_y);
 
+/* ---- measurement aid (no reference counterpart) ----
+ * dst[i] = src[idx[i]] for i < n: 16-byte records, one lane per record, on the
+ * default stream, synchronised before return.  The traversal kernels' memory
+ * pattern — one 16-byte load per lane from an address unrelated to its
+ * neighbours' — with a known byte count, so that rocprofv3's FETCH_SIZE of
+ * scattered reads can be calibrated against it.  src, idx and dst are device
+ * pointers; an index outside [0, src_records) reads record 0.
+ * KPT_ERR_INVALID_ARG on null pointers or negative sizes, KPT_ERR_HIP when
+ * the launch fails. */
+int kpt_measure_gather16(const void* src, int64_t src_records, const int32_t* idx, int64_t n, void* dst);
+
 #ifdef __cplusplus
 }
 #endif

@@ -251,6 +251,7 @@ SIGNATURES = [
     ("kpt_write_png_rgba8", C.c_int, [C.c_char_p, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     ("kpt_write_pfm", C.c_int, [C.c_char_p, _P, C.c_int32, C.c_int32, C.c_int32]),
     ("kpt_write_exr", C.c_int, [C.c_char_p, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    ("kpt_measure_gather16", C.c_int, [_P, C.c_int64, _P, C.c_int64, _P]),
     ("kpt_set_settings", C.c_int, [_P, C.POINTER(KptSettings)]),
     ("kpt_get_settings", C.c_int, [_P, C.POINTER(KptSettings)]),
     ("kpt_set_view", C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),

@@ -694,6 +694,15 @@ static hipError_t replay_captured(kpt_ctx* ctx, hipGraphExec_t* exec, Fn&& enque
     return e;
 }
 
+// Scattered 16-byte reads with a known byte count (kpt_measure_gather16).
+__global__ __launch_bounds__(256) void k_gather16(const float4* src, int64_t src_records, const int32_t* idx,
+                                                  int64_t n, float4* dst) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int32_t k = idx[i];
+    dst[i] = src[(k >= 0 && k < src_records) ? k : 0];
+}
+
 extern "C" {
 
 int kpt_get_abi_version(void) { return KPT_ABI_VERSION; }
@@ -1933,6 +1942,15 @@ int kpt_get_stage_stats(const kpt_ctx* ctx, int32_t stage, kpt_stage_stats* out)
 int kpt_last_timing(const kpt_ctx* ctx, double out[4]) {
     if (!ctx || !out) return KPT_ERR_INVALID_ARG;
     for (int i = 0; i < 4; ++i) out[i] = ctx->timing[i];
+    return KPT_OK;
+}
+
+int kpt_measure_gather16(const void* src, int64_t src_records, const int32_t* idx, int64_t n, void* dst) {
+    if (!src || !idx || !dst || src_records <= 0 || n < 0) return KPT_ERR_INVALID_ARG;
+    if (n == 0) return KPT_OK;
+    hipLaunchKernelGGL(k_gather16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr,
+                       static_cast<const float4*>(src), src_records, idx, n, static_cast<float4*>(dst));
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return KPT_ERR_HIP;
     return KPT_OK;
 }
 
