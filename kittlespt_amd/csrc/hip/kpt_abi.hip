@@ -616,7 +616,7 @@ void free_wavefront(kpt_ctx* c) {
     dev_free(&w.ray_o); dev_free(&w.ray_d); dev_free(&w.beta); dev_free(&w.L); dev_free(&w.prev_pos);
     dev_free(&w.vis_a); dev_free(&w.vis_n); dev_free(&w.vis_p); dev_free(&w.hit); dev_free(&w.hit_v);
     dev_free(&w.queue[0]); dev_free(&w.queue[1]); dev_free(&w.shadow_queue); dev_free(&w.shadow_o);
-    dev_free(&w.shadow_d); dev_free(&w.shadow_c); dev_free(&w.shadow_z); dev_free(&w.sort_keys);
+    dev_free(&w.shadow_d); dev_free(&w.shadow_c); dev_free(&w.shadow_z);
     dev_free(&w.sort_tmp); dev_free(&w.counters); dev_free(&w.hist); dev_free(&w.tpool);
     w.tpool_waves = 0;
     for (int k = 0; k < 2; ++k) { dev_free(&w.rkeys[k]); dev_free(&w.rvals[k]); }
@@ -640,7 +640,7 @@ int alloc_wavefront(kpt_ctx* ctx, int64_t n) {
     HIPCHK(hipMalloc((void**)&w.shadow_queue, i4));
     HIPCHK(hipMalloc((void**)&w.shadow_o, f4)); HIPCHK(hipMalloc((void**)&w.shadow_d, f4));
     HIPCHK(hipMalloc((void**)&w.shadow_c, f4)); HIPCHK(hipMalloc((void**)&w.shadow_z, f4));
-    HIPCHK(hipMalloc((void**)&w.sort_keys, i4)); HIPCHK(hipMalloc((void**)&w.sort_tmp, i4));
+    HIPCHK(hipMalloc((void**)&w.sort_tmp, i4));
     HIPCHK(hipMalloc((void**)&w.counters, (size_t)WF_COUNTER_WORDS * sizeof(int32_t)));
     // parked tail traces: the waves of the default tail grid (8 workgroups of
     // 256 lanes per CU), 256 bytes per slot; larger grids park in these only

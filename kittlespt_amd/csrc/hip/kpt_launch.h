@@ -21,7 +21,6 @@ struct FrameBuffers {
     float4* output;
 };
 
-// Per-path state of the wavefront tracer (struct of arrays in HBM).
 // W.counters: WF_COUNTER_SLOTS queue counters, WF_COUNTER_STRIDE words
 // (256 bytes) apart
 constexpr int WF_COUNTER_SLOTS = 128;
@@ -30,6 +29,7 @@ constexpr int WF_COUNTER_WORDS = WF_COUNTER_SLOTS * WF_COUNTER_STRIDE;
 // W.tpool: bytes per wave (64 parked traces of 256 bytes)
 constexpr int TAIL_POOL_BYTES = 64 * 256;
 
+// Per-path state of the wavefront tracer (struct of arrays in HBM).
 struct WavefrontState {
     int64_t capacity;
     // path records
@@ -51,7 +51,6 @@ struct WavefrontState {
     float4* shadow_d;     // xyz direction
     float4* shadow_c;     // contribution when unoccluded
     float4* shadow_z;     // contribution when occluded ((0*beta)/p)
-    int32_t* sort_keys;
     int32_t* sort_tmp;
     int32_t* counters;    // queue sizes and pull counters (WF_COUNTER_WORDS words)
     int32_t* hist;        // material histogram for the shading sort

@@ -543,12 +543,7 @@ template <bool COUNT>
 KD bool trace_any(const SceneView& S, const Ray& w, float tmin, float tmax, const StackShape& sh, bool* overflow,
                   TraceCounts* cnt) {
     uint32_t spill[SPILL_STACK];
-    AnyTrav tr;
-    if (tr.init(S, w, tmin, tmax, sh, spill)) {
-        while (tr.template step<COUNT>(S, cnt)) {}
-    }
-    *overflow = tr.st.overflow;
-    return tr.hit;
+    return trace_any_in<COUNT>(S, w, tmin, tmax, sh, spill, overflow, cnt);
 }
 
 // XCD-aware workgroup swizzle: the dispatcher hands consecutive workgroup ids
