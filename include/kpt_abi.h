@@ -391,7 +391,13 @@ int kpt_get_heat_map(kpt_ctx* ctx, int32_t* counts, float* rgba, int64_t n_pixel
  * vertex's closest-hit trace, and the wave still claims new paths, the
  * unfinished traces are parked (their path state in the path arrays, their
  * traversal state and stack, up to 48 entries, in a per-wave pool) and taken
- * back by the wave's next idle lanes; the image does not change.
+ * back by the wave's next idle lanes; the image does not change, [13] (when
+ * n >= 14) two frames in flight (1, default) or one (0): a render of several
+ * frames runs every other frame's path stages on a second stream with its
+ * own path state, each frame's accumulation and post chain after the
+ * previous frame's, so the outputs are the one-at-a-time frames' (frames
+ * that train or query the radiance cache, counted frames and captured graphs
+ * run one at a time).
  * n >= 5. */
 int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n);
 int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n);

@@ -29,8 +29,9 @@ SPACE = {
     "stack": [8, 16, 24, 32],
     "block": [64, 128, 256],
     "suspend": [0, 8, 16, 24],
+    "overlap": [0, 1],
 }
-ORDER = ["tail", "suspend", "ray_sort", "refill", "blocks_per_cu", "stack", "block", "sort", "graph", "chunk",
+ORDER = ["overlap", "tail", "suspend", "ray_sort", "refill", "blocks_per_cu", "stack", "block", "sort", "graph", "chunk",
          "refill_min"]
 
 

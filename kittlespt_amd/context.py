@@ -229,7 +229,7 @@ class Context:
         return out[0], bool(out[1])
 
     TUNING_KEYS = ("refill", "chunk", "refill_min", "blocks_per_cu", "sort", "graph", "ray_sort", "tail",
-                   "tlas_device", "blas_device", "stack", "block", "suspend")
+                   "tlas_device", "blas_device", "stack", "block", "suspend", "overlap")
 
     def tuning(self):
         n = len(self.TUNING_KEYS)

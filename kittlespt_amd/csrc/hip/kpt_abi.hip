@@ -102,6 +102,12 @@ struct kpt_ctx {
     int32_t out_width = 0, out_height = 0;   // output size (resizeResolution's arguments)
     FrameBuffers fb{};
     WavefrontState ws{};
+    // a second frame in flight (the overlap knob): its path state and stream,
+    // and per stream the event marking its last frame's finished outputs
+    int overlap = 1;
+    WavefrontState ws2{};
+    hipStream_t stream2 = nullptr;
+    hipEvent_t lane_done[2] = {nullptr, nullptr};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, tev0 = nullptr, tev1 = nullptr;
     static constexpr int INS_RING = 4;   // frames whose trace events may be in flight
     WfInstrument ins[INS_RING]{};
@@ -611,8 +617,7 @@ void default_exposure(ExposureState* e) {
     e->avg_lum = 0.0f;
 }
 
-void free_wavefront(kpt_ctx* c) {
-    WavefrontState& w = c->ws;
+void free_wf_state(WavefrontState& w) {
     dev_free(&w.ray_o); dev_free(&w.ray_d); dev_free(&w.beta); dev_free(&w.L); dev_free(&w.prev_pos);
     dev_free(&w.vis_a); dev_free(&w.vis_n); dev_free(&w.vis_p); dev_free(&w.hit); dev_free(&w.hit_v);
     dev_free(&w.queue[0]); dev_free(&w.queue[1]); dev_free(&w.shadow_queue); dev_free(&w.shadow_o);
@@ -625,10 +630,15 @@ void free_wavefront(kpt_ctx* c) {
     w.capacity = 0;
 }
 
-int alloc_wavefront(kpt_ctx* ctx, int64_t n) {
-    WavefrontState& w = ctx->ws;
+void free_wavefront(kpt_ctx* c) {
+    free_wf_state(c->ws);
+    free_wf_state(c->ws2);
+}
+
+// Path state for n paths in w (ctx->ws, or ctx->ws2 for a second frame in flight).
+int alloc_wf_state(kpt_ctx* ctx, WavefrontState& w, int64_t n) {
     if (w.capacity >= n) return KPT_OK;
-    free_wavefront(ctx);
+    free_wf_state(w);
     size_t f4 = (size_t)n * sizeof(float4);
     size_t i4 = (size_t)n * sizeof(int32_t);
     HIPCHK(hipMalloc((void**)&w.ray_o, f4)); HIPCHK(hipMalloc((void**)&w.ray_d, f4));
@@ -663,6 +673,8 @@ int alloc_wavefront(kpt_ctx* ctx, int64_t n) {
     w.capacity = n;
     return KPT_OK;
 }
+
+int alloc_wavefront(kpt_ctx* ctx, int64_t n) { return alloc_wf_state(ctx, ctx->ws, n); }
 
 }  // namespace
 
@@ -781,6 +793,10 @@ int kpt_ctx_destroy(kpt_ctx* ctx) {
     if (!ctx->host_only) {
         (void)hipSetDevice(ctx->device);
         if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
+        for (int k = 0; k < 2; ++k)
+            if (ctx->lane_done[k]) (void)hipEventDestroy(ctx->lane_done[k]);
+        if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
         dev_free(&ctx->d_blas); dev_free(&ctx->d_tlas); dev_free(&ctx->d_tris); dev_free(&ctx->d_attrs);
         dev_free(&ctx->d_bblocks); dev_free(&ctx->d_tblocks);
         dev_free(&ctx->d_itrav); dev_free(&ctx->d_ishade); dev_free(&ctx->d_mat); dev_free(&ctx->d_tex);
@@ -1508,7 +1524,31 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
         ctx->ins[r].tail_launches = 0;
         for (int x = 0; x < WF_XSTAGES; ++x) ctx->ins[r].xlaunches[x] = 0;
     }
+    // Two frames in flight: odd frames run their path stages on a second
+    // stream with their own path state, so one frame's stages fill the GPU
+    // while the other's drain.  Frames stay independent up to their finish:
+    // each frame's finishing launch (accumulation, G-buffer) waits for the
+    // previous frame's outputs (finish, post chain, G-buffer copy), so the
+    // outputs are the sequential frames' bit for bit.  Frames that train or
+    // query the radiance cache (which carries state from frame to frame),
+    // counted frames and captured graphs run one at a time.
+    const bool overlap = wavefront && !graph && !use_cache && !ctx->count_trace && ctx->overlap && spp > 1;
+    if (overlap) {
+        if (!ctx->stream2) HIPCHK(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+        for (int k = 0; k < 2; ++k)
+            if (!ctx->lane_done[k]) HIPCHK(hipEventCreateWithFlags(&ctx->lane_done[k], hipEventDisableTiming));
+        const int rc = alloc_wf_state(ctx, ctx->ws2, npx);
+        if (rc != KPT_OK) return rc;
+        ctx->ws2.heat = nullptr;
+        // the second stream starts after everything enqueued on the first
+        HIPCHK(hipEventRecord(ctx->lane_done[1], ctx->stream));
+        HIPCHK(hipStreamWaitEvent(ctx->stream2, ctx->lane_done[1], 0));
+    }
+    hipEvent_t prev_done = nullptr;
     for (int32_t k = 0; k < spp; ++k) {
+        const int lane = overlap ? (k & 1) : 0;
+        hipStream_t const fs = lane ? ctx->stream2 : ctx->stream;
+        WavefrontState& fw = lane ? ctx->ws2 : ctx->ws;
         FrameConst fc = make_frame(ctx, ctx->frame_index);
         WfInstrument* ins = nullptr;
         if (wavefront && !graph) {
@@ -1525,9 +1565,9 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
                 // the library radix sort stays out of captured graphs
                 TraceTuning tune = ctx->tune;
                 if (graph) tune.ray_sort = 0;
-                e = wavefront_frame(ctx->stream, S, fc, ctx->frame_index, ctx->fb, ctx->ws, ctx->d_overflow,
+                e = wavefront_frame(fs, S, fc, ctx->frame_index, ctx->fb, fw, ctx->d_overflow,
                                     (int32_t)ctx->materials.size(), ctx->sort_by_material, tune, ins,
-                                    use_cache ? &hooks : nullptr);
+                                    use_cache ? &hooks : nullptr, nullptr, prev_done);
                 if (e != hipSuccess) return e;
                 if (use_cache && cs.train) {
                     // Adam steps over this frame's suffixes (they shape the next frames)
@@ -1565,11 +1605,11 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
             pp.upscale = (ctx->settings.upscale_enable || ctx->out_width != ctx->width ||
                           ctx->out_height != ctx->height) ? 1 : 0;
             pp.fsr = (pp.upscale && ctx->settings.upscale_fsr) ? 1 : 0;
-            e = launch_post_frame(ctx->stream, ctx->fb, ctx->pb, pp, ctx->num_cus);
+            e = launch_post_frame(fs, ctx->fb, ctx->pb, pp, ctx->num_cus);
             if (e != hipSuccess) return e;
             // end of executeRendering: gbuffer -> prev_gbuffer (renderer.cu:186)
             return hipMemcpyAsync(ctx->fb.prev_gbuffer, ctx->fb.gbuffer, (size_t)npx * sizeof(uint4),
-                                  hipMemcpyDeviceToDevice, ctx->stream);
+                                  hipMemcpyDeviceToDevice, fs);
         };
         if (graph) {
             // the frame's ~80 launches captured once and replayed as one graph;
@@ -1578,6 +1618,10 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
         } else {
             HIPCHK(enqueue());
         }
+        if (overlap) {
+            HIPCHK(hipEventRecord(ctx->lane_done[lane], fs));
+            prev_done = ctx->lane_done[lane];
+        }
         if (!wavefront) trace_launches += 1;
         if (use_cache) ctx->cache_adapt = cs.train != 0;
         // camera prev <- curr, frame_index++ (renderer.cu:188-194)
@@ -1585,6 +1629,8 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
         std::memcpy(ctx->cam.prev_inv_proj, ctx->cam.inv_proj, 64);
         ctx->frame_index++;
     }
+    // the first stream takes up what the second one enqueued
+    if (overlap) HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->lane_done[1], 0));
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     HIPCHK(hipEventSynchronize(ctx->ev1));
     for (int r = 0; r < kpt_ctx::INS_RING; ++r) HIPCHK(harvest(ctx->ins[r]));
@@ -1872,6 +1918,10 @@ int kpt_set_tuning(kpt_ctx* ctx, const int32_t* knobs, int32_t n) {
             return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
         ctx->tune.block = knobs[11];
     }
+    if (n >= 14) {
+        if (knobs[13] < 0 || knobs[13] > 1) return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
+        ctx->overlap = knobs[13];
+    }
     if (n >= 13) {
         if (knobs[12] < 0 || knobs[12] > 64) return fail(ctx, KPT_ERR_INVALID_ARG, "tuning knob out of range");
         ctx->tune.suspend = knobs[12];
@@ -1894,6 +1944,7 @@ int kpt_get_tuning(const kpt_ctx* ctx, int32_t* knobs, int32_t n) {
     if (n >= 11) knobs[10] = ctx->tune.stack;
     if (n >= 12) knobs[11] = ctx->tune.block;
     if (n >= 13) knobs[12] = ctx->tune.suspend;
+    if (n >= 14) knobs[13] = ctx->overlap;
     return KPT_OK;
 }
 

@@ -213,10 +213,14 @@ struct CacheHooks {
 // blas pops, 0.05 * tlas pops, 1) of heat[2i], heat[2i + 1].
 hipError_t launch_heat_colour(hipStream_t s, const int32_t* heat, int64_t n, float4* rgba);
 
+// finish_after (optional): an event the frame's finishing launch (the
+// accumulation and G-buffer writes) waits for, so that frames whose path
+// stages run on two streams still finish in frame order.
 hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& fc, int32_t frame,
                            const FrameBuffers& fb, WavefrontState& W, int32_t* overflow, int32_t num_materials,
                            int32_t sort_by_material, const TraceTuning& tune, WfInstrument* ins,
-                           const CacheHooks* cache, const ViewBatch* views = nullptr);
+                           const CacheHooks* cache, const ViewBatch* views = nullptr,
+                           hipEvent_t finish_after = nullptr);
 
 // Radiance-cache kernels (kpt_cache.hip).
 int cache_train_workgroups(int32_t batch);

@@ -832,7 +832,7 @@ int g_num_cus = 0;
 hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& fc, int32_t frame,
                            const FrameBuffers& fb, WavefrontState& W, int32_t* overflow, int32_t num_materials,
                            int32_t sort_by_material, const TraceTuning& tune, WfInstrument* ins,
-                           const CacheHooks* cache, const ViewBatch* views) {
+                           const CacheHooks* cache, const ViewBatch* views, hipEvent_t finish_after) {
     if (ins) {
         ins->launches = 0;
         ins->tail_launches = 0;
@@ -987,6 +987,10 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
     }
     if (cf.train) {
         hipError_t e = launch_cache_targets(s, cb, W.L, cache->enc.scale);
+        if (e != hipSuccess) return e;
+    }
+    if (finish_after) {
+        const hipError_t e = hipStreamWaitEvent(s, finish_after, 0);
         if (e != hipSuccess) return e;
     }
     if (views)

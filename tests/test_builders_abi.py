@@ -86,6 +86,13 @@ def test_stack_and_workgroup_knobs():
             c.set_tuning(suspend=bad)
         assert e.value.code == abi.KPT_ERR_INVALID_ARG
     assert c.tuning()["suspend"] == 64
+    assert c.tuning()["overlap"] == 1
+    c.set_tuning(overlap=0)
+    assert c.tuning()["overlap"] == 0
+    for bad in (-1, 2):
+        with pytest.raises(abi.KptError) as e:
+            c.set_tuning(overlap=bad)
+        assert e.value.code == abi.KPT_ERR_INVALID_ARG
     c.close()
 
 

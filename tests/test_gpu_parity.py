@@ -176,7 +176,8 @@ def test_wavefront_equals_megakernel(plain):
                                    {"stack": 8, "block": 64}, {"stack": 32, "block": 128, "tail": 1},
                                    {"stack": 24, "block": 64, "refill": 1, "tail": 2},
                                    {"tail": 1, "suspend": 0}, {"tail": 1, "suspend": 64, "blocks_per_cu": 1},
-                                   {"tail": 1, "suspend": 64, "blocks_per_cu": 1, "stack": 8, "block": 64}])
+                                   {"tail": 1, "suspend": 64, "blocks_per_cu": 1, "stack": 8, "block": 64},
+                                   {"overlap": 0}, {"overlap": 0, "tail": 1}])
 def test_launch_shapes_do_not_change_the_image(inst_small, knobs):
     """Every tuning knob only reorders work: the image and G-buffer are the
     default launch shape's, bit for bit (80x64: the camera rays run in 8x8
@@ -186,7 +187,7 @@ def test_launch_shapes_do_not_change_the_image(inst_small, knobs):
     out = []
     for kn in ({}, knobs):
         ctx.set_tuning(refill=0, chunk=128, refill_min=16, blocks_per_cu=8, sort=1, graph=0, ray_sort=0, tail=0,
-                       stack=16, block=256, suspend=16)
+                       stack=16, block=256, suspend=16, overlap=1)
         ctx.set_tuning(**kn)
         ctx.set_settings(settings(max_depth=6, integrator=abi.INTEGRATOR_WAVEFRONT, sun_theta=sc.sun_theta))
         ctx.resize(W, H)
@@ -196,7 +197,7 @@ def test_launch_shapes_do_not_change_the_image(inst_small, knobs):
         ctx.render(3)
         out.append((ctx.readback(1), ctx.readback(2)))
     ctx.set_tuning(refill=0, chunk=128, refill_min=16, blocks_per_cu=8, sort=1, graph=0, ray_sort=0, tail=0,
-                   stack=16, block=256, suspend=16)
+                   stack=16, block=256, suspend=16, overlap=1)
     assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
     assert np.array_equal(out[0][1], out[1][1])
 
