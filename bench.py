@@ -208,19 +208,24 @@ def rel_rmse(est, ref):
 def run_curve(ctx, pixels, ref, frames, checkpoints, warmup_frames=0, label="", cache_acc=None):
     """Renders `frames` frames of 1 spp (after `warmup_frames` frames whose
     time counts but whose samples are discarded); returns [(seconds, spp,
-    rmse)] at the checkpoints.  Seconds are HIP-event frame times.
+    rmse)] at the checkpoints.  Seconds are HIP-event times of the render
+    calls; the frames between two checkpoints are one call, so the tracer
+    runs with its frames in flight as in the headline.
     cache_acc (a dict, or None): per accumulated frame, the cache stages'
     HIP-event ms and launches and the frame's queries and training records
-    are added to it."""
+    are added to it (a call's last frame's counts stand for each of its
+    frames)."""
     seconds = 0.0
-    for _ in range(warmup_frames):
-        ctx.render(1)
-        seconds += ctx.timing()[0] / 1e3
     if warmup_frames:
+        ctx.render(warmup_frames)
+        seconds += ctx.timing()[0] / 1e3
         ctx.reset()
     out = []
-    for f in range(1, frames + 1):
-        ctx.render(1)
+    done = 0
+    for f in sorted(c for c in set(checkpoints) | {frames} if 0 < c <= frames):
+        n = f - done
+        ctx.render(n)
+        done = f
         seconds += ctx.timing()[0] / 1e3
         if cache_acc is not None:
             for stage in ("cache_infer", "cache_train"):
@@ -228,12 +233,12 @@ def run_curve(ctx, pixels, ref, frames, checkpoints, warmup_frames=0, label="", 
                 cache_acc[stage + "_ms"] = cache_acc.get(stage + "_ms", 0.0) + sst["ms"]
                 cache_acc[stage + "_launches"] = cache_acc.get(stage + "_launches", 0) + sst["launches"]
             cs = ctx.cache_stats()
-            cache_acc["queries"] = cache_acc.get("queries", 0) + int(cs["queries"])
-            cache_acc["train_records"] = cache_acc.get("train_records", 0) + int(cs["train_records"])
+            cache_acc["queries"] = cache_acc.get("queries", 0) + n * int(cs["queries"])
+            cache_acc["train_records"] = cache_acc.get("train_records", 0) + n * int(cs["train_records"])
             # the frame's Adam steps consume at most train_samples of its records
-            cache_acc["trained"] = cache_acc.get("trained", 0) + min(int(cs["train_records"]),
-                                                                   cache_acc.get("train_samples", 1 << 62))
-            cache_acc["frames"] = cache_acc.get("frames", 0) + 1
+            cache_acc["trained"] = cache_acc.get("trained", 0) + n * min(int(cs["train_records"]),
+                                                                       cache_acc.get("train_samples", 1 << 62))
+            cache_acc["frames"] = cache_acc.get("frames", 0) + n
         if f in checkpoints:
             est = ctx.estimate().reshape(-1, 3)[pixels].astype(np.float64)
             out.append((seconds, f, rel_rmse(est, ref)))
@@ -712,9 +717,9 @@ def main():
                                               / max(cached[-1][0] - cached[0][0], 1e-9) / 1e6, 3),
                "cache_warmup_frames": args.cache_warmup, "cache_nee_at_vertex": args.cache_nee,
                "cache_wide_roughness": args.cache_wide,
-               "frames_in_flight": ("one, for both tracers (one frame per render call); the plain tracer "
-                                    "renders faster with two in flight (the headline), the cached one "
-                                    "cannot yet: each frame's training feeds the next frame's inference"),
+               "frames_in_flight": ("two, for both tracers (the frames between checkpoints are one render "
+                                    "call); a cached frame waits for the previous one before its bounce-1 "
+                                    "work, whose cache records and inference follow the previous training"),
                "reference": f"FP64 oracle render of {args.ttt_pixels} random pixels x {args.ref_spp} spp "
                             f"(host cores)",
                "curve_plain": [[round(s, 4), n, round(e, 5)] for s, n, e in plain],

@@ -1529,10 +1529,13 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
     // while the other's drain.  Frames stay independent up to their finish:
     // each frame's finishing launch (accumulation, G-buffer) waits for the
     // previous frame's outputs (finish, post chain, G-buffer copy), so the
-    // outputs are the sequential frames' bit for bit.  Frames that train or
-    // query the radiance cache (which carries state from frame to frame),
-    // counted frames and captured graphs run one at a time.
-    const bool overlap = wavefront && !graph && !use_cache && !ctx->count_trace && ctx->overlap && spp > 1;
+    // outputs are the sequential frames' bit for bit.  A frame with the
+    // radiance cache waits for the previous frame already before its bounce-1
+    // work (its cache vertex, queries and records), so only the primary
+    // stages overlap and the cache's records, weights and Adam state pass
+    // from frame to frame as before.  Counted frames and captured graphs run
+    // one at a time.
+    const bool overlap = wavefront && !graph && !ctx->count_trace && ctx->overlap && spp > 1;
     if (overlap) {
         if (!ctx->stream2) HIPCHK(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
         for (int k = 0; k < 2; ++k)
@@ -1571,16 +1574,16 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
                 if (e != hipSuccess) return e;
                 if (use_cache && cs.train) {
                     // Adam steps over this frame's suffixes (they shape the next frames)
-                    if (ins) (void)hipEventRecord(ins->xbegin[WF_X_TRAIN], ctx->stream);
+                    if (ins) (void)hipEventRecord(ins->xbegin[WF_X_TRAIN], fs);
                     for (int32_t st = 0; st < std::max(1, cs.train_steps); ++st) {
-                        e = launch_cache_train_step(ctx->stream, ctx->cb, hooks.enc, ctx->cb.t_pos, ctx->cb.t_dir,
+                        e = launch_cache_train_step(fs, ctx->cb, hooks.enc, ctx->cb.t_pos, ctx->cb.t_dir,
                                                     ctx->cb.t_target, ctx->cb.counters + 1, 0, ctx->cb.train_cap,
                                                     (int64_t)st * batch, batch, cs.learning_rate,
                                                     ++ctx->cache_steps);
                         if (e != hipSuccess) return e;
                     }
                     if (ins) {
-                        (void)hipEventRecord(ins->xend[WF_X_TRAIN], ctx->stream);
+                        (void)hipEventRecord(ins->xend[WF_X_TRAIN], fs);
                         ins->xlaunches[WF_X_TRAIN] = 1;
                     }
                 }

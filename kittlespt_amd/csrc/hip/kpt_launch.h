@@ -215,7 +215,11 @@ hipError_t launch_heat_colour(hipStream_t s, const int32_t* heat, int64_t n, flo
 
 // finish_after (optional): an event the frame's finishing launch (the
 // accumulation and G-buffer writes) waits for, so that frames whose path
-// stages run on two streams still finish in frame order.
+// stages run on two streams still finish in frame order.  A frame with the
+// radiance cache then also waits for it before its first launch that reads
+// or writes the cache (its bounce-1 work: the cache vertex is never the
+// primary one), and starts its cache frame there, so that its records,
+// queries and inference follow the previous frame's training.
 hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& fc, int32_t frame,
                            const FrameBuffers& fb, WavefrontState& W, int32_t* overflow, int32_t num_materials,
                            int32_t sort_by_material, const TraceTuning& tune, WfInstrument* ins,

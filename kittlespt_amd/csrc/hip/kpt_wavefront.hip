@@ -840,11 +840,26 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
     }
     CacheBuffers cb{};
     CacheFrame cf{};
+    // the cache frame starts here, or (finish_after given) once the previous
+    // frame is done, before this frame's first launch that touches the cache
+    bool cache_started = true;
+    auto start_cache = [&]() -> hipError_t {
+        if (cache_started) return hipSuccess;
+        cache_started = true;
+        if (finish_after) {
+            const hipError_t e = hipStreamWaitEvent(s, finish_after, 0);
+            if (e != hipSuccess) return e;
+        }
+        return launch_cache_frame_start(s, cb, cf);
+    };
     if (cache && cache->cb && (cache->cf.enabled || cache->cf.train)) {
         cb = *cache->cb;
         cf = cache->cf;
-        hipError_t e = launch_cache_frame_start(s, cb, cf);
-        if (e != hipSuccess) return e;
+        cache_started = false;
+        if (!finish_after) {
+            hipError_t e = start_cache();
+            if (e != hipSuccess) return e;
+        }
     }
     if (g_num_cus == 0) {
         int dev = 0;
@@ -885,6 +900,10 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
     if (cf.enabled && !(cf.wide_roughness > 0.0f)) max_bounces = std::min(max_bounces, 3);
     for (int bounce = 0; bounce < max_bounces; ++bounce) {
         const int parity = bounce & 1;
+        if (bounce == 1) {
+            const hipError_t e = start_cache();
+            if (e != hipSuccess) return e;
+        }
         if (tune.tail > 0 && bounce >= tune.tail) {
             // the tail's paths in spatial order of their rays (ray_sort knob)
             const int32_t* tail_q = nullptr;
@@ -975,6 +994,10 @@ hipError_t wavefront_frame(hipStream_t s, const SceneView& S, const FrameConst& 
         }
         hipLaunchKernelGGL(k_shadow, dim3(tgrid), dim3(tblock), tlds, s, S, W, C_PULL + 3 * bounce + 2, overflow,
                            tune, shadow_order, cb.t_lbefore);
+    }
+    {
+        const hipError_t e = start_cache();   // frames that end at bounce 0
+        if (e != hipSuccess) return e;
     }
     if (cf.enabled) {
         if (ins) (void)hipEventRecord(ins->xbegin[WF_X_INFER], s);
