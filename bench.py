@@ -744,6 +744,9 @@ def main():
                             "peak": MFMA_PEAK_TFLOPS, "frac": round(fl / avg_s / 1e12 / MFMA_PEAK_TFLOPS, 5),
                             "samples_per_frame": int(used), "flop_per_frame": int(fl),
                             "avg_ms_per_frame": round(avg_s * 1e3, 4)}
+        for v in mlp.values():
+            v["duration"] = ("HIP events over the cached curve's render calls, two frames in flight: the next "
+                             "frame's primary stages share the GPU with these launches")
         ttt["mlp_roofline"] = mlp
         if roof is not None:
             roof.setdefault("stages", {}).update({"cache_" + k: v for k, v in mlp.items()})
