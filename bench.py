@@ -477,8 +477,9 @@ def main():
         t0 = time.time()
         # per-launch trace events need the eager launch path: graph replay is
         # measured separately below
-        best, ms, _ = autotune.tune(ctx, spp=1, log=log, skip=("graph",))
-        log(f"[bench] autotuned in {time.time() - t0:.1f}s: {best} ({ms:.2f} ms per 1-spp frame)")
+        # four frames per trial, so that two frames in flight take part in the choice
+        best, ms, _ = autotune.tune(ctx, spp=4, log=log, skip=("graph",))
+        log(f"[bench] autotuned in {time.time() - t0:.1f}s: {best} ({ms / 4:.2f} ms per 1-spp frame)")
     tuning = ctx.tuning() if integ == abi.INTEGRATOR_WAVEFRONT else None
 
     if args.profile_cached:
