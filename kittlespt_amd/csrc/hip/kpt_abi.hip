@@ -1536,6 +1536,12 @@ int kpt_render(kpt_ctx* ctx, int32_t spp) {
     // from frame to frame as before.  Counted frames and captured graphs run
     // one at a time.
     const bool overlap = wavefront && !graph && !ctx->count_trace && ctx->overlap && spp > 1;
+    // whatever the second stream still holds (a render that stopped on an
+    // error) comes before this render's work
+    if (ctx->stream2 && ctx->lane_done[0]) {
+        HIPCHK(hipEventRecord(ctx->lane_done[0], ctx->stream2));
+        HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->lane_done[0], 0));
+    }
     if (overlap) {
         if (!ctx->stream2) HIPCHK(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
         for (int k = 0; k < 2; ++k)
@@ -1877,6 +1883,7 @@ int kpt_sync(kpt_ctx* ctx) {
     if (!ctx) return KPT_ERR_INVALID_ARG;
     if (ctx->host_only) return KPT_OK;
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->stream2) HIPCHK(hipStreamSynchronize(ctx->stream2));
     return report_overflow(ctx);
 }
 
